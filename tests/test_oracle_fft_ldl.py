@@ -1,0 +1,184 @@
+"""Pins the FFT conventions and the LDL model restatement of oracle/:
+brute-force DFT, Parseval, adjoint identities, the reference VJP rules'
+compositions, finite differences and an independent torch float64 autograd
+restatement.  CPU only."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ldl_oracle as O
+
+
+def brute_r2c(x):
+    n0, n1, n2 = x.shape
+    out = np.zeros((n0, n1, n2 // 2 + 1), complex)
+    idx = np.stack(np.meshgrid(*[np.arange(n) for n in x.shape], indexing="ij"), -1)
+    for k0 in range(n0):
+        for k1 in range(n1):
+            for k2 in range(n2 // 2 + 1):
+                ph = -2j * np.pi * (k0 * idx[..., 0] / n0 + k1 * idx[..., 1] / n1 + k2 * idx[..., 2] / n2)
+                out[k0, k1, k2] = np.sum(x * np.exp(ph))
+    return out
+
+
+def test_pfft_brute_force_and_pifft():
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((4, 6, 8))
+    X = O.pfft(x)
+    np.testing.assert_allclose(X, brute_r2c(x), atol=1e-10)
+    np.testing.assert_allclose(O.pifft(X, x.shape), x * x.size, atol=1e-10)
+    # Parseval on the half spectrum with the Hermitian weights
+    c = O.hermitian_weight(8)[None, None, :]
+    assert abs(np.sum(c * np.abs(X) ** 2) / x.size - np.sum(x ** 2)) < 1e-9
+
+
+def test_pifft_ignores_imag_of_self_conjugate_planes():
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((4, 4, 5)) + 1j * rng.standard_normal((4, 4, 5))
+    a = O.pifft(X, (4, 4, 8))
+    # closed form: Re(sum_H c X e^{+ik.n})
+    idx = np.stack(np.meshgrid(*[np.arange(n) for n in (4, 4, 8)], indexing="ij"), -1)
+    c = O.hermitian_weight(8)
+    b = np.zeros((4, 4, 8))
+    for k0 in range(4):
+        for k1 in range(4):
+            for k2 in range(5):
+                ph = 2j * np.pi * (k0 * idx[..., 0] / 4 + k1 * idx[..., 1] / 4 + k2 * idx[..., 2] / 8)
+                b += np.real(c[k2] * X[k0, k1, k2] * np.exp(ph))
+    np.testing.assert_allclose(a, b, atol=1e-10)
+
+
+def test_exact_adjoints_and_reference_rules():
+    rng = np.random.default_rng(2)
+    shape = (4, 6, 8)
+    x = rng.standard_normal(shape)
+    Y = rng.standard_normal((4, 6, 5)) + 1j * rng.standard_normal((4, 6, 5))
+    g = rng.standard_normal(shape)
+    # <Y, pfft x> = <pfft^T Y, x>, <g, pifft Y> = <pifft^T g, Y>
+    lhs = np.real(np.sum(np.conj(Y) * O.pfft(x)))
+    assert abs(lhs - np.sum(O.exact_pfft_bwd(Y, shape) * x)) < 1e-9
+    lhs = np.sum(g * O.pifft(Y, shape))
+    assert abs(lhs - np.real(np.sum(np.conj(O.exact_pifft_bwd(g)) * Y))) < 1e-9
+    # the reference's pifft rule is 1/c of the exact one, its pfft rule c times
+    c = O.hermitian_weight(8)[None, None, :]
+    np.testing.assert_allclose(O.ref_rule_pifft_bwd(g) * c, O.exact_pifft_bwd(g), atol=1e-10)
+    np.testing.assert_allclose(O.ref_rule_pfft_bwd(Y, shape), O.exact_pfft_bwd(Y * c, shape), atol=1e-10)
+    # composition pfft_bwd(pifft_bwd(.)) is exact (c and 1/c cancel) on a real kernel
+    K = rng.random((4, 6, 5))
+    exact = O.exact_pfft_bwd(K * O.exact_pifft_bwd(g), shape)
+    ref = O.ref_rule_pfft_bwd(K * O.ref_rule_pifft_bwd(g), shape)
+    np.testing.assert_allclose(ref, exact, atol=1e-9)
+    # c2f doubles interior k_z planes of a real cotangent (model.py:23-29)
+    v = rng.standard_normal((4, 4, 3))
+    np.testing.assert_allclose(O.ref_rule_c2f_bwd(v), v * O.hermitian_weight(4)[None, None, :])
+
+
+def _setup(N=8, nstep=2, seed=83, L=40.0):
+    X = O.lattice_positions(N, L) + O.zeldovich_displacement(N, L, 1.5, seed)
+    rng = np.random.default_rng(seed + 1)
+    tgt = rng.random((N, N, N)) + 0.5
+    p = np.array([0.7, 1.2, 0.3, 1.4, 0.2] * nstep + [1.1, 0.9, 0.05])
+    return X, tgt, p, L
+
+
+@pytest.mark.parametrize("smoothing,L1", [(None, True), ("gauss", False), (None, False)])
+def test_ldl_param_grad_finite_difference(smoothing, L1):
+    X, tgt, p, L = _setup()
+    sm = ("gaussian", L / 8) if smoothing == "gauss" else None
+    m = O.SMModelOracle(8, L, 2, tgt, L1=L1, smoothing=sm)
+    loss, g = m.loss_and_grad(p, X)
+    for i in range(len(p)):
+        h = 1e-6 * max(abs(p[i]), 1.0)
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (m.loss(pp, X) - m.loss(pm, X)) / (2 * h)
+        assert abs(g[i] - fd) <= 1e-5 * abs(fd) + 1e-9, (i, g[i], fd)
+
+
+def test_ldl_position_grad_finite_difference():
+    X, tgt, p, L = _setup()
+    m = O.SMModelOracle(8, L, 2, tgt, L1=False)
+    _, _, gX = m.loss_and_grad(p, X, want_pos_grad=True)
+    for pi, d in [(3, 0), (77, 1), (300, 2)]:
+        h = 1e-6
+        Xp, Xm = X.copy(), X.copy()
+        Xp[pi, d] += h
+        Xm[pi, d] -= h
+        fd = (m.loss(p, Xp) - m.loss(p, Xm)) / (2 * h)
+        assert abs(gX[pi, d] - fd) <= 1e-5 * abs(fd) + 1e-10
+
+
+def test_reference_rule_changes_only_alpha():
+    X, tgt, p, L = _setup()
+    a = O.SMModelOracle(8, L, 2, tgt, L1=False)
+    b = O.SMModelOracle(8, L, 2, tgt, L1=False, grad_rule="reference")
+    _, ga = a.loss_and_grad(p, X)
+    _, gb = b.loss_and_grad(p, X)
+    alpha = [0, 5]
+    other = [i for i in range(len(p)) if i not in alpha]
+    np.testing.assert_allclose(ga[other], gb[other], rtol=1e-12)
+    assert np.all(np.abs(gb[alpha]) < np.abs(ga[alpha]))
+
+
+# ---------------- independent torch float64 autograd restatement ----------
+def _t_paint(x, mass, nm):
+    fl = torch.floor(x)
+    f = x - fl
+    i0 = fl.long()
+    out = torch.zeros(nm[0] * nm[1] * nm[2], dtype=x.dtype)
+    for o in [(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)]:
+        w = torch.ones_like(x[:, 0]) * mass
+        idx = []
+        for d in range(3):
+            w = w * (f[:, d] if o[d] else 1 - f[:, d])
+            idx.append((i0[:, d] + o[d]) % nm[d])
+        out = out.index_add(0, (idx[0] * nm[1] + idx[1]) * nm[2] + idx[2], w)
+    return out.reshape(nm)
+
+
+def _t_readout(x, F, nm):
+    fl = torch.floor(x)
+    f = x - fl
+    i0 = fl.long()
+    Ff = F.reshape(-1)
+    acc = torch.zeros_like(x[:, 0])
+    for o in [(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)]:
+        w = torch.ones_like(x[:, 0])
+        idx = []
+        for d in range(3):
+            w = w * (f[:, d] if o[d] else 1 - f[:, d])
+            idx.append((i0[:, d] + o[d]) % nm[d])
+        acc = acc + w * Ff[(idx[0] * nm[1] + idx[1]) * nm[2] + idx[2]]
+    return acc
+
+
+def test_ldl_against_torch_autograd():
+    N, L = 8, 40.0
+    X, tgt, p, _ = _setup(N=N, L=L)
+    m = O.SMModelOracle(N, L, 2, tgt, L1=False)
+    loss, g = m.loss_and_grad(p, X)
+    nm = (N, N, N)
+    pt = torch.tensor(p, requires_grad=True)
+    Xt = torch.tensor(X)
+    kn = torch.tensor(O.knorm(nm, L))
+    D = [torch.tensor(np.broadcast_to(d, (N, N, N // 2 + 1)).copy()) for d in O.fd_derivative_coeff(nm, L)]
+    cell = L / N
+    fact = N ** 3 / X.shape[0]
+    for i in range(2):
+        a, gam, kh, kl, nu = pt[5 * i: 5 * i + 5]
+        x = torch.remainder(Xt, L) / cell
+        rho = _t_paint(x, 1.0, nm) * fact
+        U = torch.fft.rfftn((rho + 1e-8) ** gam) / N ** 3
+        G = -torch.exp(-kn ** 2 / kl ** 2 - kh ** 2 / kn ** 2) * kn ** nu
+        P = 1e-2 * a * U * G
+        s = torch.stack([_t_readout(x, torch.fft.irfftn(-1j * D[d] * P, s=nm) * N ** 3, nm) for d in range(3)], -1)
+        Xt = Xt + s
+    x = torch.remainder(Xt, L) / cell
+    rho = _t_paint(x, 1.0, nm) * fact
+    mu, w, b = pt[10:13]
+    F = torch.where(w * (rho + 1e-8) ** mu + b > 1e-12, w * (rho + 1e-8) ** mu + b, torch.tensor(1e-12, dtype=torch.float64))
+    lt = torch.sum((F - torch.tensor(tgt)) ** 2) / N ** 3
+    lt.backward()
+    assert abs(lt.item() - loss) < 1e-12 * max(1, abs(loss))
+    np.testing.assert_allclose(pt.grad.numpy(), g, rtol=1e-9, atol=1e-14)
