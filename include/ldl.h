@@ -1,0 +1,212 @@
+/* libldl: MI355X (gfx950) kernels for JERALD's Lagrangian Deep Learning hot
+ * path -- CIC paint/readout, 3D r2c/c2r FFTs, the sm-kind LDL displacement
+ * operator, the ReLU response, the loss, and their hand-written adjoints.
+ *
+ * C ABI: plain pointers and sizes, no torch or HIP types.  Every entry point
+ *   * takes DEVICE pointers (caller-owned, e.g. torch-allocated) for arrays,
+ *     host pointers for the small int64_t[3] / double[k] parameter arrays;
+ *   * is enqueued on the HIP stream passed as ``stream`` (a hipStream_t, NULL
+ *     = the null stream) and returns without synchronising;
+ *   * returns LDL_OK (0) or an error code; ldl_last_error() returns the
+ *     calling thread's message for the last failure.
+ * Scratch memory comes from a caller workspace whose size the matching
+ * *_workspace_bytes() query gives.  The library's only hidden state is a
+ * mutex-guarded cache of FFT twiddle tables (one per device and precision).
+ *
+ * The reference binds its CPU kernels as XLA custom calls registered from
+ * lib/cpu.cc:271-293 (`void target(void* out, void** in)`), called from
+ * jaxpops_bind.py.  Each entry point below names the registration it replaces.
+ * Differences that are part of the contract:
+ *   * positions arrive in box units and are wrapped/scaled on the GPU with the
+ *     reference's formula ``(pos % L) / (L / N)`` (jaxpops_bind.py:163-164)
+ *     instead of on the host; or, for the fused model path, as FP32/FP64
+ *     displacements from a Lagrangian lattice (LDL_POS_LATTICE);
+ *   * readouts read at the positions they are given (the reference reads at
+ *     the positions of the last paint, lib/cpu.cc:93);
+ *   * one GPU holds the whole mesh: no MPI communicator, layout or edges.
+ */
+#ifndef LDL_H
+#define LDL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LDL_OK 0
+#define LDL_EINVAL 1      /* bad argument (shape, size, null pointer, dtype) */
+#define LDL_EHIP 2        /* a HIP runtime call failed */
+#define LDL_EUNSUPPORTED 3 /* valid request this build does not implement */
+
+#define LDL_POS_PHYSICAL 0 /* pos (Np,3) in box units, any value (wrapped periodically) */
+#define LDL_POS_LATTICE 1  /* pos (Np,3) = displacement in MESH units from lattice point
+                              (i,j,k)*lattice_step, p = (i*l1 + j)*l2 + k */
+
+typedef struct ldl_pos_desc {
+    int32_t kind;            /* LDL_POS_PHYSICAL or LDL_POS_LATTICE */
+    int32_t reserved;        /* must be 0 */
+    double box[3];           /* box size (both kinds: gives the cell size L/N) */
+    int64_t lattice[3];      /* LATTICE: lattice points per axis (Np = product) */
+    double lattice_step[3];  /* LATTICE: mesh cells between lattice points */
+} ldl_pos_desc;
+
+/* paint flags */
+#define LDL_PAINT_FIXED 0        /* 64-bit fixed-point integer atomics: deterministic, order free */
+#define LDL_PAINT_FLOAT_ATOMIC 1 /* floating-point atomics in the working precision */
+#define LDL_PAINT_WS_ZEROED 2    /* with LDL_PAINT_FIXED: the workspace accumulator is known to be
+                                    zero (every fixed-point paint leaves it zero), skip clearing it */
+
+const char* ldl_last_error(void);
+int ldl_version(void);
+
+/* ------------------------------------------------------------------ plan */
+/* Replaces buildplan_f32/_f64 (lib/cpu.cc:140-167, lib/parfft.h:33-76).
+ * Validates the mesh (each axis a power of two in [4, 1024]; the last one
+ * also >= 8) and prepares the twiddle tables for the current device.
+ * out[0] = local x extent (= N0), out[1] = local x start (= 0). */
+int ldl_buildplan(const int64_t nmesh[3], int32_t out[2]);
+
+/* ----------------------------------------------------------------- paint */
+/* Workspace for a paint of ``ncomp`` (1 or 3) components. */
+size_t ldl_paint_workspace_bytes(const int64_t nmesh[3], int32_t ncomp);
+
+/* Replaces ppaint_f32/_f64 (lib/cpu.cc:51-72 -> lib/parops_fast.h:483-531).
+ * out (N0,N1,N2) = out_scale * CIC deposit of ``mass`` (Np,) or, if mass is
+ * NULL, of the scalar mass_scalar.  Output is overwritten (zeroed first). */
+int ldl_ppaint_f32(const float* pos, const ldl_pos_desc* pd, int64_t np, const float* mass,
+                   double mass_scalar, const int64_t nmesh[3], double out_scale, int32_t flags,
+                   float* out, void* workspace, size_t workspace_bytes, void* stream);
+int ldl_ppaint_f64(const double* pos, const ldl_pos_desc* pd, int64_t np, const double* mass,
+                   double mass_scalar, const int64_t nmesh[3], double out_scale, int32_t flags,
+                   double* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Replaces ppaint3D_f32/_f64 (lib/cpu.cc:96-116 -> lib/parops_fast.h:678-725).
+ * mass3 (Np,3) row-major -> out (3,N0,N1,N2). */
+int ldl_ppaint3D_f32(const float* pos, const ldl_pos_desc* pd, int64_t np, const float* mass3,
+                     const int64_t nmesh[3], double out_scale, int32_t flags, float* out,
+                     void* workspace, size_t workspace_bytes, void* stream);
+int ldl_ppaint3D_f64(const double* pos, const ldl_pos_desc* pd, int64_t np, const double* mass3,
+                     const int64_t nmesh[3], double out_scale, int32_t flags, double* out,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* --------------------------------------------------------------- readout */
+/* Replaces preadout_f32/_f64 (lib/cpu.cc:74-94 -> lib/parops_fast.h:533-567).
+ * out (Np,) = CIC interpolation of field at pos; with vjpdim in {0,1,2} the
+ * derivative along that axis in box units (weights -N/L, +N/L,
+ * lib/parops_fast.h:448-451). */
+int ldl_preadout_f32(const float* pos, const ldl_pos_desc* pd, int64_t np, const float* field,
+                     const int64_t nmesh[3], int32_t vjpdim, float* out, void* stream);
+int ldl_preadout_f64(const double* pos, const ldl_pos_desc* pd, int64_t np, const double* field,
+                     const int64_t nmesh[3], int32_t vjpdim, double* out, void* stream);
+
+/* Replaces preadout3D_f32/_f64 (lib/cpu.cc:118-138 -> lib/parops_fast.h:727-857).
+ * vjp == 0: field (3,N0,N1,N2) -> out (Np,3) interpolations;
+ * vjp != 0: field (N0,N1,N2) -> out (Np,3) gradient in box units.
+ * pscale (Np,) optional (NULL = 1): multiplies each particle's row -- the
+ * ``(out1.T * mass).T`` of the reference VJPs (jaxpops_bind.py:247, :363). */
+int ldl_preadout3D_f32(const float* pos, const ldl_pos_desc* pd, int64_t np, const float* field,
+                       const int64_t nmesh[3], int32_t vjp, const float* pscale, float* out,
+                       void* stream);
+int ldl_preadout3D_f64(const double* pos, const ldl_pos_desc* pd, int64_t np, const double* field,
+                       const int64_t nmesh[3], int32_t vjp, const double* pscale, double* out,
+                       void* stream);
+
+/* Adjoint helper of preadout3D (jaxpops_bind.py:578-581, fused):
+ * out (Np,3) = sum_d v[:,d] * grad(interp F_d) for fields (3,N0,N1,N2). */
+int ldl_readout_grad3_f32(const float* pos, const ldl_pos_desc* pd, int64_t np, const float* fields3,
+                          const float* v3, const int64_t nmesh[3], float* out, void* stream);
+int ldl_readout_grad3_f64(const double* pos, const ldl_pos_desc* pd, int64_t np, const double* fields3,
+                          const double* v3, const int64_t nmesh[3], double* out, void* stream);
+
+/* ------------------------------------------------------------------- FFT */
+/* Scratch for ldl_pifft_*: one half-spectrum. */
+size_t ldl_fft_workspace_bytes(const int64_t nmesh[3], int32_t is_f64);
+
+/* Replaces pfft_f32/_f64 (lib/cpu.cc:169-200 -> lib/parfft.h:78-94, :111-125).
+ * Real (N0,N1,N2) -> complex (N0,N1,N2/2+1) interleaved (re, im), times
+ * ``scale`` (1 = FFTW's unnormalised r2c; 1/N^3 = PMesh.r2c). */
+int ldl_pfft_f32(const float* in, float* out, const int64_t nmesh[3], double scale, void* stream);
+int ldl_pfft_f64(const double* in, double* out, const int64_t nmesh[3], double scale, void* stream);
+
+/* Replaces pifft_f32/_f64 (lib/cpu.cc:202-245 -> lib/parfft.h:96-109, :127-138).
+ * Complex (N0,N1,N2/2+1) -> real (N0,N1,N2), unnormalised c2r times ``scale``;
+ * the imaginary parts of the k_z = 0 and N2/2 planes are ignored (FFTW). The
+ * input is not modified. */
+int ldl_pifft_f32(const float* in, float* out, const int64_t nmesh[3], double scale,
+                  void* workspace, size_t workspace_bytes, void* stream);
+int ldl_pifft_f64(const double* in, double* out, const int64_t nmesh[3], double scale,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------- fused sm-kind LDL layer
+ * One displacement layer of JERALDModel.potential_gradient_sm
+ * (model.py:185-212) with parameters prm = {alpha, gamma, kh, kl, nu}
+ * (model.py:268-272):  rho = fact * paint(X, 1); U = r2c((rho+1e-8)^gamma);
+ * P = 1e-2 alpha G(k) U with G = -exp(-k^2/kl^2 - kh^2/k^2) k^nu;
+ * F_d = c2r(-i D_d(k) P) with D_d the 6th-order FD symbol; s_d = readout(X, F_d).
+ * Positions are LDL_POS_LATTICE displacements (mesh units) or physical. */
+size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64);
+
+/* Forward of one layer.  Inputs pos (Np,3).  Writes pos_out (Np,3) = pos +
+ * s / cell (same kind as pos; may alias pos), and the saved tensors the
+ * backward needs: rho (N0,N1,N2), U (N0,N1,N2/2+1 complex, = r2c of
+ * (rho+1e-8)^gamma), forces F (3,N0,N1,N2). */
+int ldl_sm_layer_fwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[5], double fact, int32_t paint_flags,
+                         float* pos_out, float* rho, float* U, float* F,
+                         void* workspace, size_t workspace_bytes, void* stream);
+int ldl_sm_layer_fwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[5], double fact, int32_t paint_flags,
+                         double* pos_out, double* rho, double* U, double* F,
+                         void* workspace, size_t workspace_bytes, void* stream);
+
+/* Backward of one layer.  g_out (Np,3) = dL/dX_out (box units).  Writes
+ * g_in (Np,3) = dL/dX_in (may alias g_out) and ACCUMULATES dL/d{alpha,
+ * gamma, kh, kl, nu} into grad5 (DEVICE double[5]).  ``alpha_rule`` 0 = exact
+ * gradient, 1 = the reference's rule (interior k_z modes weighted 1 instead
+ * of 2 in dL/dalpha, see jaxpops_bind.py:758-759 and model.py:198). */
+int ldl_sm_layer_bwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[5], double fact, int32_t alpha_rule,
+                         int32_t paint_flags, const float* rho, const float* U, const float* F,
+                         const float* g_out, float* g_in, double* grad5,
+                         void* workspace, size_t workspace_bytes, void* stream);
+int ldl_sm_layer_bwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[5], double fact, int32_t alpha_rule,
+                         int32_t paint_flags, const double* rho, const double* U, const double* F,
+                         const double* g_out, double* g_in, double* grad5,
+                         void* workspace, size_t workspace_bytes, void* stream);
+
+/* Response + loss + their gradient (model.py:338-344, :350-403 / :415-468).
+ * rho (N0,N1,N2) final density (already times fact); prm3 = {mu, w, b}.
+ * F = ReLU(w (rho+1e-8)^mu + b); Fi = F^index (* field2); diff = Fi - target;
+ * optional smoothing diff = c2r(W(k) r2c(diff)) with
+ *   smooth_kind 0: none; 1: W = knorm^-n + 1 (reference, knorm(0) = 1e-8);
+ *   2: W = exp(-k^2 sigma^2 / 2);  smooth_par = n or sigma (box units);
+ * loss = sum(|diff|^p * mask) / sum(mask) with p = 1 (L1) or 2; mask/field2 may
+ * be NULL; mask_valid (NULL) adds the validation loss of lossv.
+ * Writes out5 (DEVICE double[5]) = {loss, dL/dmu, dL/dw, dL/db, loss_valid}
+ * and g_rho (N0,N1,N2) = dL/drho (NULL: value only). */
+int ldl_sm_loss_f32(const float* rho, const float* target, const float* mask, const float* mask_valid,
+                    const float* field2, const int64_t nmesh[3], const double box[3],
+                    const double prm3[3], double index, int32_t p_norm, int32_t smooth_kind,
+                    double smooth_par, double* out5, float* g_rho,
+                    void* workspace, size_t workspace_bytes, void* stream);
+int ldl_sm_loss_f64(const double* rho, const double* target, const double* mask, const double* mask_valid,
+                    const double* field2, const int64_t nmesh[3], const double box[3],
+                    const double prm3[3], double index, int32_t p_norm, int32_t smooth_kind,
+                    double smooth_par, double* out5, double* g_rho,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* Final-layer adjoint gather (fused readout-gradient): g (Np,3) =
+ * fact * grad(interp g_rho) at pos  (the ppaint VJP with unit mass,
+ * jaxpops_bind.py:242-247). */
+int ldl_paint_vjp_pos_f32(const float* pos, const ldl_pos_desc* pd, int64_t np, const float* g_rho,
+                          const int64_t nmesh[3], double fact, float* g, void* stream);
+int ldl_paint_vjp_pos_f64(const double* pos, const ldl_pos_desc* pd, int64_t np, const double* g_rho,
+                          const int64_t nmesh[3], double fact, double* g, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LDL_H */

@@ -1,0 +1,259 @@
+// Device-side CIC building blocks shared by the CIC entry points and the
+// fused LDL layer kernels (see cic.hip for the algorithm notes).
+#pragma once
+
+#include <cstring>
+
+#include "ldl_common.h"
+
+namespace ldl {
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ int fixed_shift(unsigned long long maxbits, int64_t np) {
+    double mx = __longlong_as_double((long long)maxbits);
+    if (mx == 0.0 || np <= 0) return 0;
+    int e = ilogb(mx) + 1;  // mx < 2^e (frexp exponent)
+    int lg = np > 1 ? 64 - __clzll((unsigned long long)(np - 1)) : 0;
+    return 61 - e - lg;
+}
+
+static __global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n, unsigned long long v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// header[c] = bits of max |mass[:, c]| (as double); positive doubles order
+// like their bit patterns, so an integer atomicMax is exact and order free.
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_absmax(const T* __restrict__ mass, int64_t np, unsigned long long* __restrict__ hdr) {
+    double mx[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) mx[c] = 0.0;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) mx[c] = fmax(mx[c], fabs((double)mass[p * NC + c]));
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        double v = mx[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(&hdr[c], (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+// Fixed-point deposit.  acc has NC consecutive meshes.
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_paint_fixed(const T* __restrict__ pos, PosDec<T> pd, int64_t np,
+                                                     const T* __restrict__ mass, T mscalar, Mesh m,
+                                                     unsigned long long* __restrict__ acc,
+                                                     const unsigned long long* __restrict__ hdr) {
+    int s[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) s[c] = fixed_shift(hdr[c], np);
+    const int64_t cells = m.cells();
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t idx[3];
+        T f[3];
+        pd.decode(pos, p, idx, f);
+        const double wx[2] = {(double)(T(1) - f[0]), (double)f[0]};
+        const double wy[2] = {(double)(T(1) - f[1]), (double)f[1]};
+        const double wz[2] = {(double)(T(1) - f[2]), (double)f[2]};
+        const int64_t ii[2] = {idx[0], idx[0] + 1 == m.n0 ? 0 : idx[0] + 1};
+        const int64_t jj[2] = {idx[1], idx[1] + 1 == m.n1 ? 0 : idx[1] + 1};
+        const int64_t kk[2] = {idx[2], idx[2] + 1 == m.n2 ? 0 : idx[2] + 1};
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const double mm = mass ? (double)mass[p * NC + c] : (double)mscalar;
+            const long long tot = __double2ll_rn(ldexp(mm, s[c]));
+            long long part = 0;
+            unsigned long long* a = acc + c * cells;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int ax = q >> 2, by = (q >> 1) & 1, cz = q & 1;
+                long long v;
+                if (q == 7) {
+                    v = tot - part;
+                } else {
+                    // ((w_x * w_y) * w_z) * m, the same order as the CPU emulation
+                    double w = wx[ax] * wy[by];
+                    w = w * wz[cz];
+                    w = w * mm;
+                    v = __double2ll_rn(ldexp(w, s[c]));
+                    part += v;
+                }
+                if (v != 0) atomicAdd(&a[(ii[ax] * m.n1 + jj[by]) * m.n2 + kk[cz]], (unsigned long long)v);
+            }
+        }
+    }
+}
+
+// Floating-point atomic deposit (non-deterministic order), for comparison.
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_paint_float(const T* __restrict__ pos, PosDec<T> pd, int64_t np,
+                                                     const T* __restrict__ mass, T mscalar, T scale, Mesh m,
+                                                     T* __restrict__ out) {
+    const int64_t cells = m.cells();
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t idx[3];
+        T f[3];
+        pd.decode(pos, p, idx, f);
+        const T wx[2] = {T(1) - f[0], f[0]}, wy[2] = {T(1) - f[1], f[1]}, wz[2] = {T(1) - f[2], f[2]};
+        const int64_t ii[2] = {idx[0], idx[0] + 1 == m.n0 ? 0 : idx[0] + 1};
+        const int64_t jj[2] = {idx[1], idx[1] + 1 == m.n1 ? 0 : idx[1] + 1};
+        const int64_t kk[2] = {idx[2], idx[2] + 1 == m.n2 ? 0 : idx[2] + 1};
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const T mm = (mass ? mass[p * NC + c] : mscalar) * scale;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int ax = q >> 2, by = (q >> 1) & 1, cz = q & 1;
+                atomicAdd(&out[c * cells + (ii[ax] * m.n1 + jj[by]) * m.n2 + kk[cz]], wx[ax] * wy[by] * wz[cz] * mm);
+            }
+        }
+    }
+}
+
+// out = acc * 2^-s * scale (rounded once), and acc reset to zero.
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_fixed_to_float(unsigned long long* __restrict__ acc,
+                                                        const unsigned long long* __restrict__ hdr, int64_t np,
+                                                        int64_t cells, double scale, T* __restrict__ out) {
+    double f[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) f[c] = ldexp(scale, -fixed_shift(hdr[c], np));
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cells; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const long long v = (long long)acc[c * cells + i];
+            out[c * cells + i] = (T)((double)v * f[c]);
+            acc[c * cells + i] = 0ull;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- gathers
+template <typename T>
+__device__ __forceinline__ void corners(const PosDec<T>& pd, const Mesh& m, const T* pos, int64_t p, int64_t cidx[8], T f[3]) {
+    int64_t idx[3];
+    pd.decode(pos, p, idx, f);
+    const int64_t ii[2] = {idx[0], idx[0] + 1 == m.n0 ? 0 : idx[0] + 1};
+    const int64_t jj[2] = {idx[1], idx[1] + 1 == m.n1 ? 0 : idx[1] + 1};
+    const int64_t kk[2] = {idx[2], idx[2] + 1 == m.n2 ? 0 : idx[2] + 1};
+#pragma unroll
+    for (int q = 0; q < 8; q++) cidx[q] = (ii[q >> 2] * m.n1 + jj[(q >> 1) & 1]) * m.n2 + kk[q & 1];
+}
+
+// value (deriv < 0) or derivative along ``deriv`` (box units) of one field
+template <typename T>
+__device__ __forceinline__ T interp(const T* __restrict__ F, const int64_t cidx[8], const T f[3], int deriv, const T cellinv[3]) {
+    T w[3][2];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d == deriv) { w[d][0] = -cellinv[d]; w[d][1] = cellinv[d]; }
+        else { w[d][0] = T(1) - f[d]; w[d][1] = f[d]; }
+    }
+    T acc = T(0);
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc += w[0][q >> 2] * w[1][(q >> 1) & 1] * w[2][q & 1] * F[cidx[q]];
+    return acc;
+}
+
+// value and full gradient of one field from the same 8 loads
+template <typename T>
+__device__ __forceinline__ void interp_vg(const T* __restrict__ F, const int64_t cidx[8], const T f[3], const T cellinv[3], T& val, T g[3]) {
+    T v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = F[cidx[q]];
+    const T t[3] = {T(1) - f[0], T(1) - f[1], T(1) - f[2]};
+    // along z first
+    T a00 = t[2] * v[0] + f[2] * v[1], a01 = t[2] * v[2] + f[2] * v[3];
+    T a10 = t[2] * v[4] + f[2] * v[5], a11 = t[2] * v[6] + f[2] * v[7];
+    T dz00 = v[1] - v[0], dz01 = v[3] - v[2], dz10 = v[5] - v[4], dz11 = v[7] - v[6];
+    T b0 = t[1] * a00 + f[1] * a01, b1 = t[1] * a10 + f[1] * a11;
+    val = t[0] * b0 + f[0] * b1;
+    g[0] = (b1 - b0) * cellinv[0];
+    g[1] = (t[0] * (a01 - a00) + f[0] * (a11 - a10)) * cellinv[1];
+    g[2] = (t[0] * (t[1] * dz00 + f[1] * dz01) + f[0] * (t[1] * dz10 + f[1] * dz11)) * cellinv[2];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_readout(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F,
+                                                 Mesh m, int deriv, T* __restrict__ out) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c[8];
+        T f[3];
+        corners(pd, m, pos, p, c, f);
+        out[p] = interp(F, c, f, deriv, pd.cellinv);
+    }
+}
+
+// 3 fields -> (Np,3)
+template <typename T>
+__global__ void __launch_bounds__(256) k_readout3(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
+                                                  Mesh m, T* __restrict__ out) {
+    const int64_t cells = m.cells();
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c[8];
+        T f[3];
+        corners(pd, m, pos, p, c, f);
+#pragma unroll
+        for (int d = 0; d < 3; d++) out[3 * p + d] = interp(F3 + d * cells, c, f, -1, pd.cellinv);
+    }
+}
+
+// gradient of one field -> (Np,3), times pscale[p]
+template <typename T>
+__global__ void __launch_bounds__(256) k_readout_grad(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F,
+                                                      Mesh m, const T* __restrict__ pscale, T* __restrict__ out) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c[8];
+        T f[3];
+        corners(pd, m, pos, p, c, f);
+        T val, g[3];
+        interp_vg(F, c, f, pd.cellinv, val, g);
+        const T s = pscale ? pscale[p] : T(1);
+#pragma unroll
+        for (int d = 0; d < 3; d++) out[3 * p + d] = g[d] * s;
+    }
+}
+
+// sum_d v[:,d] grad(F_d) -> (Np,3)
+template <typename T>
+__global__ void __launch_bounds__(256) k_readout_grad3(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
+                                                       const T* __restrict__ v3, Mesh m, T* __restrict__ out) {
+    const int64_t cells = m.cells();
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c[8];
+        T f[3];
+        corners(pd, m, pos, p, c, f);
+        T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            T val, g[3];
+            interp_vg(F3 + d * cells, c, f, pd.cellinv, val, g);
+            const T w = v3[3 * p + d];
+#pragma unroll
+            for (int e = 0; e < 3; e++) acc[e] += w * g[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 3; e++) out[3 * p + e] = acc[e];
+    }
+}
+
+size_t paint_ws_bytes(const Mesh& m, int nc);
+template <typename T>
+int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,
+               const int64_t nmesh[3], double out_scale, int32_t flags, T* out, void* ws, size_t ws_bytes,
+               hipStream_t st);
+template <typename T>
+int readout_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* field, const int64_t nmesh[3],
+                 int kind, int vjpdim, const T* aux, T* out, hipStream_t st);
+
+}  // namespace ldl

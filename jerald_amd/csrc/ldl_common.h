@@ -1,0 +1,180 @@
+// Shared definitions for the libldl HIP sources (gfx950 / CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/ldl.h"
+
+namespace ldl {
+
+// ------------------------------------------------------------ error channel
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define LDL_HIP_TRY(expr)                                                              \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return ::ldl::fail(LDL_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define LDL_LAUNCH_CHECK()                                                             \
+    do {                                                                               \
+        hipError_t _e = hipGetLastError();                                             \
+        if (_e != hipSuccess)                                                          \
+            return ::ldl::fail(LDL_EHIP, std::string("kernel launch: ") + hipGetErrorString(_e)); \
+    } while (0)
+
+// ------------------------------------------------------------- vector types
+template <typename T> struct V2;
+template <> struct V2<float> { using t = float2; };
+template <> struct V2<double> { using t = double2; };
+template <typename T> using c2 = typename V2<T>::t;
+
+template <typename C> __host__ __device__ __forceinline__ C cmul(C a, C b) {
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename C> __host__ __device__ __forceinline__ C cmulc(C a, C b) {  // a * conj(b)
+    return C{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename C> __host__ __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __host__ __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __host__ __device__ __forceinline__ C cconj(C a) { return C{a.x, -a.y}; }
+
+// ----------------------------------------------------------- mesh geometry
+struct Mesh {
+    int64_t n0, n1, n2;
+    __host__ __device__ int64_t cells() const { return n0 * n1 * n2; }
+    __host__ __device__ int64_t nh() const { return n2 / 2 + 1; }
+    __host__ __device__ int64_t hcells() const { return n0 * n1 * (n2 / 2 + 1); }
+};
+
+__host__ __device__ __forceinline__ int64_t wrapi(int64_t i, int64_t n) {
+    int64_t r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// ------------------------------------------------------ position decoding
+// Device-side copy of ldl_pos_desc, specialised to the working precision.
+template <typename T> struct PosDec {
+    int kind;
+    T box[3];
+    T cell[3];     // box / N in T (reference: BoxSize / Nmesh in the working dtype)
+    T cellinv[3];  // N / box in T (reference: Nmesh[d] / BoxSize[d], lib/parops_fast.h:748-750)
+    uint32_t l12, l2;  // lattice strides (Np < 2^32 for the lattice form)
+    double lstep[3];
+    int64_t n[3];
+
+    // CIC lower corner (wrapped) and fractional offsets of particle p.
+    __device__ __forceinline__ void decode(const T* __restrict__ pos, int64_t p, int64_t idx[3], T f[3]) const {
+        const T* q = pos + 3 * p;
+        T x[3] = {q[0], q[1], q[2]};
+        int64_t base[3] = {0, 0, 0};
+        if (kind == LDL_POS_PHYSICAL) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                // numpy remainder semantics, then division (jaxpops_bind.py:163-164)
+                T r = fmod(x[d], box[d]);
+                if (r != T(0) && ((r < T(0)) != (box[d] < T(0)))) r += box[d];
+                x[d] = r / cell[d];
+            }
+        } else {
+            uint32_t pp = (uint32_t)p;
+            uint32_t li[3];
+            li[0] = pp / l12;
+            uint32_t rem = pp - li[0] * l12;
+            li[1] = rem / l2;
+            li[2] = rem - li[1] * l2;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double b = (double)li[d] * lstep[d];
+                double bf = floor(b);
+                base[d] = (int64_t)bf;
+                x[d] = x[d] + (T)(b - bf);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            T fl = floor(x[d]);
+            f[d] = x[d] - fl;
+            idx[d] = wrapi(base[d] + (int64_t)fl, n[d]);
+        }
+    }
+};
+
+template <typename T>
+inline int make_posdec(const ldl_pos_desc* pd, const int64_t nmesh[3], int64_t np, PosDec<T>* out) {
+    if (!pd) return fail(LDL_EINVAL, "pos_desc is NULL");
+    if (pd->kind != LDL_POS_PHYSICAL && pd->kind != LDL_POS_LATTICE)
+        return fail(LDL_EINVAL, "pos_desc.kind must be LDL_POS_PHYSICAL or LDL_POS_LATTICE");
+    out->kind = pd->kind;
+    for (int d = 0; d < 3; d++) {
+        if (!(pd->box[d] > 0.0) || !std::isfinite(pd->box[d])) return fail(LDL_EINVAL, "box sizes must be positive and finite");
+        out->box[d] = (T)pd->box[d];
+        out->cell[d] = out->box[d] / (T)nmesh[d];
+        out->cellinv[d] = (T)nmesh[d] / out->box[d];
+        out->n[d] = nmesh[d];
+        out->lstep[d] = pd->lattice_step[d];
+    }
+    out->l12 = out->l2 = 1;
+    if (pd->kind == LDL_POS_LATTICE) {
+        int64_t l0 = pd->lattice[0], l1 = pd->lattice[1], l2 = pd->lattice[2];
+        if (l0 <= 0 || l1 <= 0 || l2 <= 0) return fail(LDL_EINVAL, "lattice sizes must be positive");
+        if (l0 * l1 * l2 != np) return fail(LDL_EINVAL, "lattice[0]*lattice[1]*lattice[2] must equal np");
+        if (np > (int64_t)0xFFFFFFFFll) return fail(LDL_EUNSUPPORTED, "lattice form supports np < 2^32");
+        out->l12 = (uint32_t)(l1 * l2);
+        out->l2 = (uint32_t)l2;
+        for (int d = 0; d < 3; d++)
+            if (!(pd->lattice_step[d] > 0.0)) return fail(LDL_EINVAL, "lattice_step must be positive");
+    }
+    return LDL_OK;
+}
+
+inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+inline int check_mesh(const int64_t nmesh[3], Mesh* m) {
+    if (!nmesh) return fail(LDL_EINVAL, "nmesh is NULL");
+    for (int d = 0; d < 3; d++)
+        if (nmesh[d] < 1) return fail(LDL_EINVAL, "mesh sizes must be >= 1");
+    m->n0 = nmesh[0];
+    m->n1 = nmesh[1];
+    m->n2 = nmesh[2];
+    return LDL_OK;
+}
+
+// FFT-capable mesh: each axis a power of two in [4, 1024], last axis >= 8.
+inline int check_fft_mesh(const int64_t nmesh[3], Mesh* m) {
+    int rc = check_mesh(nmesh, m);
+    if (rc) return rc;
+    for (int d = 0; d < 3; d++)
+        if (!is_pow2(nmesh[d]) || nmesh[d] < 4 || nmesh[d] > 1024)
+            return fail(LDL_EUNSUPPORTED, "FFT mesh axes must be powers of two in [4, 1024]");
+    if (nmesh[2] < 8) return fail(LDL_EUNSUPPORTED, "FFT mesh last axis must be >= 8");
+    return LDL_OK;
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {
+    int64_t g = (n + block - 1) / block;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+// Twiddle table for the current device: tw[L + m] = exp(-2 pi i m / L) for
+// every power of two L in [1, 2048], built in FP64 on the host.
+template <typename T> int twiddles(const c2<T>** out);
+
+// --------------------------------------------------------- reductions
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace ldl

@@ -1,0 +1,835 @@
+// Fused LDL displacement layer (sm kind), response and loss on gfx950, and
+// their hand-written adjoints.
+//
+// Forward of one layer (JERALDModel.potential_gradient_sm, model.py:185-212):
+//   rho = fact * paint(X, 1)                       fixed-point paint (cic.h)
+//   U   = r2c((rho + 1e-8)^gamma) / N^3            z pass (power fused) + y pass
+//   P   = 1e-2 alpha G(k) U,  G = -exp(-k^2/kl^2 - kh^2/k^2) k^nu
+//   F_d = c2r(-i D_d(k) P)                          x pass: forward, filter, three
+//                                                   inverse transforms in one kernel
+//   X  += readout(X, F_d)                           fused gather + update
+// Backward (exact adjoint; r2c^T = c2r(./c), c2r^T = c r2c with c = 1 on the
+// k_z = 0, N/2 planes and 2 elsewhere):
+//   gF_d = paint(X, g_d)                            3-component fixed-point paint
+//   Q    = sum_d i D_d r2c_unnorm(gF_d)             z, y passes; x pass fused with
+//   dL/dalpha = 1e-2 sum_H c G Re(conj(Q) U)        the FP64 parameter reductions
+//   dL/dG     = 1e-2 alpha c Re(conj(Q) U)  -> kh, kl, nu
+//   g_u  = c2r(1e-2 alpha G Q) / N^3                x inverse, y inverse, z c2r
+//   g_rho = g_u gamma (rho+1e-8)^(gamma-1),  dL/dgamma = sum g_u u ln(rho+1e-8)
+//   g_X  = g + sum_d g_d grad F_d + fact grad g_rho  fused gather
+// Parameter reductions are per-workgroup FP64 partials summed in a fixed
+// order, so gradients are bit-reproducible run to run.
+#include "cic.h"
+#include "fft.h"
+
+namespace ldl {
+
+static constexpr int kBlk = 256;
+
+// ---------------------------------------------------------------- tables
+// Per-axis wavenumbers k = 2 pi fftfreq(N, cell) (pmesh.py:174-186; the z
+// axis keeps N/2+1 entries, so its last one is -pi/cell) and the 6th-order
+// finite-difference gradient symbol (45 sin kD - 9 sin 2kD + sin 3kD)/(30 D)
+// (model.py:200-201), evaluated in FP64.
+template <typename T>
+__global__ void k_axis_tables(T* __restrict__ k, T* __restrict__ D, int64_t n, int64_t count, double cell) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int64_t m = i < n / 2 ? i : i - n;
+    const double th = 2.0 * (double)m / (double)n;  // k cell / pi
+    k[i] = (T)(2.0 * M_PI * (double)m / ((double)n * cell));
+    D[i] = (T)((45.0 * sinpi(th) - 9.0 * sinpi(2.0 * th) + sinpi(3.0 * th)) / (30.0 * cell));
+}
+
+template <typename T> struct KTabs {
+    const T* k[3];
+    const T* D[3];
+    __device__ __forceinline__ T knorm(int64_t ix, int64_t iy, int64_t iz) const {
+        const T a = k[0][ix], b = k[1][iy], c = k[2][iz];
+        T kn = sqrt(a * a + b * b + c * c);
+        return kn == T(0) ? T(1e-8) : kn;  // model.py:102
+    }
+};
+
+template <typename T> struct SmFilter {
+    T kh2, inv_kl2, nu;
+    __device__ __forceinline__ T G(T kn) const {  // model.py:195
+        const T k2 = kn * kn;
+        return -exp(-k2 * inv_kl2 - kh2 / k2) * pow(kn, nu);
+    }
+};
+
+// Workspace carve-up of ldl_sm_* calls.
+struct SmWs {
+    size_t paint, spec3, mesh3, tabs, part, total;
+    int64_t part_doubles;
+};
+static SmWs sm_ws(const Mesh& m, int tsize) {
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    SmWs w;
+    w.paint = al(paint_ws_bytes(m, 3));
+    w.spec3 = al((size_t)3 * m.hcells() * 2 * tsize);
+    w.mesh3 = al((size_t)3 * m.cells() * tsize);
+    w.tabs = al((size_t)2 * (m.n0 + m.n1 + m.nh()) * tsize);
+    w.part_doubles = 8 * (m.n0 * m.n1 + m.n1 * (m.nh() + 1) + 4096);
+    w.part = al((size_t)w.part_doubles * 8);
+    w.total = w.paint + w.spec3 + w.mesh3 + w.tabs + w.part;
+    return w;
+}
+
+template <typename T> struct SmPtrs {
+    void* paint;
+    c2<T>* spec3;
+    T* mesh3;
+    T* tabs;
+    double* part;
+};
+template <typename T>
+static SmPtrs<T> sm_ptrs(void* ws, const SmWs& w) {
+    char* b = reinterpret_cast<char*>(ws);
+    SmPtrs<T> p;
+    p.paint = b;
+    p.spec3 = reinterpret_cast<c2<T>*>(b + w.paint);
+    p.mesh3 = reinterpret_cast<T*>(b + w.paint + w.spec3);
+    p.tabs = reinterpret_cast<T*>(b + w.paint + w.spec3 + w.mesh3);
+    p.part = reinterpret_cast<double*>(b + w.paint + w.spec3 + w.mesh3 + w.tabs);
+    return p;
+}
+
+template <typename T>
+static KTabs<T> build_tables(T* tabs, const Mesh& m, const double box[3], hipStream_t st) {
+    const int64_t cnt[3] = {m.n0, m.n1, m.nh()};
+    const int64_t nn[3] = {m.n0, m.n1, m.n2};
+    KTabs<T> kt;
+    T* p = tabs;
+    for (int d = 0; d < 3; d++) {
+        T* kk = p;
+        T* DD = p + cnt[d];
+        hipLaunchKernelGGL(k_axis_tables<T>, dim3(grid_for(cnt[d], kBlk)), dim3(kBlk), 0, st, kk, DD, nn[d], cnt[d], box[d] / (double)nn[d]);
+        kt.k[d] = kk;
+        kt.D[d] = DD;
+        p += 2 * cnt[d];
+    }
+    return kt;
+}
+
+// Sum column j of a (nblocks x stride) FP64 array and add it to out[map[j]]
+// (or store when ``assign``); fixed order, one workgroup.
+struct RedMap {
+    int n;
+    int map[8];
+};
+__global__ void __launch_bounds__(256) k_reduce_cols(const double* __restrict__ part, int64_t nblocks, int stride, RedMap rm,
+                                                     double* __restrict__ out, int assign) {
+    __shared__ double red[4];
+    for (int j = 0; j < rm.n; j++) {
+        double s = 0.0;
+        for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) s += part[b * stride + j];
+        s = wave_sum(s);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double t = (red[0] + red[1]) + (red[2] + red[3]);
+            if (assign) out[rm.map[j]] = t;
+            else out[rm.map[j]] += t;
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------- z-pass functors
+template <typename T> struct PowRowLoad {  // (rho + 1e-8)^gamma, model.py:193
+    const T* rho;
+    int64_t n2;
+    T gamma;
+    __device__ __forceinline__ c2<T> load2(int64_t row, int m) const {
+        const c2<T> r = *reinterpret_cast<const c2<T>*>(rho + row * n2 + 2 * m);
+        return c2<T>{pow(r.x + T(1e-8), gamma), pow(r.y + T(1e-8), gamma)};
+    }
+};
+
+// g_u -> g_rho = g_u gamma (rho+1e-8)^(gamma-1); dL/dgamma partial.
+template <typename T> struct GammaBwdStore {
+    static constexpr int NR = 1;
+    const T* rho;
+    T* grho;
+    int64_t n2;
+    T scale, gamma;
+    double* part;
+    __device__ __forceinline__ void store2(int64_t row, int m, c2<T> v, double* acc) const {
+        const int64_t i = row * n2 + 2 * m;
+        const c2<T> r = *reinterpret_cast<const c2<T>*>(rho + i);
+        const T g0 = v.x * scale, g1 = v.y * scale;
+        const T r0 = r.x + T(1e-8), r1 = r.y + T(1e-8);
+        const T l0 = log(r0), l1 = log(r1);
+        acc[0] += (double)(g0 * pow(r0, gamma)) * (double)l0 + (double)(g1 * pow(r1, gamma)) * (double)l1;
+        *reinterpret_cast<c2<T>*>(grho + i) = c2<T>{g0 * gamma * pow(r0, gamma - T(1)), g1 * gamma * pow(r1, gamma - T(1))};
+    }
+    __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const { part[b] = acc[0]; }
+};
+
+// ------------------------------------------------------- fused x passes
+// Forward: X FFT, save U = . / N^3, P = 1e-2 alpha G U, three inverse X FFTs of
+// -i D_d P into out3 (3 spectra, hcells apart).
+template <typename T, int N>
+__global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
+k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>* __restrict__ out3, int64_t hcells,
+           KTabs<T> kt, SmFilter<T> fl, T a1e2, T invM) {
+    using Cfg = FftCfg<T, N>;
+    using C = c2<T>;
+    constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
+    __shared__ C lds[Cfg::LDS_ELEMS];
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t plane = tile / g.tiles_per_plane;
+    const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
+    const int64_t base = plane * g.plane_stride + col0;
+    const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
+    for (int i = threadIdx.x; i < N * P; i += TH) {
+        const int n = i / P, p = i - n * P;
+        lds[n * LS + p] = p < pv ? U[base + n * g.es + p] : C{T(0), T(0)};
+    }
+    __syncthreads();
+    lds_fft<T, N, P, Cfg::TPP, -1>(lds, tw + N);
+    C pk[EPT];
+#pragma unroll
+    for (int s = 0; s < EPT; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        C u = lds[n * LS + p];
+        u = C{u.x * invM, u.y * invM};
+        pk[s] = C{T(0), T(0)};
+        if (p < pv) {
+            U[base + n * g.es + p] = u;
+            const T G = fl.G(kt.knorm(n, plane, col0 + p));
+            pk[s] = C{u.x * a1e2 * G, u.y * a1e2 * G};
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int s = 0; s < EPT; s++) {
+            const int i = threadIdx.x + s * TH;
+            const int n = i / P, p = i - n * P;
+            const T Dd = d == 0 ? kt.D[0][n] : d == 1 ? kt.D[1][plane] : (p < pv ? kt.D[2][col0 + p] : T(0));
+            lds[n * LS + p] = C{Dd * pk[s].y, -Dd * pk[s].x};  // -i D P
+        }
+        __syncthreads();
+        lds_fft<T, N, P, Cfg::TPP, +1>(lds, tw + N);
+        for (int i = threadIdx.x; i < N * P; i += TH) {
+            const int n = i / P, p = i - n * P;
+            if (p < pv) out3[d * hcells + base + n * g.es + p] = lds[n * LS + p];
+        }
+        __syncthreads();
+    }
+}
+
+// Backward: three forward X FFTs combined into Q = sum_d i D_d S_d, the FP64
+// parameter partials (alpha, kh, kl, nu) against the saved U, then one
+// inverse X FFT of 1e-2 alpha G Q into out.
+template <typename T, int N>
+__global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
+k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3, const c2<T>* __restrict__ U,
+           c2<T>* __restrict__ out, int64_t hcells, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double kh,
+           double kl, double alpha, int alpha_rule, double* __restrict__ part) {
+    using Cfg = FftCfg<T, N>;
+    using C = c2<T>;
+    constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
+    __shared__ C lds[Cfg::LDS_ELEMS];
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t plane = tile / g.tiles_per_plane;
+    const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
+    const int64_t base = plane * g.plane_stride + col0;
+    const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
+    C q[EPT];
+#pragma unroll
+    for (int s = 0; s < EPT; s++) q[s] = C{T(0), T(0)};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        for (int i = threadIdx.x; i < N * P; i += TH) {
+            const int n = i / P, p = i - n * P;
+            lds[n * LS + p] = p < pv ? S3[d * hcells + base + n * g.es + p] : C{T(0), T(0)};
+        }
+        __syncthreads();
+        lds_fft<T, N, P, Cfg::TPP, -1>(lds, tw + N);
+#pragma unroll
+        for (int s = 0; s < EPT; s++) {
+            const int i = threadIdx.x + s * TH;
+            const int n = i / P, p = i - n * P;
+            const T Dd = d == 0 ? kt.D[0][n] : d == 1 ? kt.D[1][plane] : (p < pv ? kt.D[2][col0 + p] : T(0));
+            const C v = lds[n * LS + p];
+            q[s] = C{q[s].x - Dd * v.y, q[s].y + Dd * v.x};  // += i D v
+        }
+        __syncthreads();
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double a1e2d = 1e-2 * alpha;
+    const double gkh = -2.0 * kh, gkl = 2.0 / (kl * kl * kl);
+#pragma unroll
+    for (int s = 0; s < EPT; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        C y = C{T(0), T(0)};
+        if (p < pv) {
+            const int64_t kz = col0 + p;
+            const T kn = kt.knorm(n, plane, kz);
+            const T G = fl.G(kn);
+            const C u = U[base + n * g.es + p];
+            const double re = (double)q[s].x * (double)u.x + (double)q[s].y * (double)u.y;
+            const double c = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
+            const double Gd = (double)G, knd = (double)kn;
+            acc[0] += (alpha_rule ? 1.0 : c) * 1e-2 * Gd * re;
+            const double gG = a1e2d * c * re * Gd;  // dL/dG * G
+            acc[1] += gG * gkh / (knd * knd);
+            acc[2] += gG * gkl * (knd * knd);
+            acc[3] += gG * log(knd);
+            y = C{a1e2 * G * q[s].x, a1e2 * G * q[s].y};
+        }
+        lds[n * LS + p] = y;
+    }
+    __syncthreads();
+    lds_fft<T, N, P, Cfg::TPP, +1>(lds, tw + N);
+    for (int i = threadIdx.x; i < N * P; i += TH) {
+        const int n = i / P, p = i - n * P;
+        if (p < pv) out[base + n * g.es + p] = lds[n * LS + p];
+    }
+    block_sum<4, TH>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) part[blockIdx.x * 4 + j] = acc[j];
+    }
+}
+
+// X pass with a real isotropic weight: forward, times W(k) scale, inverse.
+// kind 1: W = knorm^-n + 1 (knorm(0) = 1e-8, model.py:376-381); kind 2:
+// W = exp(-k^2 sigma^2 / 2).
+template <typename T, int N>
+__global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
+k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, KTabs<T> kt, int kind, T par, T scale) {
+    using Cfg = FftCfg<T, N>;
+    using C = c2<T>;
+    constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS;
+    __shared__ C lds[Cfg::LDS_ELEMS];
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t plane = tile / g.tiles_per_plane;
+    const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
+    const int64_t base = plane * g.plane_stride + col0;
+    const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
+    for (int i = threadIdx.x; i < N * P; i += TH) {
+        const int n = i / P, p = i - n * P;
+        lds[n * LS + p] = p < pv ? io[base + n * g.es + p] : C{T(0), T(0)};
+    }
+    __syncthreads();
+    lds_fft<T, N, P, Cfg::TPP, -1>(lds, tw + N);
+    for (int i = threadIdx.x; i < N * P; i += TH) {
+        const int n = i / P, p = i - n * P;
+        if (p >= pv) continue;
+        T w;
+        if (kind == 1) {
+            w = pow(kt.knorm(n, plane, col0 + p), -par) + T(1);
+        } else {
+            const T a = kt.k[0][n], b = kt.k[1][plane], c = kt.k[2][col0 + p];
+            w = exp(T(-0.5) * (a * a + b * b + c * c) * par * par);
+        }
+        w *= scale;
+        const C v = lds[n * LS + p];
+        lds[n * LS + p] = C{v.x * w, v.y * w};
+    }
+    __syncthreads();
+    lds_fft<T, N, P, Cfg::TPP, +1>(lds, tw + N);
+    for (int i = threadIdx.x; i < N * P; i += TH) {
+        const int n = i / P, p = i - n * P;
+        if (p < pv) io[base + n * g.es + p] = lds[n * LS + p];
+    }
+}
+
+// --------------------------------------------------------------- gathers
+// pos_out = pos + readout3(F) * step (step = 1/cell in mesh units, 1 in box units)
+template <typename T>
+__global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
+                                                         Mesh m, T s0, T s1, T s2, T* __restrict__ out) {
+    const int64_t cells = m.cells();
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c[8];
+        T f[3];
+        corners(pd, m, pos, p, c, f);
+        const T v0 = interp(F3, c, f, -1, pd.cellinv);
+        const T v1 = interp(F3 + cells, c, f, -1, pd.cellinv);
+        const T v2 = interp(F3 + 2 * cells, c, f, -1, pd.cellinv);
+        const T x0 = pos[3 * p], x1 = pos[3 * p + 1], x2 = pos[3 * p + 2];
+        out[3 * p] = x0 + v0 * s0;
+        out[3 * p + 1] = x1 + v1 * s1;
+        out[3 * p + 2] = x2 + v2 * s2;
+    }
+}
+
+// g_in = g + sum_d g_d grad F_d + fact grad g_rho  (box units)
+template <typename T>
+__global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
+                                                       const T* __restrict__ grho, Mesh m, T fact, const T* __restrict__ gout,
+                                                       T* __restrict__ gin) {
+    const int64_t cells = m.cells();
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c[8];
+        T f[3];
+        corners(pd, m, pos, p, c, f);
+        const T g[3] = {gout[3 * p], gout[3 * p + 1], gout[3 * p + 2]};
+        T acc[3] = {g[0], g[1], g[2]};
+        T val, gr[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            interp_vg(F3 + d * cells, c, f, pd.cellinv, val, gr);
+#pragma unroll
+            for (int e = 0; e < 3; e++) acc[e] += g[d] * gr[e];
+        }
+        interp_vg(grho, c, f, pd.cellinv, val, gr);
+#pragma unroll
+        for (int e = 0; e < 3; e++) gin[3 * p + e] = acc[e] + fact * gr[e];
+    }
+}
+
+// g = fact * grad(interp g_rho)
+template <typename T>
+__global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ grho,
+                                                       Mesh m, T fact, T* __restrict__ g) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c[8];
+        T f[3];
+        corners(pd, m, pos, p, c, f);
+        T val, gr[3];
+        interp_vg(grho, c, f, pd.cellinv, val, gr);
+#pragma unroll
+        for (int e = 0; e < 3; e++) g[3 * p + e] = fact * gr[e];
+    }
+}
+
+// ------------------------------------------------------------------ loss
+template <typename T> struct RespPar {
+    T mu, w, b, index;
+    int l1;
+    const T* field2;
+};
+
+// Forward response value at a cell: y, F, Fi (model.py:344, :358-362)
+template <typename T>
+__device__ __forceinline__ void response(const RespPar<T>& rp, T rho, T f2, T& base, T& y, T& F, T& Fi) {
+    base = pow(rho + T(1e-8), rp.mu);
+    y = rp.w * base + rp.b;
+    F = y > T(1e-12) ? y : T(1e-12);
+    Fi = rp.index == T(1) ? F : pow(F, rp.index);
+    Fi = Fi * f2;
+}
+
+// d loss / d y, given d loss / d Fi (times field2 and the index power)
+template <typename T>
+__device__ __forceinline__ T response_bwd(const RespPar<T>& rp, T gFi, T f2, T y, T F) {
+    T g = gFi * f2;
+    if (rp.index != T(1)) g = g * rp.index * pow(F, rp.index - T(1));
+    return y > T(1e-12) ? g : T(0);
+}
+
+
+// mask sums -> part (2 per block)
+template <typename T>
+__global__ void __launch_bounds__(256) k_mask_sums(const T* __restrict__ mask, const T* __restrict__ maskv, int64_t n, double* __restrict__ part) {
+    double a[2] = {0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (mask) a[0] += (double)mask[i];
+        if (maskv) a[1] += (double)maskv[i];
+    }
+    block_sum<2, 256>(a);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x * 2] = a[0];
+        part[blockIdx.x * 2 + 1] = a[1];
+    }
+}
+
+// Unsmoothed loss and its gradient in one pass (npix from device memory).
+// part per block: {loss, g_mu, g_w, g_b, loss_v}
+template <typename T>
+__global__ void __launch_bounds__(256) k_loss_direct(const T* __restrict__ rho, const T* __restrict__ target, const T* __restrict__ mask,
+                                                     const T* __restrict__ maskv, RespPar<T> rp, int64_t n, const double* __restrict__ npix,
+                                                     T* __restrict__ grho, double* __restrict__ part) {
+    const double inv = 1.0 / npix[0];
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T r = rho[i];
+        const T f2 = rp.field2 ? rp.field2[i] : T(1);
+        T base, y, F, Fi;
+        response(rp, r, f2, base, y, F, Fi);
+        const T diff = Fi - target[i];
+        const T ad = fabs(diff);
+        const T mk = mask ? mask[i] : T(1);
+        const double l = rp.l1 ? (double)ad : (double)ad * (double)ad;
+        a[0] += (double)mk * l;
+        if (maskv) a[4] += (double)maskv[i] * l;
+        if (grho) {
+            const T sgn = diff > T(0) ? T(1) : diff < T(0) ? T(-1) : T(0);
+            const T gd = (T)((double)mk * inv) * (rp.l1 ? sgn : T(2) * diff);
+            const T gy = response_bwd(rp, gd, f2, y, F);
+            const T lr = log(r + T(1e-8));
+            a[1] += (double)(gy * rp.w * base) * (double)lr;
+            a[2] += (double)(gy * base);
+            a[3] += (double)gy;
+            grho[i] = gy * rp.w * rp.mu * pow(r + T(1e-8), rp.mu - T(1));
+        }
+    }
+    block_sum<5, 256>(a);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 5; j++) part[blockIdx.x * 5 + j] = a[j];
+}
+
+// diff = Fi - target (smoothed-loss path)
+template <typename T>
+__global__ void __launch_bounds__(256) k_loss_diff(const T* __restrict__ rho, const T* __restrict__ target, RespPar<T> rp, int64_t n,
+                                                   T* __restrict__ diff) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T f2 = rp.field2 ? rp.field2[i] : T(1);
+        T base, y, F, Fi;
+        response(rp, rho[i], f2, base, y, F, Fi);
+        diff[i] = Fi - target[i];
+    }
+}
+
+// z c2r epilogue of the smoothed residual: loss partials and g_ds.
+template <typename T> struct LossStore {
+    static constexpr int NR = 2;
+    const T* mask;
+    const T* maskv;
+    T* gds;
+    const double* npix;
+    int64_t n2;
+    T scale;
+    int l1;
+    double* part;
+    __device__ __forceinline__ void store2(int64_t row, int m, c2<T> v, double* acc) const {
+        const double inv = 1.0 / npix[0];
+        const int64_t i0 = row * n2 + 2 * m;
+        const T ds[2] = {v.x * scale, v.y * scale};
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int64_t i = i0 + e;
+            const T ad = fabs(ds[e]);
+            const double l = l1 ? (double)ad : (double)ad * (double)ad;
+            const T mk = mask ? mask[i] : T(1);
+            acc[0] += (double)mk * l;
+            if (maskv) acc[1] += (double)maskv[i] * l;
+            if (gds) {
+                const T sgn = ds[e] > T(0) ? T(1) : ds[e] < T(0) ? T(-1) : T(0);
+                gds[i] = (T)((double)mk * inv) * (l1 ? sgn : T(2) * ds[e]);
+            }
+        }
+    }
+    __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const {
+        part[b * 5 + 0] = acc[0];
+        part[b * 5 + 4] = acc[1];
+    }
+};
+
+// z c2r epilogue of the smoothed gradient: response backward -> g_rho.
+template <typename T> struct RespBwdStore {
+    static constexpr int NR = 3;
+    const T* rho;
+    T* grho;
+    RespPar<T> rp;
+    int64_t n2;
+    T scale;
+    double* part;
+    __device__ __forceinline__ void store2(int64_t row, int m, c2<T> v, double* acc) const {
+        const int64_t i0 = row * n2 + 2 * m;
+        const T gv[2] = {v.x * scale, v.y * scale};
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int64_t i = i0 + e;
+            const T r = rho[i];
+            const T f2 = rp.field2 ? rp.field2[i] : T(1);
+            T base, y, F, Fi;
+            response(rp, r, f2, base, y, F, Fi);
+            const T gy = response_bwd(rp, gv[e], f2, y, F);
+            acc[0] += (double)(gy * rp.w * base) * (double)log(r + T(1e-8));
+            acc[1] += (double)(gy * base);
+            acc[2] += (double)gy;
+            grho[i] = gy * rp.w * rp.mu * pow(r + T(1e-8), rp.mu - T(1));
+        }
+    }
+    __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const {
+        part[b * 5 + 1] = acc[0];
+        part[b * 5 + 2] = acc[1];
+        part[b * 5 + 3] = acc[2];
+    }
+};
+
+// out5 = {loss / npix, g_mu, g_w, g_b, loss_v / npix_v}
+__global__ void k_loss_finish(const double* __restrict__ sums, const double* __restrict__ npix, double* __restrict__ out5) {
+    if (threadIdx.x == 0) {
+        out5[0] = sums[0] / npix[0];
+        out5[1] = sums[1];
+        out5[2] = sums[2];
+        out5[3] = sums[3];
+        out5[4] = npix[1] > 0.0 ? sums[4] / npix[1] : 0.0;
+    }
+}
+
+// ------------------------------------------------------------ host side
+template <typename T>
+static int check_prm(const double* prm, int n) {
+    if (!prm) return fail(LDL_EINVAL, "parameter array is NULL");
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(prm[i])) return fail(LDL_EINVAL, "parameters must be finite");
+    return LDL_OK;
+}
+
+template <typename T>
+static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double prm[5], double fact,
+                        int32_t pflags, T* pos_out, T* rho, T* Ubuf, T* F, void* ws, size_t wsb, hipStream_t st) {
+    Mesh m;
+    int rc = check_fft_mesh(nmesh, &m);
+    if (rc) return rc;
+    if ((rc = check_prm<T>(prm, 5))) return rc;
+    if (!pos || !pos_out || !rho || !Ubuf || !F) return fail(LDL_EINVAL, "NULL array argument");
+    PosDec<T> pd;
+    if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
+    const SmWs w = sm_ws(m, sizeof(T));
+    if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
+    const SmPtrs<T> P = sm_ptrs<T>(ws, w);
+    const c2<T>* tw;
+    if ((rc = twiddles<T>(&tw))) return rc;
+    // 1. density
+    if ((rc = paint_impl<T>(pos, pdesc, np, nullptr, 1.0, 1, nmesh, fact, pflags, rho, P.paint, w.paint, st))) return rc;
+    const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
+    const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hc = m.hcells();
+    c2<T>* U = reinterpret_cast<c2<T>*>(Ubuf);
+    // 2. z pass with the gamma power, y pass
+    dispatch_n(m.n2 / 2, [&](auto H) {
+        launch_zr2c<T, decltype(H)::value>(nrows, tw, PowRowLoad<T>{rho, m.n2, (T)prm[1]}, HalfRowStore<T>{U, nh, T(1)}, st);
+    });
+    dispatch_n(m.n1, [&](auto N) {
+        ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
+        launch_colpass<T, decltype(N)::value, -1>(g, tw, PlainColOp<T>{U, U, T(1)}, st);
+    });
+    // 3. x pass + filter + three inverse x passes
+    const SmFilter<T> fl{(T)(prm[2] * prm[2]), (T)(1.0 / (prm[3] * prm[3])), (T)prm[4]};
+    dispatch_n(m.n0, [&](auto N) {
+        constexpr int NN = decltype(N)::value;
+        ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
+        hipLaunchKernelGGL((k_sm_x_fwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, hc, kt, fl,
+                           (T)(1e-2 * prm[0]), (T)(1.0 / (double)m.cells()));
+    });
+    // 4. inverse y passes (3 spectra as 3*N0 planes) and z c2r into F
+    dispatch_n(m.n1, [&](auto N) {
+        ColGeom g = col_geom<T, decltype(N)::value>(3 * m.n0, nh, m.n1 * nh, nh);
+        launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st);
+    });
+    dispatch_n(m.n2 / 2, [&](auto H) {
+        launch_zc2r<T, decltype(H)::value>(3 * nrows, tw, HalfRowLoad<T>{P.spec3, nh}, RealRowStore<T>{F, m.n2, T(1)}, st);
+    });
+    // 5. readout + position update
+    T s[3];
+    for (int d = 0; d < 3; d++) s[d] = pdesc->kind == LDL_POS_LATTICE ? pd.cellinv[d] : T(1);
+    if (np > 0)
+        hipLaunchKernelGGL(k_readout3_update<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, F, m, s[0], s[1], s[2], pos_out);
+    LDL_LAUNCH_CHECK();
+    return LDL_OK;
+}
+
+template <typename T>
+static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double prm[5], double fact,
+                        int32_t alpha_rule, int32_t pflags, const T* rho, const T* Ubuf, const T* F, const T* gout, T* gin,
+                        double* grad5, void* ws, size_t wsb, hipStream_t st) {
+    Mesh m;
+    int rc = check_fft_mesh(nmesh, &m);
+    if (rc) return rc;
+    if ((rc = check_prm<T>(prm, 5))) return rc;
+    if (!pos || !rho || !Ubuf || !F || !gout || !gin || !grad5) return fail(LDL_EINVAL, "NULL array argument");
+    PosDec<T> pd;
+    if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
+    const SmWs w = sm_ws(m, sizeof(T));
+    if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
+    const SmPtrs<T> P = sm_ptrs<T>(ws, w);
+    const c2<T>* tw;
+    if ((rc = twiddles<T>(&tw))) return rc;
+    const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hc = m.hcells(), cells = m.cells();
+    // 1. gF_d = paint(X, g_d)  (3 meshes in mesh3)
+    if ((rc = paint_impl<T>(pos, pdesc, np, gout, 0.0, 3, nmesh, 1.0, pflags, P.mesh3, P.paint, w.paint, st))) return rc;
+    const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
+    // 2. unnormalised r2c of the three meshes: z and y passes
+    dispatch_n(m.n2 / 2, [&](auto H) {
+        launch_zr2c<T, decltype(H)::value>(3 * nrows, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore<T>{P.spec3, nh, T(1)}, st);
+    });
+    dispatch_n(m.n1, [&](auto N) {
+        ColGeom g = col_geom<T, decltype(N)::value>(3 * m.n0, nh, m.n1 * nh, nh);
+        launch_colpass<T, decltype(N)::value, -1>(g, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st);
+    });
+    // 3. fused x pass: Q, parameter partials, inverse x of 1e-2 alpha G Q
+    const SmFilter<T> fl{(T)(prm[2] * prm[2]), (T)(1.0 / (prm[3] * prm[3])), (T)prm[4]};
+    const c2<T>* U = reinterpret_cast<const c2<T>*>(Ubuf);
+    c2<T>* S = P.spec3;  // result reuses the first spectrum (read before written per tile)
+    int64_t ntx = 0;
+    dispatch_n(m.n0, [&](auto N) {
+        constexpr int NN = decltype(N)::value;
+        ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
+        ntx = g.ntiles;
+        hipLaunchKernelGGL((k_sm_x_bwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U, S, hc, m.n2, kt,
+                           fl, (T)(1e-2 * prm[0]), prm[2], prm[3], prm[0], (int)alpha_rule, P.part);
+    });
+    RedMap rx{4, {0, 2, 3, 4}};
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, ntx, 4, rx, grad5, 0);
+    // 4. inverse y, z c2r with the gamma-power adjoint -> g_rho (mesh3[0])
+    dispatch_n(m.n1, [&](auto N) {
+        ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
+        launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{S, S, T(1)}, st);
+    });
+    T* grho = P.mesh3;
+    int64_t nzb = 0;
+    dispatch_n(m.n2 / 2, [&](auto H) {
+        constexpr int HH = decltype(H)::value;
+        nzb = (nrows + FftCfg<T, HH>::P - 1) / FftCfg<T, HH>::P;
+        launch_zc2r<T, HH>(nrows, tw, HalfRowLoad<T>{S, nh},
+                           GammaBwdStore<T>{rho, grho, m.n2, (T)(1.0 / (double)cells), (T)prm[1], P.part}, st);
+    });
+    RedMap rz{1, {1}};
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, nzb, 1, rz, grad5, 0);
+    // 5. position adjoint
+    if (np > 0)
+        hipLaunchKernelGGL(k_sm_bwd_gather<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, F, grho, m, (T)fact, gout, gin);
+    LDL_LAUNCH_CHECK();
+    return LDL_OK;
+}
+
+template <typename T>
+static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv, const T* field2, const int64_t nmesh[3],
+                   const double box[3], const double prm3[3], double index, int32_t p_norm, int32_t smooth_kind, double smooth_par,
+                   double* out5, T* grho, void* ws, size_t wsb, hipStream_t st) {
+    Mesh m;
+    int rc = smooth_kind ? check_fft_mesh(nmesh, &m) : check_mesh(nmesh, &m);
+    if (rc) return rc;
+    if ((rc = check_prm<T>(prm3, 3))) return rc;
+    if (!rho || !target || !out5) return fail(LDL_EINVAL, "NULL array argument");
+    if (p_norm != 1 && p_norm != 2) return fail(LDL_EINVAL, "p_norm must be 1 or 2");
+    if (smooth_kind < 0 || smooth_kind > 2) return fail(LDL_EINVAL, "smooth_kind must be 0, 1 or 2");
+    if (smooth_kind && (!box || !(box[0] > 0 && box[1] > 0 && box[2] > 0))) return fail(LDL_EINVAL, "box must be positive");
+    const SmWs w = sm_ws(m, sizeof(T));
+    if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
+    const SmPtrs<T> P = sm_ptrs<T>(ws, w);
+    const int64_t cells = m.cells();
+    // part layout: [0, 16) npix scratch and sums; block partials from 64
+    double* npix = P.part;          // {npix, npix_v}
+    double* sums = P.part + 8;      // 5 sums
+    double* bpart = P.part + 64;
+    const unsigned gb = 2048;
+    hipLaunchKernelGGL(k_mask_sums<T>, dim3(gb), dim3(256), 0, st, mask, maskv, cells, bpart);
+    RedMap rn{2, {0, 1}};
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, (int64_t)gb, 2, rn, npix, 1);
+    if (!mask) {  // npix = number of cells
+        const double nc = (double)cells;
+        hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(1), 0, st, npix, (int64_t)1, nc);
+    }
+    const RespPar<T> rp{(T)prm3[0], (T)prm3[1], (T)prm3[2], (T)index, p_norm == 1, field2};
+    RedMap r5{5, {0, 1, 2, 3, 4}};
+    if (smooth_kind == 0) {
+        hipLaunchKernelGGL(k_loss_direct<T>, dim3(gb), dim3(256), 0, st, rho, target, mask, maskv, rp, cells, npix, grho, bpart);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, (int64_t)gb, 5, r5, sums, 1);
+    } else {
+        const c2<T>* tw;
+        if ((rc = twiddles<T>(&tw))) return rc;
+        const KTabs<T> kt = build_tables<T>(P.tabs, m, box, st);
+        const int64_t nrows = m.n0 * m.n1, nh = m.nh();
+        T* diff = P.mesh3;
+        T* gds = P.mesh3 + cells;
+        c2<T>* S = P.spec3;
+        const T invM = (T)(1.0 / (double)cells);
+        auto smooth = [&](const T* src) {  // S = W(k) r2c(src) / M, inverse x and y passes
+            dispatch_n(m.n2 / 2, [&](auto H) {
+                launch_zr2c<T, decltype(H)::value>(nrows, tw, RealRowLoad<T>{src, m.n2}, HalfRowStore<T>{S, nh, T(1)}, st);
+            });
+            dispatch_n(m.n1, [&](auto N) {
+                ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
+                launch_colpass<T, decltype(N)::value, -1>(g, tw, PlainColOp<T>{S, S, T(1)}, st);
+            });
+            dispatch_n(m.n0, [&](auto N) {
+                constexpr int NN = decltype(N)::value;
+                ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
+                hipLaunchKernelGGL((k_x_realfilter<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, S, kt,
+                                   (int)smooth_kind, (T)smooth_par, invM);
+            });
+            dispatch_n(m.n1, [&](auto N) {
+                ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
+                launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{S, S, T(1)}, st);
+            });
+        };
+        int64_t nzb = 0;
+        dispatch_n(m.n2 / 2, [&](auto H) { nzb = (nrows + FftCfg<T, decltype(H)::value>::P - 1) / FftCfg<T, decltype(H)::value>::P; });
+        // every block writes the loss columns; the gradient columns stay zero without g_rho
+        hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nzb * 5, 256)), dim3(256), 0, st, bpart, nzb * 5, 0.0);
+        hipLaunchKernelGGL(k_loss_diff<T>, dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff);
+        smooth(diff);
+        dispatch_n(m.n2 / 2, [&](auto H) {
+            launch_zc2r<T, decltype(H)::value>(nrows, tw, HalfRowLoad<T>{S, nh},
+                                               LossStore<T>{mask, maskv, grho ? gds : nullptr, npix, m.n2, T(1), p_norm == 1, bpart}, st);
+        });
+        if (grho) {
+            smooth(gds);
+            dispatch_n(m.n2 / 2, [&](auto H) {
+                launch_zc2r<T, decltype(H)::value>(nrows, tw, HalfRowLoad<T>{S, nh}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart}, st);
+            });
+        }
+        hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, nzb, 5, r5, sums, 1);
+    }
+    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(64), 0, st, sums, npix, out5);
+    LDL_LAUNCH_CHECK();
+    return LDL_OK;
+}
+
+template <typename T>
+static int paint_vjp_pos(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* grho, const int64_t nmesh[3], double fact, T* g,
+                         hipStream_t st) {
+    Mesh m;
+    int rc = check_mesh(nmesh, &m);
+    if (rc) return rc;
+    if (np == 0) return LDL_OK;
+    if (!pos || !grho || !g) return fail(LDL_EINVAL, "NULL array argument");
+    PosDec<T> pd;
+    if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
+    hipLaunchKernelGGL(k_paint_vjp_pos<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, grho, m, (T)fact, g);
+    LDL_LAUNCH_CHECK();
+    return LDL_OK;
+}
+
+}  // namespace ldl
+
+using namespace ldl;
+
+extern "C" {
+
+size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64) {
+    Mesh m;
+    if (check_mesh(nmesh, &m)) return 0;
+    return sm_ws(m, is_f64 ? 8 : 4).total;
+}
+
+#define SM_ENTRY(SUF, T)                                                                                                    \
+    int ldl_sm_layer_fwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
+                               double fact, int32_t pflags, T* pos_out, T* rho, T* U, T* F, void* ws, size_t wsb, void* stream) { \
+        return sm_layer_fwd<T>(pos, pd, np, nmesh, prm, fact, pflags, pos_out, rho, U, F, ws, wsb, as_stream(stream));     \
+    }                                                                                                                       \
+    int ldl_sm_layer_bwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
+                               double fact, int32_t alpha_rule, int32_t pflags, const T* rho, const T* U, const T* F,      \
+                               const T* g_out, T* g_in, double* grad5, void* ws, size_t wsb, void* stream) {              \
+        return sm_layer_bwd<T>(pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, rho, U, F, g_out, g_in, grad5, ws, wsb,   \
+                               as_stream(stream));                                                                          \
+    }                                                                                                                       \
+    int ldl_sm_loss_##SUF(const T* rho, const T* target, const T* mask, const T* mask_valid, const T* field2,               \
+                          const int64_t nmesh[3], const double box[3], const double prm3[3], double index, int32_t p_norm,  \
+                          int32_t smooth_kind, double smooth_par, double* out5, T* g_rho, void* ws, size_t wsb,             \
+                          void* stream) {                                                                                   \
+        return sm_loss<T>(rho, target, mask, mask_valid, field2, nmesh, box, prm3, index, p_norm, smooth_kind, smooth_par,  \
+                          out5, g_rho, ws, wsb, as_stream(stream));                                                        \
+    }                                                                                                                       \
+    int ldl_paint_vjp_pos_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const T* g_rho, const int64_t nmesh[3],   \
+                                double fact, T* g, void* stream) {                                                          \
+        return paint_vjp_pos<T>(pos, pd, np, g_rho, nmesh, fact, g, as_stream(stream));                                     \
+    }
+
+SM_ENTRY(f32, float)
+SM_ENTRY(f64, double)
+
+}  // extern "C"

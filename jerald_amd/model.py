@@ -1,0 +1,281 @@
+"""JERALD's LDL model, stellar-mass ('sm') kind, on one MI355X.
+
+Mirrors ``JERALDModel`` (model.py:62-479): ``displace``, ``evaluate``,
+``loss``, ``loss_grad``, ``loss_and_grad``, ``lossv``, ``lossv_and_grad``
+with the reference's parameter layout ``[alpha, gamma, kh, kl, nu] * Nstep +
+[mu, w, b]`` (model.py:268-272, :339-341) and loss options (``n``, ``index``,
+``L1``, ``field2``, ``masktrain``, ``maskvalid``; model.py:350-468).
+
+The whole loss is one ``torch.autograd.Function`` whose forward runs the fused
+layer kernels (libldl ``ldl_sm_layer_fwd_*``) and the response/loss kernel, and
+whose backward runs the hand-written adjoint kernels (``ldl_sm_layer_bwd_*``).
+Per layer the forward keeps the layer's input positions and, with
+``save="all"``, its density, spectrum and force meshes; with
+``save="positions"`` only the positions at layer boundaries are kept and the
+backward recomputes each layer's forward (the memory-lean mode for 1024^3).
+
+Extensions over the reference, all opt-in:
+  * ``lattice=(l0, l1, l2)``: X is in lattice order and positions are held as
+    FP32/FP64 displacements (mesh units) from the lattice, so CIC fractions keep
+    full precision independent of the box coordinate;
+  * ``smoothing=("gaussian", sigma)``: Gaussian loss weight exp(-k^2 sigma^2/2);
+  * ``grad_rule``: "exact" (default) or "reference" -- the reference's custom
+    FFT VJP rules weight interior k_z modes by 1 instead of 2 in dL/dalpha
+    (jaxpops_bind.py:758-759 with model.py:198); every other gradient agrees;
+  * ``paint_mode``: "fixed" (64-bit fixed point, deterministic) or "float".
+Only the 'sm' kind is implemented; 'dm' and 'HI' raise NotImplementedError.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _as_mesh(a, nm, dtype, device):
+    if a is None:
+        return None
+    t = torch.as_tensor(a)
+    return t.to(device=device, dtype=dtype).reshape(nm).contiguous()
+
+
+class JERALDModel:
+    def __init__(self, X, target, pm, Nstep=2, kind="sm", n=0, index=1.0, L1=None, field2=None, masktrain=None,
+                 maskvalid=None, starmap=None, *, smoothing=None, grad_rule="exact", paint_mode="fixed",
+                 lattice=None, save="all", dtype=None):
+        if kind in ("dm", "HI"):
+            raise NotImplementedError(f"kind '{kind}' is not implemented; only 'sm' is on this path")
+        self.kind = "sm"  # unknown kinds use the sm model (model.py:110-112)
+        self.pm = pm
+        self.device = pm.device
+        self.dtype = dtype if dtype is not None else pm.dtype
+        self.Nstep = int(Nstep)
+        self.n = float(n)
+        self.index = float(index)
+        self.L1 = True if L1 is None else bool(L1)  # model.py:115-122 (True for non-dm kinds)
+        self.nm = tuple(int(v) for v in pm.Nmesh)
+        self.box = tuple(float(v) for v in pm.BoxSize)
+        self.cell = np.array(self.box) / np.array(self.nm)
+        if grad_rule not in ("exact", "reference"):
+            raise ValueError("grad_rule must be 'exact' or 'reference'")
+        self.alpha_rule = 0 if grad_rule == "exact" else 1
+        self.paint_flags = {"fixed": L.LDL_PAINT_FIXED | L.LDL_PAINT_WS_ZEROED,
+                            "float": L.LDL_PAINT_FLOAT_ATOMIC}[paint_mode]
+        if save not in ("all", "positions"):
+            raise ValueError("save must be 'all' or 'positions'")
+        self.save = save
+        # loss smoothing: reference k^-n + 1 when n > 0 (model.py:376-381)
+        if smoothing is None:
+            self.smooth_kind, self.smooth_par = (1, self.n) if self.n > 0 else (0, 0.0)
+        elif smoothing == "reference":
+            self.smooth_kind, self.smooth_par = 1, self.n
+        elif isinstance(smoothing, tuple) and smoothing[0] == "gaussian":
+            self.smooth_kind, self.smooth_par = 2, float(smoothing[1])
+        else:
+            raise ValueError("smoothing must be None, 'reference' or ('gaussian', sigma)")
+        self.target = _as_mesh(target, self.nm, self.dtype, self.device)
+        self.field2 = _as_mesh(field2, self.nm, self.dtype, self.device)
+        self.masktrain = _as_mesh(masktrain, self.nm, self.dtype, self.device)
+        self.maskvalid = _as_mesh(maskvalid, self.nm, self.dtype, self.device)
+        if starmap is not None:
+            raise NotImplementedError("starmap is only used by the HI kind")
+        self.lattice = None if lattice is None else tuple(int(v) for v in lattice)
+        self.X = None if X is None else self._encode(X)
+
+    # ------------------------------------------------------ positions
+    def _lattice_points(self):
+        """Lattice points in mesh units (float64) in particle order."""
+        l0, l1, l2 = self.lattice
+        step = np.array(self.nm, dtype=np.float64) / np.array(self.lattice, dtype=np.float64)
+        i = torch.arange(l0 * l1 * l2, device=self.device, dtype=torch.int64)
+        idx = torch.stack((i // (l1 * l2), (i // l2) % l1, i % l2), dim=1).to(torch.float64)
+        return idx * torch.tensor(step, device=self.device, dtype=torch.float64)
+
+    def _encode(self, X):
+        X = torch.as_tensor(X).to(self.device)
+        if X.dim() != 2 or X.shape[1] != 3:
+            raise ValueError("X must be (Np, 3)")
+        if self.lattice is None:
+            return X.to(self.dtype).contiguous()
+        if int(np.prod(self.lattice)) != X.shape[0]:
+            raise ValueError("lattice sizes must multiply to the number of particles")
+        cell = torch.tensor(self.cell, device=self.device, dtype=torch.float64)
+        d = X.to(torch.float64) / cell - self._lattice_points()
+        return d.to(self.dtype).contiguous()
+
+    def _decode(self, P):
+        if self.lattice is None:
+            return P
+        cell = torch.tensor(self.cell, device=self.device, dtype=torch.float64)
+        return ((self._lattice_points() + P.to(torch.float64)) * cell).to(self.dtype)
+
+    def _pd(self):
+        if self.lattice is None:
+            return L.pos_desc(self.box)
+        step = [self.nm[d] / self.lattice[d] for d in range(3)]
+        return L.pos_desc(self.box, L.LDL_POS_LATTICE, self.lattice, step)
+
+    # ------------------------------------------------------- kernels
+    def _ws(self):
+        nb = L.load().ldl_sm_workspace_bytes(L.i64x3(self.nm), int(self.dtype == torch.float64))
+        return L.workspaces.get(self.device, f"sm{self.dtype}", nb)
+
+    def _layer_fwd(self, P, prm, fact):
+        dt, dev, nm = self.dtype, self.device, self.nm
+        rho = torch.empty(nm, dtype=dt, device=dev)
+        U = torch.empty(nm[0], nm[1], nm[2] // 2 + 1, 2, dtype=dt, device=dev)
+        F = torch.empty((3,) + nm, dtype=dt, device=dev)
+        out = torch.empty_like(P)
+        ws = self._ws()
+        L.call(f"ldl_sm_layer_fwd_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], L.i64x3(nm), L.dvec(prm), float(fact),
+               self.paint_flags, L.ptr(out), L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+        return out, rho, U, F
+
+    def _paint_density(self, P, fact):
+        dt, dev, nm = self.dtype, self.device, self.nm
+        rho = torch.empty(nm, dtype=dt, device=dev)
+        nb = L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 1)
+        ws = L.workspaces.get(dev, f"paint1", nb)
+        L.call(f"ldl_ppaint_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], None, 1.0, L.i64x3(nm), float(fact),
+               self.paint_flags, L.ptr(rho), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+        return rho
+
+    def _loss_kernel(self, rho, prm3, need_grad):
+        dt, dev, nm = self.dtype, self.device, self.nm
+        out5 = torch.zeros(5, dtype=torch.float64, device=dev)
+        g_rho = torch.empty(nm, dtype=dt, device=dev) if need_grad else None
+        ws = self._ws()
+        L.call(f"ldl_sm_loss_{L.sfx(dt)}", L.ptr(rho), L.ptr(self.target), L.ptr(self.masktrain), L.ptr(self.maskvalid),
+               L.ptr(self.field2), L.i64x3(nm), L.dvec(self.box), L.dvec(prm3), self.index, 1 if self.L1 else 2,
+               self.smooth_kind, self.smooth_par, L.ptr(out5), L.ptr(g_rho), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+        return out5, g_rho
+
+    # ------------------------------------------------- model pieces
+    @staticmethod
+    def _params(params):
+        p = params.detach().to("cpu", torch.float64).numpy() if torch.is_tensor(params) else np.asarray(params, np.float64)
+        return p
+
+    def _check(self, p, Nstep):
+        if p.shape != (5 * Nstep + 3,):
+            raise ValueError(f"expected {5 * Nstep + 3} parameters, got {p.shape}")
+
+    def displace(self, params, X=None, Nstep=None):
+        """Positions after Nstep layers (model.py:249-289), box units."""
+        Nstep = self.Nstep if Nstep is None else int(Nstep)
+        p = self._params(params)
+        P = self.X if X is None else self._encode(X)
+        fact = float(np.prod(self.nm)) / P.shape[0]
+        for i in range(Nstep):
+            P = self._layer_fwd(P, p[5 * i: 5 * i + 5], fact)[0]
+        return self._decode(P)
+
+    def evaluate(self, params, X=None, Nstep=None):
+        """Model map ReLU(w (rho+1e-8)^mu + b) (model.py:291-347)."""
+        Nstep = self.Nstep if Nstep is None else int(Nstep)
+        p = self._params(params)
+        P = self.X if X is None else self._encode(X)
+        fact = float(np.prod(self.nm)) / P.shape[0]
+        for i in range(Nstep):
+            P = self._layer_fwd(P, p[5 * i: 5 * i + 5], fact)[0]
+        rho = self._paint_density(P, fact)
+        mu, w, b = (float(v) for v in p[5 * Nstep: 5 * Nstep + 3])
+        y = w * (rho + 1e-8) ** mu + b
+        return torch.where(y > 1e-12, y, torch.full_like(y, 1e-12))
+
+    def _forward(self, p, need_grad):
+        self._check(p, self.Nstep)
+        P = self.X
+        fact = float(np.prod(self.nm)) / P.shape[0]
+        saved = []
+        for i in range(self.Nstep):
+            out, rho, U, F = self._layer_fwd(P, p[5 * i: 5 * i + 5], fact)
+            if need_grad:
+                saved.append((P, rho, U, F) if self.save == "all" else (P,))
+            P = out
+        rho_f = self._paint_density(P, fact)
+        out5, g_rho = self._loss_kernel(rho_f, p[5 * self.Nstep: 5 * self.Nstep + 3], need_grad)
+        return out5, dict(saved=saved, P=P, g_rho=g_rho, fact=fact, out5=out5)
+
+    def _backward(self, p, st):
+        dt, dev, nm = self.dtype, self.device, self.nm
+        P = st["P"]
+        fact = st["fact"]
+        g = torch.empty_like(P)
+        L.call(f"ldl_paint_vjp_pos_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], L.ptr(st["g_rho"]), L.i64x3(nm),
+               float(fact), L.ptr(g), L.stream_ptr(dev))
+        grads = torch.zeros(5 * self.Nstep, dtype=torch.float64, device=dev)
+        ws = self._ws()
+        for i in reversed(range(self.Nstep)):
+            prm = p[5 * i: 5 * i + 5]
+            sv = st["saved"][i]
+            if self.save == "all":
+                Pi, rho, U, F = sv
+            else:
+                Pi = sv[0]
+                _, rho, U, F = self._layer_fwd(Pi, prm, fact)
+            gp = C_ptr_offset(grads, 5 * i)
+            L.call(f"ldl_sm_layer_bwd_{L.sfx(dt)}", L.ptr(Pi), self._pd(), Pi.shape[0], L.i64x3(nm), L.dvec(prm),
+                   float(fact), self.alpha_rule, self.paint_flags, L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(g), L.ptr(g),
+                   gp, L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+            st["saved"][i] = None
+        return torch.cat([grads, st["out5"][1:4]]), g
+
+    # ------------------------------------------------------- losses
+    def loss(self, params):
+        """Training loss (model.py:350-403) as a differentiable 0-d tensor."""
+        return _LDLLoss.apply(torch.as_tensor(params, dtype=torch.float64), self)[0]
+
+    def lossv(self, params):
+        """(loss, validation loss) (model.py:415-468)."""
+        p = self._params(params)
+        out5, _ = self._forward(p, False)
+        return out5[0], out5[4]
+
+    def loss_grad(self, params):
+        return self.loss_and_grad(params)[1]
+
+    def loss_and_grad(self, params):
+        p = self._params(params)
+        out5, st = self._forward(p, True)
+        grad, _ = self._backward(p, st)
+        return out5[0], grad
+
+    def lossv_and_grad(self, params):
+        """(loss, validation loss, gradient) as the reference returns them
+        (model.py:476-479)."""
+        p = self._params(params)
+        out5, st = self._forward(p, True)
+        grad, _ = self._backward(p, st)
+        return out5[0], out5[4], grad
+
+    def loss_and_grad_positions(self, params):
+        """Loss and dL/dX of the initial positions (box units)."""
+        p = self._params(params)
+        out5, st = self._forward(p, True)
+        grad, g = self._backward(p, st)
+        return out5[0], grad, g
+
+
+def C_ptr_offset(t, n_elems):
+    """Device pointer ``n_elems`` elements past the start of ``t``."""
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr() + n_elems * t.element_size())
+
+
+class _LDLLoss(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, params, model):
+        p = model._params(params)
+        need = bool(ctx.needs_input_grad[0]) if hasattr(ctx, "needs_input_grad") else True
+        out5, st = model._forward(p, need)
+        ctx.model, ctx.p, ctx.st = model, p, st
+        ctx.param_device, ctx.param_dtype = params.device, params.dtype
+        return out5[0].clone(), out5[4].clone()
+
+    @staticmethod
+    def backward(ctx, gloss, glossv):
+        grad, _ = ctx.model._backward(ctx.p, ctx.st)
+        ctx.st = None
+        return (grad * gloss).to(ctx.param_device, ctx.param_dtype), None

@@ -1,0 +1,166 @@
+"""GPU parity of the CIC kernels through the C ABI (jaxpops mirror) against
+the oracle: bit-exact fixed-point paint, gathers within FP tolerance, the
+reference's own golden vectors, autograd adjoints, determinism."""
+import numpy as np
+import pytest
+import torch
+
+from jerald_amd import jaxpops as J
+from oracle import ldl_oracle as O
+from oracle import native as NV
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+TOL = {torch.float32: dict(rtol=2e-5, atol=2e-5), torch.float64: dict(rtol=1e-12, atol=1e-12)}
+
+
+def _inputs(np_=3000, nm=(16, 12, 20), box=(32.0, 24.0, 50.0), seed=0):
+    rng = np.random.default_rng(seed)
+    pos = rng.random((np_, 3)) * np.array(box) * 3 - np.array(box)
+    return pos, nm, np.array(box)
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_paint_fixed_point_bit_exact(dt):
+    pos, nm, box = _inputs()
+    ndt = np.float32 if dt == torch.float32 else np.float64
+    p = pos.astype(ndt)
+    m = (np.random.default_rng(1).standard_normal(len(p))).astype(ndt)
+    for mass in (1.0, m, 2.5):
+        got = J.ppaint(torch.tensor(p, device=DEV), torch.tensor(mass, device=DEV) if np.ndim(mass) else mass,
+                       nm, box).cpu().numpy()
+        acc, s = NV.paint_fixed(p, mass, nm, box=box, dtype=ndt)
+        want = NV.fixed_to_float(acc, s, ndt)
+        np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_paint_deterministic_and_conserves_mass(dt):
+    pos, nm, box = _inputs(np_=200000, nm=(32, 32, 32), box=(64.0, 64.0, 64.0))
+    t = torch.tensor(pos, device=DEV, dtype=dt)
+    a = J.ppaint(t, 1.0, nm, box)
+    for _ in range(3):
+        b = J.ppaint(t[torch.randperm(t.shape[0], device=DEV)], 1.0, nm, box)
+        assert torch.equal(a, b)  # order independent, bit for bit
+    assert abs(a.double().sum().item() - 200000.0) < (1e-9 if dt == torch.float64 else 2.0)
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_paint_float_atomic_mode(dt):
+    pos, nm, box = _inputs()
+    x = O.mesh_units(pos, box, nm, np.float32 if dt == torch.float32 else np.float64).astype(np.float64)
+    want = O.paint(x, 1.0, nm)
+    J.set_paint_mode("float")
+    try:
+        got = J.ppaint(torch.tensor(pos, device=DEV, dtype=dt), 1.0, nm, box).cpu().numpy()
+    finally:
+        J.set_paint_mode("fixed")
+    np.testing.assert_allclose(got, want, **TOL[dt])
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_gathers_match_oracle(dt):
+    pos, nm, box = _inputs()
+    ndt = np.float32 if dt == torch.float32 else np.float64
+    x = O.mesh_units(pos, box, nm, ndt).astype(np.float64)
+    rng = np.random.default_rng(2)
+    F = rng.standard_normal(nm)
+    F3 = rng.standard_normal((3,) + nm)
+    v3 = rng.standard_normal((len(pos), 3))
+    tp = torch.tensor(pos, device=DEV, dtype=dt)
+    ci = np.array(nm) / box
+    Fd = F.astype(ndt).astype(np.float64)
+    F3d = F3.astype(ndt).astype(np.float64)
+    np.testing.assert_allclose(J.preadout(tp, torch.tensor(F, device=DEV, dtype=dt), nm, box).cpu().numpy(),
+                               O.readout(x, Fd), **TOL[dt])
+    for d in range(3):
+        np.testing.assert_allclose(
+            J.preadout(tp, torch.tensor(F, device=DEV, dtype=dt), nm, box, vjpdim=d).cpu().numpy(),
+            O.readout(x, Fd, d, ci), rtol=TOL[dt]["rtol"], atol=TOL[dt]["atol"] * 10)
+    np.testing.assert_allclose(J.preadout3D(tp, torch.tensor(F3, device=DEV, dtype=dt), nm, box).cpu().numpy(),
+                               O.readout3(x, F3d), **TOL[dt])
+    np.testing.assert_allclose(J.preadout3D(tp, torch.tensor(F, device=DEV, dtype=dt), nm, box, vjp=True).cpu().numpy(),
+                               O.readout_grad(x, Fd, ci), rtol=TOL[dt]["rtol"], atol=TOL[dt]["atol"] * 10)
+    from jerald_amd.jaxpops import _readout_grad3_raw
+    want = sum(v3[:, d:d + 1].astype(ndt) * O.readout_grad(x, F3d[d], ci) for d in range(3))
+    got = _readout_grad3_raw(tp, torch.tensor(F3, device=DEV, dtype=dt), torch.tensor(v3, device=DEV, dtype=dt), nm,
+                             tuple(box)).cpu().numpy()
+    np.testing.assert_allclose(got, want, rtol=TOL[dt]["rtol"] * 10, atol=TOL[dt]["atol"] * 50)
+
+
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+def test_reference_golden_vectors(golden_cic, tag):
+    """The reference's own lib/parops_fast.h outputs (oracle/_ref)."""
+    g = golden_cic
+    dt = torch.float64 if tag == "f64" else torch.float32
+    tol = dict(rtol=0, atol=1e-12) if tag == "f64" else dict(rtol=0, atol=3e-6)
+    nm, box = tuple(int(v) for v in g["nmesh"]), g["box"]
+    pos = torch.tensor(g["pos"], device=DEV, dtype=dt)
+    np.testing.assert_allclose(J.ppaint(pos, torch.tensor(g["mass"], device=DEV, dtype=dt), nm, box).cpu().numpy(),
+                               g[f"paint_{tag}"], **tol)
+    np.testing.assert_allclose(J.ppaint(pos, 1.0, nm, box).cpu().numpy(), g[f"paint_unit_{tag}"], **tol)
+    f = torch.tensor(g["field"], device=DEV, dtype=dt)
+    np.testing.assert_allclose(J.preadout(pos, f, nm, box).cpu().numpy(), g[f"readout_{tag}"], **tol)
+    np.testing.assert_allclose(J.preadout3D(pos, f, nm, box, vjp=True).cpu().numpy(), g[f"readout_grad_{tag}"],
+                               rtol=0, atol=tol["atol"] * 10)
+    f3 = torch.tensor(g["fields3"], device=DEV, dtype=dt)
+    np.testing.assert_allclose(J.preadout3D(pos, f3, nm, box).cpu().numpy(), g[f"readout3_{tag}"], **tol)
+    m3 = torch.tensor(g["mass3"], device=DEV, dtype=dt)
+    np.testing.assert_allclose(J.ppaint3D(pos, m3, nm, box).cpu().numpy(), g[f"paint3_{tag}"], **tol)
+
+
+def test_lattice_positions_match_physical():
+    """Displacement-from-lattice form == absolute positions (fixed point)."""
+    from jerald_amd import _lib as L
+    n, N, box = 16, 32, 64.0
+    X = O.lattice_positions(n, box) + O.zeldovich_displacement(n, box, 2.0, 3)
+    q = O.lattice_positions(n, box) / (box / N)
+    d = (X / (box / N) - q).astype(np.float32)
+    nm = (N, N, N)
+    acc, s = NV.paint_fixed(d, 1.0, nm, lattice=(n, n, n), lattice_step=(2.0, 2.0, 2.0))
+    out = torch.empty(nm, device=DEV)
+    ws = torch.zeros(L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 1), dtype=torch.uint8, device=DEV)
+    pd = L.pos_desc((box,) * 3, L.LDL_POS_LATTICE, (n, n, n), (2.0, 2.0, 2.0))
+    td = torch.tensor(d, device=DEV)
+    L.call("ldl_ppaint_f32", L.ptr(td), pd, td.shape[0], None, 1.0, L.i64x3(nm), 1.0, 0, L.ptr(out), L.ptr(ws),
+           ws.numel(), L.stream_ptr(td.device))
+    np.testing.assert_array_equal(out.cpu().numpy(), NV.fixed_to_float(acc, s, np.float32))
+    ref = O.paint(O.mesh_units(X, box, nm), 1.0, nm)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, atol=1e-5)
+
+
+@pytest.mark.parametrize("dt", [torch.float64])
+def test_autograd_adjoints(dt):
+    """ppaint/preadout/ppaint3D/preadout3D backward == oracle VJP formulas."""
+    pos, nm, box = _inputs(np_=500, nm=(8, 8, 8), box=(16.0, 16.0, 16.0))
+    rng = np.random.default_rng(4)
+    x = O.mesh_units(pos, box, nm)
+    ci = np.array(nm) / box
+    tp = torch.tensor(pos, device=DEV, dtype=dt, requires_grad=True)
+    m = torch.tensor(rng.random(500), device=DEV, dtype=dt, requires_grad=True)
+    v = rng.standard_normal(nm)
+    out = J.ppaint(tp, m, nm, box)
+    out.backward(torch.tensor(v, device=DEV, dtype=dt))
+    np.testing.assert_allclose(tp.grad.cpu().numpy(), O.readout_grad(x, v, ci) * m.detach().cpu().numpy()[:, None],
+                               rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(m.grad.cpu().numpy(), O.readout(x, v), rtol=1e-10, atol=1e-10)
+    tp.grad = None
+    F = torch.tensor(rng.standard_normal(nm), device=DEV, dtype=dt, requires_grad=True)
+    vm = rng.standard_normal(500)
+    J.preadout(tp, F, nm, box).backward(torch.tensor(vm, device=DEV, dtype=dt))
+    np.testing.assert_allclose(tp.grad.cpu().numpy(), O.readout_grad(x, F.detach().cpu().numpy(), ci) * vm[:, None],
+                               rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(F.grad.cpu().numpy(), O.paint(x, vm, nm), rtol=1e-10, atol=1e-10)
+    tp.grad = None
+    F3 = torch.tensor(rng.standard_normal((3,) + nm), device=DEV, dtype=dt, requires_grad=True)
+    v3 = rng.standard_normal((500, 3))
+    J.preadout3D(tp, F3, nm, box).backward(torch.tensor(v3, device=DEV, dtype=dt))
+    F3n = F3.detach().cpu().numpy()
+    np.testing.assert_allclose(tp.grad.cpu().numpy(), sum(v3[:, d:d + 1] * O.readout_grad(x, F3n[d], ci) for d in range(3)),
+                               rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(F3.grad.cpu().numpy(), O.paint3(x, v3, nm), rtol=1e-10, atol=1e-10)
+    # numerical gradient check of the paint->readout composition (positions off cell faces)
+    tp2 = torch.tensor(np.floor(x) + 0.25 + 0.5 * rng.random(x.shape), device=DEV, dtype=dt) * torch.tensor(box / np.array(nm), device=DEV, dtype=dt)
+    tp2.requires_grad_(True)
+    Fg = torch.tensor(rng.standard_normal(nm), device=DEV, dtype=dt)
+    assert torch.autograd.gradcheck(lambda p: J.preadout(p, Fg, nm, box), (tp2,), eps=1e-6, atol=1e-6)

@@ -1,0 +1,165 @@
+"""GPU parity of the 3D FFTs and of the fused LDL model against the oracle.
+
+Tolerances: FP64 kernels vs the FP64 oracle to ~1e-10 relative; FP32 runs
+vs the FP64 oracle to the 1e-5 relative the LDL path targets for the loss and
+parameter gradients."""
+import numpy as np
+import pytest
+import torch
+
+from jerald_amd import jaxpops as J
+from jerald_amd.model import JERALDModel
+from jerald_amd.pmesh import PMesh
+from oracle import ldl_oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+SHAPES = [(8, 8, 8), (4, 16, 8), (32, 16, 64), (64, 64, 64), (128, 32, 16), (16, 8, 256), (8, 4, 1024), (512, 4, 8)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_pfft_pifft_match_numpy(shape, dt):
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal(shape)
+    X = np.fft.rfftn(x)
+    got = J.pfft(torch.tensor(x, device=DEV, dtype=dt)).cpu().numpy()
+    scale = np.abs(X).max()
+    rt = 3e-6 if dt == torch.float32 else 1e-13
+    assert np.abs(got - X).max() <= rt * scale * np.log2(x.size)
+    # c2r of a non-Hermitian spectrum (random imaginary parts on the k_z = 0, N/2 planes)
+    Y = rng.standard_normal(X.shape) + 1j * rng.standard_normal(X.shape)
+    want = O.pifft(Y, shape)
+    cdt = torch.complex64 if dt == torch.float32 else torch.complex128
+    Yt = torch.tensor(Y, device=DEV, dtype=cdt)
+    got = J.pifft(Yt, shape).cpu().numpy()
+    assert np.abs(got - want).max() <= rt * np.abs(want).max() * np.log2(x.size)
+    assert torch.equal(Yt, torch.tensor(Y, device=DEV, dtype=cdt))  # input untouched
+
+
+def test_fft_round_trip_large():
+    """Size-independent property at 256^3: c2r(r2c(x)) = N^3 x, Parseval."""
+    shape = (256, 256, 256)
+    x = torch.randn(shape, device=DEV, generator=torch.Generator(DEV).manual_seed(0))
+    X = J.pfft(x)
+    c = J.hermitian_weight(256, DEV, torch.float64)
+    e_k = ((X.abs().double() ** 2) * c).sum() / x.numel()
+    e_x = (x.double() ** 2).sum()
+    assert abs(e_k.item() / e_x.item() - 1) < 1e-5
+    y = J.pifft(X, shape) / x.numel()
+    assert (y - x).abs().max().item() < 1e-5
+
+
+@pytest.mark.parametrize("rule", ["exact", "reference"])
+def test_fft_vjp_rules(rule):
+    shape = (8, 16, 8)
+    rng = np.random.default_rng(7)
+    x = torch.tensor(rng.standard_normal(shape), device=DEV, dtype=torch.float64, requires_grad=True)
+    G = rng.standard_normal((8, 16, 5)) + 1j * rng.standard_normal((8, 16, 5))
+    J.set_vjp_rule(rule)
+    try:
+        J.pfft(x).backward(torch.tensor(G, device=DEV))
+        want = O.exact_pfft_bwd(G, shape) if rule == "exact" else O.ref_rule_pfft_bwd(G, shape)
+        np.testing.assert_allclose(x.grad.cpu().numpy(), want, rtol=1e-10, atol=1e-10)
+        Y = torch.tensor(G, device=DEV, requires_grad=True)
+        g = rng.standard_normal(shape)
+        J.pifft(Y, shape).backward(torch.tensor(g, device=DEV))
+        want = O.exact_pifft_bwd(g) if rule == "exact" else O.ref_rule_pifft_bwd(g)
+        np.testing.assert_allclose(Y.grad.cpu().numpy(), want, rtol=1e-10, atol=1e-10)
+        if rule == "exact":  # agrees with torch's own FFT autograd
+            x2 = x.detach().clone().requires_grad_(True)
+            torch.fft.rfftn(x2).backward(torch.tensor(G, device=DEV))
+            np.testing.assert_allclose(x.grad.cpu().numpy(), x2.grad.cpu().numpy(), rtol=1e-10, atol=1e-10)
+    finally:
+        J.set_vjp_rule("reference")
+
+
+def _problem(N=16, nstep=2, L=64.0, seed=83, smoothing=None, L1=True, mask=False, rms=1.5):
+    X = O.lattice_positions(N, L) + O.zeldovich_displacement(N, L, rms, seed)
+    rng = np.random.default_rng(seed + 1)
+    tgt = rng.random((N, N, N)) + 0.5
+    mk = (rng.random((N, N, N)) > 0.3).astype(float) if mask else None
+    p = np.array([0.8, 1.2, 0.3, 1.5, 0.2] * nstep + [1.1, 0.9, 0.05])
+    orc = O.SMModelOracle(N, L, nstep, tgt, L1=L1, smoothing=smoothing, mask=mk)
+    return X, tgt, mk, p, orc
+
+
+def _model(X, tgt, mk, N, L, nstep, dt, **kw):
+    pm = PMesh(N, L, device=DEV, dtype=dt)
+    return JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, kind="sm", masktrain=mk, **kw)
+
+
+@pytest.mark.parametrize("smoothing,L1,mask", [(None, True, True), (None, False, False),
+                                               (("gaussian", 4.0), False, False), (("gaussian", 4.0), True, True)])
+@pytest.mark.parametrize("lattice", [None, (16, 16, 16)])
+def test_ldl_fp64_matches_oracle(smoothing, L1, mask, lattice):
+    N, L, nstep = 16, 64.0, 2
+    X, tgt, mk, p, orc = _problem(N, nstep, L, smoothing=smoothing, L1=L1, mask=mask)
+    loss0, g0 = orc.loss_and_grad(p, X)
+    m = _model(X, tgt, mk, N, L, nstep, torch.float64, L1=L1, smoothing=smoothing, lattice=lattice)
+    loss, grad = m.loss_and_grad(torch.tensor(p))
+    assert abs(loss.item() - loss0) <= 1e-10 * abs(loss0)
+    np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
+
+
+def test_ldl_fp32_within_1e5_of_fp64():
+    N, L, nstep = 32, 128.0, 2
+    X, tgt, mk, p, orc = _problem(N, nstep, L, smoothing=("gaussian", 4.0), L1=False, rms=2.0)
+    out = {}
+    for dt in (torch.float64, torch.float32):
+        m = _model(X, tgt, mk, N, L, nstep, dt, L1=False, smoothing=("gaussian", 4.0), lattice=(N, N, N))
+        loss, grad = m.loss_and_grad(torch.tensor(p))
+        out[dt] = (loss.item(), grad.cpu().numpy())
+    l64, g64 = out[torch.float64]
+    l32, g32 = out[torch.float32]
+    assert abs(l32 - l64) <= 1e-5 * abs(l64)
+    rel = np.abs(g32 - g64) / np.abs(g64)
+    assert np.all(rel < 1e-4), rel
+    loss0, g0 = orc.loss_and_grad(p, X)
+    assert abs(l64 - loss0) <= 1e-10 * abs(loss0)
+
+
+def test_ldl_reference_alpha_rule_and_position_grad():
+    N, L, nstep = 16, 64.0, 2
+    X, tgt, mk, p, _ = _problem(N, nstep, L, L1=False)
+    orc = O.SMModelOracle(N, L, nstep, tgt, L1=False, grad_rule="reference")
+    loss0, g0, gX0 = orc.loss_and_grad(p, X, want_pos_grad=True)
+    m = _model(X, tgt, mk, N, L, nstep, torch.float64, L1=False, grad_rule="reference")
+    loss, grad, gX = m.loss_and_grad_positions(torch.tensor(p))
+    np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-8)
+    np.testing.assert_allclose(gX.cpu().numpy(), gX0, rtol=1e-7, atol=1e-12 * np.abs(gX0).max())
+
+
+def test_ldl_autograd_function_and_recompute_mode():
+    N, L, nstep = 16, 64.0, 2
+    X, tgt, mk, p, orc = _problem(N, nstep, L, L1=False)
+    _, g0 = orc.loss_and_grad(p, X)
+    m = _model(X, tgt, mk, N, L, nstep, torch.float64, L1=False, save="positions")
+    pt = torch.tensor(p, requires_grad=True)
+    loss = m.loss(pt)
+    loss.backward()
+    np.testing.assert_allclose(pt.grad.numpy(), g0, rtol=1e-8)
+    # displace/evaluate agree with the oracle
+    Xf = m.displace(torch.tensor(p)).cpu().numpy()
+    np.testing.assert_allclose(Xf, orc.displace(p, X), rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(m.evaluate(torch.tensor(p)).cpu().numpy(), orc.evaluate(p, X), rtol=1e-9, atol=1e-12)
+
+
+def test_lossv_and_reference_smoothing():
+    N, L, nstep = 16, 64.0, 1
+    X = O.lattice_positions(N, L) + O.zeldovich_displacement(N, L, 1.5, 5)
+    rng = np.random.default_rng(6)
+    tgt = rng.random((N, N, N)) + 0.5
+    mt = np.ones((N, N, N))
+    mv = (rng.random((N, N, N)) > 0.5).astype(float)
+    p = np.array([0.5, 0.9, 0.2, 1.3, 0.1, 1.0, 1.0, 0.0])
+    orc = O.SMModelOracle(N, L, nstep, tgt, L1=True, n=1.0, mask=mt)
+    orv = O.SMModelOracle(N, L, nstep, tgt, L1=True, n=1.0, mask=mv)
+    pm = PMesh(N, L, device=DEV, dtype=torch.float64)
+    m = JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, kind="sm", n=1.0, masktrain=mt, maskvalid=mv)
+    loss, lossv, grad = m.lossv_and_grad(torch.tensor(p))
+    l0, g0 = orc.loss_and_grad(p, X)
+    assert abs(loss.item() - l0) <= 1e-9 * abs(l0)
+    assert abs(lossv.item() - orv.loss(p, X)) <= 1e-9 * abs(l0)
+    np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-7)
