@@ -1,0 +1,242 @@
+#!/usr/bin/env python3
+"""LDL forward+backward throughput on one MI355X.
+
+Workload (the headline configuration): 512^3 particles on a 512^3 mesh, box
+L = 409.6 Mpc/h (0.8 Mpc/h cells), positions held as FP32 displacements from
+the Lagrangian lattice (seeded Zel'dovich field, rms 2 cells), 2 sm-kind LDL
+layers, parameters [alpha, gamma, kh, kl, nu] = [0.01, 1, 0.1, 1, 0] per layer
+and [mu, w, b] = [1, 1, 0]; target = the model at seed-perturbed (+-10%)
+parameters plus 1% Gaussian noise; loss = L2 of the residual smoothed with a
+1-cell Gaussian.  One step = loss and gradient w.r.t. all 13 parameters.
+
+Prints one JSON line (rank 0) with the metric, the dominant kernel's roofline
+(HIP-event timing of its launches inside the timed region), the plain FP64
+GPU path on the same inputs, the FP32-vs-FP64 relative errors and a CPU
+baseline (the FP64 oracle restatement on a bounded sample).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public spec
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512, help="particles per side = mesh per side")
+    ap.add_argument("--nstep", type=int, default=2, help="LDL layers")
+    ap.add_argument("--fp64-steps", type=int, default=3, help="timed steps of the plain FP64 path (0: skip)")
+    ap.add_argument("--cpu-n", type=int, default=256, help="per-side size of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--save", default="all", choices=["all", "positions"])
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ inputs
+def zeldovich_gpu(n, box, rms_cells, seed, device):
+    """Lattice + Zel'dovich displacement (box units, float64), built on the
+    GPU with torch.fft (input setup only; not part of the timed path).  Same
+    recipe as oracle.ldl_oracle.zeldovich_displacement."""
+    import torch
+    g = torch.Generator(device=device).manual_seed(seed)
+    white = torch.randn((n, n, n), generator=g, device=device, dtype=torch.float64)
+    wk = torch.fft.rfftn(white)
+    del white
+    cell = box / n
+    k1 = torch.fft.fftfreq(n, d=cell, device=device, dtype=torch.float64) * 2 * math.pi
+    kz = k1[: n // 2 + 1]
+    k2 = k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kz[None, None, :] ** 2
+    k2s = torch.where(k2 > 0, k2, torch.ones_like(k2))
+    pk = torch.where(k2 > 0, torch.exp(-k2 * (2 * cell) ** 2) / k2s, torch.zeros_like(k2))
+    dk = torch.sqrt(pk) * wk / k2s
+    del wk, pk
+    psi = torch.empty((n ** 3, 3), device=device, dtype=torch.float64)
+    for d, kk in enumerate((k1[:, None, None], k1[None, :, None], kz[None, None, :])):
+        psi[:, d] = torch.fft.irfftn(1j * kk * dk, s=(n, n, n)).reshape(-1)
+    del dk, k2, k2s
+    psi *= rms_cells * cell / torch.sqrt(torch.mean(psi ** 2))
+    i = torch.arange(n, device=device, dtype=torch.float64) * cell
+    q = torch.stack(torch.meshgrid(i, i, i, indexing="ij"), dim=-1).reshape(-1, 3)
+    return q + psi
+
+
+def params0(nstep):
+    import numpy as np
+    return np.array([0.01, 1.0, 0.1, 1.0, 0.0] * nstep + [1.0, 1.0, 0.0])
+
+
+# --------------------------------------------------------- algorithmic bytes
+def kernel_bytes(name, n, b):
+    """Compulsory HBM bytes of one launch at n^3 particles / n^3 mesh, b bytes
+    per real.  Np = Nm = n^3, Nh = n^2 (n/2+1)."""
+    Np = n ** 3
+    Nm = n ** 3
+    Nh = n * n * (n // 2 + 1)
+    return {
+        "readout3_update": 3 * b * Np + 3 * b * Nm + 3 * b * Np,        # pos in, 3 fields, pos out
+        "sm_bwd_gather": 6 * b * Np + 4 * b * Nm + 3 * b * Np,          # pos + g in, 4 fields, g out
+        "paint_deposit": 3 * b * Np + 8 * Nm * 1,                       # pos in, int64 mesh once
+        "sm_x_fwd": 2 * b * Nh + 2 * b * Nh + 3 * 2 * b * Nh,           # U in, U out, 3 spectra out
+        "sm_x_bwd": 3 * 2 * b * Nh + 2 * b * Nh + 2 * b * Nh,           # 3 spectra + U in, 1 out
+    }.get(name)
+
+
+def time_steps(model, p, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        model.loss_and_grad(p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = model.loss_and_grad(p)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the LDL path here is a "
+              f"single-GPU build; multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from jerald_amd import _lib as L
+    from jerald_amd.model import JERALDModel
+    from jerald_amd.pmesh import PMesh
+
+    dev = torch.device("cuda:0")
+    n, nstep = args.n, args.nstep
+    box = 0.8 * n
+    cell = box / n
+    p = params0(nstep)
+    log(f"[bench] building inputs n={n} box={box}")
+    X = zeldovich_gpu(n, box, 2.0, 83, dev)
+    pm32 = PMesh(n, box, device=dev, dtype=torch.float32)
+    kw = dict(Nstep=nstep, kind="sm", L1=False, smoothing=("gaussian", cell))
+    gen = JERALDModel(X, None, pm32, lattice=(n, n, n), **kw)
+    rng = np.random.default_rng(84)
+    pt = p * (1 + 0.1 * (2 * rng.random(p.shape) - 1))
+    target = gen.evaluate(torch.tensor(pt)).double()
+    gnoise = torch.Generator(device=dev).manual_seed(85)
+    target += 0.01 * target.std() * torch.randn(target.shape, generator=gnoise, device=dev, dtype=torch.float64)
+    del gen
+    model = JERALDModel(X, target, pm32, lattice=(n, n, n), save=args.save, **kw)
+    log("[bench] FP32 warmup + timed steps")
+    L.timer_enable(False)
+    for _ in range(args.warmup):
+        model.loss_and_grad(p)
+    torch.cuda.synchronize()
+    L.timer_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss32, grad32 = model.loss_and_grad(p)
+    torch.cuda.synchronize()
+    dt32 = time.perf_counter() - t0
+    kern = {}
+    for name in ("paint_deposit", "paint_convert", "zpass_r2c", "zpass_c2r", "colpass", "sm_x_fwd", "sm_x_bwd",
+                 "readout3_update", "sm_bwd_gather", "paint_vjp_pos", "loss", "x_realfilter"):
+        tot, cnt = L.timer_read(name)
+        if cnt:
+            kern[name] = (tot, cnt)
+    L.timer_enable(False)
+    loss32 = loss32.item()
+    grad32 = grad32.cpu().numpy()
+    ms = 1e3 * dt32 / args.steps
+    log(f"[bench] FP32: {ms:.2f} ms/step, loss {loss32:.8g}")
+    # dominant instrumented kernel with a defined algorithmic byte count
+    total_k = sum(t for t, _ in kern.values())
+    ranked = sorted(kern.items(), key=lambda kv: -kv[1][0])
+    dom = next((k for k, _ in ranked if kernel_bytes(k, n, 4) is not None), None)
+    roof = None
+    if dom is not None:
+        tot, cnt = kern[dom]
+        avg_s = tot / cnt / 1e3
+        by = kernel_bytes(dom, n, 4)
+        ach = by / avg_s / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 4),
+                "algorithmic_bytes": by, "share_of_kernel_time": round(tot / total_k, 3)}
+    kernel_table = {k: {"ms_per_step": round(t / args.steps, 3), "launches_per_step": c / args.steps}
+                    for k, (t, c) in ranked}
+    # plain FP64 GPU path: FP64 absolute positions, FP64 atomics, same kernels
+    fp64 = None
+    if args.fp64_steps > 0:
+        log("[bench] FP64 plain path")
+        del model
+        torch.cuda.empty_cache()
+        pm64 = PMesh(n, box, device=dev, dtype=torch.float64)
+        m64 = JERALDModel(X, target, pm64, paint_mode="float", save=args.save, **kw)
+        t64, (l64, g64) = time_steps(m64, p, args.fp64_steps, 1)
+        l64 = l64.item()
+        g64 = g64.cpu().numpy()
+        rel = np.abs(grad32 - g64) / np.maximum(np.abs(g64), 1e-300)
+        fp64 = {"value": round(args.fp64_steps / t64, 4), "unit": "it/s", "ms_per_step": round(1e3 * t64 / args.fp64_steps, 2),
+                "loss": l64, "loss_rel_err_fp32": abs(loss32 - l64) / abs(l64),
+                "grad_rel_err_fp32_max": float(rel.max()), "grad_rel_err_fp32": [float(v) for v in rel]}
+        del m64
+        torch.cuda.empty_cache()
+    # CPU baseline: FP64 oracle restatement (C CIC + pocketfft) on a bounded sample
+    cpu = None
+    if args.cpu_n > 0:
+        log(f"[bench] CPU baseline sample n={args.cpu_n}")
+        from oracle import ldl_oracle as O
+        nc = args.cpu_n
+        boxc = 0.8 * nc
+        Xc = zeldovich_gpu(nc, boxc, 2.0, 83, dev).cpu().numpy()
+        tgtc = 1.0 + 0.1 * np.random.default_rng(86).standard_normal((nc, nc, nc))
+        orc = O.SMModelOracle(nc, boxc, nstep, tgtc, L1=False, smoothing=("gaussian", boxc / nc), backend="native",
+                              threads=args.cpu_threads)
+        t0 = time.perf_counter()
+        orc.loss_and_grad(p, Xc)
+        tc = time.perf_counter() - t0
+        scale = (n / nc) ** 3
+        cpu = {"value": round(1.0 / (tc * scale), 6), "unit": "it/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"one FP64 fwd+bwd iteration at {nc}^3 particles/mesh took {tc:.2f} s on "
+                         f"{args.cpu_threads} threads (oracle C CIC + pocketfft); value scaled by "
+                         f"({n}/{nc})^3 = {scale:.0f} to {n}^3"}
+    line = {
+        "metric": "LDL fwd+bwd iters/s @512^3 FP32 (2 layers, 1 GPU)",
+        "value": round(args.steps / dt32, 4),
+        "unit": "it/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Zel'dovich lattice, generated on the GPU)",
+        "config": {"workload": f"{n}^3 particles on {n}^3 mesh, {nstep} sm LDL layers, L2 loss with 1-cell "
+                               f"Gaussian smoothing, loss+grad of {5 * nstep + 3} params",
+                   "n": n, "nstep": nstep, "box": box, "positions": "fp32 displacements from lattice",
+                   "paint": "64-bit fixed point", "save": args.save},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64_plain_gpu": fp64,
+        "loss_fp32": loss32,
+        "kernels": kernel_table,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

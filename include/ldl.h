@@ -61,6 +61,15 @@ typedef struct ldl_pos_desc {
 const char* ldl_last_error(void);
 int ldl_version(void);
 
+/* Benchmark instrumentation (no counterpart in the reference).  While enabled,
+ * the main kernels are bracketed by HIP events on their launch stream;
+ * ldl_timer_read() waits for them and returns the summed device time and the
+ * launch count of one kernel name ("paint_deposit", "paint_convert", "zpass_r2c",
+ * "zpass_c2r", "colpass", "sm_x_fwd", "sm_x_bwd", "readout3_update",
+ * "sm_bwd_gather", "readout", "loss").  Enabling clears previous records. */
+int ldl_timer_enable(int32_t on);
+int ldl_timer_read(const char* name, double* total_ms, int64_t* count);
+
 /* ------------------------------------------------------------------ plan */
 /* Replaces buildplan_f32/_f64 (lib/cpu.cc:140-167, lib/parfft.h:33-76).
  * Validates the mesh (each axis a power of two in [4, 1024]; the last one

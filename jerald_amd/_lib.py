@@ -52,6 +52,8 @@ _SIGS = {
     "ldl_paint_workspace_bytes": [_I64x3, _I32],
     "ldl_fft_workspace_bytes": [_I64x3, _I32],
     "ldl_sm_workspace_bytes": [_I64x3, _I32],
+    "ldl_timer_enable": [_I32],
+    "ldl_timer_read": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
 }
 for _t in ("f32", "f64"):
     _SIGS.update({
@@ -102,6 +104,18 @@ def call(name: str, *args):
     if rc != 0:
         msg = load().ldl_last_error().decode(errors="replace")
         raise LdlError(f"{name} failed ({rc}): {msg}")
+
+
+def timer_enable(on: bool):
+    call("ldl_timer_enable", int(bool(on)))
+
+
+def timer_read(name: str):
+    """(total device ms, launch count) of an instrumented kernel."""
+    t = C.c_double(0.0)
+    n = C.c_int64(0)
+    call("ldl_timer_read", name.encode(), C.byref(t), C.byref(n))
+    return t.value, n.value
 
 
 def sfx(dtype: torch.dtype) -> str:

@@ -42,9 +42,9 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         hipLaunchKernelGGL(k_fill<T>, dim3(gm), dim3(kBlock), 0, st, out, tot, T(0));
         if (np > 0) {
             if (nc == 1)
-                hipLaunchKernelGGL((k_paint_float<T, 1>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, (T)out_scale, m, out);
+                { KTimer _t("paint_deposit", st); hipLaunchKernelGGL((k_paint_float<T, 1>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, (T)out_scale, m, out); }
             else
-                hipLaunchKernelGGL((k_paint_float<T, 3>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, (T)out_scale, m, out);
+                { KTimer _t("paint_deposit", st); hipLaunchKernelGGL((k_paint_float<T, 3>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, (T)out_scale, m, out); }
         }
         LDL_LAUNCH_CHECK();
         return LDL_OK;
@@ -67,14 +67,14 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     }
     if (np > 0) {
         if (nc == 1)
-            hipLaunchKernelGGL((k_paint_fixed<T, 1>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, m, acc, hdr);
+            { KTimer _t("paint_deposit", st); hipLaunchKernelGGL((k_paint_fixed<T, 1>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, m, acc, hdr); }
         else
-            hipLaunchKernelGGL((k_paint_fixed<T, 3>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, m, acc, hdr);
+            { KTimer _t("paint_deposit", st); hipLaunchKernelGGL((k_paint_fixed<T, 3>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, m, acc, hdr); }
     }
     if (nc == 1)
-        hipLaunchKernelGGL((k_fixed_to_float<T, 1>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, hdr, np, cells, out_scale, out);
+        { KTimer _t("paint_convert", st); hipLaunchKernelGGL((k_fixed_to_float<T, 1>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, hdr, np, cells, out_scale, out); }
     else
-        hipLaunchKernelGGL((k_fixed_to_float<T, 3>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, hdr, np, cells, out_scale, out);
+        { KTimer _t("paint_convert", st); hipLaunchKernelGGL((k_fixed_to_float<T, 3>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, hdr, np, cells, out_scale, out); }
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }
@@ -94,13 +94,13 @@ int readout_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* f
     switch (kind) {
         case 0:
             if (vjpdim < -1 || vjpdim > 2) return fail(LDL_EINVAL, "vjpdim must be between -1 and 2");
-            hipLaunchKernelGGL(k_readout<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, m, vjpdim, out);
+            { KTimer _t("readout", st); hipLaunchKernelGGL(k_readout<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, m, vjpdim, out); }
             break;
-        case 1: hipLaunchKernelGGL(k_readout3<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, m, out); break;
-        case 2: hipLaunchKernelGGL(k_readout_grad<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, m, aux, out); break;
+        case 1: { KTimer _t("readout", st); hipLaunchKernelGGL(k_readout3<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, m, out); } break;
+        case 2: { KTimer _t("readout", st); hipLaunchKernelGGL(k_readout_grad<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, m, aux, out); } break;
         case 3:
             if (!aux) return fail(LDL_EINVAL, "v3 is NULL");
-            hipLaunchKernelGGL(k_readout_grad3<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, aux, m, out);
+            { KTimer _t("readout", st); hipLaunchKernelGGL(k_readout_grad3<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, aux, m, out); }
             break;
     }
     LDL_LAUNCH_CHECK();

@@ -258,6 +258,7 @@ __global__ void __launch_bounds__((FftCfg<T, H>::THREADS)) k_zpass_c2r(int64_t n
     for (int s = 0; s < (HP * P + TH - 1) / TH; s++) {
         const int i = threadIdx.x + s * TH;
         ia[s] = -1;
+        ib[s] = -1;
         if (i < HP * P) {
             const int p = i / HP, k = i - p * HP;
             const C xk = lds[k * LS + p];
@@ -268,7 +269,6 @@ __global__ void __launch_bounds__((FftCfg<T, H>::THREADS)) k_zpass_c2r(int64_t n
             C e = cadd(xk, cconj(xc)), o = cmul(csub(xk, cconj(xc)), w);
             za[s] = cadd(e, C{-o.y, o.x});
             ia[s] = k * LS + p;
-            ib[s] = -1;
             if (k != 0 && 2 * k != H) {
                 C w2 = twN2[H - k];
                 w2.y = -w2.y;
@@ -351,12 +351,14 @@ template <class F> inline bool dispatch_n(int64_t n, F&& f) {
 
 template <typename T, int N, int DIR, class Op>
 inline void launch_colpass(const ColGeom& g, const c2<T>* tw, const Op& op, hipStream_t st) {
+    KTimer _t("colpass", st);
     hipLaunchKernelGGL((k_colpass<T, N, DIR, Op>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, N>::THREADS), 0, st, g, tw, op);
 }
 
 template <typename T, int H, class Pro, class Epi>
 inline void launch_zr2c(int64_t nrows, const c2<T>* tw, const Pro& pro, const Epi& epi, hipStream_t st) {
     constexpr int P = FftCfg<T, H>::P;
+    KTimer _t("zpass_r2c", st);
     hipLaunchKernelGGL((k_zpass_r2c<T, H, Pro, Epi>), dim3((unsigned)((nrows + P - 1) / P)), dim3(FftCfg<T, H>::THREADS), 0, st,
                        nrows, tw, pro, epi);
 }
@@ -364,6 +366,7 @@ inline void launch_zr2c(int64_t nrows, const c2<T>* tw, const Pro& pro, const Ep
 template <typename T, int H, class Pro, class Epi>
 inline void launch_zc2r(int64_t nrows, const c2<T>* tw, const Pro& pro, const Epi& epi, hipStream_t st) {
     constexpr int P = FftCfg<T, H>::P;
+    KTimer _t("zpass_c2r", st);
     hipLaunchKernelGGL((k_zpass_c2r<T, H, Pro, Epi>), dim3((unsigned)((nrows + P - 1) / P)), dim3(FftCfg<T, H>::THREADS), 0, st,
                        nrows, tw, pro, epi);
 }

@@ -170,6 +170,21 @@ inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {
 // every power of two L in [1, 2048], built in FP64 on the host.
 template <typename T> int twiddles(const c2<T>** out);
 
+// ------------------------------------------------- kernel timing (bench)
+bool timing_enabled();
+void timer_begin(const char* name, hipStream_t st, void** token);
+void timer_end(const char* name, hipStream_t st, void* token);
+
+// Scope guard that brackets the launches inside it with HIP events on ``st``
+// when ldl_timer_enable(1) is active.
+struct KTimer {
+    const char* name;
+    hipStream_t st;
+    void* tok;
+    KTimer(const char* n, hipStream_t s) : name(n), st(s), tok(nullptr) { timer_begin(n, s, &tok); }
+    ~KTimer() { timer_end(name, st, tok); }
+};
+
 // --------------------------------------------------------- reductions
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -611,8 +611,8 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     dispatch_n(m.n0, [&](auto N) {
         constexpr int NN = decltype(N)::value;
         ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
-        hipLaunchKernelGGL((k_sm_x_fwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, hc, kt, fl,
-                           (T)(1e-2 * prm[0]), (T)(1.0 / (double)m.cells()));
+        { KTimer _t("sm_x_fwd", st); hipLaunchKernelGGL((k_sm_x_fwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, hc, kt, fl,
+                           (T)(1e-2 * prm[0]), (T)(1.0 / (double)m.cells())); }
     });
     // 4. inverse y passes (3 spectra as 3*N0 planes) and z c2r into F
     dispatch_n(m.n1, [&](auto N) {
@@ -626,7 +626,7 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     T s[3];
     for (int d = 0; d < 3; d++) s[d] = pdesc->kind == LDL_POS_LATTICE ? pd.cellinv[d] : T(1);
     if (np > 0)
-        hipLaunchKernelGGL(k_readout3_update<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, F, m, s[0], s[1], s[2], pos_out);
+        { KTimer _t("readout3_update", st); hipLaunchKernelGGL(k_readout3_update<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, F, m, s[0], s[1], s[2], pos_out); }
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }
@@ -668,8 +668,8 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
         constexpr int NN = decltype(N)::value;
         ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
         ntx = g.ntiles;
-        hipLaunchKernelGGL((k_sm_x_bwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U, S, hc, m.n2, kt,
-                           fl, (T)(1e-2 * prm[0]), prm[2], prm[3], prm[0], (int)alpha_rule, P.part);
+        { KTimer _t("sm_x_bwd", st); hipLaunchKernelGGL((k_sm_x_bwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U, S, hc, m.n2, kt,
+                           fl, (T)(1e-2 * prm[0]), prm[2], prm[3], prm[0], (int)alpha_rule, P.part); }
     });
     RedMap rx{4, {0, 2, 3, 4}};
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, ntx, 4, rx, grad5, 0);
@@ -690,7 +690,7 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, nzb, 1, rz, grad5, 0);
     // 5. position adjoint
     if (np > 0)
-        hipLaunchKernelGGL(k_sm_bwd_gather<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, F, grho, m, (T)fact, gout, gin);
+        { KTimer _t("sm_bwd_gather", st); hipLaunchKernelGGL(k_sm_bwd_gather<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, F, grho, m, (T)fact, gout, gin); }
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }
@@ -726,7 +726,7 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
     const RespPar<T> rp{(T)prm3[0], (T)prm3[1], (T)prm3[2], (T)index, p_norm == 1, field2};
     RedMap r5{5, {0, 1, 2, 3, 4}};
     if (smooth_kind == 0) {
-        hipLaunchKernelGGL(k_loss_direct<T>, dim3(gb), dim3(256), 0, st, rho, target, mask, maskv, rp, cells, npix, grho, bpart);
+        { KTimer _t("loss", st); hipLaunchKernelGGL(k_loss_direct<T>, dim3(gb), dim3(256), 0, st, rho, target, mask, maskv, rp, cells, npix, grho, bpart); }
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, (int64_t)gb, 5, r5, sums, 1);
     } else {
         const c2<T>* tw;
@@ -748,8 +748,8 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
             dispatch_n(m.n0, [&](auto N) {
                 constexpr int NN = decltype(N)::value;
                 ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
-                hipLaunchKernelGGL((k_x_realfilter<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, S, kt,
-                                   (int)smooth_kind, (T)smooth_par, invM);
+                { KTimer _t("x_realfilter", st); hipLaunchKernelGGL((k_x_realfilter<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, S, kt,
+                                   (int)smooth_kind, (T)smooth_par, invM); }
             });
             dispatch_n(m.n1, [&](auto N) {
                 ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
@@ -789,7 +789,7 @@ static int paint_vjp_pos(const T* pos, const ldl_pos_desc* pdesc, int64_t np, co
     if (!pos || !grho || !g) return fail(LDL_EINVAL, "NULL array argument");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
-    hipLaunchKernelGGL(k_paint_vjp_pos<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, grho, m, (T)fact, g);
+    { KTimer _t("paint_vjp_pos", st); hipLaunchKernelGGL(k_paint_vjp_pos<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, grho, m, (T)fact, g); }
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }

@@ -1,6 +1,8 @@
 // libldl runtime: error channel, version, twiddle-table cache, buildplan.
+#include <map>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ldl_common.h"
@@ -57,6 +59,45 @@ int twiddles(const c2<T>** out) {
 template int twiddles<float>(const c2<float>**);
 template int twiddles<double>(const c2<double>**);
 
+// ------------------------------------------------- kernel timing (bench)
+// When enabled, instrumented launches are bracketed by HIP events recorded on
+// the launch stream; totals are read back (and the events released) later.
+static std::mutex g_timer_mutex;
+static bool g_timing = false;
+static std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> g_events;
+
+bool timing_enabled() { return g_timing; }
+
+void timer_begin(const char* name, hipStream_t st, void** token) {
+    *token = nullptr;
+    if (!g_timing) return;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return;
+    if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+    (void)hipEventRecord(a, st);
+    std::lock_guard<std::mutex> lock(g_timer_mutex);
+    auto& v = g_events[name];
+    v.emplace_back(a, b);
+    *token = reinterpret_cast<void*>(v.size());
+}
+
+void timer_end(const char* name, hipStream_t st, void* token) {
+    if (!token) return;
+    std::lock_guard<std::mutex> lock(g_timer_mutex);
+    auto& v = g_events[name];
+    const size_t i = reinterpret_cast<size_t>(token) - 1;
+    if (i < v.size()) (void)hipEventRecord(v[i].second, st);
+}
+
+static void timer_clear_locked() {
+    for (auto& kv : g_events)
+        for (auto& e : kv.second) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    g_events.clear();
+}
+
 }  // namespace ldl
 
 extern "C" {
@@ -64,6 +105,30 @@ extern "C" {
 const char* ldl_last_error(void) { return ldl::g_last_error.c_str(); }
 
 int ldl_version(void) { return 1; }
+
+int ldl_timer_enable(int32_t on) {
+    std::lock_guard<std::mutex> lock(ldl::g_timer_mutex);
+    ldl::timer_clear_locked();
+    ldl::g_timing = on != 0;
+    return LDL_OK;
+}
+
+int ldl_timer_read(const char* name, double* total_ms, int64_t* count) {
+    if (!name || !total_ms || !count) return ldl::fail(LDL_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lock(ldl::g_timer_mutex);
+    *total_ms = 0.0;
+    *count = 0;
+    auto it = ldl::g_events.find(name);
+    if (it == ldl::g_events.end()) return LDL_OK;
+    for (auto& e : it->second) {
+        LDL_HIP_TRY(hipEventSynchronize(e.second));
+        float ms = 0.f;
+        LDL_HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        *total_ms += ms;
+        *count += 1;
+    }
+    return LDL_OK;
+}
 
 int ldl_buildplan(const int64_t nmesh[3], int32_t out[2]) {
     ldl::Mesh m;

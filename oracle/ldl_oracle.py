@@ -259,7 +259,7 @@ class SMModelOracle:
     ``smoothing`` = None | "reference" | ("gaussian", sigma)."""
 
     def __init__(self, nmesh, box, nstep, target, L1=True, n=0.0, smoothing=None,
-                 index=1.0, field2=None, mask=None, grad_rule="exact"):
+                 index=1.0, field2=None, mask=None, grad_rule="exact", backend="numpy", threads=0):
         self.nmesh = tuple(int(v) for v in np.broadcast_to(np.asarray(nmesh), (3,)))
         self.box = np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)).copy()
         self.nstep = int(nstep)
@@ -285,21 +285,58 @@ class SMModelOracle:
         if grad_rule not in ("exact", "reference"):
             raise ValueError(grad_rule)
         self.grad_rule = grad_rule
+        # backend "numpy": the restatement above; "native": the same algorithm
+        # with the C CIC loops (oracle/cic_oracle.c, OpenMP) and pocketfft with
+        # ``threads`` workers -- used as the timed CPU baseline.
+        if backend not in ("numpy", "native"):
+            raise ValueError(backend)
+        self.backend = backend
+        self.threads = int(threads)
 
     # -- pieces -------------------------------------------------------------
     def _x(self, X):
         return mesh_units(X, self.box, self.nmesh)
 
+    def _paint(self, x, m):
+        if self.backend == "native":
+            from . import native
+            return native.paint_f64(x, m, self.nmesh, self.threads)
+        return paint(x, m, self.nmesh)
+
+    def _readout(self, x, F):
+        if self.backend == "native":
+            from . import native
+            return native.readout_f64(x, F, -1, (1.0, 1.0, 1.0), self.threads)
+        return readout(x, F)
+
+    def _readout_grad(self, x, F):
+        if self.backend == "native":
+            from . import native
+            return native.readout_grad_f64(x, F, self.cellinv, self.threads)
+        return readout_grad(x, F, self.cellinv)
+
+    def _rfftn(self, x):
+        if self.backend == "native":
+            import scipy.fft
+            return scipy.fft.rfftn(x, workers=self.threads or -1)
+        return np.fft.rfftn(x)
+
+    def _irfftn(self, X):
+        if self.backend == "native":
+            import scipy.fft
+            return scipy.fft.irfftn(X, s=self.nmesh, workers=self.threads or -1)
+        return np.fft.irfftn(X, s=self.nmesh, axes=(0, 1, 2))
+
     def layer_forward(self, X, p, fact):
         alpha, gamma, kh, kl, nu = p
         x = self._x(X)
-        rho = paint(x, 1.0, self.nmesh) * fact
+        rho = self._paint(x, 1.0) * fact
         u = (rho + EPS_RHO) ** gamma
-        U = r2c(u)
+        U = self._rfftn(u) / self.M
         G = green_sm(self.kn, kh, kl, nu)
         P = ALPHA_SCALE * alpha * U * G
-        F = [c2r(-1j * self.D[d] * P, self.nmesh) for d in range(3)]
-        s = np.stack([readout(x, F[d]) for d in range(3)], axis=-1)
+        F = [self._irfftn(-1j * self.D[d] * P) * self.M for d in range(3)]
+        s = np.stack([self._readout(x, F[d]) for d in range(3)], axis=-1)
         cache = dict(x=x, rho=rho, u=u, U=U, G=G, F=F)
         return X + s, cache
 
@@ -318,7 +355,7 @@ class SMModelOracle:
         fact = self.M / X.shape[0]
         Xf, caches = self.displace(params, X, keep=True)
         xf = self._x(Xf)
-        rho = paint(xf, 1.0, self.nmesh) * fact
+        rho = self._paint(xf, 1.0) * fact
         mu, w, b = params[5 * self.nstep: 5 * self.nstep + 3]
         base = (rho + EPS_RHO) ** mu
         y = w * base + b
@@ -333,7 +370,7 @@ class SMModelOracle:
             Fi = Fi * self.field2
         diff = Fi - self.target
         if self.W is not None:
-            diff = c2r(r2c(diff) * self.W, self.nmesh)
+            diff = self._irfftn(self._rfftn(diff) * self.W)
         return diff
 
     def _reduce(self, diff):
@@ -360,7 +397,7 @@ class SMModelOracle:
         g = (np.sign(diff) if self.L1 else 2.0 * diff) * m / npix
         if self.W is not None:
             # diff_s = C(W R(diff) / M) -> g_diff = C(W R(g)) / M
-            g = np.fft.irfftn(np.fft.rfftn(g) * self.W, s=self.nmesh, axes=(0, 1, 2))
+            g = self._irfftn(self._rfftn(g) * self.W)
         if self.field2 is not None:
             g = g * self.field2
         if self.index != 1.0:
@@ -373,7 +410,7 @@ class SMModelOracle:
         grad[5 * self.nstep + 2] = np.sum(g_y)
         g_rho = g_y * w * mu * rhoe ** (mu - 1.0)
         fact = st["fact"]
-        gX = fact * readout_grad(st["xf"], g_rho, self.cellinv)
+        gX = fact * self._readout_grad(st["xf"], g_rho)
         for i in reversed(range(self.nstep)):
             gX = self._layer_backward(params[5 * i: 5 * i + 5], st["caches"][i], gX, fact, grad, 5 * i)
         return (loss, grad, gX) if want_pos_grad else (loss, grad)
@@ -382,10 +419,10 @@ class SMModelOracle:
         alpha, gamma, kh, kl, nu = p
         x = cache["x"]
         gX = v.copy()
-        gF = [paint(x, v[:, d], self.nmesh) for d in range(3)]
+        gF = [self._paint(x, np.ascontiguousarray(v[:, d])) for d in range(3)]
         for d in range(3):
-            gX += v[:, d:d + 1] * readout_grad(x, cache["F"][d], self.cellinv)
-        Q = sum(1j * self.D[d] * pfft(gF[d]) for d in range(3))
+            gX += v[:, d:d + 1] * self._readout_grad(x, cache["F"][d])
+        Q = sum(1j * self.D[d] * self._rfftn(gF[d]) for d in range(3))
         U, G = cache["U"], cache["G"]
         cw = self.c if self.grad_rule == "exact" else 1.0
         kn = self.kn
@@ -395,11 +432,11 @@ class SMModelOracle:
         grad[off + 2] = np.sum(gG * G * (-2.0 * kh / kn ** 2))
         grad[off + 3] = np.sum(gG * G * (2.0 * kn ** 2 / kl ** 3))
         grad[off + 4] = np.sum(gG * G * np.log(kn))
-        g_u = np.fft.irfftn(ALPHA_SCALE * alpha * G * Q, s=self.nmesh, axes=(0, 1, 2))
+        g_u = self._irfftn(ALPHA_SCALE * alpha * G * Q)
         rhoe = cache["rho"] + EPS_RHO
         grad[off + 1] = np.sum(g_u * cache["u"] * np.log(rhoe))
         g_rho = g_u * gamma * rhoe ** (gamma - 1.0)
-        gX += fact * readout_grad(x, g_rho, self.cellinv)
+        gX += fact * self._readout_grad(x, g_rho)
         return gX
 
 

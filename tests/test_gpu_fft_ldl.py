@@ -147,12 +147,17 @@ def test_ldl_autograd_function_and_recompute_mode():
 
 
 def test_lossv_and_reference_smoothing():
+    """Reference smoothing k^-n + 1 with knorm(0) = 1e-8 (W(0) = 1e8 + 1,
+    model.py:102, :376-381) and train/validation masks (model.py:430-466).
+    A random train mask keeps the residual gradient non-uniform: with an
+    all-ones mask the DC term makes g_rho exactly uniform and every layer
+    gradient exactly zero."""
     N, L, nstep = 16, 64.0, 1
     X = O.lattice_positions(N, L) + O.zeldovich_displacement(N, L, 1.5, 5)
     rng = np.random.default_rng(6)
     tgt = rng.random((N, N, N)) + 0.5
-    mt = np.ones((N, N, N))
-    mv = (rng.random((N, N, N)) > 0.5).astype(float)
+    mt = (rng.random((N, N, N)) > 0.4).astype(float)
+    mv = 1.0 - mt
     p = np.array([0.5, 0.9, 0.2, 1.3, 0.1, 1.0, 1.0, 0.0])
     orc = O.SMModelOracle(N, L, nstep, tgt, L1=True, n=1.0, mask=mt)
     orv = O.SMModelOracle(N, L, nstep, tgt, L1=True, n=1.0, mask=mv)
@@ -160,6 +165,9 @@ def test_lossv_and_reference_smoothing():
     m = JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, kind="sm", n=1.0, masktrain=mt, maskvalid=mv)
     loss, lossv, grad = m.lossv_and_grad(torch.tensor(p))
     l0, g0 = orc.loss_and_grad(p, X)
+    lv0 = orv.loss(p, X)
     assert abs(loss.item() - l0) <= 1e-9 * abs(l0)
-    assert abs(lossv.item() - orv.loss(p, X)) <= 1e-9 * abs(l0)
-    np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-7)
+    assert abs(lossv.item() - lv0) <= 1e-9 * abs(lv0)
+    # the layer gradients come from the non-DC part of a residual whose DC
+    # weight is 1e8, so they carry ~1e-8 relative rounding from either side
+    np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-5)
