@@ -55,8 +55,10 @@ typedef struct ldl_pos_desc {
 /* paint flags */
 #define LDL_PAINT_FIXED 0        /* 64-bit fixed-point integer atomics: deterministic, order free */
 #define LDL_PAINT_FLOAT_ATOMIC 1 /* floating-point atomics in the working precision */
-#define LDL_PAINT_WS_ZEROED 2    /* with LDL_PAINT_FIXED: the workspace accumulator is known to be
-                                    zero (every fixed-point paint leaves it zero), skip clearing it */
+#define LDL_PAINT_WS_ZEROED 2    /* with LDL_PAINT_DIRECT: the workspace accumulator is known to be
+                                    zero (every direct paint leaves it zero), skip clearing it */
+#define LDL_PAINT_DIRECT 4       /* with LDL_PAINT_FIXED: one device-scope 64-bit atomic per corner
+                                    instead of the default tile-binned LDS accumulation (same bits) */
 
 const char* ldl_last_error(void);
 int ldl_version(void);
@@ -64,7 +66,8 @@ int ldl_version(void);
 /* Benchmark instrumentation (no counterpart in the reference).  While enabled,
  * the main kernels are bracketed by HIP events on their launch stream;
  * ldl_timer_read() waits for them and returns the summed device time and the
- * launch count of one kernel name ("paint_deposit", "paint_convert", "zpass_r2c",
+ * launch count of one kernel name ("paint_bin", "paint_scatter", "paint_deposit",
+ * "paint_convert", "zpass_r2c",
  * "zpass_c2r", "colpass", "sm_x_fwd", "sm_x_bwd", "readout3_update",
  * "sm_bwd_gather", "readout", "loss").  Enabling clears previous records. */
 int ldl_timer_enable(int32_t on);
@@ -78,8 +81,8 @@ int ldl_timer_read(const char* name, double* total_ms, int64_t* count);
 int ldl_buildplan(const int64_t nmesh[3], int32_t out[2]);
 
 /* ----------------------------------------------------------------- paint */
-/* Workspace for a paint of ``ncomp`` (1 or 3) components. */
-size_t ldl_paint_workspace_bytes(const int64_t nmesh[3], int32_t ncomp);
+/* Workspace for a paint of ``ncomp`` (1 or 3) components of ``np`` particles. */
+size_t ldl_paint_workspace_bytes(const int64_t nmesh[3], int32_t ncomp, int64_t np);
 
 /* Replaces ppaint_f32/_f64 (lib/cpu.cc:51-72 -> lib/parops_fast.h:483-531).
  * out (N0,N1,N2) = out_scale * CIC deposit of ``mass`` (Np,) or, if mass is
@@ -155,7 +158,7 @@ int ldl_pifft_f64(const double* in, double* out, const int64_t nmesh[3], double 
  * P = 1e-2 alpha G(k) U with G = -exp(-k^2/kl^2 - kh^2/k^2) k^nu;
  * F_d = c2r(-i D_d(k) P) with D_d the 6th-order FD symbol; s_d = readout(X, F_d).
  * Positions are LDL_POS_LATTICE displacements (mesh units) or physical. */
-size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64);
+size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np);
 
 /* Forward of one layer.  Inputs pos (Np,3).  Writes pos_out (Np,3) = pos +
  * s / cell (same kind as pos; may alias pos), and the saved tensors the

@@ -19,6 +19,7 @@ LDL_POS_LATTICE = 1
 LDL_PAINT_FIXED = 0
 LDL_PAINT_FLOAT_ATOMIC = 1
 LDL_PAINT_WS_ZEROED = 2
+LDL_PAINT_DIRECT = 4
 
 
 class LdlError(RuntimeError):
@@ -49,9 +50,9 @@ _SIGS = {
     "ldl_last_error": [],
     "ldl_version": [],
     "ldl_buildplan": [_I64x3, C.POINTER(C.c_int32)],
-    "ldl_paint_workspace_bytes": [_I64x3, _I32],
+    "ldl_paint_workspace_bytes": [_I64x3, _I32, _I64],
     "ldl_fft_workspace_bytes": [_I64x3, _I32],
-    "ldl_sm_workspace_bytes": [_I64x3, _I32],
+    "ldl_sm_workspace_bytes": [_I64x3, _I32, _I64],
     "ldl_timer_enable": [_I32],
     "ldl_timer_read": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
 }

@@ -247,7 +247,218 @@ __global__ void __launch_bounds__(256) k_readout_grad3(const T* __restrict__ pos
     }
 }
 
-size_t paint_ws_bytes(const Mesh& m, int nc);
+// ------------------------------------------------------- tiled paint
+// Eulerian tiles of TS^3 cells.  A particle belongs to the tile of its lower
+// CIC corner, so its 8 corners fall in the tile's (TS+1)^3 box.  Each box is
+// accumulated in LDS with integer atomics; the core TS^3 cells are written
+// with plain stores (no other tile writes them) and the three +1 faces go to
+// a per-tile halo buffer that the convert pass adds to the lower faces of the
+// neighbouring tiles.  No global atomics touch the mesh.
+struct TileGeom {
+    int ts;              // tile side in cells
+    int64_t nt[3];       // tiles per axis (last tile may be partial)
+    int64_t ntiles;
+    int64_t hstride;     // halo entries per tile, 3 ts^2 + 3 ts + 1
+    __host__ __device__ int64_t extent(int d, int64_t t, const Mesh& m) const {
+        const int64_t n = d == 0 ? m.n0 : d == 1 ? m.n1 : m.n2;
+        const int64_t lo = t * ts;
+        return n - lo < ts ? n - lo : ts;
+    }
+};
+
+inline TileGeom make_tiles(const Mesh& m, int ts) {
+    TileGeom g;
+    g.ts = ts;
+    g.nt[0] = (m.n0 + ts - 1) / ts;
+    g.nt[1] = (m.n1 + ts - 1) / ts;
+    g.nt[2] = (m.n2 + ts - 1) / ts;
+    g.ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+    g.hstride = 3ll * ts * ts + 3ll * ts + 1;
+    return g;
+}
+
+// Position of box cell (a, b, c) -- on one of the +1 faces -- in a tile's
+// halo record; ex, ey, ez are the tile's extents.
+__device__ __forceinline__ int64_t halo_index(int64_t a, int64_t b, int64_t c, int64_t ex, int64_t ey, int64_t ez) {
+    if (a == ex) return b * (ez + 1) + c;
+    if (b == ey) return (ey + 1) * (ez + 1) + a * (ez + 1) + c;
+    return (ey + 1) * (ez + 1) + ex * (ez + 1) + a * ey + b;
+}
+
+// Counting-sort pass 1: tile key of each particle and its rank among the
+// particles of the same tile (wave-aggregated atomics on the tile counters).
+template <typename T>
+__global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Mesh m, TileGeom tg,
+                                                  unsigned* __restrict__ counts, unsigned* __restrict__ key,
+                                                  unsigned* __restrict__ slot) {
+    const int lane = __lane_id();
+    for (int64_t p0 = blockIdx.x * (int64_t)blockDim.x; p0 < np; p0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = p0 + threadIdx.x;
+        unsigned k = 0xFFFFFFFFu;
+        if (p < np) {
+            int64_t idx[3];
+            T f[3];
+            pd.decode(pos, p, idx, f);
+            k = (unsigned)(((idx[0] / tg.ts) * tg.nt[1] + idx[1] / tg.ts) * tg.nt[2] + idx[2] / tg.ts);
+        }
+        unsigned long long pending = __ballot(p < np);
+        unsigned myslot = 0;
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const unsigned lk = __shfl(k, leader, 64);
+            const unsigned long long same = __ballot(k == lk) & pending;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(&counts[lk], (unsigned)__popcll(same));
+            base = __shfl(base, leader, 64);
+            if ((same >> lane) & 1ull) myslot = base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+            pending &= ~same;
+        }
+        if (p < np) {
+            key[p] = k;
+            slot[p] = myslot;
+        }
+    }
+}
+
+// Exclusive scan of the tile counts into offsets[0..ntiles]; counts reset.
+static __global__ void __launch_bounds__(1024) k_scan_counts(unsigned* __restrict__ counts, unsigned* __restrict__ offsets, int64_t n) {
+    __shared__ unsigned long long part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t b = t * per, e = b + per < n ? b + per : n;
+    unsigned long long s = 0;
+    for (int64_t i = b; i < e; i++) s += counts[i];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+        unsigned long long v = t >= o ? part[t - o] : 0ull;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    unsigned long long run = t ? part[t - 1] : 0ull;
+    for (int64_t i = b; i < e; i++) {
+        offsets[i] = (unsigned)run;
+        run += counts[i];
+        counts[i] = 0u;
+    }
+    if (t == 1023) offsets[n] = (unsigned)part[1023];
+}
+
+// Counting-sort pass 2: particle indices grouped by tile.
+static __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned* __restrict__ key, const unsigned* __restrict__ slot,
+                                                     const unsigned* __restrict__ offsets, unsigned* __restrict__ sorted) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x)
+        sorted[offsets[key[p]] + slot[p]] = (unsigned)p;
+}
+
+// One workgroup per tile: LDS fixed-point deposit, core + halo write-out.
+template <typename T, int NC, int TS>
+__global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ mass,
+                                                     T mscalar, Mesh m, TileGeom tg, const unsigned* __restrict__ offsets,
+                                                     const unsigned* __restrict__ sorted, unsigned long long* __restrict__ acc,
+                                                     unsigned long long* __restrict__ halo, const unsigned long long* __restrict__ hdr) {
+    constexpr int B = TS + 1, B3 = B * B * B;
+    __shared__ unsigned long long box[NC * B3];
+    const int64_t t = blockIdx.x;
+    const int64_t tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
+    const int64_t lo[3] = {tx * TS, ty * TS, tz * TS};
+    const int64_t ex = tg.extent(0, tx, m), ey = tg.extent(1, ty, m), ez = tg.extent(2, tz, m);
+    for (int i = threadIdx.x; i < NC * B3; i += 256) box[i] = 0ull;
+    int s[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) s[c] = fixed_shift(hdr[c], np);
+    __syncthreads();
+    const unsigned j0 = offsets[t], j1 = offsets[t + 1];
+    for (unsigned j = j0 + threadIdx.x; j < j1; j += 256) {
+        const int64_t p = sorted[j];
+        int64_t idx[3];
+        T f[3];
+        pd.decode(pos, p, idx, f);
+        const int a = (int)(idx[0] - lo[0]), b = (int)(idx[1] - lo[1]), cz = (int)(idx[2] - lo[2]);
+        const double wx[2] = {(double)(T(1) - f[0]), (double)f[0]};
+        const double wy[2] = {(double)(T(1) - f[1]), (double)f[1]};
+        const double wz[2] = {(double)(T(1) - f[2]), (double)f[2]};
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const double mm = mass ? (double)mass[p * NC + c] : (double)mscalar;
+            const long long tot = __double2ll_rn(ldexp(mm, s[c]));
+            long long part = 0;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int ax = q >> 2, by = (q >> 1) & 1, zc = q & 1;
+                long long v;
+                if (q == 7) {
+                    v = tot - part;
+                } else {
+                    double w = wx[ax] * wy[by];
+                    w = w * wz[zc];
+                    w = w * mm;
+                    v = __double2ll_rn(ldexp(w, s[c]));
+                    part += v;
+                }
+                if (v != 0) atomicAdd(&box[c * B3 + ((a + ax) * B + (b + by)) * B + (cz + zc)], (unsigned long long)v);
+            }
+        }
+    }
+    __syncthreads();
+    const int64_t cells = m.cells();
+    const int64_t nbox = (ex + 1) * (ey + 1) * (ez + 1);
+    for (int64_t i = threadIdx.x; i < nbox; i += 256) {
+        const int64_t cc = i % (ez + 1), bb = (i / (ez + 1)) % (ey + 1), aa = i / ((ez + 1) * (ey + 1));
+        const int li = (int)((aa * B + bb) * B + cc);
+        if (aa < ex && bb < ey && cc < ez) {
+            const int64_t g = ((lo[0] + aa) * m.n1 + (lo[1] + bb)) * m.n2 + (lo[2] + cc);
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[c * cells + g] = box[c * B3 + li];
+        } else {
+            const int64_t h = t * tg.hstride + halo_index(aa, bb, cc, ex, ey, ez);
+#pragma unroll
+            for (int c = 0; c < NC; c++) halo[c * tg.ntiles * tg.hstride + h] = box[c * B3 + li];
+        }
+    }
+}
+
+// out = (core + halos of the lower neighbour tiles) * 2^-s * scale.
+template <typename T, int NC>
+__global__ void __launch_bounds__(256) k_tiles_to_float(const unsigned long long* __restrict__ acc,
+                                                        const unsigned long long* __restrict__ halo,
+                                                        const unsigned long long* __restrict__ hdr, int64_t np, Mesh m,
+                                                        TileGeom tg, double scale, T* __restrict__ out) {
+    double f[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) f[c] = ldexp(scale, -fixed_shift(hdr[c], np));
+    const int64_t cells = m.cells();
+    const int ts = tg.ts;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cells; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t z = i % m.n2, y = (i / m.n2) % m.n1, x = i / (m.n2 * m.n1);
+        const int64_t tx = x / ts, ty = y / ts, tz = z / ts;
+        const int64_t a = x - tx * ts, b = y - ty * ts, c = z - tz * ts;
+        long long v[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) v[k] = (long long)acc[k * cells + i];
+        if (a == 0 || b == 0 || c == 0) {
+#pragma unroll
+            for (int q = 1; q < 8; q++) {
+                const int sx = q >> 2, sy = (q >> 1) & 1, sz = q & 1;
+                if ((sx && a) || (sy && b) || (sz && c)) continue;
+                const int64_t nx = sx ? (tx == 0 ? tg.nt[0] - 1 : tx - 1) : tx;
+                const int64_t ny = sy ? (ty == 0 ? tg.nt[1] - 1 : ty - 1) : ty;
+                const int64_t nz = sz ? (tz == 0 ? tg.nt[2] - 1 : tz - 1) : tz;
+                const int64_t ex = tg.extent(0, nx, m), ey = tg.extent(1, ny, m), ez = tg.extent(2, nz, m);
+                const int64_t nt = (nx * tg.nt[1] + ny) * tg.nt[2] + nz;
+                const int64_t h = nt * tg.hstride + halo_index(sx ? ex : a, sy ? ey : b, sz ? ez : c, ex, ey, ez);
+#pragma unroll
+                for (int k = 0; k < NC; k++) v[k] += (long long)halo[k * tg.ntiles * tg.hstride + h];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++) out[k * cells + i] = (T)((double)v[k] * f[k]);
+    }
+}
+
+size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np);
+
 template <typename T>
 int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,
                const int64_t nmesh[3], double out_scale, int32_t flags, T* out, void* ws, size_t ws_bytes,

@@ -18,7 +18,30 @@ namespace ldl {
 static constexpr int kBlock = 256;
 static constexpr int64_t kHdrBytes = 256;
 
-size_t paint_ws_bytes(const Mesh& m, int nc) { return (size_t)kHdrBytes + (size_t)nc * (size_t)m.cells() * 8; }
+static constexpr int kTileSide1 = 16;  // one-component paint: 17^3 x 8 B = 39 KB of LDS
+static constexpr int kTileSide3 = 8;   // three components: 3 x 9^3 x 8 B = 17.5 KB of LDS
+
+static inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Workspace: header | fixed-point mesh (nc) | tile halos (nc) | counts | offsets | key | slot | sorted
+struct PaintWs {
+    size_t acc, halo, counts, offsets, key, slot, sorted, total;
+};
+static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
+    const TileGeom tg = make_tiles(m, nc == 1 ? kTileSide1 : kTileSide3);
+    PaintWs w;
+    w.acc = al256((size_t)kHdrBytes);
+    w.halo = w.acc + al256((size_t)nc * m.cells() * 8);
+    w.counts = w.halo + al256((size_t)nc * tg.ntiles * tg.hstride * 8);
+    w.offsets = w.counts + al256((size_t)(tg.ntiles + 1) * 4);
+    w.key = w.offsets + al256((size_t)(tg.ntiles + 1) * 4);
+    w.slot = w.key + al256((size_t)np * 4);
+    w.sorted = w.slot + al256((size_t)np * 4);
+    w.total = w.sorted + al256((size_t)np * 4);
+    return w;
+}
+
+size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np) { return paint_ws(m, nc, np < 0 ? 0 : np).total; }
 
 template <typename T>
 int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,
@@ -49,9 +72,12 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         LDL_LAUNCH_CHECK();
         return LDL_OK;
     }
-    if (!ws || ws_bytes < paint_ws_bytes(m, nc)) return fail(LDL_EINVAL, "paint workspace too small (see ldl_paint_workspace_bytes)");
-    auto* hdr = reinterpret_cast<unsigned long long*>(ws);
-    auto* acc = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(ws) + kHdrBytes);
+    if (np > (int64_t)0xFFFFFFFFll) return fail(LDL_EUNSUPPORTED, "paint supports np < 2^32");
+    const PaintWs w = paint_ws(m, nc, np);
+    if (!ws || ws_bytes < w.total) return fail(LDL_EINVAL, "paint workspace too small (see ldl_paint_workspace_bytes)");
+    char* wb = reinterpret_cast<char*>(ws);
+    auto* hdr = reinterpret_cast<unsigned long long*>(wb);
+    auto* acc = reinterpret_cast<unsigned long long*>(wb + w.acc);
     // header: max |m| bits; for a scalar mass it is known on the host
     unsigned long long hbits = 0;
     if (!mass) {
@@ -59,22 +85,59 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         std::memcpy(&hbits, &a, 8);
     }
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, hdr, (int64_t)(kHdrBytes / 8), hbits);
-    if (!(flags & 2))  // LDL_PAINT_WS_ZEROED (internal): accumulator already zero
-        hipLaunchKernelGGL(k_fill_u64, dim3(gm), dim3(kBlock), 0, st, acc, tot, 0ull);
     if (mass && np > 0) {
         if (nc == 1) hipLaunchKernelGGL((k_absmax<T, 1>), dim3(grid_for(np, kBlock, 4096)), dim3(kBlock), 0, st, mass, np, hdr);
         else hipLaunchKernelGGL((k_absmax<T, 3>), dim3(grid_for(np, kBlock, 4096)), dim3(kBlock), 0, st, mass, np, hdr);
     }
-    if (np > 0) {
+    if (flags & LDL_PAINT_DIRECT) {
+        // reference variant: one device-scope 64-bit atomic per corner
+        if (!(flags & LDL_PAINT_WS_ZEROED))
+            hipLaunchKernelGGL(k_fill_u64, dim3(gm), dim3(kBlock), 0, st, acc, tot, 0ull);
+        if (np > 0) {
+            if (nc == 1)
+                { KTimer _t("paint_deposit", st); hipLaunchKernelGGL((k_paint_fixed<T, 1>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, m, acc, hdr); }
+            else
+                { KTimer _t("paint_deposit", st); hipLaunchKernelGGL((k_paint_fixed<T, 3>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, m, acc, hdr); }
+        }
         if (nc == 1)
-            { KTimer _t("paint_deposit", st); hipLaunchKernelGGL((k_paint_fixed<T, 1>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, m, acc, hdr); }
+            { KTimer _t("paint_convert", st); hipLaunchKernelGGL((k_fixed_to_float<T, 1>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, hdr, np, cells, out_scale, out); }
         else
-            { KTimer _t("paint_deposit", st); hipLaunchKernelGGL((k_paint_fixed<T, 3>), dim3(gp), dim3(kBlock), 0, st, pos, pd, np, mass, (T)mscalar, m, acc, hdr); }
+            { KTimer _t("paint_convert", st); hipLaunchKernelGGL((k_fixed_to_float<T, 3>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, hdr, np, cells, out_scale, out); }
+        LDL_LAUNCH_CHECK();
+        return LDL_OK;
     }
-    if (nc == 1)
-        { KTimer _t("paint_convert", st); hipLaunchKernelGGL((k_fixed_to_float<T, 1>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, hdr, np, cells, out_scale, out); }
-    else
-        { KTimer _t("paint_convert", st); hipLaunchKernelGGL((k_fixed_to_float<T, 3>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, hdr, np, cells, out_scale, out); }
+    // tiled path: counting sort by tile, LDS deposit, halo fold
+    const TileGeom tg = make_tiles(m, nc == 1 ? kTileSide1 : kTileSide3);
+    auto* halo = reinterpret_cast<unsigned long long*>(wb + w.halo);
+    auto* counts = reinterpret_cast<unsigned*>(wb + w.counts);
+    auto* offsets = reinterpret_cast<unsigned*>(wb + w.offsets);
+    auto* key = reinterpret_cast<unsigned*>(wb + w.key);
+    auto* slot = reinterpret_cast<unsigned*>(wb + w.slot);
+    auto* sorted = reinterpret_cast<unsigned*>(wb + w.sorted);
+    hipLaunchKernelGGL(k_fill<unsigned>, dim3(grid_for(tg.ntiles + 1, kBlock, 1 << 16)), dim3(kBlock), 0, st, counts, tg.ntiles + 1, 0u);
+    if (np > 0) {
+        { KTimer _t("paint_bin", st); hipLaunchKernelGGL(k_bin_keys<T>, dim3(gp), dim3(kBlock), 0, st, pos, pd, np, m, tg, counts, key, slot); }
+    }
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, offsets, tg.ntiles);
+    if (np > 0) {
+        { KTimer _t("paint_scatter", st); hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, st, np, key, slot, offsets, sorted); }
+    }
+    if (nc == 1) {
+        KTimer _t("paint_deposit", st);
+        hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide1>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, tg,
+                           offsets, sorted, acc, halo, hdr);
+    } else {
+        KTimer _t("paint_deposit", st);
+        hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide3>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, tg,
+                           offsets, sorted, acc, halo, hdr);
+    }
+    {
+        KTimer _t("paint_convert", st);
+        if (nc == 1)
+            hipLaunchKernelGGL((k_tiles_to_float<T, 1>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, halo, hdr, np, m, tg, out_scale, out);
+        else
+            hipLaunchKernelGGL((k_tiles_to_float<T, 3>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, halo, hdr, np, m, tg, out_scale, out);
+    }
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }
@@ -118,10 +181,10 @@ using namespace ldl;
 
 extern "C" {
 
-size_t ldl_paint_workspace_bytes(const int64_t nmesh[3], int32_t ncomp) {
+size_t ldl_paint_workspace_bytes(const int64_t nmesh[3], int32_t ncomp, int64_t np) {
     Mesh m;
-    if (check_mesh(nmesh, &m) || (ncomp != 1 && ncomp != 3)) return 0;
-    return paint_ws_bytes(m, ncomp);
+    if (check_mesh(nmesh, &m) || (ncomp != 1 && ncomp != 3) || np < 0) return 0;
+    return paint_ws_bytes(m, ncomp, np);
 }
 
 #define PAINT_ENTRY(SUF, T)                                                                                      \

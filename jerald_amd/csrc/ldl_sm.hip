@@ -64,10 +64,10 @@ struct SmWs {
     size_t paint, spec3, mesh3, tabs, part, total;
     int64_t part_doubles;
 };
-static SmWs sm_ws(const Mesh& m, int tsize) {
+static SmWs sm_ws(const Mesh& m, int tsize, int64_t np) {
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     SmWs w;
-    w.paint = al(paint_ws_bytes(m, 3));
+    w.paint = al(paint_ws_bytes(m, 3, np));
     w.spec3 = al((size_t)3 * m.hcells() * 2 * tsize);
     w.mesh3 = al((size_t)3 * m.cells() * tsize);
     w.tabs = al((size_t)2 * (m.n0 + m.n1 + m.nh()) * tsize);
@@ -588,7 +588,7 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     if (!pos || !pos_out || !rho || !Ubuf || !F) return fail(LDL_EINVAL, "NULL array argument");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
-    const SmWs w = sm_ws(m, sizeof(T));
+    const SmWs w = sm_ws(m, sizeof(T), np);
     if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
     const SmPtrs<T> P = sm_ptrs<T>(ws, w);
     const c2<T>* tw;
@@ -642,7 +642,7 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     if (!pos || !rho || !Ubuf || !F || !gout || !gin || !grad5) return fail(LDL_EINVAL, "NULL array argument");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
-    const SmWs w = sm_ws(m, sizeof(T));
+    const SmWs w = sm_ws(m, sizeof(T), np);
     if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
     const SmPtrs<T> P = sm_ptrs<T>(ws, w);
     const c2<T>* tw;
@@ -707,7 +707,7 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
     if (p_norm != 1 && p_norm != 2) return fail(LDL_EINVAL, "p_norm must be 1 or 2");
     if (smooth_kind < 0 || smooth_kind > 2) return fail(LDL_EINVAL, "smooth_kind must be 0, 1 or 2");
     if (smooth_kind && (!box || !(box[0] > 0 && box[1] > 0 && box[2] > 0))) return fail(LDL_EINVAL, "box must be positive");
-    const SmWs w = sm_ws(m, sizeof(T));
+    const SmWs w = sm_ws(m, sizeof(T), 0);  // no paint in the loss
     if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
     const SmPtrs<T> P = sm_ptrs<T>(ws, w);
     const int64_t cells = m.cells();
@@ -800,10 +800,10 @@ using namespace ldl;
 
 extern "C" {
 
-size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64) {
+size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np) {
     Mesh m;
-    if (check_mesh(nmesh, &m)) return 0;
-    return sm_ws(m, is_f64 ? 8 : 4).total;
+    if (check_mesh(nmesh, &m) || np < 0) return 0;
+    return sm_ws(m, is_f64 ? 8 : 4, np).total;
 }
 
 #define SM_ENTRY(SUF, T)                                                                                                    \

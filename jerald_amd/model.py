@@ -60,7 +60,8 @@ class JERALDModel:
         if grad_rule not in ("exact", "reference"):
             raise ValueError("grad_rule must be 'exact' or 'reference'")
         self.alpha_rule = 0 if grad_rule == "exact" else 1
-        self.paint_flags = {"fixed": L.LDL_PAINT_FIXED | L.LDL_PAINT_WS_ZEROED,
+        self.paint_flags = {"fixed": L.LDL_PAINT_FIXED,
+                            "direct": L.LDL_PAINT_FIXED | L.LDL_PAINT_DIRECT,
                             "float": L.LDL_PAINT_FLOAT_ATOMIC}[paint_mode]
         if save not in ("all", "positions"):
             raise ValueError("save must be 'all' or 'positions'")
@@ -118,7 +119,8 @@ class JERALDModel:
 
     # ------------------------------------------------------- kernels
     def _ws(self):
-        nb = L.load().ldl_sm_workspace_bytes(L.i64x3(self.nm), int(self.dtype == torch.float64))
+        np_ = 0 if self.X is None else self.X.shape[0]
+        nb = L.load().ldl_sm_workspace_bytes(L.i64x3(self.nm), int(self.dtype == torch.float64), np_)
         return L.workspaces.get(self.device, f"sm{self.dtype}", nb)
 
     def _layer_fwd(self, P, prm, fact):
@@ -135,7 +137,7 @@ class JERALDModel:
     def _paint_density(self, P, fact):
         dt, dev, nm = self.dtype, self.device, self.nm
         rho = torch.empty(nm, dtype=dt, device=dev)
-        nb = L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 1)
+        nb = L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 1, P.shape[0])
         ws = L.workspaces.get(dev, f"paint1", nb)
         L.call(f"ldl_ppaint_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], None, 1.0, L.i64x3(nm), float(fact),
                self.paint_flags, L.ptr(rho), L.ptr(ws), ws.numel(), L.stream_ptr(dev))

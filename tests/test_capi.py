@@ -30,11 +30,13 @@ def test_library_exports_every_declared_symbol():
 def test_workspace_queries():
     lib = L.load()
     nm = L.i64x3((64, 64, 64))
-    assert lib.ldl_paint_workspace_bytes(nm, 1) == 256 + 64 ** 3 * 8
-    assert lib.ldl_paint_workspace_bytes(nm, 3) == 256 + 3 * 64 ** 3 * 8
-    assert lib.ldl_paint_workspace_bytes(nm, 2) == 0
+    one = lib.ldl_paint_workspace_bytes(nm, 1, 1000)
+    assert one >= 64 ** 3 * 8 + 3 * 1000 * 4
+    assert lib.ldl_paint_workspace_bytes(nm, 3, 1000) > one
+    assert lib.ldl_paint_workspace_bytes(nm, 2, 1000) == 0
+    assert lib.ldl_paint_workspace_bytes(nm, 1, -1) == 0
     assert lib.ldl_fft_workspace_bytes(nm, 0) == 64 * 64 * 33 * 8
-    assert lib.ldl_sm_workspace_bytes(nm, 1) > lib.ldl_sm_workspace_bytes(nm, 0) > 0
+    assert lib.ldl_sm_workspace_bytes(nm, 1, 1000) > lib.ldl_sm_workspace_bytes(nm, 0, 1000) > 0
 
 
 @pytest.mark.parametrize("shape", [(12, 8, 8), (8, 8, 4), (2048, 8, 8), (8, 8, 6)])
