@@ -151,7 +151,7 @@ def main():
     torch.cuda.synchronize()
     dt32 = time.perf_counter() - t0
     kern = {}
-    for name in ("paint_deposit", "paint_convert", "zpass_r2c", "zpass_c2r", "colpass", "sm_x_fwd", "sm_x_bwd",
+    for name in ("paint_bin", "paint_scatter", "paint_deposit", "paint_convert", "zpass_r2c", "zpass_c2r", "colpass", "sm_x_fwd", "sm_x_bwd",
                  "readout3_update", "sm_bwd_gather", "paint_vjp_pos", "loss", "x_realfilter"):
         tot, cnt = L.timer_read(name)
         if cnt:

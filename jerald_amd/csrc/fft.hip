@@ -7,7 +7,7 @@ namespace ldl {
 template <typename T>
 int r2c3d(const T* in, c2<T>* spec, const Mesh& m, T scale, const c2<T>* tw, hipStream_t st) {
     const int64_t nrows = m.n0 * m.n1, nh = m.nh();
-    dispatch_n(m.n2 / 2, [&](auto H) {
+    dispatch_h(m.n2 / 2, [&](auto H) {
         launch_zr2c<T, decltype(H)::value>(nrows, tw, RealRowLoad<T>{in, m.n2}, HalfRowStore<T>{spec, nh, T(1)}, st);
     });
     dispatch_n(m.n1, [&](auto N) {
@@ -33,7 +33,7 @@ int c2r3d(const c2<T>* spec, c2<T>* tmp, T* out, const Mesh& m, T scale, const c
         ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
         launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{tmp, tmp, T(1)}, st);
     });
-    dispatch_n(m.n2 / 2, [&](auto H) {
+    dispatch_h(m.n2 / 2, [&](auto H) {
         launch_zc2r<T, decltype(H)::value>(nrows, tw, HalfRowLoad<T>{tmp, nh}, RealRowStore<T>{out, m.n2, scale}, st);
     });
     LDL_LAUNCH_CHECK();

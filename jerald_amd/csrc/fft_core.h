@@ -28,7 +28,9 @@ constexpr int radix_of(int rem) { return rem == 2 ? 2 : (rem == 4 || rem == 16) 
 
 template <typename T, int N> struct FftCfg {
     static constexpr int TPP = N <= 8 ? 1 : N <= 32 ? 4 : N == 64 ? 8 : N <= 512 ? 16 : 32;
-    static constexpr int P = sizeof(T) == 4 ? (N <= 8 ? 64 : N <= 64 ? 32 : 16) : (N <= 8 ? 32 : N <= 64 ? 16 : 8);
+    // FP64 at N = 1024 keeps 4 pencils so data + twiddles fit the 160 KB of LDS
+    static constexpr int P = sizeof(T) == 4 ? (N <= 8 ? 64 : N <= 64 ? 32 : 16)
+                                            : (N <= 8 ? 32 : N <= 64 ? 16 : N <= 512 ? 8 : 4);
     static constexpr int THREADS = P * TPP;
     static constexpr int LS = P + 1;
     static constexpr int LDS_ELEMS = N * LS;
@@ -78,9 +80,10 @@ template <int R, int DIR, typename C> __device__ __forceinline__ void dft(C* v) 
 }
 
 // ------------------------------------------------------------ LDS FFT
-// In-place transform of the P pencils held in lds.  twN = tables + N.
-template <typename T, int N, int P, int TPP, int DIR, int Ns = 1>
-__device__ __forceinline__ void lds_fft(c2<T>* lds, const c2<T>* __restrict__ twN) {
+// In-place transform of the P pencils held in lds.  twN[m * TWS] =
+// exp(-2 pi i m / N): the workgroup's LDS copy of the twiddle table.
+template <typename T, int N, int P, int TPP, int DIR, int TWS = 1, int Ns = 1>
+__device__ __forceinline__ void lds_fft(c2<T>* lds, const c2<T>* twN) {
     if constexpr (Ns < N) {
         using C = c2<T>;
         constexpr int R = radix_of(N / Ns);
@@ -104,7 +107,7 @@ __device__ __forceinline__ void lds_fft(c2<T>* lds, const c2<T>* __restrict__ tw
             if constexpr (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    C w = twN[r * k * (N / (Ns * R))];
+                    C w = twN[r * k * (N / (Ns * R)) * TWS];
                     if constexpr (DIR > 0) w.y = -w.y;
                     v[b][r] = cmul(v[b][r], w);
                 }
@@ -115,7 +118,46 @@ __device__ __forceinline__ void lds_fft(c2<T>* lds, const c2<T>* __restrict__ tw
             for (int r = 0; r < R; r++) lds[(base + r * Ns) * LS + p] = v[b][r];
         }
         __syncthreads();
-        lds_fft<T, N, P, TPP, DIR, Ns * R>(lds, twN);
+        lds_fft<T, N, P, TPP, DIR, TWS, Ns * R>(lds, twN);
+    }
+}
+
+// Copy the length-L twiddle table (tw + L) into LDS.
+template <typename T, int L, int TH>
+__device__ __forceinline__ void stage_twiddles(c2<T>* twl, const c2<T>* __restrict__ tw) {
+    for (int i = threadIdx.x; i < L; i += TH) twl[i] = tw[L + i];
+}
+
+// Tile of N x P complex values: every thread first issues all of its global
+// loads (N / TPP of them), then writes LDS, so each wave keeps many
+// requests in flight.  f(n, p) loads element n of pencil p (p < pv).
+template <typename T, int N, int P, int TH, class F>
+__device__ __forceinline__ void tile_load(c2<T>* lds, int pv, F&& f) {
+    constexpr int NL = N * P / TH;
+    c2<T> v[NL];
+#pragma unroll
+    for (int s = 0; s < NL; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        v[s] = p < pv ? f(n, p) : c2<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int s = 0; s < NL; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        lds[n * (P + 1) + p] = v[s];
+    }
+}
+
+// f(n, p, value) stores element n of pencil p (p < pv).
+template <typename T, int N, int P, int TH, class F>
+__device__ __forceinline__ void tile_store(const c2<T>* lds, int pv, F&& f) {
+    constexpr int NL = N * P / TH;
+#pragma unroll
+    for (int s = 0; s < NL; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        if (p < pv) f(n, p, lds[n * (P + 1) + p]);
     }
 }
 
@@ -178,23 +220,19 @@ template <typename T> struct PlainColOp {
 template <typename T, int N, int DIR, class Op>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS)) k_colpass(ColGeom g, const c2<T>* __restrict__ tw, Op op) {
     using Cfg = FftCfg<T, N>;
-    constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS;
+    constexpr int P = Cfg::P, TH = Cfg::THREADS;
     __shared__ c2<T> lds[Cfg::LDS_ELEMS];
+    __shared__ c2<T> twl[N];
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t plane = tile / g.tiles_per_plane;
     const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
-    for (int i = threadIdx.x; i < N * P; i += TH) {
-        const int n = i / P, p = i - n * P;
-        lds[n * LS + p] = p < pv ? op.load(base + n * g.es + p, n, plane, col0 + p) : c2<T>{T(0), T(0)};
-    }
+    stage_twiddles<T, N, TH>(twl, tw);
+    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return op.load(base + n * g.es + p, n, plane, col0 + p); });
     __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, DIR>(lds, tw + N);
-    for (int i = threadIdx.x; i < N * P; i += TH) {
-        const int n = i / P, p = i - n * P;
-        if (p < pv) op.store(base + n * g.es + p, n, plane, col0 + p, lds[n * LS + p]);
-    }
+    lds_fft<T, N, P, Cfg::TPP, DIR>(lds, twl);
+    tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, c2<T> v) { op.store(base + n * g.es + p, n, plane, col0 + p, v); });
 }
 
 // ------------------------------------------------------------ z passes
@@ -207,14 +245,28 @@ __global__ void __launch_bounds__((FftCfg<T, H>::THREADS)) k_zpass_r2c(int64_t n
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS;
     constexpr int N2 = 2 * H;
     __shared__ C lds[Cfg::LDS_ELEMS];
+    __shared__ C twl[N2];
     const int64_t row0 = xcd_tile(blockIdx.x, gridDim.x) * P;
-    for (int i = threadIdx.x; i < H * P; i += TH) {
-        const int p = i / H, m = i - p * H;
-        lds[m * LS + p] = row0 + p < nrows ? pro.load2(row0 + p, m) : C{T(0), T(0)};
+    stage_twiddles<T, N2, TH>(twl, tw);
+    {
+        constexpr int NL = H * P / TH;
+        C v[NL];
+#pragma unroll
+        for (int s = 0; s < NL; s++) {
+            const int i = threadIdx.x + s * TH;
+            const int p = i / H, m = i - p * H;
+            v[s] = row0 + p < nrows ? pro.load2(row0 + p, m) : C{T(0), T(0)};
+        }
+#pragma unroll
+        for (int s = 0; s < NL; s++) {
+            const int i = threadIdx.x + s * TH;
+            const int p = i / H, m = i - p * H;
+            lds[m * LS + p] = v[s];
+        }
     }
     __syncthreads();
-    lds_fft<T, H, P, Cfg::TPP, -1>(lds, tw + H);
-    const C* twN2 = tw + N2;
+    lds_fft<T, H, P, Cfg::TPP, -1, 2>(lds, twl);
+    const C* twN2 = twl;
     for (int i = threadIdx.x; i < (H + 1) * P; i += TH) {
         const int p = i / (H + 1), k = i - p * (H + 1);
         if (row0 + p >= nrows) continue;
@@ -239,18 +291,31 @@ __global__ void __launch_bounds__((FftCfg<T, H>::THREADS)) k_zpass_c2r(int64_t n
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS;
     constexpr int N2 = 2 * H;
     __shared__ C lds[(H + 1) * LS];
+    __shared__ C twl[N2];
     const int64_t row0 = xcd_tile(blockIdx.x, gridDim.x) * P;
-    for (int i = threadIdx.x; i < (H + 1) * P; i += TH) {
-        const int p = i / (H + 1), k = i - p * (H + 1);
-        C v = C{T(0), T(0)};
-        if (row0 + p < nrows) {
-            v = pro.load(row0 + p, k);
-            if (k == 0 || k == H) v.y = T(0);  // FFTW ignores these imaginary parts
+    stage_twiddles<T, N2, TH>(twl, tw);
+    {
+        constexpr int NL = ((H + 1) * P + TH - 1) / TH;
+        C v[NL];
+#pragma unroll
+        for (int s = 0; s < NL; s++) {
+            const int i = threadIdx.x + s * TH;
+            const int p = i / (H + 1), k = i - p * (H + 1);
+            v[s] = C{T(0), T(0)};
+            if (i < (H + 1) * P && row0 + p < nrows) {
+                v[s] = pro.load(row0 + p, k);
+                if (k == 0 || k == H) v[s].y = T(0);  // FFTW ignores these imaginary parts
+            }
         }
-        lds[k * LS + p] = v;
+#pragma unroll
+        for (int s = 0; s < NL; s++) {
+            const int i = threadIdx.x + s * TH;
+            const int p = i / (H + 1), k = i - p * (H + 1);
+            if (i < (H + 1) * P) lds[k * LS + p] = v[s];
+        }
     }
     __syncthreads();
-    const C* twN2 = tw + N2;
+    const C* twN2 = twl;
     constexpr int HP = H / 2 + 1;
     C za[(HP * P + TH - 1) / TH], zb[(HP * P + TH - 1) / TH];
     int ia[(HP * P + TH - 1) / TH], ib[(HP * P + TH - 1) / TH];
@@ -285,7 +350,7 @@ __global__ void __launch_bounds__((FftCfg<T, H>::THREADS)) k_zpass_c2r(int64_t n
         if (ib[s] >= 0) lds[ib[s]] = zb[s];
     }
     __syncthreads();
-    lds_fft<T, H, P, Cfg::TPP, +1>(lds, tw + H);
+    lds_fft<T, H, P, Cfg::TPP, +1, 2>(lds, twl);
     constexpr int NR = Epi::NR;
     double acc[NR > 0 ? NR : 1] = {};
     for (int i = threadIdx.x; i < H * P; i += TH) {
@@ -333,8 +398,8 @@ template <typename T> struct RealRowStore {
 // ------------------------------------------------------------ dispatch
 template <int N> using IC = std::integral_constant<int, N>;
 
-// Calls f(IC<N>{}) for a runtime power of two N in [4, 1024]; false otherwise.
-template <class F> inline bool dispatch_n(int64_t n, F&& f) {
+// Calls f(IC<N>{}) for a runtime power of two N in [4, MAXN]; false otherwise.
+template <int MAXN = 1024, class F> inline bool dispatch_n(int64_t n, F&& f) {
     switch (n) {
         case 4: f(IC<4>{}); return true;
         case 8: f(IC<8>{}); return true;
@@ -344,10 +409,15 @@ template <class F> inline bool dispatch_n(int64_t n, F&& f) {
         case 128: f(IC<128>{}); return true;
         case 256: f(IC<256>{}); return true;
         case 512: f(IC<512>{}); return true;
-        case 1024: f(IC<1024>{}); return true;
+        case 1024:
+            if constexpr (MAXN >= 1024) { f(IC<1024>{}); return true; }
+            return false;
         default: return false;
     }
 }
+
+// z passes run half-length FFTs: H = N2 / 2 <= 512.
+template <class F> inline bool dispatch_h(int64_t h, F&& f) { return dispatch_n<512>(h, f); }
 
 template <typename T, int N, int DIR, class Op>
 inline void launch_colpass(const ColGeom& g, const c2<T>* tw, const Op& op, hipStream_t st) {

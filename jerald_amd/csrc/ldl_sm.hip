@@ -179,17 +179,16 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
     __shared__ C lds[Cfg::LDS_ELEMS];
+    __shared__ C twl[N];
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t plane = tile / g.tiles_per_plane;
     const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
-    for (int i = threadIdx.x; i < N * P; i += TH) {
-        const int n = i / P, p = i - n * P;
-        lds[n * LS + p] = p < pv ? U[base + n * g.es + p] : C{T(0), T(0)};
-    }
+    stage_twiddles<T, N, TH>(twl, tw);
+    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return U[base + n * g.es + p]; });
     __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, -1>(lds, tw + N);
+    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
     C pk[EPT];
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
@@ -214,11 +213,9 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
             lds[n * LS + p] = C{Dd * pk[s].y, -Dd * pk[s].x};  // -i D P
         }
         __syncthreads();
-        lds_fft<T, N, P, Cfg::TPP, +1>(lds, tw + N);
-        for (int i = threadIdx.x; i < N * P; i += TH) {
-            const int n = i / P, p = i - n * P;
-            if (p < pv) out3[d * hcells + base + n * g.es + p] = lds[n * LS + p];
-        }
+        lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
+        C* o = out3 + d * hcells + base;
+        tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { o[n * g.es + p] = v; });
         __syncthreads();
     }
 }
@@ -235,22 +232,22 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
     __shared__ C lds[Cfg::LDS_ELEMS];
+    __shared__ C twl[N];
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t plane = tile / g.tiles_per_plane;
     const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
+    stage_twiddles<T, N, TH>(twl, tw);
     C q[EPT];
 #pragma unroll
     for (int s = 0; s < EPT; s++) q[s] = C{T(0), T(0)};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        for (int i = threadIdx.x; i < N * P; i += TH) {
-            const int n = i / P, p = i - n * P;
-            lds[n * LS + p] = p < pv ? S3[d * hcells + base + n * g.es + p] : C{T(0), T(0)};
-        }
+        const C* src = S3 + d * hcells + base;
+        tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return src[n * g.es + p]; });
         __syncthreads();
-        lds_fft<T, N, P, Cfg::TPP, -1>(lds, tw + N);
+        lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
 #pragma unroll
         for (int s = 0; s < EPT; s++) {
             const int i = threadIdx.x + s * TH;
@@ -287,11 +284,8 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
         lds[n * LS + p] = y;
     }
     __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, +1>(lds, tw + N);
-    for (int i = threadIdx.x; i < N * P; i += TH) {
-        const int n = i / P, p = i - n * P;
-        if (p < pv) out[base + n * g.es + p] = lds[n * LS + p];
-    }
+    lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
+    tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
     block_sum<4, TH>(acc);
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -309,17 +303,16 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS;
     __shared__ C lds[Cfg::LDS_ELEMS];
+    __shared__ C twl[N];
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t plane = tile / g.tiles_per_plane;
     const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
-    for (int i = threadIdx.x; i < N * P; i += TH) {
-        const int n = i / P, p = i - n * P;
-        lds[n * LS + p] = p < pv ? io[base + n * g.es + p] : C{T(0), T(0)};
-    }
+    stage_twiddles<T, N, TH>(twl, tw);
+    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return io[base + n * g.es + p]; });
     __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, -1>(lds, tw + N);
+    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
     for (int i = threadIdx.x; i < N * P; i += TH) {
         const int n = i / P, p = i - n * P;
         if (p >= pv) continue;
@@ -335,11 +328,8 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
         lds[n * LS + p] = C{v.x * w, v.y * w};
     }
     __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, +1>(lds, tw + N);
-    for (int i = threadIdx.x; i < N * P; i += TH) {
-        const int n = i / P, p = i - n * P;
-        if (p < pv) io[base + n * g.es + p] = lds[n * LS + p];
-    }
+    lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
+    tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { io[base + n * g.es + p] = v; });
 }
 
 // --------------------------------------------------------------- gathers
@@ -599,7 +589,7 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hc = m.hcells();
     c2<T>* U = reinterpret_cast<c2<T>*>(Ubuf);
     // 2. z pass with the gamma power, y pass
-    dispatch_n(m.n2 / 2, [&](auto H) {
+    dispatch_h(m.n2 / 2, [&](auto H) {
         launch_zr2c<T, decltype(H)::value>(nrows, tw, PowRowLoad<T>{rho, m.n2, (T)prm[1]}, HalfRowStore<T>{U, nh, T(1)}, st);
     });
     dispatch_n(m.n1, [&](auto N) {
@@ -619,7 +609,7 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
         ColGeom g = col_geom<T, decltype(N)::value>(3 * m.n0, nh, m.n1 * nh, nh);
         launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st);
     });
-    dispatch_n(m.n2 / 2, [&](auto H) {
+    dispatch_h(m.n2 / 2, [&](auto H) {
         launch_zc2r<T, decltype(H)::value>(3 * nrows, tw, HalfRowLoad<T>{P.spec3, nh}, RealRowStore<T>{F, m.n2, T(1)}, st);
     });
     // 5. readout + position update
@@ -652,7 +642,7 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     if ((rc = paint_impl<T>(pos, pdesc, np, gout, 0.0, 3, nmesh, 1.0, pflags, P.mesh3, P.paint, w.paint, st))) return rc;
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
     // 2. unnormalised r2c of the three meshes: z and y passes
-    dispatch_n(m.n2 / 2, [&](auto H) {
+    dispatch_h(m.n2 / 2, [&](auto H) {
         launch_zr2c<T, decltype(H)::value>(3 * nrows, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore<T>{P.spec3, nh, T(1)}, st);
     });
     dispatch_n(m.n1, [&](auto N) {
@@ -680,7 +670,7 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     });
     T* grho = P.mesh3;
     int64_t nzb = 0;
-    dispatch_n(m.n2 / 2, [&](auto H) {
+    dispatch_h(m.n2 / 2, [&](auto H) {
         constexpr int HH = decltype(H)::value;
         nzb = (nrows + FftCfg<T, HH>::P - 1) / FftCfg<T, HH>::P;
         launch_zc2r<T, HH>(nrows, tw, HalfRowLoad<T>{S, nh},
@@ -738,7 +728,7 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
         c2<T>* S = P.spec3;
         const T invM = (T)(1.0 / (double)cells);
         auto smooth = [&](const T* src) {  // S = W(k) r2c(src) / M, inverse x and y passes
-            dispatch_n(m.n2 / 2, [&](auto H) {
+            dispatch_h(m.n2 / 2, [&](auto H) {
                 launch_zr2c<T, decltype(H)::value>(nrows, tw, RealRowLoad<T>{src, m.n2}, HalfRowStore<T>{S, nh, T(1)}, st);
             });
             dispatch_n(m.n1, [&](auto N) {
@@ -757,18 +747,18 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
             });
         };
         int64_t nzb = 0;
-        dispatch_n(m.n2 / 2, [&](auto H) { nzb = (nrows + FftCfg<T, decltype(H)::value>::P - 1) / FftCfg<T, decltype(H)::value>::P; });
+        dispatch_h(m.n2 / 2, [&](auto H) { nzb = (nrows + FftCfg<T, decltype(H)::value>::P - 1) / FftCfg<T, decltype(H)::value>::P; });
         // every block writes the loss columns; the gradient columns stay zero without g_rho
         hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nzb * 5, 256)), dim3(256), 0, st, bpart, nzb * 5, 0.0);
         hipLaunchKernelGGL(k_loss_diff<T>, dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff);
         smooth(diff);
-        dispatch_n(m.n2 / 2, [&](auto H) {
+        dispatch_h(m.n2 / 2, [&](auto H) {
             launch_zc2r<T, decltype(H)::value>(nrows, tw, HalfRowLoad<T>{S, nh},
                                                LossStore<T>{mask, maskv, grho ? gds : nullptr, npix, m.n2, T(1), p_norm == 1, bpart}, st);
         });
         if (grho) {
             smooth(gds);
-            dispatch_n(m.n2 / 2, [&](auto H) {
+            dispatch_h(m.n2 / 2, [&](auto H) {
                 launch_zc2r<T, decltype(H)::value>(nrows, tw, HalfRowLoad<T>{S, nh}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart}, st);
             });
         }
