@@ -259,6 +259,7 @@ struct TileGeom {
     int64_t nt[3];       // tiles per axis (last tile may be partial)
     int64_t ntiles;
     int64_t hstride;     // halo entries per tile, 3 ts^2 + 3 ts + 1
+    int64_t lstride;     // low-face entries per tile, 3 ts^2 - 3 ts + 1
     __host__ __device__ int64_t extent(int d, int64_t t, const Mesh& m) const {
         const int64_t n = d == 0 ? m.n0 : d == 1 ? m.n1 : m.n2;
         const int64_t lo = t * ts;
@@ -274,6 +275,7 @@ inline TileGeom make_tiles(const Mesh& m, int ts) {
     g.nt[2] = (m.n2 + ts - 1) / ts;
     g.ntiles = g.nt[0] * g.nt[1] * g.nt[2];
     g.hstride = 3ll * ts * ts + 3ll * ts + 1;
+    g.lstride = 3ll * ts * ts - 3ll * ts + 1;
     return g;
 }
 
@@ -286,37 +288,59 @@ __device__ __forceinline__ int64_t halo_index(int64_t a, int64_t b, int64_t c, i
 }
 
 // Counting-sort pass 1: tile key of each particle and its rank among the
-// particles of the same tile (wave-aggregated atomics on the tile counters).
-template <typename T>
+// particles of the same tile.  A workgroup takes a chunk of 256*PPT
+// particles, counts them per tile in an LDS hash table (LDS atomics give the
+// local ranks), then reserves each distinct tile's range with one global
+// atomic -- issued by different threads in parallel.
+template <typename T, int PPT>
 __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Mesh m, TileGeom tg,
                                                   unsigned* __restrict__ counts, unsigned* __restrict__ key,
                                                   unsigned* __restrict__ slot) {
-    const int lane = __lane_id();
-    for (int64_t p0 = blockIdx.x * (int64_t)blockDim.x; p0 < np; p0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = p0 + threadIdx.x;
-        unsigned k = 0xFFFFFFFFu;
-        if (p < np) {
-            int64_t idx[3];
-            T f[3];
-            pd.decode(pos, p, idx, f);
-            k = (unsigned)(((idx[0] / tg.ts) * tg.nt[1] + idx[1] / tg.ts) * tg.nt[2] + idx[2] / tg.ts);
+    constexpr int CH = 256 * PPT;
+    constexpr int HS = 2 * CH;  // at most CH distinct keys: the table is at most half full
+    constexpr unsigned EMPTY = 0xFFFFFFFFu;
+    __shared__ unsigned hkey[HS];
+    __shared__ unsigned hcnt[HS];
+    const int64_t nchunks = (np + CH - 1) / CH;
+    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        for (int h = threadIdx.x; h < HS; h += 256) {
+            hkey[h] = EMPTY;
+            hcnt[h] = 0u;
         }
-        unsigned long long pending = __ballot(p < np);
-        unsigned myslot = 0;
-        while (pending) {
-            const int leader = __ffsll((unsigned long long)pending) - 1;
-            const unsigned lk = __shfl(k, leader, 64);
-            const unsigned long long same = __ballot(k == lk) & pending;
-            unsigned base = 0;
-            if (lane == leader) base = atomicAdd(&counts[lk], (unsigned)__popcll(same));
-            base = __shfl(base, leader, 64);
-            if ((same >> lane) & 1ull) myslot = base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
-            pending &= ~same;
+        __syncthreads();
+        unsigned k[PPT], hs[PPT], r[PPT];
+#pragma unroll
+        for (int q = 0; q < PPT; q++) {
+            const int64_t p = ch * CH + q * 256 + threadIdx.x;
+            k[q] = EMPTY;
+            if (p < np) {
+                int64_t idx[3];
+                T f[3];
+                pd.decode(pos, p, idx, f);
+                k[q] = (unsigned)(((idx[0] / tg.ts) * tg.nt[1] + idx[1] / tg.ts) * tg.nt[2] + idx[2] / tg.ts);
+                unsigned h = (k[q] * 2654435761u) & (HS - 1);
+                while (true) {  // terminates: fewer than HS distinct keys per chunk
+                    const unsigned prev = atomicCAS(&hkey[h], EMPTY, k[q]);
+                    if (prev == EMPTY || prev == k[q]) break;
+                    h = (h + 1) & (HS - 1);
+                }
+                hs[q] = h;
+                r[q] = atomicAdd(&hcnt[h], 1u);
+            }
         }
-        if (p < np) {
-            key[p] = k;
-            slot[p] = myslot;
+        __syncthreads();
+        for (int h = threadIdx.x; h < HS; h += 256)
+            if (hkey[h] != EMPTY) hcnt[h] = atomicAdd(&counts[hkey[h]], hcnt[h]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PPT; q++) {
+            const int64_t p = ch * CH + q * 256 + threadIdx.x;
+            if (p < np) {
+                key[p] = k[q];
+                slot[p] = hcnt[hs[q]] + r[q];
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -352,22 +376,47 @@ static __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const un
         sorted[offsets[key[p]] + slot[p]] = (unsigned)p;
 }
 
-// One workgroup per tile: LDS fixed-point deposit, core + halo write-out.
+// Position of core cell (a, b, c) with a == 0, b == 0 or c == 0 in a tile's
+// low-face record (ex, ey, ez: the tile's extents); inverse: low_cell.
+__device__ __forceinline__ int low_index(int a, int b, int c, int ex, int ey, int ez) {
+    if (a == 0) return b * ez + c;
+    if (b == 0) return ey * ez + (a - 1) * ez + c;
+    return ey * ez + (ex - 1) * ez + (a - 1) * (ey - 1) + (b - 1);
+}
+__device__ __forceinline__ void low_cell(int i, int ex, int ey, int ez, int& a, int& b, int& c) {
+    if (i < ey * ez) { a = 0; b = i / ez; c = i - b * ez; return; }
+    i -= ey * ez;
+    if (i < (ex - 1) * ez) { a = 1 + i / ez; b = 0; c = i - (a - 1) * ez; return; }
+    i -= (ex - 1) * ez;
+    a = 1 + i / (ey - 1);
+    b = 1 + (i - (a - 1) * (ey - 1));
+    c = 0;
+}
+
+// One workgroup per tile: LDS fixed-point deposit.  Interior cells (no
+// neighbour contributes) are final and written as FP values; low-face cells
+// go to the ``low`` record and the +1 faces to the ``halo`` record, both as
+// integers, for k_tiles_fixup.
 template <typename T, int NC, int TS>
 __global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ mass,
                                                      T mscalar, Mesh m, TileGeom tg, const unsigned* __restrict__ offsets,
-                                                     const unsigned* __restrict__ sorted, unsigned long long* __restrict__ acc,
-                                                     unsigned long long* __restrict__ halo, const unsigned long long* __restrict__ hdr) {
+                                                     const unsigned* __restrict__ sorted, unsigned long long* __restrict__ low,
+                                                     unsigned long long* __restrict__ halo, const unsigned long long* __restrict__ hdr,
+                                                     double scale, T* __restrict__ out) {
     constexpr int B = TS + 1, B3 = B * B * B;
     __shared__ unsigned long long box[NC * B3];
     const int64_t t = blockIdx.x;
     const int64_t tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
     const int64_t lo[3] = {tx * TS, ty * TS, tz * TS};
-    const int64_t ex = tg.extent(0, tx, m), ey = tg.extent(1, ty, m), ez = tg.extent(2, tz, m);
+    const int ex = (int)tg.extent(0, tx, m), ey = (int)tg.extent(1, ty, m), ez = (int)tg.extent(2, tz, m);
     for (int i = threadIdx.x; i < NC * B3; i += 256) box[i] = 0ull;
     int s[NC];
+    double fs[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) s[c] = fixed_shift(hdr[c], np);
+    for (int c = 0; c < NC; c++) {
+        s[c] = fixed_shift(hdr[c], np);
+        fs[c] = ldexp(scale, -s[c]);
+    }
     __syncthreads();
     const unsigned j0 = offsets[t], j1 = offsets[t + 1];
     for (unsigned j = j0 + threadIdx.x; j < j1; j += 256) {
@@ -403,57 +452,69 @@ __global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, 
     }
     __syncthreads();
     const int64_t cells = m.cells();
-    const int64_t nbox = (ex + 1) * (ey + 1) * (ez + 1);
-    for (int64_t i = threadIdx.x; i < nbox; i += 256) {
-        const int64_t cc = i % (ez + 1), bb = (i / (ez + 1)) % (ey + 1), aa = i / ((ez + 1) * (ey + 1));
-        const int li = (int)((aa * B + bb) * B + cc);
+    const int64_t lstride = tg.lstride, hstride = tg.hstride;
+    const int e1z = ez + 1, e1y = ey + 1;
+    const int nbox = (ex + 1) * e1y * e1z;
+    for (int i = threadIdx.x; i < nbox; i += 256) {
+        const int cc = i % e1z, bb = (i / e1z) % e1y, aa = i / (e1z * e1y);
+        const int li = (aa * B + bb) * B + cc;
         if (aa < ex && bb < ey && cc < ez) {
-            const int64_t g = ((lo[0] + aa) * m.n1 + (lo[1] + bb)) * m.n2 + (lo[2] + cc);
+            if (aa && bb && cc) {
+                const int64_t g = ((lo[0] + aa) * m.n1 + (lo[1] + bb)) * m.n2 + (lo[2] + cc);
 #pragma unroll
-            for (int c = 0; c < NC; c++) acc[c * cells + g] = box[c * B3 + li];
+                for (int c = 0; c < NC; c++) out[c * cells + g] = (T)((double)(long long)box[c * B3 + li] * fs[c]);
+            } else {
+                const int64_t h = t * lstride + low_index(aa, bb, cc, ex, ey, ez);
+#pragma unroll
+                for (int c = 0; c < NC; c++) low[c * tg.ntiles * lstride + h] = box[c * B3 + li];
+            }
         } else {
-            const int64_t h = t * tg.hstride + halo_index(aa, bb, cc, ex, ey, ez);
+            const int64_t h = t * hstride + halo_index(aa, bb, cc, ex, ey, ez);
 #pragma unroll
-            for (int c = 0; c < NC; c++) halo[c * tg.ntiles * tg.hstride + h] = box[c * B3 + li];
+            for (int c = 0; c < NC; c++) halo[c * tg.ntiles * hstride + h] = box[c * B3 + li];
         }
     }
 }
 
-// out = (core + halos of the lower neighbour tiles) * 2^-s * scale.
+// Low-face cells: own value + the +1 faces of the lower neighbour tiles,
+// then the FP conversion.  One thread per low-face cell.
 template <typename T, int NC>
-__global__ void __launch_bounds__(256) k_tiles_to_float(const unsigned long long* __restrict__ acc,
-                                                        const unsigned long long* __restrict__ halo,
-                                                        const unsigned long long* __restrict__ hdr, int64_t np, Mesh m,
-                                                        TileGeom tg, double scale, T* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_tiles_fixup(const unsigned long long* __restrict__ low,
+                                                     const unsigned long long* __restrict__ halo,
+                                                     const unsigned long long* __restrict__ hdr, int64_t np, Mesh m,
+                                                     TileGeom tg, double scale, T* __restrict__ out) {
     double f[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) f[c] = ldexp(scale, -fixed_shift(hdr[c], np));
     const int64_t cells = m.cells();
-    const int ts = tg.ts;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cells; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t z = i % m.n2, y = (i / m.n2) % m.n1, x = i / (m.n2 * m.n1);
-        const int64_t tx = x / ts, ty = y / ts, tz = z / ts;
-        const int64_t a = x - tx * ts, b = y - ty * ts, c = z - tz * ts;
+    const int64_t total = tg.ntiles * tg.lstride;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = i / tg.lstride;
+        const int li = (int)(i - t * tg.lstride);
+        const int64_t tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
+        const int ex = (int)tg.extent(0, tx, m), ey = (int)tg.extent(1, ty, m), ez = (int)tg.extent(2, tz, m);
+        if (li >= ex * ey * ez - (ex - 1) * (ey - 1) * (ez - 1)) continue;  // partial tile: shorter record
+        int a, b, c;
+        low_cell(li, ex, ey, ez, a, b, c);
         long long v[NC];
 #pragma unroll
-        for (int k = 0; k < NC; k++) v[k] = (long long)acc[k * cells + i];
-        if (a == 0 || b == 0 || c == 0) {
+        for (int k = 0; k < NC; k++) v[k] = (long long)low[k * tg.ntiles * tg.lstride + i];
 #pragma unroll
-            for (int q = 1; q < 8; q++) {
-                const int sx = q >> 2, sy = (q >> 1) & 1, sz = q & 1;
-                if ((sx && a) || (sy && b) || (sz && c)) continue;
-                const int64_t nx = sx ? (tx == 0 ? tg.nt[0] - 1 : tx - 1) : tx;
-                const int64_t ny = sy ? (ty == 0 ? tg.nt[1] - 1 : ty - 1) : ty;
-                const int64_t nz = sz ? (tz == 0 ? tg.nt[2] - 1 : tz - 1) : tz;
-                const int64_t ex = tg.extent(0, nx, m), ey = tg.extent(1, ny, m), ez = tg.extent(2, nz, m);
-                const int64_t nt = (nx * tg.nt[1] + ny) * tg.nt[2] + nz;
-                const int64_t h = nt * tg.hstride + halo_index(sx ? ex : a, sy ? ey : b, sz ? ez : c, ex, ey, ez);
+        for (int q = 1; q < 8; q++) {
+            const int sx = q >> 2, sy = (q >> 1) & 1, sz = q & 1;
+            if ((sx && a) || (sy && b) || (sz && c)) continue;
+            const int64_t nx = sx ? (tx == 0 ? tg.nt[0] - 1 : tx - 1) : tx;
+            const int64_t ny = sy ? (ty == 0 ? tg.nt[1] - 1 : ty - 1) : ty;
+            const int64_t nz = sz ? (tz == 0 ? tg.nt[2] - 1 : tz - 1) : tz;
+            const int nex = (int)tg.extent(0, nx, m), ney = (int)tg.extent(1, ny, m), nez = (int)tg.extent(2, nz, m);
+            const int64_t nt = (nx * tg.nt[1] + ny) * tg.nt[2] + nz;
+            const int64_t h = nt * tg.hstride + halo_index(sx ? nex : a, sy ? ney : b, sz ? nez : c, nex, ney, nez);
 #pragma unroll
-                for (int k = 0; k < NC; k++) v[k] += (long long)halo[k * tg.ntiles * tg.hstride + h];
-            }
+            for (int k = 0; k < NC; k++) v[k] += (long long)halo[k * tg.ntiles * tg.hstride + h];
         }
+        const int64_t g = ((tx * tg.ts + a) * m.n1 + (ty * tg.ts + b)) * m.n2 + (tz * tg.ts + c);
 #pragma unroll
-        for (int k = 0; k < NC; k++) out[k * cells + i] = (T)((double)v[k] * f[k]);
+        for (int k = 0; k < NC; k++) out[k * cells + g] = (T)((double)v[k] * f[k]);
     }
 }
 

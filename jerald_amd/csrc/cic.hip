@@ -23,15 +23,17 @@ static constexpr int kTileSide3 = 8;   // three components: 3 x 9^3 x 8 B = 17.5
 
 static inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Workspace: header | fixed-point mesh (nc) | tile halos (nc) | counts | offsets | key | slot | sorted
+// Workspace: header | direct-mode mesh or tile low faces (nc) | tile halos (nc) |
+// counts | offsets | key | slot | sorted
 struct PaintWs {
     size_t acc, halo, counts, offsets, key, slot, sorted, total;
 };
 static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
     const TileGeom tg = make_tiles(m, nc == 1 ? kTileSide1 : kTileSide3);
+    const size_t low = (size_t)nc * tg.ntiles * tg.lstride * 8, direct = (size_t)nc * m.cells() * 8;
     PaintWs w;
     w.acc = al256((size_t)kHdrBytes);
-    w.halo = w.acc + al256((size_t)nc * m.cells() * 8);
+    w.halo = w.acc + al256(low > direct ? low : direct);
     w.counts = w.halo + al256((size_t)nc * tg.ntiles * tg.hstride * 8);
     w.offsets = w.counts + al256((size_t)(tg.ntiles + 1) * 4);
     w.key = w.offsets + al256((size_t)(tg.ntiles + 1) * 4);
@@ -116,27 +118,32 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     auto* sorted = reinterpret_cast<unsigned*>(wb + w.sorted);
     hipLaunchKernelGGL(k_fill<unsigned>, dim3(grid_for(tg.ntiles + 1, kBlock, 1 << 16)), dim3(kBlock), 0, st, counts, tg.ntiles + 1, 0u);
     if (np > 0) {
-        { KTimer _t("paint_bin", st); hipLaunchKernelGGL(k_bin_keys<T>, dim3(gp), dim3(kBlock), 0, st, pos, pd, np, m, tg, counts, key, slot); }
+        constexpr int PPT = 8;
+        KTimer _t("paint_bin", st);
+        hipLaunchKernelGGL((k_bin_keys<T, PPT>), dim3(grid_for((np + 256 * PPT - 1) / (256 * PPT), 1, 1 << 15)), dim3(256), 0, st,
+                           pos, pd, np, m, tg, counts, key, slot);
     }
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, offsets, tg.ntiles);
     if (np > 0) {
         { KTimer _t("paint_scatter", st); hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, st, np, key, slot, offsets, sorted); }
     }
+    auto* low = acc;
     if (nc == 1) {
         KTimer _t("paint_deposit", st);
         hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide1>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, tg,
-                           offsets, sorted, acc, halo, hdr);
+                           offsets, sorted, low, halo, hdr, out_scale, out);
     } else {
         KTimer _t("paint_deposit", st);
         hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide3>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, tg,
-                           offsets, sorted, acc, halo, hdr);
+                           offsets, sorted, low, halo, hdr, out_scale, out);
     }
     {
         KTimer _t("paint_convert", st);
+        const unsigned gf = grid_for(tg.ntiles * tg.lstride, kBlock, 1 << 16);
         if (nc == 1)
-            hipLaunchKernelGGL((k_tiles_to_float<T, 1>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, halo, hdr, np, m, tg, out_scale, out);
+            hipLaunchKernelGGL((k_tiles_fixup<T, 1>), dim3(gf), dim3(kBlock), 0, st, low, halo, hdr, np, m, tg, out_scale, out);
         else
-            hipLaunchKernelGGL((k_tiles_to_float<T, 3>), dim3(grid_for(cells, kBlock, 1 << 16)), dim3(kBlock), 0, st, acc, halo, hdr, np, m, tg, out_scale, out);
+            hipLaunchKernelGGL((k_tiles_fixup<T, 3>), dim3(gf), dim3(kBlock), 0, st, low, halo, hdr, np, m, tg, out_scale, out);
     }
     LDL_LAUNCH_CHECK();
     return LDL_OK;

@@ -130,7 +130,9 @@ __device__ __forceinline__ void stage_twiddles(c2<T>* twl, const c2<T>* __restri
 
 // Tile of N x P complex values: every thread first issues all of its global
 // loads (N / TPP of them), then writes LDS, so each wave keeps many
-// requests in flight.  f(n, p) loads element n of pencil p (p < pv).
+// requests in flight.  f(n, p) loads element n of pencil p.  Pencils past
+// pv (a partial tile) load a valid pencil and are zeroed afterwards, so the
+// loads stay branch free (a branch per load would serialise them).
 template <typename T, int N, int P, int TH, class F>
 __device__ __forceinline__ void tile_load(c2<T>* lds, int pv, F&& f) {
     constexpr int NL = N * P / TH;
@@ -139,13 +141,13 @@ __device__ __forceinline__ void tile_load(c2<T>* lds, int pv, F&& f) {
     for (int s = 0; s < NL; s++) {
         const int i = threadIdx.x + s * TH;
         const int n = i / P, p = i - n * P;
-        v[s] = p < pv ? f(n, p) : c2<T>{T(0), T(0)};
+        v[s] = f(n, p < pv ? p : pv - 1);
     }
 #pragma unroll
     for (int s = 0; s < NL; s++) {
         const int i = threadIdx.x + s * TH;
         const int n = i / P, p = i - n * P;
-        lds[n * (P + 1) + p] = v[s];
+        lds[n * (P + 1) + p] = p < pv ? v[s] : c2<T>{T(0), T(0)};
     }
 }
 
@@ -255,13 +257,13 @@ __global__ void __launch_bounds__((FftCfg<T, H>::THREADS)) k_zpass_r2c(int64_t n
         for (int s = 0; s < NL; s++) {
             const int i = threadIdx.x + s * TH;
             const int p = i / H, m = i - p * H;
-            v[s] = row0 + p < nrows ? pro.load2(row0 + p, m) : C{T(0), T(0)};
+            v[s] = pro.load2(row0 + p < nrows ? row0 + p : nrows - 1, m);  // branch free, see tile_load
         }
 #pragma unroll
         for (int s = 0; s < NL; s++) {
             const int i = threadIdx.x + s * TH;
             const int p = i / H, m = i - p * H;
-            lds[m * LS + p] = v[s];
+            lds[m * LS + p] = row0 + p < nrows ? v[s] : C{T(0), T(0)};
         }
     }
     __syncthreads();
@@ -300,18 +302,17 @@ __global__ void __launch_bounds__((FftCfg<T, H>::THREADS)) k_zpass_c2r(int64_t n
 #pragma unroll
         for (int s = 0; s < NL; s++) {
             const int i = threadIdx.x + s * TH;
-            const int p = i / (H + 1), k = i - p * (H + 1);
-            v[s] = C{T(0), T(0)};
-            if (i < (H + 1) * P && row0 + p < nrows) {
-                v[s] = pro.load(row0 + p, k);
-                if (k == 0 || k == H) v[s].y = T(0);  // FFTW ignores these imaginary parts
-            }
+            const int ii = i < (H + 1) * P ? i : (H + 1) * P - 1;
+            const int p = ii / (H + 1), k = ii - p * (H + 1);
+            v[s] = pro.load(row0 + p < nrows ? row0 + p : nrows - 1, k);  // branch free, see tile_load
         }
 #pragma unroll
         for (int s = 0; s < NL; s++) {
             const int i = threadIdx.x + s * TH;
             const int p = i / (H + 1), k = i - p * (H + 1);
-            if (i < (H + 1) * P) lds[k * LS + p] = v[s];
+            C w = row0 + p < nrows ? v[s] : C{T(0), T(0)};
+            if (k == 0 || k == H) w.y = T(0);  // FFTW ignores these imaginary parts
+            if (i < (H + 1) * P) lds[k * LS + p] = w;
         }
     }
     __syncthreads();
