@@ -139,20 +139,61 @@ __global__ void __launch_bounds__(256) k_fixed_to_float(unsigned long long* __re
 }
 
 // ---------------------------------------------------------------- gathers
+// CIC stencil of one particle: the four (x, y) rows of the 8 corners and the
+// lower z index.  Corner q = 4a + 2b + c sits at row[2a + b] + (k + c) mod N2.
+template <typename T> struct Stencil {
+    int64_t row[4];
+    int64_t k;
+    T f[3];
+};
+
 template <typename T>
-__device__ __forceinline__ void corners(const PosDec<T>& pd, const Mesh& m, const T* pos, int64_t p, int64_t cidx[8], T f[3]) {
+__device__ __forceinline__ void locate(const PosDec<T>& pd, const Mesh& m, T x0, T x1, T x2, int64_t p, Stencil<T>& st) {
     int64_t idx[3];
-    pd.decode(pos, p, idx, f);
-    const int64_t ii[2] = {idx[0], idx[0] + 1 == m.n0 ? 0 : idx[0] + 1};
-    const int64_t jj[2] = {idx[1], idx[1] + 1 == m.n1 ? 0 : idx[1] + 1};
-    const int64_t kk[2] = {idx[2], idx[2] + 1 == m.n2 ? 0 : idx[2] + 1};
-#pragma unroll
-    for (int q = 0; q < 8; q++) cidx[q] = (ii[q >> 2] * m.n1 + jj[(q >> 1) & 1]) * m.n2 + kk[q & 1];
+    pd.decode_xyz(x0, x1, x2, p, idx, st.f);
+    const int64_t i1 = idx[0] + 1 == m.n0 ? 0 : idx[0] + 1;
+    const int64_t j1 = idx[1] + 1 == m.n1 ? 0 : idx[1] + 1;
+    st.row[0] = (idx[0] * m.n1 + idx[1]) * m.n2;
+    st.row[1] = (idx[0] * m.n1 + j1) * m.n2;
+    st.row[2] = (i1 * m.n1 + idx[1]) * m.n2;
+    st.row[3] = (i1 * m.n1 + j1) * m.n2;
+    st.k = idx[2];
 }
 
-// value (deriv < 0) or derivative along ``deriv`` (box units) of one field
 template <typename T>
-__device__ __forceinline__ T interp(const T* __restrict__ F, const int64_t cidx[8], const T f[3], int deriv, const T cellinv[3]) {
+__device__ __forceinline__ void locate(const PosDec<T>& pd, const Mesh& m, const T* __restrict__ pos, int64_t p, Stencil<T>& st) {
+    const T* q = pos + 3 * p;
+    locate(pd, m, q[0], q[1], q[2], p, st);
+}
+
+// Pair of adjacent values at an element-aligned (not pair-aligned) address.
+template <typename T> struct alignas(sizeof(T)) Pair { T a, b; };
+
+// The 8 corner values: one two-element load per row (z-adjacent corners are
+// contiguous), plus a separate load of the wrapped corner when k = N2 - 1.
+template <typename T>
+__device__ __forceinline__ void gather8(const T* __restrict__ F, const Stencil<T>& st, int64_t n2, T v[8]) {
+    const bool wrap = st.k + 1 == n2;
+    const int64_t kk = wrap ? (n2 >= 2 ? n2 - 2 : 0) : st.k;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        if (n2 >= 2) {
+            const Pair<T> pr = *reinterpret_cast<const Pair<T>*>(F + st.row[r] + kk);
+            v[2 * r] = wrap ? pr.b : pr.a;
+            v[2 * r + 1] = pr.b;
+        } else {
+            v[2 * r] = v[2 * r + 1] = F[st.row[r]];
+        }
+    }
+    if (wrap && n2 >= 2) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[2 * r + 1] = F[st.row[r]];
+    }
+}
+
+// value (deriv < 0) or derivative along ``deriv`` (box units)
+template <typename T>
+__device__ __forceinline__ T interp(const T v[8], const T f[3], int deriv, const T cellinv[3]) {
     T w[3][2];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -161,18 +202,14 @@ __device__ __forceinline__ T interp(const T* __restrict__ F, const int64_t cidx[
     }
     T acc = T(0);
 #pragma unroll
-    for (int q = 0; q < 8; q++) acc += w[0][q >> 2] * w[1][(q >> 1) & 1] * w[2][q & 1] * F[cidx[q]];
+    for (int q = 0; q < 8; q++) acc += w[0][q >> 2] * w[1][(q >> 1) & 1] * w[2][q & 1] * v[q];
     return acc;
 }
 
-// value and full gradient of one field from the same 8 loads
+// value and full gradient from the same 8 corner values
 template <typename T>
-__device__ __forceinline__ void interp_vg(const T* __restrict__ F, const int64_t cidx[8], const T f[3], const T cellinv[3], T& val, T g[3]) {
-    T v[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) v[q] = F[cidx[q]];
+__device__ __forceinline__ void interp_vg(const T v[8], const T f[3], const T cellinv[3], T& val, T g[3]) {
     const T t[3] = {T(1) - f[0], T(1) - f[1], T(1) - f[2]};
-    // along z first
     T a00 = t[2] * v[0] + f[2] * v[1], a01 = t[2] * v[2] + f[2] * v[3];
     T a10 = t[2] * v[4] + f[2] * v[5], a11 = t[2] * v[6] + f[2] * v[7];
     T dz00 = v[1] - v[0], dz01 = v[3] - v[2], dz10 = v[5] - v[4], dz11 = v[7] - v[6];
@@ -187,10 +224,11 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_readout(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F,
                                                  Mesh m, int deriv, T* __restrict__ out) {
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c[8];
-        T f[3];
-        corners(pd, m, pos, p, c, f);
-        out[p] = interp(F, c, f, deriv, pd.cellinv);
+        Stencil<T> st;
+        locate(pd, m, pos, p, st);
+        T v[8];
+        gather8(F, st, m.n2, v);
+        out[p] = interp(v, st.f, deriv, pd.cellinv);
     }
 }
 
@@ -200,11 +238,13 @@ __global__ void __launch_bounds__(256) k_readout3(const T* __restrict__ pos, Pos
                                                   Mesh m, T* __restrict__ out) {
     const int64_t cells = m.cells();
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c[8];
-        T f[3];
-        corners(pd, m, pos, p, c, f);
+        Stencil<T> st;
+        locate(pd, m, pos, p, st);
+        T v[3][8];
 #pragma unroll
-        for (int d = 0; d < 3; d++) out[3 * p + d] = interp(F3 + d * cells, c, f, -1, pd.cellinv);
+        for (int d = 0; d < 3; d++) gather8(F3 + d * cells, st, m.n2, v[d]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) out[3 * p + d] = interp(v[d], st.f, -1, pd.cellinv);
     }
 }
 
@@ -213,11 +253,11 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_readout_grad(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F,
                                                       Mesh m, const T* __restrict__ pscale, T* __restrict__ out) {
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c[8];
-        T f[3];
-        corners(pd, m, pos, p, c, f);
-        T val, g[3];
-        interp_vg(F, c, f, pd.cellinv, val, g);
+        Stencil<T> st;
+        locate(pd, m, pos, p, st);
+        T v[8], val, g[3];
+        gather8(F, st, m.n2, v);
+        interp_vg(v, st.f, pd.cellinv, val, g);
         const T s = pscale ? pscale[p] : T(1);
 #pragma unroll
         for (int d = 0; d < 3; d++) out[3 * p + d] = g[d] * s;
@@ -230,14 +270,16 @@ __global__ void __launch_bounds__(256) k_readout_grad3(const T* __restrict__ pos
                                                        const T* __restrict__ v3, Mesh m, T* __restrict__ out) {
     const int64_t cells = m.cells();
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c[8];
-        T f[3];
-        corners(pd, m, pos, p, c, f);
+        Stencil<T> st;
+        locate(pd, m, pos, p, st);
+        T v[3][8];
+#pragma unroll
+        for (int d = 0; d < 3; d++) gather8(F3 + d * cells, st, m.n2, v[d]);
         T acc[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             T val, g[3];
-            interp_vg(F3 + d * cells, c, f, pd.cellinv, val, g);
+            interp_vg(v[d], st.f, pd.cellinv, val, g);
             const T w = v3[3 * p + d];
 #pragma unroll
             for (int e = 0; e < 3; e++) acc[e] += w * g[e];
@@ -419,18 +461,41 @@ __global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, 
     }
     __syncthreads();
     const unsigned j0 = offsets[t], j1 = offsets[t + 1];
-    for (unsigned j = j0 + threadIdx.x; j < j1; j += 256) {
-        const int64_t p = sorted[j];
+    // KB particles per thread per round; all their loads are issued before any
+    // deposit so the dependent sorted[j] -> pos[p] chains overlap
+    constexpr int KB = 4;
+    for (unsigned jb = j0; jb < j1; jb += 256 * KB) {
+        unsigned pi[KB];
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const unsigned j = jb + k * 256 + threadIdx.x;
+            pi[k] = sorted[j < j1 ? j : j1 - 1];
+        }
+        T xs[KB][3];
+        T ms[KB][NC];
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const T* q = pos + 3 * (int64_t)pi[k];
+            xs[k][0] = q[0];
+            xs[k][1] = q[1];
+            xs[k][2] = q[2];
+#pragma unroll
+            for (int c = 0; c < NC; c++) ms[k][c] = mass ? mass[(int64_t)pi[k] * NC + c] : mscalar;
+        }
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+        if (jb + k * 256 + threadIdx.x >= j1) continue;
+        const int64_t p = pi[k];
         int64_t idx[3];
         T f[3];
-        pd.decode(pos, p, idx, f);
+        pd.decode_xyz(xs[k][0], xs[k][1], xs[k][2], p, idx, f);
         const int a = (int)(idx[0] - lo[0]), b = (int)(idx[1] - lo[1]), cz = (int)(idx[2] - lo[2]);
         const double wx[2] = {(double)(T(1) - f[0]), (double)f[0]};
         const double wy[2] = {(double)(T(1) - f[1]), (double)f[1]};
         const double wz[2] = {(double)(T(1) - f[2]), (double)f[2]};
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const double mm = mass ? (double)mass[p * NC + c] : (double)mscalar;
+            const double mm = (double)ms[k][c];
             const long long tot = __double2ll_rn(ldexp(mm, s[c]));
             long long part = 0;
 #pragma unroll
@@ -448,6 +513,7 @@ __global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, 
                 }
                 if (v != 0) atomicAdd(&box[c * B3 + ((a + ax) * B + (b + by)) * B + (cz + zc)], (unsigned long long)v);
             }
+        }
         }
     }
     __syncthreads();

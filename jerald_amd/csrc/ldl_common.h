@@ -72,7 +72,12 @@ template <typename T> struct PosDec {
     // CIC lower corner (wrapped) and fractional offsets of particle p.
     __device__ __forceinline__ void decode(const T* __restrict__ pos, int64_t p, int64_t idx[3], T f[3]) const {
         const T* q = pos + 3 * p;
-        T x[3] = {q[0], q[1], q[2]};
+        decode_xyz(q[0], q[1], q[2], p, idx, f);
+    }
+
+    // Same, from the already loaded coordinates (x0, x1, x2) of particle p.
+    __device__ __forceinline__ void decode_xyz(T x0, T x1, T x2, int64_t p, int64_t idx[3], T f[3]) const {
+        T x[3] = {x0, x1, x2};
         int64_t base[3] = {0, 0, 0};
         if (kind == LDL_POS_PHYSICAL) {
 #pragma unroll

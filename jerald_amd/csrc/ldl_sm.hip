@@ -339,16 +339,15 @@ __global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ p
                                                          Mesh m, T s0, T s1, T s2, T* __restrict__ out) {
     const int64_t cells = m.cells();
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c[8];
-        T f[3];
-        corners(pd, m, pos, p, c, f);
-        const T v0 = interp(F3, c, f, -1, pd.cellinv);
-        const T v1 = interp(F3 + cells, c, f, -1, pd.cellinv);
-        const T v2 = interp(F3 + 2 * cells, c, f, -1, pd.cellinv);
         const T x0 = pos[3 * p], x1 = pos[3 * p + 1], x2 = pos[3 * p + 2];
-        out[3 * p] = x0 + v0 * s0;
-        out[3 * p + 1] = x1 + v1 * s1;
-        out[3 * p + 2] = x2 + v2 * s2;
+        Stencil<T> st;
+        locate(pd, m, x0, x1, x2, p, st);
+        T v[3][8];
+#pragma unroll
+        for (int d = 0; d < 3; d++) gather8(F3 + d * cells, st, m.n2, v[d]);
+        out[3 * p] = x0 + interp(v[0], st.f, -1, pd.cellinv) * s0;
+        out[3 * p + 1] = x1 + interp(v[1], st.f, -1, pd.cellinv) * s1;
+        out[3 * p + 2] = x2 + interp(v[2], st.f, -1, pd.cellinv) * s2;
     }
 }
 
@@ -359,19 +358,22 @@ __global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos
                                                        T* __restrict__ gin) {
     const int64_t cells = m.cells();
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c[8];
-        T f[3];
-        corners(pd, m, pos, p, c, f);
+        Stencil<T> st;
+        locate(pd, m, pos, p, st);
+        T v[4][8];
+#pragma unroll
+        for (int d = 0; d < 3; d++) gather8(F3 + d * cells, st, m.n2, v[d]);
+        gather8(grho, st, m.n2, v[3]);
         const T g[3] = {gout[3 * p], gout[3 * p + 1], gout[3 * p + 2]};
         T acc[3] = {g[0], g[1], g[2]};
         T val, gr[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            interp_vg(F3 + d * cells, c, f, pd.cellinv, val, gr);
+            interp_vg(v[d], st.f, pd.cellinv, val, gr);
 #pragma unroll
             for (int e = 0; e < 3; e++) acc[e] += g[d] * gr[e];
         }
-        interp_vg(grho, c, f, pd.cellinv, val, gr);
+        interp_vg(v[3], st.f, pd.cellinv, val, gr);
 #pragma unroll
         for (int e = 0; e < 3; e++) gin[3 * p + e] = acc[e] + fact * gr[e];
     }
@@ -382,11 +384,11 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ grho,
                                                        Mesh m, T fact, T* __restrict__ g) {
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c[8];
-        T f[3];
-        corners(pd, m, pos, p, c, f);
-        T val, gr[3];
-        interp_vg(grho, c, f, pd.cellinv, val, gr);
+        Stencil<T> st;
+        locate(pd, m, pos, p, st);
+        T v[8], val, gr[3];
+        gather8(grho, st, m.n2, v);
+        interp_vg(v, st.f, pd.cellinv, val, gr);
 #pragma unroll
         for (int e = 0; e < 3; e++) g[3 * p + e] = fact * gr[e];
     }
