@@ -9,12 +9,46 @@
 namespace ldl {
 
 // ---------------------------------------------------------------- helpers
+// Fixed-point exponent s (deposits are rint(w m 2^s)), from max |m| < 2^e:
+//  * FP32 path: s = 30 - e, so every deposit fits in 31 bits and converts with
+//    the native double -> int32 instruction; a cell (or the total) stays below
+//    Np 2^30 < 2^62 for Np < 2^32;
+//  * FP64 path: s = 61 - e - ceil(log2 Np), the largest scale for which the
+//    total stays below 2^61.
+template <typename T>
 __device__ __forceinline__ int fixed_shift(unsigned long long maxbits, int64_t np) {
     double mx = __longlong_as_double((long long)maxbits);
     if (mx == 0.0 || np <= 0) return 0;
     int e = ilogb(mx) + 1;  // mx < 2^e (frexp exponent)
+    if (sizeof(T) == 4) return 30 - e;
     int lg = np > 1 ? 64 - __clzll((unsigned long long)(np - 1)) : 0;
     return 61 - e - lg;
+}
+
+// The 8 fixed-point CIC deposits of one particle with fractions f and mass
+// m at scale 2^s: q[4a+2b+c] = rint((w_x[a] w_y[b]) (w_z[c] (m 2^s))), and
+// q[7] = rint(m 2^s) - (q[0] + ... + q[6]) so the deposits sum exactly to
+// the rounded mass.  The CPU emulation (oracle/cic_oracle.c) performs the
+// same operations in the same order.
+template <typename T>
+__device__ __forceinline__ void deposits(const T f[3], double m, int s, long long q[8]) {
+    const double wx[2] = {(double)(T(1) - f[0]), (double)f[0]};
+    const double wy[2] = {(double)(T(1) - f[1]), (double)f[1]};
+    const double ms = ldexp(m, s);
+    const double wzm[2] = {(double)(T(1) - f[2]) * ms, (double)f[2] * ms};
+    const double wxy[4] = {wx[0] * wy[0], wx[0] * wy[1], wx[1] * wy[0], wx[1] * wy[1]};
+    long long part = 0;
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        const double w = wxy[k >> 1] * wzm[k & 1];
+        long long v;
+        if (sizeof(T) == 4) v = (long long)__double2int_rn(w);  // |w| < 2^30
+        else v = __double2ll_rn(w);
+        q[k] = v;
+        part += v;
+    }
+    const long long tot = sizeof(T) == 4 ? (long long)__double2int_rn(ms) : __double2ll_rn(ms);
+    q[7] = tot - part;
 }
 
 static __global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n, unsigned long long v) {
@@ -56,40 +90,24 @@ __global__ void __launch_bounds__(256) k_paint_fixed(const T* __restrict__ pos, 
                                                      const unsigned long long* __restrict__ hdr) {
     int s[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) s[c] = fixed_shift(hdr[c], np);
+    for (int c = 0; c < NC; c++) s[c] = fixed_shift<T>(hdr[c], np);
     const int64_t cells = m.cells();
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
         int64_t idx[3];
         T f[3];
         pd.decode(pos, p, idx, f);
-        const double wx[2] = {(double)(T(1) - f[0]), (double)f[0]};
-        const double wy[2] = {(double)(T(1) - f[1]), (double)f[1]};
-        const double wz[2] = {(double)(T(1) - f[2]), (double)f[2]};
         const int64_t ii[2] = {idx[0], idx[0] + 1 == m.n0 ? 0 : idx[0] + 1};
         const int64_t jj[2] = {idx[1], idx[1] + 1 == m.n1 ? 0 : idx[1] + 1};
         const int64_t kk[2] = {idx[2], idx[2] + 1 == m.n2 ? 0 : idx[2] + 1};
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             const double mm = mass ? (double)mass[p * NC + c] : (double)mscalar;
-            const long long tot = __double2ll_rn(ldexp(mm, s[c]));
-            long long part = 0;
+            long long q[8];
+            deposits<T>(f, mm, s[c], q);
             unsigned long long* a = acc + c * cells;
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int ax = q >> 2, by = (q >> 1) & 1, cz = q & 1;
-                long long v;
-                if (q == 7) {
-                    v = tot - part;
-                } else {
-                    // ((w_x * w_y) * w_z) * m, the same order as the CPU emulation
-                    double w = wx[ax] * wy[by];
-                    w = w * wz[cz];
-                    w = w * mm;
-                    v = __double2ll_rn(ldexp(w, s[c]));
-                    part += v;
-                }
-                if (v != 0) atomicAdd(&a[(ii[ax] * m.n1 + jj[by]) * m.n2 + kk[cz]], (unsigned long long)v);
-            }
+            for (int k = 0; k < 8; k++)
+                if (q[k] != 0) atomicAdd(&a[(ii[k >> 2] * m.n1 + jj[(k >> 1) & 1]) * m.n2 + kk[k & 1]], (unsigned long long)q[k]);
         }
     }
 }
@@ -127,7 +145,7 @@ __global__ void __launch_bounds__(256) k_fixed_to_float(unsigned long long* __re
                                                         int64_t cells, double scale, T* __restrict__ out) {
     double f[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) f[c] = ldexp(scale, -fixed_shift(hdr[c], np));
+    for (int c = 0; c < NC; c++) f[c] = ldexp(scale, -fixed_shift<T>(hdr[c], np));
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cells; i += (int64_t)gridDim.x * blockDim.x) {
 #pragma unroll
         for (int c = 0; c < NC; c++) {
@@ -456,7 +474,7 @@ __global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, 
     double fs[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-        s[c] = fixed_shift(hdr[c], np);
+        s[c] = fixed_shift<T>(hdr[c], np);
         fs[c] = ldexp(scale, -s[c]);
     }
     __syncthreads();
@@ -490,29 +508,14 @@ __global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, 
         T f[3];
         pd.decode_xyz(xs[k][0], xs[k][1], xs[k][2], p, idx, f);
         const int a = (int)(idx[0] - lo[0]), b = (int)(idx[1] - lo[1]), cz = (int)(idx[2] - lo[2]);
-        const double wx[2] = {(double)(T(1) - f[0]), (double)f[0]};
-        const double wy[2] = {(double)(T(1) - f[1]), (double)f[1]};
-        const double wz[2] = {(double)(T(1) - f[2]), (double)f[2]};
+        const int base = (a * B + b) * B + cz;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const double mm = (double)ms[k][c];
-            const long long tot = __double2ll_rn(ldexp(mm, s[c]));
-            long long part = 0;
+            long long q[8];
+            deposits<T>(f, (double)ms[k][c], s[c], q);
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int ax = q >> 2, by = (q >> 1) & 1, zc = q & 1;
-                long long v;
-                if (q == 7) {
-                    v = tot - part;
-                } else {
-                    double w = wx[ax] * wy[by];
-                    w = w * wz[zc];
-                    w = w * mm;
-                    v = __double2ll_rn(ldexp(w, s[c]));
-                    part += v;
-                }
-                if (v != 0) atomicAdd(&box[c * B3 + ((a + ax) * B + (b + by)) * B + (cz + zc)], (unsigned long long)v);
-            }
+            for (int qq = 0; qq < 8; qq++)
+                if (q[qq] != 0) atomicAdd(&box[c * B3 + base + (qq >> 2) * B * B + ((qq >> 1) & 1) * B + (qq & 1)], (unsigned long long)q[qq]);
         }
         }
     }
@@ -551,7 +554,7 @@ __global__ void __launch_bounds__(256) k_tiles_fixup(const unsigned long long* _
                                                      TileGeom tg, double scale, T* __restrict__ out) {
     double f[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) f[c] = ldexp(scale, -fixed_shift(hdr[c], np));
+    for (int c = 0; c < NC; c++) f[c] = ldexp(scale, -fixed_shift<T>(hdr[c], np));
     const int64_t cells = m.cells();
     const int64_t total = tg.ntiles * tg.lstride;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {

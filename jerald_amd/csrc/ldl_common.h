@@ -54,6 +54,9 @@ struct Mesh {
 };
 
 __host__ __device__ __forceinline__ int64_t wrapi(int64_t i, int64_t n) {
+    if ((uint64_t)i < (uint64_t)n) return i;  // common case: already in range
+    if (i == n) return 0;
+    if (i == -1) return n - 1;
     int64_t r = i % n;
     return r < 0 ? r + n : r;
 }
@@ -66,6 +69,7 @@ template <typename T> struct PosDec {
     T cell[3];     // box / N in T (reference: BoxSize / Nmesh in the working dtype)
     T cellinv[3];  // N / box in T (reference: Nmesh[d] / BoxSize[d], lib/parops_fast.h:748-750)
     uint32_t l12, l2;  // lattice strides (Np < 2^32 for the lattice form)
+    int sh12, sh2;     // log2 of the strides when they are powers of two, else -1
     double lstep[3];
     int64_t n[3];
 
@@ -82,18 +86,29 @@ template <typename T> struct PosDec {
         if (kind == LDL_POS_PHYSICAL) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                // numpy remainder semantics, then division (jaxpops_bind.py:163-164)
-                T r = fmod(x[d], box[d]);
-                if (r != T(0) && ((r < T(0)) != (box[d] < T(0)))) r += box[d];
+                // numpy remainder semantics, then division (jaxpops_bind.py:163-164);
+                // fmod is the identity on [0, L)
+                T r = x[d];
+                if (!(r >= T(0) && r < box[d])) {
+                    r = fmod(r, box[d]);
+                    if (r != T(0) && ((r < T(0)) != (box[d] < T(0)))) r += box[d];
+                }
                 x[d] = r / cell[d];
             }
         } else {
             uint32_t pp = (uint32_t)p;
             uint32_t li[3];
-            li[0] = pp / l12;
-            uint32_t rem = pp - li[0] * l12;
-            li[1] = rem / l2;
-            li[2] = rem - li[1] * l2;
+            if (sh12 >= 0 && sh2 >= 0) {
+                li[0] = pp >> sh12;
+                const uint32_t rem = pp & (l12 - 1u);
+                li[1] = rem >> sh2;
+                li[2] = rem & (l2 - 1u);
+            } else {
+                li[0] = pp / l12;
+                const uint32_t rem = pp - li[0] * l12;
+                li[1] = rem / l2;
+                li[2] = rem - li[1] * l2;
+            }
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 double b = (double)li[d] * lstep[d];
@@ -126,6 +141,7 @@ inline int make_posdec(const ldl_pos_desc* pd, const int64_t nmesh[3], int64_t n
         out->lstep[d] = pd->lattice_step[d];
     }
     out->l12 = out->l2 = 1;
+    out->sh12 = out->sh2 = 0;
     if (pd->kind == LDL_POS_LATTICE) {
         int64_t l0 = pd->lattice[0], l1 = pd->lattice[1], l2 = pd->lattice[2];
         if (l0 <= 0 || l1 <= 0 || l2 <= 0) return fail(LDL_EINVAL, "lattice sizes must be positive");
@@ -133,6 +149,9 @@ inline int make_posdec(const ldl_pos_desc* pd, const int64_t nmesh[3], int64_t n
         if (np > (int64_t)0xFFFFFFFFll) return fail(LDL_EUNSUPPORTED, "lattice form supports np < 2^32");
         out->l12 = (uint32_t)(l1 * l2);
         out->l2 = (uint32_t)l2;
+        auto lg = [](uint32_t v) { int k = 0; while ((1u << k) < v) k++; return (1u << k) == v ? k : -1; };
+        out->sh12 = lg(out->l12);
+        out->sh2 = lg(out->l2);
         for (int d = 0; d < 3; d++)
             if (!(pd->lattice_step[d] > 0.0)) return fail(LDL_EINVAL, "lattice_step must be positive");
     }

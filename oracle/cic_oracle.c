@@ -110,25 +110,27 @@ void oracle_mesh_units_f64(const double* pos, int64_t np_, const double* box, co
 }
 
 /* ---------------- fixed-point paint emulation ----------------
- * Per particle: CIC fractions in the working precision T, the 8 weights in
- * double as ((w_x * w_y) * w_z) * m, each deposit q = rint(w * 2^s) as int64;
- * the (1,1,1) corner takes rint(m * 2^s) - sum(other 7) so the mass is
- * conserved exactly.  The accumulator is summed in int64 (exact, order free).
+ * Per particle: CIC fractions in the working precision T, then in double
+ * ms = m 2^s, wzm[c] = w_z[c] ms, wxy[2a+b] = w_x[a] w_y[b] and the deposit of
+ * corner k = 4a+2b+c is rint(wxy[k>>1] * wzm[k&1]) for k < 7; corner 7 takes
+ * rint(ms) minus the other seven so the mass is conserved exactly.  The
+ * accumulator is summed in int64 (exact, order free).  s comes from the
+ * caller (FP32: 30 - e, FP64: 61 - e - ceil(log2 Np), with max|m| < 2^e).
  * kind 0: pos physical (wrapped and divided in T);  kind 1: pos is a
  * displacement in mesh units from the lattice point idx*step.            */
 #define DEPOSIT_BODY                                                               \
-    double wx[2] = {(double)tx, (double)fx_}, wy[2] = {(double)ty, (double)fy_},   \
-           wz[2] = {(double)tz, (double)fz_};                                      \
+    double wx[2] = {(double)tx, (double)fx_}, wy[2] = {(double)ty, (double)fy_};   \
+    double ms = ldexp(m, s);                                                       \
+    double wzm[2] = {(double)tz * ms, (double)fz_ * ms};                           \
+    double wxy[4] = {wx[0] * wy[0], wx[0] * wy[1], wx[1] * wy[0], wx[1] * wy[1]};   \
     int64_t ii[2] = {i, (i + 1) % n0}, jj[2] = {j, (j + 1) % n1}, kk[2] = {k, (k + 1) % n2}; \
-    int64_t tot = llrint(ldexp(m, s)), part = 0;                                   \
-    for (int a = 0; a < 2; a++)                                                    \
-        for (int b = 0; b < 2; b++)                                                \
-            for (int c = 0; c < 2; c++) {                                          \
-                int64_t q;                                                         \
-                if (a & b & c) q = tot - part;                                     \
-                else { q = llrint(ldexp(wx[a] * wy[b] * wz[c] * m, s)); part += q; } \
-                acc[(ii[a] * n1 + jj[b]) * n2 + kk[c]] += q;                       \
-            }
+    int64_t part = 0;                                                              \
+    for (int q = 0; q < 8; q++) {                                                  \
+        int64_t v;                                                                 \
+        if (q == 7) v = llrint(ms) - part;                                         \
+        else { v = llrint(wxy[q >> 1] * wzm[q & 1]); part += v; }                   \
+        acc[(ii[q >> 2] * n1 + jj[(q >> 1) & 1]) * n2 + kk[q & 1]] += v;           \
+    }
 
 void oracle_paint_fixed_f32(const float* pos, int64_t np_, int kind, const double* boxd,
                             const int64_t* lat, const double* lstep, const float* mass, double mscalar,

@@ -83,13 +83,15 @@ def readout_grad_f64(x, field, cellinv, nthreads=0):
     return np.stack([readout_f64(x, field, d, cellinv, nthreads) for d in range(3)], axis=-1)
 
 
-def fixed_point_shift(np_, maxabs):
-    """Fixed-point exponent s used by the paint: the sum of |m| over all
-    particles is below 2^(ceil(log2 Np) + e) with maxabs < 2^e, so every cell
-    (and the total) stays below 2^61 * 2^-s."""
+def fixed_point_shift(np_, maxabs, dtype=np.float32):
+    """Fixed-point exponent s of the paint (deposits are rint(w m 2^s)), with
+    max|m| < 2^e: FP32 s = 30 - e (every deposit fits in 31 bits); FP64
+    s = 61 - e - ceil(log2 Np) (the total stays below 2^61)."""
     if maxabs == 0.0 or np_ == 0:
         return 0
     e = math.frexp(maxabs)[1]
+    if np.dtype(dtype) == np.float32:
+        return 30 - e
     lg = int(np_ - 1).bit_length()
     return 61 - e - lg
 
@@ -109,7 +111,7 @@ def paint_fixed(pos, mass, nmesh, box=None, lattice=None, lattice_step=None, s=N
         m = np.ascontiguousarray(mass, dtype=dtype)
         maxabs = float(np.max(np.abs(m))) if np_ else 0.0
     if s is None:
-        s = fixed_point_shift(np_, maxabs)
+        s = fixed_point_shift(np_, maxabs, dtype)
     acc = np.empty(nm, dtype=np.int64)
     kind = 0 if lattice is None else 1
     boxa = np.ascontiguousarray(np.broadcast_to(np.asarray(box if box is not None else 1.0, np.float64), (3,)))

@@ -136,4 +136,7 @@ def test_fixed_point_emulation_close_to_f64():
     np.testing.assert_allclose(NV.fixed_to_float(acc, s, np.float64), ref, atol=2.0 ** -s * 8)
     # lattice + displacement form gives the same deposit as absolute positions
     lat = NV.paint_fixed(np.zeros((512, 3), np.float32), 1.0, nm, lattice=(8, 8, 8), lattice_step=(1, 1, 1))[0]
-    assert np.all(lat == round(2.0 ** NV.fixed_point_shift(512, 1.0)))
+    assert np.all(lat == round(2.0 ** NV.fixed_point_shift(512, 1.0, np.float32)))
+    # FP32 deposits stay below 2^30; FP64 keeps the total below 2^61
+    assert NV.fixed_point_shift(10 ** 6, 1.0, np.float32) == 29
+    assert NV.fixed_point_shift(10 ** 6, 1.0, np.float64) == 61 - 1 - 20
