@@ -27,7 +27,9 @@ namespace ldl {
 constexpr int radix_of(int rem) { return rem == 2 ? 2 : (rem == 4 || rem == 16) ? 4 : 8; }
 
 template <typename T, int N> struct FftCfg {
-    static constexpr int TPP = N <= 8 ? 1 : N <= 32 ? 4 : N == 64 ? 8 : N <= 512 ? 16 : 32;
+    // threads per pencil: each thread holds N / TPP values per stage (16 at
+    // N = 512), which keeps the fused kernels below 128 VGPRs
+    static constexpr int TPP = N <= 8 ? 1 : N <= 32 ? 4 : N == 64 ? 8 : N == 128 ? 16 : N <= 512 ? 32 : 64;
     // FP64 at N = 1024 keeps 4 pencils so data + twiddles fit the 160 KB of LDS
     static constexpr int P = sizeof(T) == 4 ? (N <= 8 ? 64 : N <= 64 ? 32 : 16)
                                             : (N <= 8 ? 32 : N <= 64 ? 16 : N <= 512 ? 8 : 4);

@@ -203,7 +203,7 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
             pk[s] = C{u.x * a1e2 * G, u.y * a1e2 * G};
         }
     }
-#pragma unroll
+#pragma unroll 1
     for (int d = 0; d < 3; d++) {
 #pragma unroll
         for (int s = 0; s < EPT; s++) {
@@ -242,7 +242,7 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
     C q[EPT];
 #pragma unroll
     for (int s = 0; s < EPT; s++) q[s] = C{T(0), T(0)};
-#pragma unroll
+#pragma unroll 1
     for (int d = 0; d < 3; d++) {
         const C* src = S3 + d * hcells + base;
         tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return src[n * g.es + p]; });
