@@ -81,18 +81,35 @@ def params0(nstep):
 
 # --------------------------------------------------------- algorithmic bytes
 def kernel_bytes(name, n, b):
-    """Compulsory HBM bytes of one launch at n^3 particles / n^3 mesh, b bytes
-    per real.  Np = Nm = n^3, Nh = n^2 (n/2+1)."""
-    Np = n ** 3
-    Nm = n ** 3
+    """Algorithmic (compulsory) HBM bytes of one launch of an instrumented
+    kernel at n^3 particles on an n^3 mesh, b bytes per real.  Np = Nm = n^3,
+    Nh = n^2 (n/2+1) complex values of 2b bytes.  The tile deposit kernel
+    carries the bytes of the whole paint operation (positions and masses in,
+    mesh out); its binning and face-fixup kernels are reported separately."""
+    Np = Nm = n ** 3
     Nh = n * n * (n // 2 + 1)
     return {
-        "readout3_update": 3 * b * Np + 3 * b * Nm + 3 * b * Np,        # pos in, 3 fields, pos out
-        "sm_bwd_gather": 6 * b * Np + 4 * b * Nm + 3 * b * Np,          # pos + g in, 4 fields, g out
-        "paint_deposit": 3 * b * Np + 8 * Nm * 1,                       # pos in, int64 mesh once
-        "sm_x_fwd": 2 * b * Nh + 2 * b * Nh + 3 * 2 * b * Nh,           # U in, U out, 3 spectra out
-        "sm_x_bwd": 3 * 2 * b * Nh + 2 * b * Nh + 2 * b * Nh,           # 3 spectra + U in, 1 out
+        "readout3_update": 6 * b * Np + 3 * b * Nm,      # pos in + out, 3 fields
+        "sm_bwd_gather": 9 * b * Np + 4 * b * Nm,        # pos, g_out in, g_in out, 4 fields
+        "paint_vjp_pos": 6 * b * Np + b * Nm,            # pos in, grad out, 1 field
+        "paint_deposit1": 3 * b * Np + b * Nm,           # unit-mass paint
+        "paint_deposit3": 6 * b * Np + 3 * b * Nm,       # (Np,3) masses -> 3 meshes
+        "zpass_r2c": b * Nm + 2 * b * Nh,
+        "zpass_r2c3": 3 * (b * Nm + 2 * b * Nh),
+        "zpass_c2r": b * Nm + 2 * b * Nh,
+        "zpass_c2r3": 3 * (b * Nm + 2 * b * Nh),
+        "colpass": 4 * b * Nh,
+        "colpass3": 12 * b * Nh,
+        "sm_x_fwd": 10 * b * Nh,                         # in, U out, 3 spectra out
+        "sm_x_bwd": 10 * b * Nh,                         # 3 spectra + U in, 1 out
+        "x_realfilter": 4 * b * Nh,
     }.get(name)
+
+
+KERNELS = ("paint_bin", "paint_scatter", "paint_deposit1", "paint_deposit3", "paint_fixup1", "paint_fixup3",
+           "paint_deposit", "paint_convert", "zpass_r2c", "zpass_r2c3", "zpass_c2r", "zpass_c2r3", "colpass",
+           "colpass3", "sm_x_fwd", "sm_x_bwd", "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
+           "loss")
 
 
 def time_steps(model, p, steps, warmup):
@@ -151,8 +168,7 @@ def main():
     torch.cuda.synchronize()
     dt32 = time.perf_counter() - t0
     kern = {}
-    for name in ("paint_bin", "paint_scatter", "paint_deposit", "paint_convert", "zpass_r2c", "zpass_c2r", "colpass", "sm_x_fwd", "sm_x_bwd",
-                 "readout3_update", "sm_bwd_gather", "paint_vjp_pos", "loss", "x_realfilter"):
+    for name in KERNELS:
         tot, cnt = L.timer_read(name)
         if cnt:
             kern[name] = (tot, cnt)
@@ -161,7 +177,7 @@ def main():
     grad32 = grad32.cpu().numpy()
     ms = 1e3 * dt32 / args.steps
     log(f"[bench] FP32: {ms:.2f} ms/step, loss {loss32:.8g}")
-    # dominant instrumented kernel with a defined algorithmic byte count
+    # roofline of the dominant instrumented kernel with a defined byte count
     total_k = sum(t for t, _ in kern.values())
     ranked = sorted(kern.items(), key=lambda kv: -kv[1][0])
     dom = next((k for k, _ in ranked if kernel_bytes(k, n, 4) is not None), None)
@@ -174,8 +190,14 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 4),
                 "algorithmic_bytes": by, "share_of_kernel_time": round(tot / total_k, 3)}
-    kernel_table = {k: {"ms_per_step": round(t / args.steps, 3), "launches_per_step": c / args.steps}
-                    for k, (t, c) in ranked}
+    kernel_table = {}
+    for k, (t, c) in ranked:
+        row = {"ms_per_step": round(t / args.steps, 3), "launches_per_step": c / args.steps,
+               "avg_ms": round(t / c, 4)}
+        by = kernel_bytes(k, n, 4)
+        if by is not None:
+            row["GBps_alg"] = round(by / (t / c / 1e3) / 1e9, 1)
+        kernel_table[k] = row
     # plain FP64 GPU path: FP64 absolute positions, FP64 atomics, same kernels
     fp64 = None
     if args.fp64_steps > 0:

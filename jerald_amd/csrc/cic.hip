@@ -129,16 +129,16 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     }
     auto* low = acc;
     if (nc == 1) {
-        KTimer _t("paint_deposit", st);
+        KTimer _t("paint_deposit1", st);
         hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide1>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, tg,
                            offsets, sorted, low, halo, hdr, out_scale, out);
     } else {
-        KTimer _t("paint_deposit", st);
+        KTimer _t("paint_deposit3", st);
         hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide3>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, tg,
                            offsets, sorted, low, halo, hdr, out_scale, out);
     }
     {
-        KTimer _t("paint_convert", st);
+        KTimer _t(nc == 1 ? "paint_fixup1" : "paint_fixup3", st);
         const unsigned gf = grid_for(tg.ntiles * tg.lstride, kBlock, 1 << 16);
         if (nc == 1)
             hipLaunchKernelGGL((k_tiles_fixup<T, 1>), dim3(gf), dim3(kBlock), 0, st, low, halo, hdr, np, m, tg, out_scale, out);

@@ -423,23 +423,25 @@ template <int MAXN = 1024, class F> inline bool dispatch_n(int64_t n, F&& f) {
 template <class F> inline bool dispatch_h(int64_t h, F&& f) { return dispatch_n<512>(h, f); }
 
 template <typename T, int N, int DIR, class Op>
-inline void launch_colpass(const ColGeom& g, const c2<T>* tw, const Op& op, hipStream_t st) {
-    KTimer _t("colpass", st);
+inline void launch_colpass(const ColGeom& g, const c2<T>* tw, const Op& op, hipStream_t st, const char* tname = "colpass") {
+    KTimer _t(tname, st);
     hipLaunchKernelGGL((k_colpass<T, N, DIR, Op>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, N>::THREADS), 0, st, g, tw, op);
 }
 
 template <typename T, int H, class Pro, class Epi>
-inline void launch_zr2c(int64_t nrows, const c2<T>* tw, const Pro& pro, const Epi& epi, hipStream_t st) {
+inline void launch_zr2c(int64_t nrows, const c2<T>* tw, const Pro& pro, const Epi& epi, hipStream_t st,
+                        const char* tname = "zpass_r2c") {
     constexpr int P = FftCfg<T, H>::P;
-    KTimer _t("zpass_r2c", st);
+    KTimer _t(tname, st);
     hipLaunchKernelGGL((k_zpass_r2c<T, H, Pro, Epi>), dim3((unsigned)((nrows + P - 1) / P)), dim3(FftCfg<T, H>::THREADS), 0, st,
                        nrows, tw, pro, epi);
 }
 
 template <typename T, int H, class Pro, class Epi>
-inline void launch_zc2r(int64_t nrows, const c2<T>* tw, const Pro& pro, const Epi& epi, hipStream_t st) {
+inline void launch_zc2r(int64_t nrows, const c2<T>* tw, const Pro& pro, const Epi& epi, hipStream_t st,
+                        const char* tname = "zpass_c2r") {
     constexpr int P = FftCfg<T, H>::P;
-    KTimer _t("zpass_c2r", st);
+    KTimer _t(tname, st);
     hipLaunchKernelGGL((k_zpass_c2r<T, H, Pro, Epi>), dim3((unsigned)((nrows + P - 1) / P)), dim3(FftCfg<T, H>::THREADS), 0, st,
                        nrows, tw, pro, epi);
 }

@@ -609,10 +609,11 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     // 4. inverse y passes (3 spectra as 3*N0 planes) and z c2r into F
     dispatch_n(m.n1, [&](auto N) {
         ColGeom g = col_geom<T, decltype(N)::value>(3 * m.n0, nh, m.n1 * nh, nh);
-        launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st);
+        launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3");
     });
     dispatch_h(m.n2 / 2, [&](auto H) {
-        launch_zc2r<T, decltype(H)::value>(3 * nrows, tw, HalfRowLoad<T>{P.spec3, nh}, RealRowStore<T>{F, m.n2, T(1)}, st);
+        launch_zc2r<T, decltype(H)::value>(3 * nrows, tw, HalfRowLoad<T>{P.spec3, nh}, RealRowStore<T>{F, m.n2, T(1)}, st,
+                                           "zpass_c2r3");
     });
     // 5. readout + position update
     T s[3];
@@ -645,11 +646,12 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
     // 2. unnormalised r2c of the three meshes: z and y passes
     dispatch_h(m.n2 / 2, [&](auto H) {
-        launch_zr2c<T, decltype(H)::value>(3 * nrows, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore<T>{P.spec3, nh, T(1)}, st);
+        launch_zr2c<T, decltype(H)::value>(3 * nrows, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore<T>{P.spec3, nh, T(1)}, st,
+                                           "zpass_r2c3");
     });
     dispatch_n(m.n1, [&](auto N) {
         ColGeom g = col_geom<T, decltype(N)::value>(3 * m.n0, nh, m.n1 * nh, nh);
-        launch_colpass<T, decltype(N)::value, -1>(g, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st);
+        launch_colpass<T, decltype(N)::value, -1>(g, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3");
     });
     // 3. fused x pass: Q, parameter partials, inverse x of 1e-2 alpha G Q
     const SmFilter<T> fl{(T)(prm[2] * prm[2]), (T)(1.0 / (prm[3] * prm[3])), (T)prm[4]};
