@@ -162,20 +162,27 @@ int ldl_pifft_f64(const double* in, double* out, const int64_t nmesh[3], double 
  * Positions are LDL_POS_LATTICE displacements (mesh units) or physical. */
 size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np);
 
+/* Size of a layer's particle sort (tile offsets + sorted indices, uint32): the
+ * forward's density paint can record it and the backward's 3-component paint
+ * of the same positions reuse it instead of sorting again. */
+size_t ldl_sm_bins_bytes(const int64_t nmesh[3], int64_t np);
+
 /* Forward of one layer.  Inputs pos (Np,3).  Writes pos_out (Np,3) = pos +
  * s / cell (same kind as pos; may alias pos), and the saved tensors the
  * backward needs: rho (N0,N1,N2), U (N0,N1,N2/2+1 complex, = r2c of
- * (rho+1e-8)^gamma), forces F (3,N0,N1,N2). */
+ * (rho+1e-8)^gamma), forces F (3,N0,N1,N2), and -- if ``bins`` is not NULL
+ * (ldl_sm_bins_bytes) -- the particle sort of the fixed-point paint. */
 int ldl_sm_layer_fwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[5], double fact, int32_t paint_flags,
-                         float* pos_out, float* rho, float* U, float* F,
+                         float* pos_out, float* rho, float* U, float* F, void* bins,
                          void* workspace, size_t workspace_bytes, void* stream);
 int ldl_sm_layer_fwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[5], double fact, int32_t paint_flags,
-                         double* pos_out, double* rho, double* U, double* F,
+                         double* pos_out, double* rho, double* U, double* F, void* bins,
                          void* workspace, size_t workspace_bytes, void* stream);
 
-/* Backward of one layer.  g_out (Np,3) = dL/dX_out (box units).  Writes
+/* Backward of one layer.  ``bins``: the forward's sort of the same positions
+ * or NULL.  g_out (Np,3) = dL/dX_out (box units).  Writes
  * g_in (Np,3) = dL/dX_in (may alias g_out) and ACCUMULATES dL/d{alpha,
  * gamma, kh, kl, nu} into grad5 (DEVICE double[5]).  ``alpha_rule`` 0 = exact
  * gradient, 1 = the reference's rule (interior k_z modes weighted 1 instead
@@ -183,12 +190,12 @@ int ldl_sm_layer_fwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
 int ldl_sm_layer_bwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[5], double fact, int32_t alpha_rule,
                          int32_t paint_flags, const float* rho, const float* U, const float* F,
-                         const float* g_out, float* g_in, double* grad5,
+                         const void* bins, const float* g_out, float* g_in, double* grad5,
                          void* workspace, size_t workspace_bytes, void* stream);
 int ldl_sm_layer_bwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[5], double fact, int32_t alpha_rule,
                          int32_t paint_flags, const double* rho, const double* U, const double* F,
-                         const double* g_out, double* g_in, double* grad5,
+                         const void* bins, const double* g_out, double* g_in, double* grad5,
                          void* workspace, size_t workspace_bytes, void* stream);
 
 /* Response + loss + their gradient (model.py:338-344, :350-403 / :415-468).

@@ -53,6 +53,7 @@ _SIGS = {
     "ldl_paint_workspace_bytes": [_I64x3, _I32, _I64],
     "ldl_fft_workspace_bytes": [_I64x3, _I32],
     "ldl_sm_workspace_bytes": [_I64x3, _I32, _I64],
+    "ldl_sm_bins_bytes": [_I64x3, _I64],
     "ldl_timer_enable": [_I32],
     "ldl_timer_read": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
 }
@@ -65,8 +66,8 @@ for _t in ("f32", "f64"):
         f"ldl_readout_grad3_{_t}": [_P, _PD, _I64, _P, _P, _I64x3, _P, _P],
         f"ldl_pfft_{_t}": [_P, _P, _I64x3, _D, _P],
         f"ldl_pifft_{_t}": [_P, _P, _I64x3, _D, _P, _SZ, _P],
-        f"ldl_sm_layer_fwd_{_t}": [_P, _PD, _I64, _I64x3, _Dp, _D, _I32, _P, _P, _P, _P, _P, _SZ, _P],
-        f"ldl_sm_layer_bwd_{_t}": [_P, _PD, _I64, _I64x3, _Dp, _D, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],
+        f"ldl_sm_layer_fwd_{_t}": [_P, _PD, _I64, _I64x3, _Dp, _D, _I32, _P, _P, _P, _P, _P, _P, _SZ, _P],
+        f"ldl_sm_layer_bwd_{_t}": [_P, _PD, _I64, _I64x3, _Dp, _D, _I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _SZ, _P],
         f"ldl_sm_loss_{_t}": [_P, _P, _P, _P, _P, _I64x3, _Dp, _Dp, _D, _I32, _I32, _D, _P, _P, _P, _SZ, _P],
         f"ldl_paint_vjp_pos_{_t}": [_P, _PD, _I64, _P, _I64x3, _D, _P, _P],
     })
@@ -75,6 +76,7 @@ _RESTYPE = {
     "ldl_paint_workspace_bytes": C.c_size_t,
     "ldl_fft_workspace_bytes": C.c_size_t,
     "ldl_sm_workspace_bytes": C.c_size_t,
+    "ldl_sm_bins_bytes": C.c_size_t,
 }
 
 EXPORTED = sorted(_SIGS)

@@ -457,8 +457,8 @@ __device__ __forceinline__ void low_cell(int i, int ex, int ey, int ez, int& a, 
 // neighbour contributes) are final and written as FP values; low-face cells
 // go to the ``low`` record and the +1 faces to the ``halo`` record, both as
 // integers, for k_tiles_fixup.
-template <typename T, int NC, int TS>
-__global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ mass,
+template <typename T, int NC, int TS, int TB = 256>
+__global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ mass,
                                                      T mscalar, Mesh m, TileGeom tg, const unsigned* __restrict__ offsets,
                                                      const unsigned* __restrict__ sorted, unsigned long long* __restrict__ low,
                                                      unsigned long long* __restrict__ halo, const unsigned long long* __restrict__ hdr,
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, 
     const int64_t tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
     const int64_t lo[3] = {tx * TS, ty * TS, tz * TS};
     const int ex = (int)tg.extent(0, tx, m), ey = (int)tg.extent(1, ty, m), ez = (int)tg.extent(2, tz, m);
-    for (int i = threadIdx.x; i < NC * B3; i += 256) box[i] = 0ull;
+    for (int i = threadIdx.x; i < NC * B3; i += TB) box[i] = 0ull;
     int s[NC];
     double fs[NC];
 #pragma unroll
@@ -482,11 +482,11 @@ __global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, 
     // KB particles per thread per round; all their loads are issued before any
     // deposit so the dependent sorted[j] -> pos[p] chains overlap
     constexpr int KB = 4;
-    for (unsigned jb = j0; jb < j1; jb += 256 * KB) {
+    for (unsigned jb = j0; jb < j1; jb += TB * KB) {
         unsigned pi[KB];
 #pragma unroll
         for (int k = 0; k < KB; k++) {
-            const unsigned j = jb + k * 256 + threadIdx.x;
+            const unsigned j = jb + k * TB + threadIdx.x;
             pi[k] = sorted[j < j1 ? j : j1 - 1];
         }
         T xs[KB][3];
@@ -502,7 +502,7 @@ __global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, 
         }
 #pragma unroll
         for (int k = 0; k < KB; k++) {
-        if (jb + k * 256 + threadIdx.x >= j1) continue;
+        if (jb + k * TB + threadIdx.x >= j1) continue;
         const int64_t p = pi[k];
         int64_t idx[3];
         T f[3];
@@ -524,7 +524,7 @@ __global__ void __launch_bounds__(256) k_paint_tiles(const T* __restrict__ pos, 
     const int64_t lstride = tg.lstride, hstride = tg.hstride;
     const int e1z = ez + 1, e1y = ey + 1;
     const int nbox = (ex + 1) * e1y * e1z;
-    for (int i = threadIdx.x; i < nbox; i += 256) {
+    for (int i = threadIdx.x; i < nbox; i += TB) {
         const int cc = i % e1z, bb = (i / e1z) % e1y, aa = i / (e1z * e1y);
         const int li = (aa * B + bb) * B + cc;
         if (aa < ex && bb < ey && cc < ez) {
@@ -592,7 +592,11 @@ size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np);
 template <typename T>
 int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,
                const int64_t nmesh[3], double out_scale, int32_t flags, T* out, void* ws, size_t ws_bytes,
-               hipStream_t st);
+               hipStream_t st, unsigned* bins_out = nullptr, const unsigned* bins_in = nullptr);
+// Particle sort by 16^3-cell tile that a 1-component paint can hand to a later
+// 3-component paint of the same positions: offsets (ntiles + 1) then sorted
+// particle indices (np), uint32.
+size_t paint_bins_bytes(const Mesh& m, int64_t np);
 template <typename T>
 int readout_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* field, const int64_t nmesh[3],
                  int kind, int vjpdim, const T* aux, T* out, hipStream_t st);

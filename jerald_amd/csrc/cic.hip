@@ -29,6 +29,7 @@ struct PaintWs {
     size_t acc, halo, counts, offsets, key, slot, sorted, total;
 };
 static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
+    // the smaller tile side has the larger face records and tile count
     const TileGeom tg = make_tiles(m, nc == 1 ? kTileSide1 : kTileSide3);
     const size_t low = (size_t)nc * tg.ntiles * tg.lstride * 8, direct = (size_t)nc * m.cells() * 8;
     PaintWs w;
@@ -45,10 +46,15 @@ static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
 
 size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np) { return paint_ws(m, nc, np < 0 ? 0 : np).total; }
 
+size_t paint_bins_bytes(const Mesh& m, int64_t np) {
+    const TileGeom tg = make_tiles(m, kTileSide1);
+    return (size_t)(tg.ntiles + 1 + (np < 0 ? 0 : np)) * 4;
+}
+
 template <typename T>
 int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,
                const int64_t nmesh[3], double out_scale, int32_t flags, T* out, void* ws, size_t ws_bytes,
-               hipStream_t st) {
+               hipStream_t st, unsigned* bins_out, const unsigned* bins_in) {
     Mesh m;
     int rc = check_mesh(nmesh, &m);
     if (rc) return rc;
@@ -108,34 +114,52 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         LDL_LAUNCH_CHECK();
         return LDL_OK;
     }
-    // tiled path: counting sort by tile, LDS deposit, halo fold
-    const TileGeom tg = make_tiles(m, nc == 1 ? kTileSide1 : kTileSide3);
+    // tiled path: counting sort by tile, LDS deposit, halo fold.  A 3-component
+    // paint that is handed the sort of an earlier 1-component paint (bins_in)
+    // uses that paint's 16^3 tiles and skips the sort.
+    const int ts = (nc == 1 || bins_in) ? kTileSide1 : kTileSide3;
+    const TileGeom tg = make_tiles(m, ts);
     auto* halo = reinterpret_cast<unsigned long long*>(wb + w.halo);
     auto* counts = reinterpret_cast<unsigned*>(wb + w.counts);
-    auto* offsets = reinterpret_cast<unsigned*>(wb + w.offsets);
     auto* key = reinterpret_cast<unsigned*>(wb + w.key);
     auto* slot = reinterpret_cast<unsigned*>(wb + w.slot);
-    auto* sorted = reinterpret_cast<unsigned*>(wb + w.sorted);
-    hipLaunchKernelGGL(k_fill<unsigned>, dim3(grid_for(tg.ntiles + 1, kBlock, 1 << 16)), dim3(kBlock), 0, st, counts, tg.ntiles + 1, 0u);
-    if (np > 0) {
-        constexpr int PPT = 8;
-        KTimer _t("paint_bin", st);
-        hipLaunchKernelGGL((k_bin_keys<T, PPT>), dim3(grid_for((np + 256 * PPT - 1) / (256 * PPT), 1, 1 << 15)), dim3(256), 0, st,
-                           pos, pd, np, m, tg, counts, key, slot);
+    unsigned* offsets = reinterpret_cast<unsigned*>(wb + w.offsets);
+    unsigned* sorted = reinterpret_cast<unsigned*>(wb + w.sorted);
+    if (bins_out) {
+        offsets = bins_out;
+        sorted = bins_out + tg.ntiles + 1;
     }
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, offsets, tg.ntiles);
-    if (np > 0) {
-        { KTimer _t("paint_scatter", st); hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, st, np, key, slot, offsets, sorted); }
+    if (bins_in) {
+        offsets = const_cast<unsigned*>(bins_in);
+        sorted = const_cast<unsigned*>(bins_in) + tg.ntiles + 1;
+    } else {
+        hipLaunchKernelGGL(k_fill<unsigned>, dim3(grid_for(tg.ntiles + 1, kBlock, 1 << 16)), dim3(kBlock), 0, st, counts, tg.ntiles + 1, 0u);
+        if (np > 0) {
+            constexpr int PPT = 8;
+            KTimer _t("paint_bin", st);
+            hipLaunchKernelGGL((k_bin_keys<T, PPT>), dim3(grid_for((np + 256 * PPT - 1) / (256 * PPT), 1, 1 << 15)), dim3(256), 0, st,
+                               pos, pd, np, m, tg, counts, key, slot);
+        }
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, offsets, tg.ntiles);
+        if (np > 0) {
+            KTimer _t("paint_scatter", st);
+            hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, st, np, key, slot, offsets, sorted);
+        }
     }
     auto* low = acc;
     if (nc == 1) {
         KTimer _t("paint_deposit1", st);
         hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide1>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, tg,
                            offsets, sorted, low, halo, hdr, out_scale, out);
-    } else {
+    } else if (ts == kTileSide3) {
         KTimer _t("paint_deposit3", st);
         hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide3>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, tg,
                            offsets, sorted, low, halo, hdr, out_scale, out);
+    } else {
+        // 3 x 17^3 x 8 B = 118 KB of LDS: one workgroup per CU, so use 1024 threads
+        KTimer _t("paint_deposit3", st);
+        hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide1, 1024>), dim3((unsigned)tg.ntiles), dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m,
+                           tg, offsets, sorted, low, halo, hdr, out_scale, out);
     }
     {
         KTimer _t(nc == 1 ? "paint_fixup1" : "paint_fixup3", st);
@@ -177,8 +201,8 @@ int readout_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* f
     return LDL_OK;
 }
 
-template int paint_impl<float>(const float*, const ldl_pos_desc*, int64_t, const float*, double, int, const int64_t*, double, int32_t, float*, void*, size_t, hipStream_t);
-template int paint_impl<double>(const double*, const ldl_pos_desc*, int64_t, const double*, double, int, const int64_t*, double, int32_t, double*, void*, size_t, hipStream_t);
+template int paint_impl<float>(const float*, const ldl_pos_desc*, int64_t, const float*, double, int, const int64_t*, double, int32_t, float*, void*, size_t, hipStream_t, unsigned*, const unsigned*);
+template int paint_impl<double>(const double*, const ldl_pos_desc*, int64_t, const double*, double, int, const int64_t*, double, int32_t, double*, void*, size_t, hipStream_t, unsigned*, const unsigned*);
 template int readout_impl<float>(const float*, const ldl_pos_desc*, int64_t, const float*, const int64_t*, int, int, const float*, float*, hipStream_t);
 template int readout_impl<double>(const double*, const ldl_pos_desc*, int64_t, const double*, const int64_t*, int, int, const double*, double*, hipStream_t);
 

@@ -572,7 +572,7 @@ static int check_prm(const double* prm, int n) {
 
 template <typename T>
 static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double prm[5], double fact,
-                        int32_t pflags, T* pos_out, T* rho, T* Ubuf, T* F, void* ws, size_t wsb, hipStream_t st) {
+                        int32_t pflags, T* pos_out, T* rho, T* Ubuf, T* F, void* bins, void* ws, size_t wsb, hipStream_t st) {
     Mesh m;
     int rc = check_fft_mesh(nmesh, &m);
     if (rc) return rc;
@@ -586,7 +586,9 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     const c2<T>* tw;
     if ((rc = twiddles<T>(&tw))) return rc;
     // 1. density
-    if ((rc = paint_impl<T>(pos, pdesc, np, nullptr, 1.0, 1, nmesh, fact, pflags, rho, P.paint, w.paint, st))) return rc;
+    if ((rc = paint_impl<T>(pos, pdesc, np, nullptr, 1.0, 1, nmesh, fact, pflags, rho, P.paint, w.paint, st,
+                            (pflags & 1) ? nullptr : reinterpret_cast<unsigned*>(bins))))
+        return rc;
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
     const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hc = m.hcells();
     c2<T>* U = reinterpret_cast<c2<T>*>(Ubuf);
@@ -626,8 +628,8 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
 
 template <typename T>
 static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double prm[5], double fact,
-                        int32_t alpha_rule, int32_t pflags, const T* rho, const T* Ubuf, const T* F, const T* gout, T* gin,
-                        double* grad5, void* ws, size_t wsb, hipStream_t st) {
+                        int32_t alpha_rule, int32_t pflags, const T* rho, const T* Ubuf, const T* F, const void* bins,
+                        const T* gout, T* gin, double* grad5, void* ws, size_t wsb, hipStream_t st) {
     Mesh m;
     int rc = check_fft_mesh(nmesh, &m);
     if (rc) return rc;
@@ -642,7 +644,10 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     if ((rc = twiddles<T>(&tw))) return rc;
     const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hc = m.hcells(), cells = m.cells();
     // 1. gF_d = paint(X, g_d)  (3 meshes in mesh3)
-    if ((rc = paint_impl<T>(pos, pdesc, np, gout, 0.0, 3, nmesh, 1.0, pflags, P.mesh3, P.paint, w.paint, st))) return rc;
+    const bool reuse = bins && !(pflags & (LDL_PAINT_FLOAT_ATOMIC | LDL_PAINT_DIRECT));
+    if ((rc = paint_impl<T>(pos, pdesc, np, gout, 0.0, 3, nmesh, 1.0, pflags, P.mesh3, P.paint, w.paint, st, nullptr,
+                            reuse ? reinterpret_cast<const unsigned*>(bins) : nullptr)))
+        return rc;
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
     // 2. unnormalised r2c of the three meshes: z and y passes
     dispatch_h(m.n2 / 2, [&](auto H) {
@@ -794,6 +799,12 @@ using namespace ldl;
 
 extern "C" {
 
+size_t ldl_sm_bins_bytes(const int64_t nmesh[3], int64_t np) {
+    Mesh m;
+    if (check_mesh(nmesh, &m) || np < 0) return 0;
+    return paint_bins_bytes(m, np);
+}
+
 size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np) {
     Mesh m;
     if (check_mesh(nmesh, &m) || np < 0) return 0;
@@ -802,14 +813,16 @@ size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np
 
 #define SM_ENTRY(SUF, T)                                                                                                    \
     int ldl_sm_layer_fwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
-                               double fact, int32_t pflags, T* pos_out, T* rho, T* U, T* F, void* ws, size_t wsb, void* stream) { \
-        return sm_layer_fwd<T>(pos, pd, np, nmesh, prm, fact, pflags, pos_out, rho, U, F, ws, wsb, as_stream(stream));     \
+                               double fact, int32_t pflags, T* pos_out, T* rho, T* U, T* F, void* bins, void* ws, size_t wsb, \
+                               void* stream) {                                                                              \
+        return sm_layer_fwd<T>(pos, pd, np, nmesh, prm, fact, pflags, pos_out, rho, U, F, bins, ws, wsb, as_stream(stream)); \
     }                                                                                                                       \
     int ldl_sm_layer_bwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
                                double fact, int32_t alpha_rule, int32_t pflags, const T* rho, const T* U, const T* F,      \
-                               const T* g_out, T* g_in, double* grad5, void* ws, size_t wsb, void* stream) {              \
-        return sm_layer_bwd<T>(pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, rho, U, F, g_out, g_in, grad5, ws, wsb,   \
-                               as_stream(stream));                                                                          \
+                               const void* bins, const T* g_out, T* g_in, double* grad5, void* ws, size_t wsb,             \
+                               void* stream) {                                                                              \
+        return sm_layer_bwd<T>(pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, rho, U, F, bins, g_out, g_in, grad5, ws,  \
+                               wsb, as_stream(stream));                                                                     \
     }                                                                                                                       \
     int ldl_sm_loss_##SUF(const T* rho, const T* target, const T* mask, const T* mask_valid, const T* field2,               \
                           const int64_t nmesh[3], const double box[3], const double prm3[3], double index, int32_t p_norm,  \

@@ -43,7 +43,7 @@ def _as_mesh(a, nm, dtype, device):
 class JERALDModel:
     def __init__(self, X, target, pm, Nstep=2, kind="sm", n=0, index=1.0, L1=None, field2=None, masktrain=None,
                  maskvalid=None, starmap=None, *, smoothing=None, grad_rule="exact", paint_mode="fixed",
-                 lattice=None, save="all", dtype=None):
+                 lattice=None, save="all", dtype=None, reuse_sort=True):
         if kind in ("dm", "HI"):
             raise NotImplementedError(f"kind '{kind}' is not implemented; only 'sm' is on this path")
         self.kind = "sm"  # unknown kinds use the sm model (model.py:110-112)
@@ -66,6 +66,7 @@ class JERALDModel:
         if save not in ("all", "positions"):
             raise ValueError("save must be 'all' or 'positions'")
         self.save = save
+        self.reuse_sort = bool(reuse_sort)  # backward paint reuses the forward's particle sort
         # loss smoothing: reference k^-n + 1 when n > 0 (model.py:376-381)
         if smoothing is None:
             self.smooth_kind, self.smooth_par = (1, self.n) if self.n > 0 else (0, 0.0)
@@ -123,16 +124,21 @@ class JERALDModel:
         nb = L.load().ldl_sm_workspace_bytes(L.i64x3(self.nm), int(self.dtype == torch.float64), np_)
         return L.workspaces.get(self.device, f"sm{self.dtype}", nb)
 
-    def _layer_fwd(self, P, prm, fact):
+    def _layer_fwd(self, P, prm, fact, keep_bins=False):
         dt, dev, nm = self.dtype, self.device, self.nm
         rho = torch.empty(nm, dtype=dt, device=dev)
         U = torch.empty(nm[0], nm[1], nm[2] // 2 + 1, 2, dtype=dt, device=dev)
         F = torch.empty((3,) + nm, dtype=dt, device=dev)
         out = torch.empty_like(P)
+        bins = None
+        if keep_bins:
+            nb = L.load().ldl_sm_bins_bytes(L.i64x3(nm), P.shape[0])
+            bins = torch.empty(nb, dtype=torch.uint8, device=dev)
         ws = self._ws()
         L.call(f"ldl_sm_layer_fwd_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], L.i64x3(nm), L.dvec(prm), float(fact),
-               self.paint_flags, L.ptr(out), L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
-        return out, rho, U, F
+               self.paint_flags, L.ptr(out), L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(bins), L.ptr(ws), ws.numel(),
+               L.stream_ptr(dev))
+        return out, rho, U, F, bins
 
     def _paint_density(self, P, fact):
         dt, dev, nm = self.dtype, self.device, self.nm
@@ -192,9 +198,10 @@ class JERALDModel:
         fact = float(np.prod(self.nm)) / P.shape[0]
         saved = []
         for i in range(self.Nstep):
-            out, rho, U, F = self._layer_fwd(P, p[5 * i: 5 * i + 5], fact)
+            out, rho, U, F, bins = self._layer_fwd(P, p[5 * i: 5 * i + 5], fact,
+                                                   keep_bins=need_grad and self.reuse_sort)
             if need_grad:
-                saved.append((P, rho, U, F) if self.save == "all" else (P,))
+                saved.append((P, rho, U, F, bins) if self.save == "all" else (P,))
             P = out
         rho_f = self._paint_density(P, fact)
         out5, g_rho = self._loss_kernel(rho_f, p[5 * self.Nstep: 5 * self.Nstep + 3], need_grad)
@@ -213,14 +220,14 @@ class JERALDModel:
             prm = p[5 * i: 5 * i + 5]
             sv = st["saved"][i]
             if self.save == "all":
-                Pi, rho, U, F = sv
+                Pi, rho, U, F, bins = sv
             else:
                 Pi = sv[0]
-                _, rho, U, F = self._layer_fwd(Pi, prm, fact)
+                _, rho, U, F, bins = self._layer_fwd(Pi, prm, fact, keep_bins=self.reuse_sort)
             gp = C_ptr_offset(grads, 5 * i)
             L.call(f"ldl_sm_layer_bwd_{L.sfx(dt)}", L.ptr(Pi), self._pd(), Pi.shape[0], L.i64x3(nm), L.dvec(prm),
-                   float(fact), self.alpha_rule, self.paint_flags, L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(g), L.ptr(g),
-                   gp, L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+                   float(fact), self.alpha_rule, self.paint_flags, L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(bins),
+                   L.ptr(g), L.ptr(g), gp, L.ptr(ws), ws.numel(), L.stream_ptr(dev))
             st["saved"][i] = None
         return torch.cat([grads, st["out5"][1:4]]), g
 

@@ -120,6 +120,22 @@ def test_ldl_fp32_within_1e5_of_fp64():
     assert abs(l64 - loss0) <= 1e-10 * abs(loss0)
 
 
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_sort_reuse_is_bit_identical(dt):
+    """The backward paint on the forward's 16^3-tile sort gives the same
+    integers as its own 8^3-tile sort: identical loss and gradients."""
+    N, L, nstep = 32, 128.0, 2
+    X, tgt, mk, p, _ = _problem(N, nstep, L, smoothing=("gaussian", 4.0), L1=False, rms=2.0)
+    out = []
+    for reuse in (True, False):
+        m = _model(X, tgt, mk, N, L, nstep, dt, L1=False, smoothing=("gaussian", 4.0), lattice=(N, N, N),
+                   reuse_sort=reuse)
+        loss, grad = m.loss_and_grad(torch.tensor(p))
+        out.append((loss.item(), grad.cpu().numpy()))
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
 def test_ldl_reference_alpha_rule_and_position_grad():
     N, L, nstep = 16, 64.0, 2
     X, tgt, mk, p, _ = _problem(N, nstep, L, L1=False)
