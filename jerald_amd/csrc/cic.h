@@ -241,7 +241,9 @@ __device__ __forceinline__ void interp_vg(const T v[8], const T f[3], const T ce
 template <typename T>
 __global__ void __launch_bounds__(256) k_readout(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F,
                                                  Mesh m, int deriv, T* __restrict__ out) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    int64_t pb, pe;
+    block_particles(np, pb, pe);
+    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
         Stencil<T> st;
         locate(pd, m, pos, p, st);
         T v[8];
@@ -255,7 +257,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_readout3(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
                                                   Mesh m, T* __restrict__ out) {
     const int64_t cells = m.cells();
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    int64_t pb, pe;
+    block_particles(np, pb, pe);
+    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
         Stencil<T> st;
         locate(pd, m, pos, p, st);
         T v[3][8];
@@ -270,7 +274,9 @@ __global__ void __launch_bounds__(256) k_readout3(const T* __restrict__ pos, Pos
 template <typename T>
 __global__ void __launch_bounds__(256) k_readout_grad(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F,
                                                       Mesh m, const T* __restrict__ pscale, T* __restrict__ out) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    int64_t pb, pe;
+    block_particles(np, pb, pe);
+    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
         Stencil<T> st;
         locate(pd, m, pos, p, st);
         T v[8], val, g[3];
@@ -287,7 +293,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_readout_grad3(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
                                                        const T* __restrict__ v3, Mesh m, T* __restrict__ out) {
     const int64_t cells = m.cells();
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    int64_t pb, pe;
+    block_particles(np, pb, pe);
+    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
         Stencil<T> st;
         locate(pd, m, pos, p, st);
         T v[3][8];
@@ -315,7 +323,8 @@ __global__ void __launch_bounds__(256) k_readout_grad3(const T* __restrict__ pos
 // a per-tile halo buffer that the convert pass adds to the lower faces of the
 // neighbouring tiles.  No global atomics touch the mesh.
 struct TileGeom {
-    int ts;              // tile side in cells
+    int ts;              // tile side in cells (a power of two)
+    int tsh;             // log2(ts)
     int64_t nt[3];       // tiles per axis (last tile may be partial)
     int64_t ntiles;
     int64_t hstride;     // halo entries per tile, 3 ts^2 + 3 ts + 1
@@ -330,6 +339,8 @@ struct TileGeom {
 inline TileGeom make_tiles(const Mesh& m, int ts) {
     TileGeom g;
     g.ts = ts;
+    g.tsh = 0;
+    while ((1 << g.tsh) < ts) g.tsh++;
     g.nt[0] = (m.n0 + ts - 1) / ts;
     g.nt[1] = (m.n1 + ts - 1) / ts;
     g.nt[2] = (m.n2 + ts - 1) / ts;
@@ -377,7 +388,8 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
                 int64_t idx[3];
                 T f[3];
                 pd.decode(pos, p, idx, f);
-                k[q] = (unsigned)(((idx[0] / tg.ts) * tg.nt[1] + idx[1] / tg.ts) * tg.nt[2] + idx[2] / tg.ts);
+                k[q] = (unsigned)((((unsigned)idx[0] >> tg.tsh) * (unsigned)tg.nt[1] + ((unsigned)idx[1] >> tg.tsh)) * (unsigned)tg.nt[2] +
+                                  ((unsigned)idx[2] >> tg.tsh));
                 unsigned h = (k[q] * 2654435761u) & (HS - 1);
                 while (true) {  // terminates: fewer than HS distinct keys per chunk
                     const unsigned prev = atomicCAS(&hkey[h], EMPTY, k[q]);
@@ -556,11 +568,13 @@ __global__ void __launch_bounds__(256) k_tiles_fixup(const unsigned long long* _
 #pragma unroll
     for (int c = 0; c < NC; c++) f[c] = ldexp(scale, -fixed_shift<T>(hdr[c], np));
     const int64_t cells = m.cells();
-    const int64_t total = tg.ntiles * tg.lstride;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t t = i / tg.lstride;
-        const int li = (int)(i - t * tg.lstride);
-        const int64_t tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
+    const unsigned total = (unsigned)(tg.ntiles * tg.lstride);  // < 2^31 for meshes up to 2048^3
+    const unsigned ls = (unsigned)tg.lstride, nt1 = (unsigned)tg.nt[1], nt2 = (unsigned)tg.nt[2];
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const unsigned t = i / ls;
+        const int li = (int)(i - t * ls);
+        const unsigned tq = t / nt2;
+        const int64_t tz = t - tq * nt2, ty = tq % nt1, tx = tq / nt1;
         const int ex = (int)tg.extent(0, tx, m), ey = (int)tg.extent(1, ty, m), ez = (int)tg.extent(2, tz, m);
         if (li >= ex * ey * ez - (ex - 1) * (ey - 1) * (ez - 1)) continue;  // partial tile: shorter record
         int a, b, c;

@@ -165,15 +165,6 @@ __device__ __forceinline__ void tile_store(const c2<T>* lds, int pv, F&& f) {
     }
 }
 
-// Bijective blockIdx -> tile map that gives each of the 8 XCDs (which the
-// hardware feeds round-robin) a contiguous range of tiles, so tiles that
-// share cache lines share an L2.
-__device__ __forceinline__ int64_t xcd_tile(int64_t bid, int64_t nb) {
-    const int64_t x = bid & 7, local = bid >> 3;
-    const int64_t q = nb >> 3, r = nb & 7;
-    return x * q + (x < r ? x : r) + local;
-}
-
 // Sum NR per-thread FP64 values over the workgroup; thread 0 gets the totals.
 template <int NR, int TH>
 __device__ __forceinline__ void block_sum(double (&v)[NR]) {

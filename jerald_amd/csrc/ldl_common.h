@@ -194,6 +194,26 @@ inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {
 // every power of two L in [1, 2048], built in FP64 on the host.
 template <typename T> int twiddles(const c2<T>** out);
 
+// Bijective blockIdx -> tile map that gives each of the 8 XCDs (which the
+// hardware feeds round-robin) a contiguous range of tiles, so tiles that
+// share cache lines share an L2.
+__device__ __forceinline__ int64_t xcd_tile(int64_t bid, int64_t nb) {
+    const int64_t x = bid & 7, local = bid >> 3;
+    const int64_t q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + local;
+}
+
+// Particle range [b, e) of this workgroup: np split into gridDim.x contiguous
+// chunks, chunk order XCD-major (xcd_tile), so each XCD gathers from a
+// contiguous stretch of the Lagrangian-ordered particles and its L2 keeps the
+// mesh planes they touch.
+__device__ __forceinline__ void block_particles(int64_t np, int64_t& b, int64_t& e) {
+    const int64_t nb = gridDim.x;
+    const int64_t per = (np + nb - 1) / nb;
+    b = xcd_tile(blockIdx.x, nb) * per;
+    e = b + per < np ? b + per : np;
+}
+
 // ------------------------------------------------- kernel timing (bench)
 bool timing_enabled();
 void timer_begin(const char* name, hipStream_t st, void** token);
@@ -208,6 +228,17 @@ struct KTimer {
     KTimer(const char* n, hipStream_t s) : name(n), st(s), tok(nullptr) { timer_begin(n, s, &tok); }
     ~KTimer() { timer_end(name, st, tok); }
 };
+
+// ----------------------------------------------------- elementwise math
+// Positive-base power, log and exp: the FP32 path uses the hardware
+// exp2/log2-based intrinsics (a few ulp), the FP64 path the accurate library
+// routines.
+__device__ __forceinline__ float ppow(float x, float y) { return __powf(x, y); }
+__device__ __forceinline__ double ppow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float plog(float x) { return __logf(x); }
+__device__ __forceinline__ double plog(double x) { return log(x); }
+__device__ __forceinline__ float pexp(float x) { return __expf(x); }
+__device__ __forceinline__ double pexp(double x) { return exp(x); }
 
 // --------------------------------------------------------- reductions
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {

@@ -55,7 +55,7 @@ template <typename T> struct SmFilter {
     T kh2, inv_kl2, nu;
     __device__ __forceinline__ T G(T kn) const {  // model.py:195
         const T k2 = kn * kn;
-        return -exp(-k2 * inv_kl2 - kh2 / k2) * pow(kn, nu);
+        return -pexp(-k2 * inv_kl2 - kh2 / k2) * ppow(kn, nu);
     }
 };
 
@@ -144,7 +144,7 @@ template <typename T> struct PowRowLoad {  // (rho + 1e-8)^gamma, model.py:193
     T gamma;
     __device__ __forceinline__ c2<T> load2(int64_t row, int m) const {
         const c2<T> r = *reinterpret_cast<const c2<T>*>(rho + row * n2 + 2 * m);
-        return c2<T>{pow(r.x + T(1e-8), gamma), pow(r.y + T(1e-8), gamma)};
+        return c2<T>{ppow(r.x + T(1e-8), gamma), ppow(r.y + T(1e-8), gamma)};
     }
 };
 
@@ -161,9 +161,9 @@ template <typename T> struct GammaBwdStore {
         const c2<T> r = *reinterpret_cast<const c2<T>*>(rho + i);
         const T g0 = v.x * scale, g1 = v.y * scale;
         const T r0 = r.x + T(1e-8), r1 = r.y + T(1e-8);
-        const T l0 = log(r0), l1 = log(r1);
-        acc[0] += (double)(g0 * pow(r0, gamma)) * (double)l0 + (double)(g1 * pow(r1, gamma)) * (double)l1;
-        *reinterpret_cast<c2<T>*>(grho + i) = c2<T>{g0 * gamma * pow(r0, gamma - T(1)), g1 * gamma * pow(r1, gamma - T(1))};
+        const T l0 = plog(r0), l1 = plog(r1);
+        acc[0] += (double)(g0 * ppow(r0, gamma)) * (double)l0 + (double)(g1 * ppow(r1, gamma)) * (double)l1;
+        *reinterpret_cast<c2<T>*>(grho + i) = c2<T>{g0 * gamma * ppow(r0, gamma - T(1)), g1 * gamma * ppow(r1, gamma - T(1))};
     }
     __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const { part[b] = acc[0]; }
 };
@@ -318,10 +318,10 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
         if (p >= pv) continue;
         T w;
         if (kind == 1) {
-            w = pow(kt.knorm(n, plane, col0 + p), -par) + T(1);
+            w = ppow(kt.knorm(n, plane, col0 + p), -par) + T(1);
         } else {
             const T a = kt.k[0][n], b = kt.k[1][plane], c = kt.k[2][col0 + p];
-            w = exp(T(-0.5) * (a * a + b * b + c * c) * par * par);
+            w = pexp(T(-0.5) * (a * a + b * b + c * c) * par * par);
         }
         w *= scale;
         const C v = lds[n * LS + p];
@@ -338,7 +338,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
                                                          Mesh m, T s0, T s1, T s2, T* __restrict__ out) {
     const int64_t cells = m.cells();
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    int64_t pb, pe;
+    block_particles(np, pb, pe);
+    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
         const T x0 = pos[3 * p], x1 = pos[3 * p + 1], x2 = pos[3 * p + 2];
         Stencil<T> st;
         locate(pd, m, x0, x1, x2, p, st);
@@ -357,7 +359,9 @@ __global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos
                                                        const T* __restrict__ grho, Mesh m, T fact, const T* __restrict__ gout,
                                                        T* __restrict__ gin) {
     const int64_t cells = m.cells();
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    int64_t pb, pe;
+    block_particles(np, pb, pe);
+    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
         Stencil<T> st;
         locate(pd, m, pos, p, st);
         T v[4][8];
@@ -383,7 +387,9 @@ __global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos
 template <typename T>
 __global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ grho,
                                                        Mesh m, T fact, T* __restrict__ g) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    int64_t pb, pe;
+    block_particles(np, pb, pe);
+    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
         Stencil<T> st;
         locate(pd, m, pos, p, st);
         T v[8], val, gr[3];
@@ -404,10 +410,10 @@ template <typename T> struct RespPar {
 // Forward response value at a cell: y, F, Fi (model.py:344, :358-362)
 template <typename T>
 __device__ __forceinline__ void response(const RespPar<T>& rp, T rho, T f2, T& base, T& y, T& F, T& Fi) {
-    base = pow(rho + T(1e-8), rp.mu);
+    base = ppow(rho + T(1e-8), rp.mu);
     y = rp.w * base + rp.b;
     F = y > T(1e-12) ? y : T(1e-12);
-    Fi = rp.index == T(1) ? F : pow(F, rp.index);
+    Fi = rp.index == T(1) ? F : ppow(F, rp.index);
     Fi = Fi * f2;
 }
 
@@ -415,7 +421,7 @@ __device__ __forceinline__ void response(const RespPar<T>& rp, T rho, T f2, T& b
 template <typename T>
 __device__ __forceinline__ T response_bwd(const RespPar<T>& rp, T gFi, T f2, T y, T F) {
     T g = gFi * f2;
-    if (rp.index != T(1)) g = g * rp.index * pow(F, rp.index - T(1));
+    if (rp.index != T(1)) g = g * rp.index * ppow(F, rp.index - T(1));
     return y > T(1e-12) ? g : T(0);
 }
 
@@ -458,11 +464,11 @@ __global__ void __launch_bounds__(256) k_loss_direct(const T* __restrict__ rho, 
             const T sgn = diff > T(0) ? T(1) : diff < T(0) ? T(-1) : T(0);
             const T gd = (T)((double)mk * inv) * (rp.l1 ? sgn : T(2) * diff);
             const T gy = response_bwd(rp, gd, f2, y, F);
-            const T lr = log(r + T(1e-8));
+            const T lr = plog(r + T(1e-8));
             a[1] += (double)(gy * rp.w * base) * (double)lr;
             a[2] += (double)(gy * base);
             a[3] += (double)gy;
-            grho[i] = gy * rp.w * rp.mu * pow(r + T(1e-8), rp.mu - T(1));
+            grho[i] = gy * rp.w * rp.mu * ppow(r + T(1e-8), rp.mu - T(1));
         }
     }
     block_sum<5, 256>(a);
@@ -537,10 +543,10 @@ template <typename T> struct RespBwdStore {
             T base, y, F, Fi;
             response(rp, r, f2, base, y, F, Fi);
             const T gy = response_bwd(rp, gv[e], f2, y, F);
-            acc[0] += (double)(gy * rp.w * base) * (double)log(r + T(1e-8));
+            acc[0] += (double)(gy * rp.w * base) * (double)plog(r + T(1e-8));
             acc[1] += (double)(gy * base);
             acc[2] += (double)gy;
-            grho[i] = gy * rp.w * rp.mu * pow(r + T(1e-8), rp.mu - T(1));
+            grho[i] = gy * rp.w * rp.mu * ppow(r + T(1e-8), rp.mu - T(1));
         }
     }
     __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const {
