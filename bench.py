@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=256, help="per-side size of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--save", default="all", choices=["all", "positions"])
+    ap.add_argument("--copy-probe-mib", type=int, default=1024,
+                    help="size of the plain device copy timed during setup (its HBM counters calibrate traffic)")
     return ap.parse_args()
 
 
@@ -94,6 +96,7 @@ def kernel_bytes(name, n, b):
         "paint_vjp_pos": 6 * b * Np + b * Nm,            # pos in, grad out, 1 field
         "paint_deposit1": 3 * b * Np + b * Nm,           # unit-mass paint
         "paint_deposit3": 6 * b * Np + 3 * b * Nm,       # (Np,3) masses -> 3 meshes
+        "paint_absmax": 3 * b * Np,                      # (Np,3) masses in
         "zpass_r2c": b * Nm + 2 * b * Nh,
         "zpass_r2c3": 3 * (b * Nm + 2 * b * Nh),
         "zpass_c2r": b * Nm + 2 * b * Nh,
@@ -106,10 +109,49 @@ def kernel_bytes(name, n, b):
     }.get(name)
 
 
-KERNELS = ("paint_bin", "paint_scatter", "paint_deposit1", "paint_deposit3", "paint_fixup1", "paint_fixup3",
+KERNELS = ("paint_absmax", "paint_bin", "paint_scatter", "paint_deposit1", "paint_deposit3", "paint_fixup1", "paint_fixup3",
            "paint_deposit", "paint_convert", "zpass_r2c", "zpass_r2c3", "zpass_c2r", "zpass_c2r3", "colpass",
            "colpass3", "sm_x_fwd", "sm_x_bwd", "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
            "loss")
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_ldl512_fp32.txt")
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of an instrumented kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy-probe
+    calibration), or None when the summary does not cover this kernel at this
+    size.  Data lines: <timer name> <n> <dispatches> <fetch bytes> <write bytes>."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            t = line.split()
+            if len(t) == 5 and not line.startswith("#") and t[0] == kernel and int(t[1]) == n:
+                return int(float(t[3]) + float(t[4]))
+    return None
+
+
+def copy_probe(mib, dev):
+    """One plain device-to-device copy of a known size, timed with HIP events:
+    the achievable-bandwidth reference and the calibration dispatch for the
+    HBM byte counters."""
+    import torch
+    if mib <= 0:
+        return None
+    a = torch.ones(mib << 18, device=dev, dtype=torch.float32)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    by = 2 * (mib << 20)
+    del a, b
+    return {"bytes": by, "ms": round(ms, 4), "GBps": round(by / ms / 1e6, 1)}
 
 
 def time_steps(model, p, steps, warmup):
@@ -143,6 +185,7 @@ def main():
     box = 0.8 * n
     cell = box / n
     p = params0(nstep)
+    probe = copy_probe(args.copy_probe_mib, dev)
     log(f"[bench] building inputs n={n} box={box}")
     X = zeldovich_gpu(n, box, 2.0, 83, dev)
     pm32 = PMesh(n, box, device=dev, dtype=torch.float32)
@@ -188,7 +231,8 @@ def main():
         by = kernel_bytes(dom, n, 4)
         ach = by / avg_s / 1e9
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 4),
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, n),
+                "avg_launch_ms": round(avg_s * 1e3, 4),
                 "algorithmic_bytes": by, "share_of_kernel_time": round(tot / total_k, 3)}
     kernel_table = {}
     for k, (t, c) in ranked:
@@ -197,6 +241,9 @@ def main():
         by = kernel_bytes(k, n, 4)
         if by is not None:
             row["GBps_alg"] = round(by / (t / c / 1e3) / 1e9, 1)
+        tr = measured_traffic(k, n)
+        if tr is not None:
+            row["traffic_bytes"] = tr
         kernel_table[k] = row
     # plain FP64 GPU path: FP64 absolute positions, FP64 atomics, same kernels
     fp64 = None
@@ -254,6 +301,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp64_plain_gpu": fp64,
+        "copy_probe": probe,
         "loss_fp32": loss32,
         "kernels": kernel_table,
     }

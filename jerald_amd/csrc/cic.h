@@ -64,18 +64,55 @@ __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
 
 // header[c] = bits of max |mass[:, c]| (as double); positive doubles order
 // like their bit patterns, so an integer atomicMax is exact and order free.
+// The (np, NC) array is read flat as coalesced 16-byte vectors of W = 16 /
+// sizeof(T) elements, U vectors in flight per thread; flat element e belongs
+// to component e % NC.  Maxima of |m| are exact in T.
 template <typename T, int NC>
 __global__ void __launch_bounds__(256) k_absmax(const T* __restrict__ mass, int64_t np, unsigned long long* __restrict__ hdr) {
-    double mx[NC];
+    constexpr int W = 16 / sizeof(T), U = 4;
+    struct alignas(16) Vec { T v[W]; };
+    T mx[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) mx[c] = 0.0;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    for (int c = 0; c < NC; c++) mx[c] = T(0);
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, nth = (int64_t)gridDim.x * blockDim.x;
+    const int64_t ne = np * NC;
+    const int64_t nv = (reinterpret_cast<uintptr_t>(mass) & 15) == 0 ? ne / W : 0;
+    const Vec* mv = reinterpret_cast<const Vec*>(mass);
+    for (int64_t v0 = tid; v0 < nv; v0 += U * nth) {
+        Vec q[U];
 #pragma unroll
-        for (int c = 0; c < NC; c++) mx[c] = fmax(mx[c], fabs((double)mass[p * NC + c]));
+        for (int u = 0; u < U; u++) {
+            const int64_t v = v0 + u * nth;
+            q[u] = mv[v < nv ? v : v0];  // a repeated vector leaves the maxima unchanged
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t v = v0 + u * nth < nv ? v0 + u * nth : v0;
+            T rel[NC];  // rel[k]: max over the elements j of q[u] with j % NC == k
+#pragma unroll
+            for (int k = 0; k < NC; k++) rel[k] = T(0);
+#pragma unroll
+            for (int j = 0; j < W; j++) rel[j % NC] = fmax(rel[j % NC], fabs(q[u].v[j]));
+            const int r0 = (int)((W * v) % NC);  // component of element 0
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const int k = c - r0 < 0 ? c - r0 + NC : c - r0;
+                T val = rel[0];
+#pragma unroll
+                for (int kk = 1; kk < NC; kk++) val = k == kk ? rel[kk] : val;
+                mx[c] = fmax(mx[c], val);
+            }
+        }
+    }
+    for (int64_t e = nv * W + tid; e < ne; e += nth) {
+        const int c = (int)(e % NC);
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+            if (k == c) mx[k] = fmax(mx[k], fabs(mass[e]));
     }
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-        double v = mx[c];
+        double v = (double)mx[c];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
         if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(&hdr[c], (unsigned long long)__double_as_longlong(v));
@@ -209,6 +246,120 @@ __device__ __forceinline__ void gather8(const T* __restrict__ F, const Stencil<T
     }
 }
 
+// ---- power-of-two meshes (every FFT mesh: axes 2^k <= 1024, at most 2^30
+// cells): 32-bit byte offsets from a uniform base pointer, wraps by masking.
+struct MeshP2 {
+    uint32_t msk[3];   // n - 1 per axis
+    uint32_t lg1, lg2;  // log2 n1, log2 n2
+};
+inline MeshP2 mesh_p2(const Mesh& m) {
+    auto lg = [](int64_t v) { uint32_t k = 0; while ((int64_t(1) << k) < v) k++; return k; };
+    MeshP2 q;
+    q.msk[0] = (uint32_t)(m.n0 - 1);
+    q.msk[1] = (uint32_t)(m.n1 - 1);
+    q.msk[2] = (uint32_t)(m.n2 - 1);
+    q.lg1 = lg(m.n1);
+    q.lg2 = lg(m.n2);
+    return q;
+}
+
+template <typename T> struct StencilP2 {
+    uint32_t row[4];  // byte offsets of the four (x, y) rows
+    uint32_t k;       // lower z index
+    T f[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void locate_p2(const PosDec<T>& pd, const MeshP2& mp, T x0, T x1, T x2, int64_t p, StencilP2<T>& st) {
+    uint32_t idx[3];
+    pd.decode_p2(x0, x1, x2, p, mp.msk, idx, st.f);
+    const uint32_t i1 = (idx[0] + 1u) & mp.msk[0], j1 = (idx[1] + 1u) & mp.msk[1];
+    constexpr uint32_t lb = sizeof(T) == 4 ? 2 : 3;
+    st.row[0] = ((idx[0] << mp.lg1) | idx[1]) << (mp.lg2 + lb);
+    st.row[1] = ((idx[0] << mp.lg1) | j1) << (mp.lg2 + lb);
+    st.row[2] = ((i1 << mp.lg1) | idx[1]) << (mp.lg2 + lb);
+    st.row[3] = ((i1 << mp.lg1) | j1) << (mp.lg2 + lb);
+    st.k = idx[2];
+}
+
+// Corner values of NF fields on a power-of-two mesh: one two-element load per
+// (field, row) -- z-adjacent corners are contiguous -- all issued before any
+// is used; lanes with k = N2 - 1 load the pair ending at k and then the wrapped
+// corner (N2 >= 2).
+template <typename T, int NF>
+__device__ __forceinline__ void gather_p2(const T* const (&F)[NF], const StencilP2<T>& st, const MeshP2& mp, T (&v)[NF][8]) {
+    const bool wrap = st.k == mp.msk[2];
+    const uint32_t kb = (wrap ? st.k - 1u : st.k) * (uint32_t)sizeof(T);
+    Pair<T> pr[NF][4];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) pr[f][r] = *reinterpret_cast<const Pair<T>*>(reinterpret_cast<const char*>(F[f]) + (st.row[r] + kb));
+    T w[NF][4];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) w[f][r] = T(0);
+    if (wrap) {
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) w[f][r] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(F[f]) + st.row[r]);
+    }
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            v[f][2 * r] = wrap ? pr[f][r].b : pr[f][r].a;
+            v[f][2 * r + 1] = wrap ? w[f][r] : pr[f][r].b;
+        }
+}
+
+// Three field pointers passed by value: each is a uniform kernel argument, so
+// the gathers address all three as base + 32-bit offset.
+template <typename T> struct Fields3 {
+    const T* f[3];
+};
+template <typename T>
+inline Fields3<T> fields3(const T* F3, int64_t cells) {
+    return Fields3<T>{{F3, F3 + cells, F3 + 2 * cells}};
+}
+
+// Mesh geometry policies of the gather kernels: GeomAny works on every mesh;
+// GeomP2 on power-of-two meshes whose byte size fits 32 bits (p2_ok).
+template <typename T> struct GeomAny {
+    using St = Stencil<T>;
+    Mesh m;
+    __device__ __forceinline__ void locate(const PosDec<T>& pd, T x0, T x1, T x2, int64_t p, St& st) const {
+        ldl::locate(pd, m, x0, x1, x2, p, st);
+    }
+    template <int NF>
+    __device__ __forceinline__ void gather(const T* const (&F)[NF], const St& st, T (&v)[NF][8]) const {
+#pragma unroll
+        for (int f = 0; f < NF; f++) gather8(F[f], st, m.n2, v[f]);
+    }
+};
+template <typename T> struct GeomP2 {
+    using St = StencilP2<T>;
+    MeshP2 mp;
+    __device__ __forceinline__ void locate(const PosDec<T>& pd, T x0, T x1, T x2, int64_t p, St& st) const {
+        locate_p2(pd, mp, x0, x1, x2, p, st);
+    }
+    template <int NF>
+    __device__ __forceinline__ void gather(const T* const (&F)[NF], const St& st, T (&v)[NF][8]) const { gather_p2<T, NF>(F, st, mp, v); }
+};
+template <typename T>
+inline bool p2_ok(const Mesh& m) {
+    return is_pow2(m.n0) && is_pow2(m.n1) && is_pow2(m.n2) && m.n2 >= 2 &&
+           m.cells() * (int64_t)sizeof(T) <= (int64_t(1) << 32);
+}
+// Calls f(geom) with the fastest policy that fits the mesh.
+template <typename T, typename Fn>
+inline void with_geom(const Mesh& m, Fn&& f) {
+    if (p2_ok<T>(m)) f(GeomP2<T>{mesh_p2(m)});
+    else f(GeomAny<T>{m});
+}
+
 // value (deriv < 0) or derivative along ``deriv`` (box units)
 template <typename T>
 __device__ __forceinline__ T interp(const T v[8], const T f[3], int deriv, const T cellinv[3]) {
@@ -238,50 +389,44 @@ __device__ __forceinline__ void interp_vg(const T v[8], const T f[3], const T ce
     g[2] = (t[0] * (t[1] * dz00 + f[1] * dz01) + f[0] * (t[1] * dz10 + f[1] * dz11)) * cellinv[2];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_readout(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F,
-                                                 Mesh m, int deriv, T* __restrict__ out) {
-    int64_t pb, pe;
-    block_particles(np, pb, pe);
-    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-        Stencil<T> st;
-        locate(pd, m, pos, p, st);
-        T v[8];
-        gather8(F, st, m.n2, v);
-        out[p] = interp(v, st.f, deriv, pd.cellinv);
+template <typename T, typename G>
+__global__ void __launch_bounds__(256) k_readout(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ F,
+                                                 G geo, int deriv, T* __restrict__ out) {
+    LDL_SWEEP(sw, np, p) {
+        typename G::St st;
+        geo.locate(pd, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, st);
+        const T* fs[1] = {F};
+        T v[1][8];
+        geo.template gather<1>(fs, st, v);
+        out[p] = interp(v[0], st.f, deriv, pd.cellinv);
     }
 }
 
 // 3 fields -> (Np,3)
-template <typename T>
-__global__ void __launch_bounds__(256) k_readout3(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
-                                                  Mesh m, T* __restrict__ out) {
-    const int64_t cells = m.cells();
-    int64_t pb, pe;
-    block_particles(np, pb, pe);
-    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-        Stencil<T> st;
-        locate(pd, m, pos, p, st);
+template <typename T, typename G>
+__global__ void __launch_bounds__(256) k_readout3(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, Fields3<T> F3,
+                                                  G geo, T* __restrict__ out) {
+    LDL_SWEEP(sw, np, p) {
+        typename G::St st;
+        geo.locate(pd, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, st);
         T v[3][8];
-#pragma unroll
-        for (int d = 0; d < 3; d++) gather8(F3 + d * cells, st, m.n2, v[d]);
+        geo.template gather<3>(F3.f, st, v);
 #pragma unroll
         for (int d = 0; d < 3; d++) out[3 * p + d] = interp(v[d], st.f, -1, pd.cellinv);
     }
 }
 
 // gradient of one field -> (Np,3), times pscale[p]
-template <typename T>
-__global__ void __launch_bounds__(256) k_readout_grad(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F,
-                                                      Mesh m, const T* __restrict__ pscale, T* __restrict__ out) {
-    int64_t pb, pe;
-    block_particles(np, pb, pe);
-    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-        Stencil<T> st;
-        locate(pd, m, pos, p, st);
-        T v[8], val, g[3];
-        gather8(F, st, m.n2, v);
-        interp_vg(v, st.f, pd.cellinv, val, g);
+template <typename T, typename G>
+__global__ void __launch_bounds__(256) k_readout_grad(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ F,
+                                                      G geo, const T* __restrict__ pscale, T* __restrict__ out) {
+    LDL_SWEEP(sw, np, p) {
+        typename G::St st;
+        geo.locate(pd, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, st);
+        const T* fs[1] = {F};
+        T v[1][8], val, g[3];
+        geo.template gather<1>(fs, st, v);
+        interp_vg(v[0], st.f, pd.cellinv, val, g);
         const T s = pscale ? pscale[p] : T(1);
 #pragma unroll
         for (int d = 0; d < 3; d++) out[3 * p + d] = g[d] * s;
@@ -289,18 +434,14 @@ __global__ void __launch_bounds__(256) k_readout_grad(const T* __restrict__ pos,
 }
 
 // sum_d v[:,d] grad(F_d) -> (Np,3)
-template <typename T>
-__global__ void __launch_bounds__(256) k_readout_grad3(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
-                                                       const T* __restrict__ v3, Mesh m, T* __restrict__ out) {
-    const int64_t cells = m.cells();
-    int64_t pb, pe;
-    block_particles(np, pb, pe);
-    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-        Stencil<T> st;
-        locate(pd, m, pos, p, st);
+template <typename T, typename G>
+__global__ void __launch_bounds__(256) k_readout_grad3(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, Fields3<T> F3,
+                                                       const T* __restrict__ v3, G geo, T* __restrict__ out) {
+    LDL_SWEEP(sw, np, p) {
+        typename G::St st;
+        geo.locate(pd, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, st);
         T v[3][8];
-#pragma unroll
-        for (int d = 0; d < 3; d++) gather8(F3 + d * cells, st, m.n2, v[d]);
+        geo.template gather<3>(F3.f, st, v);
         T acc[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -315,13 +456,6 @@ __global__ void __launch_bounds__(256) k_readout_grad3(const T* __restrict__ pos
     }
 }
 
-// ------------------------------------------------------- tiled paint
-// Eulerian tiles of TS^3 cells.  A particle belongs to the tile of its lower
-// CIC corner, so its 8 corners fall in the tile's (TS+1)^3 box.  Each box is
-// accumulated in LDS with integer atomics; the core TS^3 cells are written
-// with plain stores (no other tile writes them) and the three +1 faces go to
-// a per-tile halo buffer that the convert pass adds to the lower faces of the
-// neighbouring tiles.  No global atomics touch the mesh.
 struct TileGeom {
     int ts;              // tile side in cells (a power of two)
     int tsh;             // log2(ts)
@@ -359,27 +493,23 @@ __device__ __forceinline__ int64_t halo_index(int64_t a, int64_t b, int64_t c, i
 }
 
 // Counting-sort pass 1: tile key of each particle and its rank among the
-// particles of the same tile.  A workgroup takes a chunk of 256*PPT
-// particles, counts them per tile in an LDS hash table (LDS atomics give the
-// local ranks), then reserves each distinct tile's range with one global
-// atomic -- issued by different threads in parallel.
+// particles of the same tile.  Consecutive lanes hold consecutive particles,
+// and particles stored in lattice order fall into a few runs of equal keys per
+// wavefront: each run's first lane reserves the run's ranks with one global
+// atomic and broadcasts the base to the run.  Rank order inside a tile depends
+// on atomic order, which no paint result depends on (integer sums).
 template <typename T, int PPT>
 __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Mesh m, TileGeom tg,
                                                   unsigned* __restrict__ counts, unsigned* __restrict__ key,
                                                   unsigned* __restrict__ slot) {
     constexpr int CH = 256 * PPT;
-    constexpr int HS = 2 * CH;  // at most CH distinct keys: the table is at most half full
     constexpr unsigned EMPTY = 0xFFFFFFFFu;
-    __shared__ unsigned hkey[HS];
-    __shared__ unsigned hcnt[HS];
+    const int lane = threadIdx.x & 63;
+    const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
     const int64_t nchunks = (np + CH - 1) / CH;
     for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-        for (int h = threadIdx.x; h < HS; h += 256) {
-            hkey[h] = EMPTY;
-            hcnt[h] = 0u;
-        }
-        __syncthreads();
-        unsigned k[PPT], hs[PPT], r[PPT];
+        unsigned k[PPT], base[PPT];
+        int hl[PPT];
 #pragma unroll
         for (int q = 0; q < PPT; q++) {
             const int64_t p = ch * CH + q * 256 + threadIdx.x;
@@ -388,31 +518,31 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
                 int64_t idx[3];
                 T f[3];
                 pd.decode(pos, p, idx, f);
-                k[q] = (unsigned)((((unsigned)idx[0] >> tg.tsh) * (unsigned)tg.nt[1] + ((unsigned)idx[1] >> tg.tsh)) * (unsigned)tg.nt[2] +
-                                  ((unsigned)idx[2] >> tg.tsh));
-                unsigned h = (k[q] * 2654435761u) & (HS - 1);
-                while (true) {  // terminates: fewer than HS distinct keys per chunk
-                    const unsigned prev = atomicCAS(&hkey[h], EMPTY, k[q]);
-                    if (prev == EMPTY || prev == k[q]) break;
-                    h = (h + 1) & (HS - 1);
-                }
-                hs[q] = h;
-                r[q] = atomicAdd(&hcnt[h], 1u);
+                k[q] = (((unsigned)idx[0] >> tg.tsh) * (unsigned)tg.nt[1] + ((unsigned)idx[1] >> tg.tsh)) * (unsigned)tg.nt[2] +
+                       ((unsigned)idx[2] >> tg.tsh);
             }
         }
-        __syncthreads();
-        for (int h = threadIdx.x; h < HS; h += 256)
-            if (hkey[h] != EMPTY) hcnt[h] = atomicAdd(&counts[hkey[h]], hcnt[h]);
-        __syncthreads();
+        // run heads, run lengths and one atomic per run (all issued before any result is used)
 #pragma unroll
         for (int q = 0; q < PPT; q++) {
+            const unsigned prev = __shfl_up(k[q], 1);
+            const bool head = lane == 0 || prev != k[q];
+            const unsigned long long heads = __ballot(head);
+            hl[q] = 63 - __clzll(heads & upto);
+            const unsigned long long after = heads & ~upto;
+            const int next = after ? __ffsll((long long)after) - 1 : 64;
+            base[q] = 0u;
+            if (head && k[q] != EMPTY) base[q] = atomicAdd(&counts[k[q]], (unsigned)(next - lane));
+        }
+#pragma unroll
+        for (int q = 0; q < PPT; q++) {
+            const unsigned b = __shfl(base[q], hl[q]);
             const int64_t p = ch * CH + q * 256 + threadIdx.x;
             if (p < np) {
                 key[p] = k[q];
-                slot[p] = hcnt[hs[q]] + r[q];
+                slot[p] = b + (unsigned)(lane - hl[q]);
             }
         }
-        __syncthreads();
     }
 }
 
@@ -477,7 +607,9 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
                                                      double scale, T* __restrict__ out) {
     constexpr int B = TS + 1, B3 = B * B * B;
     __shared__ unsigned long long box[NC * B3];
-    const int64_t t = blockIdx.x;
+    // XCD-contiguous tile order: the tiles an XCD runs together are z-neighbours,
+    // whose particles share cache lines of the Lagrangian-ordered arrays
+    const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
     const int64_t lo[3] = {tx * TS, ty * TS, tz * TS};
     const int ex = (int)tg.extent(0, tx, m), ey = (int)tg.extent(1, ty, m), ez = (int)tg.extent(2, tz, m);

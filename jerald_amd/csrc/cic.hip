@@ -94,6 +94,7 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     }
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, hdr, (int64_t)(kHdrBytes / 8), hbits);
     if (mass && np > 0) {
+        KTimer _t("paint_absmax", st);
         if (nc == 1) hipLaunchKernelGGL((k_absmax<T, 1>), dim3(grid_for(np, kBlock, 4096)), dim3(kBlock), 0, st, mass, np, hdr);
         else hipLaunchKernelGGL((k_absmax<T, 3>), dim3(grid_for(np, kBlock, 4096)), dim3(kBlock), 0, st, mass, np, hdr);
     }
@@ -184,17 +185,32 @@ int readout_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* f
     if (!pos || !field || !out) return fail(LDL_EINVAL, "NULL array argument");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
-    const unsigned g = grid_for(np, kBlock, 1 << 16);
+    const Sweep sw = make_sweep(pd, np);
+    const unsigned g = sweep_grid(sw);
+    static_assert(kBlock == kChunk, "gathers take one particle per thread");
+    if (kind == 0 && (vjpdim < -1 || vjpdim > 2)) return fail(LDL_EINVAL, "vjpdim must be between -1 and 2");
+    if (kind == 3 && !aux) return fail(LDL_EINVAL, "v3 is NULL");
+    KTimer _t("readout", st);
     switch (kind) {
         case 0:
-            if (vjpdim < -1 || vjpdim > 2) return fail(LDL_EINVAL, "vjpdim must be between -1 and 2");
-            { KTimer _t("readout", st); hipLaunchKernelGGL(k_readout<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, m, vjpdim, out); }
+            with_geom<T>(m, [&](auto geo) {
+                hipLaunchKernelGGL((k_readout<T, decltype(geo)>), dim3(g), dim3(kBlock), 0, st, pos, pd, np, sw, field, geo, vjpdim, out);
+            });
             break;
-        case 1: { KTimer _t("readout", st); hipLaunchKernelGGL(k_readout3<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, m, out); } break;
-        case 2: { KTimer _t("readout", st); hipLaunchKernelGGL(k_readout_grad<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, m, aux, out); } break;
+        case 1:
+            with_geom<T>(m, [&](auto geo) {
+                hipLaunchKernelGGL((k_readout3<T, decltype(geo)>), dim3(g), dim3(kBlock), 0, st, pos, pd, np, sw, fields3(field, m.cells()), geo, out);
+            });
+            break;
+        case 2:
+            with_geom<T>(m, [&](auto geo) {
+                hipLaunchKernelGGL((k_readout_grad<T, decltype(geo)>), dim3(g), dim3(kBlock), 0, st, pos, pd, np, sw, field, geo, aux, out);
+            });
+            break;
         case 3:
-            if (!aux) return fail(LDL_EINVAL, "v3 is NULL");
-            { KTimer _t("readout", st); hipLaunchKernelGGL(k_readout_grad3<T>, dim3(g), dim3(kBlock), 0, st, pos, pd, np, field, aux, m, out); }
+            with_geom<T>(m, [&](auto geo) {
+                hipLaunchKernelGGL((k_readout_grad3<T, decltype(geo)>), dim3(g), dim3(kBlock), 0, st, pos, pd, np, sw, fields3(field, m.cells()), aux, geo, out);
+            });
             break;
     }
     LDL_LAUNCH_CHECK();

@@ -73,16 +73,23 @@ template <typename T> struct PosDec {
     double lstep[3];
     int64_t n[3];
 
+    int unit;          // lattice form with lattice_step == 1 on every axis
+
     // CIC lower corner (wrapped) and fractional offsets of particle p.
     __device__ __forceinline__ void decode(const T* __restrict__ pos, int64_t p, int64_t idx[3], T f[3]) const {
         const T* q = pos + 3 * p;
         decode_xyz(q[0], q[1], q[2], p, idx, f);
     }
 
-    // Same, from the already loaded coordinates (x0, x1, x2) of particle p.
-    __device__ __forceinline__ void decode_xyz(T x0, T x1, T x2, int64_t p, int64_t idx[3], T f[3]) const {
-        T x[3] = {x0, x1, x2};
-        int64_t base[3] = {0, 0, 0};
+    // Mesh coordinates x (before the floor) of particle p from its loaded
+    // coordinates, and the integer cell offset ``base`` they are relative to
+    // (the lattice cell; 0 for physical positions).
+    template <typename I>
+    __device__ __forceinline__ void coords(T x0, T x1, T x2, int64_t p, T x[3], I base[3]) const {
+        x[0] = x0;
+        x[1] = x1;
+        x[2] = x2;
+        base[0] = base[1] = base[2] = 0;
         if (kind == LDL_POS_PHYSICAL) {
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -109,19 +116,45 @@ template <typename T> struct PosDec {
                 li[1] = rem / l2;
                 li[2] = rem - li[1] * l2;
             }
+            if (unit) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                double b = (double)li[d] * lstep[d];
-                double bf = floor(b);
-                base[d] = (int64_t)bf;
-                x[d] = x[d] + (T)(b - bf);
+                for (int d = 0; d < 3; d++) base[d] = (I)li[d];
+            } else {
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    double b = (double)li[d] * lstep[d];
+                    double bf = floor(b);
+                    base[d] = (I)bf;
+                    x[d] = x[d] + (T)(b - bf);
+                }
             }
         }
+    }
+
+    // Same, from the already loaded coordinates (x0, x1, x2) of particle p.
+    __device__ __forceinline__ void decode_xyz(T x0, T x1, T x2, int64_t p, int64_t idx[3], T f[3]) const {
+        T x[3];
+        int64_t base[3];
+        coords(x0, x1, x2, p, x, base);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             T fl = floor(x[d]);
             f[d] = x[d] - fl;
             idx[d] = wrapi(base[d] + (int64_t)fl, n[d]);
+        }
+    }
+
+    // Power-of-two mesh: 32-bit cell indices wrapped with the masks n - 1
+    // (mesh coordinates within +-2^31 cells).
+    __device__ __forceinline__ void decode_p2(T x0, T x1, T x2, int64_t p, const uint32_t msk[3], uint32_t idx[3], T f[3]) const {
+        T x[3];
+        int32_t base[3];
+        coords(x0, x1, x2, p, x, base);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            T fl = floor(x[d]);
+            f[d] = x[d] - fl;
+            idx[d] = (uint32_t)(base[d] + (int32_t)fl) & msk[d];
         }
     }
 };
@@ -142,6 +175,7 @@ inline int make_posdec(const ldl_pos_desc* pd, const int64_t nmesh[3], int64_t n
     }
     out->l12 = out->l2 = 1;
     out->sh12 = out->sh2 = 0;
+    out->unit = 0;
     if (pd->kind == LDL_POS_LATTICE) {
         int64_t l0 = pd->lattice[0], l1 = pd->lattice[1], l2 = pd->lattice[2];
         if (l0 <= 0 || l1 <= 0 || l2 <= 0) return fail(LDL_EINVAL, "lattice sizes must be positive");
@@ -154,6 +188,7 @@ inline int make_posdec(const ldl_pos_desc* pd, const int64_t nmesh[3], int64_t n
         out->sh2 = lg(out->l2);
         for (int d = 0; d < 3; d++)
             if (!(pd->lattice_step[d] > 0.0)) return fail(LDL_EINVAL, "lattice_step must be positive");
+        out->unit = pd->lattice_step[0] == 1.0 && pd->lattice_step[1] == 1.0 && pd->lattice_step[2] == 1.0;
     }
     return LDL_OK;
 }
@@ -203,16 +238,62 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t bid, int64_t nb) {
     return x * q + (x < r ? x : r) + local;
 }
 
-// Particle range [b, e) of this workgroup: np split into gridDim.x contiguous
-// chunks, chunk order XCD-major (xcd_tile), so each XCD gathers from a
-// contiguous stretch of the Lagrangian-ordered particles and its L2 keeps the
-// mesh planes they touch.
-__device__ __forceinline__ void block_particles(int64_t np, int64_t& b, int64_t& e) {
-    const int64_t nb = gridDim.x;
-    const int64_t per = (np + nb - 1) / nb;
-    b = xcd_tile(blockIdx.x, nb) * per;
-    e = b + per < np ? b + per : np;
+// Order in which the workgroups of a particle gather sweep the particles.
+// A workgroup takes chunks of kChunk consecutive particles (one per thread);
+// its chunk id comes from xcd_tile, so each XCD owns a contiguous run of chunk
+// ids.  Linear order: chunk c starts at particle c * kChunk.  Blocked order,
+// for a lattice of L0 x L1 x L2 particles stored in lattice order with L2 a
+// multiple of kChunk: chunk ids run over slabs of S lattice planes (S = L0 / 8
+// when 8 divides L0, so each XCD owns one slab), then blocks of J rows, then
+// the S planes, then the J rows, then the L2 / kChunk chunks of a row.  The
+// mesh rows a lattice plane gathers from are reused by the next plane J rows'
+// worth of particles later, while they are still in the XCD's L2; in linear
+// order that reuse distance is a whole lattice plane, whose mesh footprint
+// (several fields of 2-3 mesh planes) exceeds the L2.
+constexpr int kChunk = 256;
+struct Sweep {
+    int blocked;
+    uint32_t H, J, S, NYB, L1, L2;
+    int64_t nchunks;
+    __device__ __forceinline__ int64_t first(int64_t c) const {
+        if (!blocked) return c * kChunk;
+        uint32_t q = (uint32_t)c;
+        const uint32_t h = q % H; q /= H;
+        const uint32_t jj = q % J; q /= J;
+        const uint32_t i = q % S; q /= S;
+        const uint32_t yb = q % NYB, slab = q / NYB;
+        return (((int64_t)(slab * S + i) * L1 + yb * J + jj) * L2) + (int64_t)h * kChunk;
+    }
+};
+
+template <typename T>
+inline Sweep make_sweep(const PosDec<T>& pd, int64_t np) {
+    Sweep w{};
+    w.nchunks = (np + kChunk - 1) / kChunk;
+    if (pd.kind != LDL_POS_LATTICE || np == 0) return w;
+    const int64_t L2 = pd.l2, L1 = pd.l12 / pd.l2, L0 = np / pd.l12;
+    if (L2 % kChunk) return w;
+    w.blocked = 1;
+    w.L1 = (uint32_t)L1;
+    w.L2 = (uint32_t)L2;
+    w.H = (uint32_t)(L2 / kChunk);
+    w.J = (uint32_t)L1;
+    for (int64_t j = 32; j >= 2; j >>= 1)
+        if (L1 % j == 0) { w.J = (uint32_t)j; break; }
+    w.S = (uint32_t)(L0 % 8 == 0 ? L0 / 8 : L0);
+    w.NYB = (uint32_t)(L1 / w.J);
+    return w;
 }
+
+// Grid for a Sweep-ordered kernel (kChunk threads per workgroup).
+inline unsigned sweep_grid(const Sweep& w) { return grid_for(w.nchunks, 1, 1 << 22); }
+
+// Loop over this workgroup's particles in Sweep order; the body sees ``p``.
+#define LDL_SWEEP(sw, np, p)                                                                  \
+    for (int64_t _base = 0; _base < (sw).nchunks; _base += gridDim.x)                         \
+        for (int64_t _c = _base + xcd_tile(blockIdx.x, gridDim.x), p = (sw).first(_c) + threadIdx.x, \
+                     _once = 1;                                                                \
+             _once && _c < (sw).nchunks && p < (np); _once = 0)
 
 // ------------------------------------------------- kernel timing (bench)
 bool timing_enabled();

@@ -25,6 +25,7 @@
 namespace ldl {
 
 static constexpr int kBlk = 256;
+static_assert(kBlk == kChunk, "gathers take one particle per thread");
 
 // ---------------------------------------------------------------- tables
 // Per-axis wavenumbers k = 2 pi fftfreq(N, cell) (pmesh.py:174-186; the z
@@ -334,19 +335,15 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
 
 // --------------------------------------------------------------- gathers
 // pos_out = pos + readout3(F) * step (step = 1/cell in mesh units, 1 in box units)
-template <typename T>
-__global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
-                                                         Mesh m, T s0, T s1, T s2, T* __restrict__ out) {
-    const int64_t cells = m.cells();
-    int64_t pb, pe;
-    block_particles(np, pb, pe);
-    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
+template <typename T, typename G>
+__global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, Fields3<T> F3,
+                                                         G geo, T s0, T s1, T s2, T* __restrict__ out) {
+    LDL_SWEEP(sw, np, p) {
         const T x0 = pos[3 * p], x1 = pos[3 * p + 1], x2 = pos[3 * p + 2];
-        Stencil<T> st;
-        locate(pd, m, x0, x1, x2, p, st);
+        typename G::St st;
+        geo.locate(pd, x0, x1, x2, p, st);
         T v[3][8];
-#pragma unroll
-        for (int d = 0; d < 3; d++) gather8(F3 + d * cells, st, m.n2, v[d]);
+        geo.template gather<3>(F3.f, st, v);
         out[3 * p] = x0 + interp(v[0], st.f, -1, pd.cellinv) * s0;
         out[3 * p + 1] = x1 + interp(v[1], st.f, -1, pd.cellinv) * s1;
         out[3 * p + 2] = x2 + interp(v[2], st.f, -1, pd.cellinv) * s2;
@@ -354,20 +351,16 @@ __global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ p
 }
 
 // g_in = g + sum_d g_d grad F_d + fact grad g_rho  (box units)
-template <typename T>
-__global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ F3,
-                                                       const T* __restrict__ grho, Mesh m, T fact, const T* __restrict__ gout,
+template <typename T, typename G>
+__global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, Fields3<T> F3,
+                                                       const T* __restrict__ grho, G geo, T fact, const T* __restrict__ gout,
                                                        T* __restrict__ gin) {
-    const int64_t cells = m.cells();
-    int64_t pb, pe;
-    block_particles(np, pb, pe);
-    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-        Stencil<T> st;
-        locate(pd, m, pos, p, st);
+    LDL_SWEEP(sw, np, p) {
+        typename G::St st;
+        geo.locate(pd, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, st);
+        const T* fs[4] = {F3.f[0], F3.f[1], F3.f[2], grho};
         T v[4][8];
-#pragma unroll
-        for (int d = 0; d < 3; d++) gather8(F3 + d * cells, st, m.n2, v[d]);
-        gather8(grho, st, m.n2, v[3]);
+        geo.template gather<4>(fs, st, v);
         const T g[3] = {gout[3 * p], gout[3 * p + 1], gout[3 * p + 2]};
         T acc[3] = {g[0], g[1], g[2]};
         T val, gr[3];
@@ -384,17 +377,16 @@ __global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos
 }
 
 // g = fact * grad(interp g_rho)
-template <typename T>
-__global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ grho,
-                                                       Mesh m, T fact, T* __restrict__ g) {
-    int64_t pb, pe;
-    block_particles(np, pb, pe);
-    for (int64_t p = pb + threadIdx.x; p < pe; p += blockDim.x) {
-        Stencil<T> st;
-        locate(pd, m, pos, p, st);
-        T v[8], val, gr[3];
-        gather8(grho, st, m.n2, v);
-        interp_vg(v, st.f, pd.cellinv, val, gr);
+template <typename T, typename G>
+__global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ grho,
+                                                       G geo, T fact, T* __restrict__ g) {
+    LDL_SWEEP(sw, np, p) {
+        typename G::St st;
+        geo.locate(pd, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, st);
+        const T* fs[1] = {grho};
+        T v[1][8], val, gr[3];
+        geo.template gather<1>(fs, st, v);
+        interp_vg(v[0], st.f, pd.cellinv, val, gr);
 #pragma unroll
         for (int e = 0; e < 3; e++) g[3 * p + e] = fact * gr[e];
     }
@@ -586,6 +578,7 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     if (!pos || !pos_out || !rho || !Ubuf || !F) return fail(LDL_EINVAL, "NULL array argument");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
+    const Sweep sw = make_sweep(pd, np);
     const SmWs w = sm_ws(m, sizeof(T), np);
     if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
     const SmPtrs<T> P = sm_ptrs<T>(ws, w);
@@ -627,7 +620,13 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     T s[3];
     for (int d = 0; d < 3; d++) s[d] = pdesc->kind == LDL_POS_LATTICE ? pd.cellinv[d] : T(1);
     if (np > 0)
-        { KTimer _t("readout3_update", st); hipLaunchKernelGGL(k_readout3_update<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, F, m, s[0], s[1], s[2], pos_out); }
+        {
+            KTimer _t("readout3_update", st);
+            with_geom<T>(m, [&](auto geo) {
+                hipLaunchKernelGGL((k_readout3_update<T, decltype(geo)>), dim3(sweep_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, fields3(F, m.cells()), geo,
+                                   s[0], s[1], s[2], pos_out);
+            });
+        }
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }
@@ -643,6 +642,7 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     if (!pos || !rho || !Ubuf || !F || !gout || !gin || !grad5) return fail(LDL_EINVAL, "NULL array argument");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
+    const Sweep sw = make_sweep(pd, np);
     const SmWs w = sm_ws(m, sizeof(T), np);
     if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
     const SmPtrs<T> P = sm_ptrs<T>(ws, w);
@@ -695,7 +695,13 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, nzb, 1, rz, grad5, 0);
     // 5. position adjoint
     if (np > 0)
-        { KTimer _t("sm_bwd_gather", st); hipLaunchKernelGGL(k_sm_bwd_gather<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, F, grho, m, (T)fact, gout, gin); }
+        {
+            KTimer _t("sm_bwd_gather", st);
+            with_geom<T>(m, [&](auto geo) {
+                hipLaunchKernelGGL((k_sm_bwd_gather<T, decltype(geo)>), dim3(sweep_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, fields3(F, m.cells()), grho, geo,
+                                   (T)fact, gout, gin);
+            });
+        }
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }
@@ -794,7 +800,11 @@ static int paint_vjp_pos(const T* pos, const ldl_pos_desc* pdesc, int64_t np, co
     if (!pos || !grho || !g) return fail(LDL_EINVAL, "NULL array argument");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
-    { KTimer _t("paint_vjp_pos", st); hipLaunchKernelGGL(k_paint_vjp_pos<T>, dim3(grid_for(np, kBlk, 1 << 16)), dim3(kBlk), 0, st, pos, pd, np, grho, m, (T)fact, g); }
+    const Sweep sw = make_sweep(pd, np);
+    KTimer _t("paint_vjp_pos", st);
+    with_geom<T>(m, [&](auto geo) {
+        hipLaunchKernelGGL((k_paint_vjp_pos<T, decltype(geo)>), dim3(sweep_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, grho, geo, (T)fact, g);
+    });
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }

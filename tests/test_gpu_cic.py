@@ -35,6 +35,28 @@ def test_paint_fixed_point_bit_exact(dt):
 
 
 @pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+@pytest.mark.parametrize("np_,offset", [(3000, 0), (2999, 0), (2999, 1), (5, 2)])
+def test_paint3_fixed_point_bit_exact(dt, np_, offset):
+    """3-component paint: each component is the 1-component fixed-point paint
+    of its masses, bit for bit.  Odd counts and a view starting ``offset``
+    elements into its storage cover the vector tail and the unaligned path
+    of the per-component max |m| scan."""
+    pos, nm, box = _inputs(np_=np_)
+    ndt = np.float32 if dt == torch.float32 else np.float64
+    p = pos.astype(ndt)
+    rng = np.random.default_rng(2)
+    m3 = (rng.standard_normal((np_, 3)) * np.array([1.0, 1e-3, 40.0])).astype(ndt)
+    m3[np_ // 2, 1] = -0.5  # a component whose max |m| comes from a negative value
+    store = torch.zeros(np_ * 3 + offset, device=DEV, dtype=dt)
+    store[offset:] = torch.tensor(m3.reshape(-1), device=DEV)
+    tm = store[offset:].view(np_, 3)
+    got = J.ppaint3D(torch.tensor(p, device=DEV), tm, nm, box).cpu().numpy()
+    for c in range(3):
+        acc, s = NV.paint_fixed(p, m3[:, c], nm, box=box, dtype=ndt)
+        np.testing.assert_array_equal(got[c], NV.fixed_to_float(acc, s, ndt))
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
 def test_paint_deterministic_and_conserves_mass(dt):
     pos, nm, box = _inputs(np_=200000, nm=(32, 32, 32), box=(64.0, 64.0, 64.0))
     t = torch.tensor(pos, device=DEV, dtype=dt)

@@ -103,6 +103,24 @@ def test_ldl_fp64_matches_oracle(smoothing, L1, mask, lattice):
     np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
 
 
+@pytest.mark.parametrize("lat", [(4, 8, 256), (8, 2, 512), (3, 5, 256)])
+def test_ldl_blocked_sweep_lattice(lat):
+    """Lattices whose rows hold a multiple of 256 particles take the blocked
+    (slab, row-block) gather order: same loss and gradients as the oracle."""
+    N, L, nstep = 16, 64.0, 2
+    rng = np.random.default_rng(7)
+    idx = np.stack(np.meshgrid(*[np.arange(v) for v in lat], indexing="ij"), axis=-1).reshape(-1, 3)
+    X = idx * (L / np.array(lat)) + rng.uniform(-1.5, 1.5, idx.shape) * (L / N)
+    tgt = rng.random((N, N, N)) + 0.5
+    p = np.array([0.8, 1.2, 0.3, 1.5, 0.2] * nstep + [1.1, 0.9, 0.05])
+    orc = O.SMModelOracle(N, L, nstep, tgt, L1=False, smoothing=("gaussian", 4.0))
+    loss0, g0 = orc.loss_and_grad(p, X)
+    m = _model(X, tgt, None, N, L, nstep, torch.float64, L1=False, smoothing=("gaussian", 4.0), lattice=lat)
+    loss, grad = m.loss_and_grad(torch.tensor(p))
+    assert abs(loss.item() - loss0) <= 1e-10 * abs(loss0)
+    np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
+
+
 def test_ldl_fp32_within_1e5_of_fp64():
     N, L, nstep = 32, 128.0, 2
     X, tgt, mk, p, orc = _problem(N, nstep, L, smoothing=("gaussian", 4.0), L1=False, rms=2.0)
