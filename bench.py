@@ -109,7 +109,7 @@ def kernel_bytes(name, n, b):
     }.get(name)
 
 
-KERNELS = ("paint_absmax", "paint_bin", "paint_scatter", "paint_deposit1", "paint_deposit3", "paint_fixup1", "paint_fixup3",
+KERNELS = ("paint_absmax", "paint_bin", "paint_scan", "paint_scatter", "paint_deposit1", "paint_deposit3",
            "paint_deposit", "paint_convert", "zpass_r2c", "zpass_r2c3", "zpass_c2r", "zpass_c2r3", "colpass",
            "colpass3", "sm_x_fwd", "sm_x_bwd", "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
            "loss")

@@ -66,8 +66,8 @@ int ldl_version(void);
 /* Benchmark instrumentation (no counterpart in the reference).  While enabled,
  * the main kernels are bracketed by HIP events on their launch stream;
  * ldl_timer_read() waits for them and returns the summed device time and the
- * launch count of one kernel name: "paint_absmax", "paint_bin", "paint_scatter",
- * "paint_deposit1"/"3", "paint_fixup1"/"3" (1- or 3-component tiled paint),
+ * launch count of one kernel name: "paint_absmax", "paint_bin", "paint_scan",
+ * "paint_scatter", "paint_deposit1"/"3" (1- or 3-component tiled paint),
  * "paint_deposit", "paint_convert" (direct and float modes), "zpass_r2c"/"3",
  * "zpass_c2r"/"3", "colpass"/"3" (one or three spectra), "sm_x_fwd", "sm_x_bwd",
  * "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",

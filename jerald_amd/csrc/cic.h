@@ -62,6 +62,15 @@ __global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
         p[i] = v;
 }
 
+// e % NC for NC in {1, 3} with 32-bit arithmetic (2^32 = 1 mod 3).
+template <int NC>
+__device__ __forceinline__ int flat_mod(uint64_t e) {
+    static_assert(NC == 1 || NC == 3, "1 or 3 components");
+    if (NC == 1) return 0;
+    const uint32_t s = (uint32_t)(e >> 32) % 3u + (uint32_t)e % 3u;
+    return (int)(s % 3u);
+}
+
 // header[c] = bits of max |mass[:, c]| (as double); positive doubles order
 // like their bit patterns, so an integer atomicMax is exact and order free.
 // The (np, NC) array is read flat as coalesced 16-byte vectors of W = 16 /
@@ -93,7 +102,7 @@ __global__ void __launch_bounds__(256) k_absmax(const T* __restrict__ mass, int6
             for (int k = 0; k < NC; k++) rel[k] = T(0);
 #pragma unroll
             for (int j = 0; j < W; j++) rel[j % NC] = fmax(rel[j % NC], fabs(q[u].v[j]));
-            const int r0 = (int)((W * v) % NC);  // component of element 0
+            const int r0 = flat_mod<NC>((uint64_t)v * W);  // component of element 0
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 const int k = c - r0 < 0 ? c - r0 + NC : c - r0;
@@ -105,17 +114,25 @@ __global__ void __launch_bounds__(256) k_absmax(const T* __restrict__ mass, int6
         }
     }
     for (int64_t e = nv * W + tid; e < ne; e += nth) {
-        const int c = (int)(e % NC);
+        const int c = flat_mod<NC>((uint64_t)e);
 #pragma unroll
         for (int k = 0; k < NC; k++)
             if (k == c) mx[k] = fmax(mx[k], fabs(mass[e]));
     }
+    // wavefront, then workgroup maxima: one atomic per workgroup and component
+    __shared__ double wmax[NC][4];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         double v = (double)mx[c];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-        if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(&hdr[c], (unsigned long long)__double_as_longlong(v));
+        if ((threadIdx.x & 63) == 0) wmax[c][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NC) {
+        const int c = threadIdx.x;
+        const double v = fmax(fmax(wmax[c][0], wmax[c][1]), fmax(wmax[c][2], wmax[c][3]));
+        if (v > 0.0) atomicMax(&hdr[c], (unsigned long long)__double_as_longlong(v));
     }
 }
 
@@ -457,12 +474,11 @@ __global__ void __launch_bounds__(256) k_readout_grad3(const T* __restrict__ pos
 }
 
 struct TileGeom {
-    int ts;              // tile side in cells (a power of two)
-    int tsh;             // log2(ts)
-    int64_t nt[3];       // tiles per axis (last tile may be partial)
+    int ts;         // tile side in cells (a power of two)
+    int tsh;        // log2(ts)
+    int64_t nt[3];  // tiles per axis (the last tile of an axis may be partial)
     int64_t ntiles;
-    int64_t hstride;     // halo entries per tile, 3 ts^2 + 3 ts + 1
-    int64_t lstride;     // low-face entries per tile, 3 ts^2 - 3 ts + 1
+    int64_t nkeys() const { return ntiles * 8; }
     __host__ __device__ int64_t extent(int d, int64_t t, const Mesh& m) const {
         const int64_t n = d == 0 ? m.n0 : d == 1 ? m.n1 : m.n2;
         const int64_t lo = t * ts;
@@ -479,24 +495,30 @@ inline TileGeom make_tiles(const Mesh& m, int ts) {
     g.nt[1] = (m.n1 + ts - 1) / ts;
     g.nt[2] = (m.n2 + ts - 1) / ts;
     g.ntiles = g.nt[0] * g.nt[1] * g.nt[2];
-    g.hstride = 3ll * ts * ts + 3ll * ts + 1;
-    g.lstride = 3ll * ts * ts - 3ll * ts + 1;
     return g;
 }
 
-// Position of box cell (a, b, c) -- on one of the +1 faces -- in a tile's
-// halo record; ex, ey, ez are the tile's extents.
-__device__ __forceinline__ int64_t halo_index(int64_t a, int64_t b, int64_t c, int64_t ex, int64_t ey, int64_t ez) {
-    if (a == ex) return b * (ez + 1) + c;
-    if (b == ey) return (ey + 1) * (ez + 1) + a * (ez + 1) + c;
-    return (ey + 1) * (ez + 1) + ex * (ez + 1) + a * ey + b;
+// Sort key of a particle whose CIC lower corner is cell idx: 8 * tile + face
+// class, where face-class bit d is set when the corner lies on the tile's
+// upper face along axis d -- exactly the particles whose stencil also reaches
+// the next tile along d.
+__device__ __forceinline__ unsigned tile_key(const int64_t idx[3], const TileGeom& tg, const Mesh& m) {
+    unsigned t[3], fc = 0;
+    const int64_t n[3] = {m.n0, m.n1, m.n2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        t[d] = (unsigned)idx[d] >> tg.tsh;
+        const int64_t hi = ((int64_t)t[d] + 1) * tg.ts;  // one past the tile's last cell
+        if (idx[d] + 1 == (hi < n[d] ? hi : n[d])) fc |= 1u << d;
+    }
+    return (((t[0] * (unsigned)tg.nt[1] + t[1]) * (unsigned)tg.nt[2] + t[2]) << 3) | fc;
 }
 
-// Counting-sort pass 1: tile key of each particle and its rank among the
-// particles of the same tile.  Consecutive lanes hold consecutive particles,
-// and particles stored in lattice order fall into a few runs of equal keys per
+// Counting-sort pass 1: key of each particle and its rank among the particles
+// with the same key.  Consecutive lanes hold consecutive particles, and
+// particles stored in lattice order fall into a few runs of equal keys per
 // wavefront: each run's first lane reserves the run's ranks with one global
-// atomic and broadcasts the base to the run.  Rank order inside a tile depends
+// atomic and broadcasts the base to the run.  Rank order inside a key depends
 // on atomic order, which no paint result depends on (integer sums).
 template <typename T, int PPT>
 __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Mesh m, TileGeom tg,
@@ -518,8 +540,7 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
                 int64_t idx[3];
                 T f[3];
                 pd.decode(pos, p, idx, f);
-                k[q] = (((unsigned)idx[0] >> tg.tsh) * (unsigned)tg.nt[1] + ((unsigned)idx[1] >> tg.tsh)) * (unsigned)tg.nt[2] +
-                       ((unsigned)idx[2] >> tg.tsh);
+                k[q] = tile_key(idx, tg, m);
             }
         }
         // run heads, run lengths and one atomic per run (all issued before any result is used)
@@ -546,67 +567,115 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
     }
 }
 
-// Exclusive scan of the tile counts into offsets[0..ntiles]; counts reset.
-static __global__ void __launch_bounds__(1024) k_scan_counts(unsigned* __restrict__ counts, unsigned* __restrict__ offsets, int64_t n) {
-    __shared__ unsigned long long part[1024];
-    const int t = threadIdx.x;
-    const int64_t per = (n + 1023) / 1024;
-    const int64_t b = t * per, e = b + per < n ? b + per : n;
-    unsigned long long s = 0;
-    for (int64_t i = b; i < e; i++) s += counts[i];
-    part[t] = s;
+// Exclusive scan of the key counts into offsets[0..n] in two passes over
+// segments of kScanSeg counts: segment sums, then each segment adds the sums
+// of the segments before it and scans itself.
+constexpr int kScanSeg = 8192;  // 1024 threads x 8 counts
+
+__device__ __forceinline__ unsigned block_exclusive_scan_1024(unsigned v, unsigned* sh, unsigned& total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    unsigned x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[w] = x;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
-        unsigned long long v = t >= o ? part[t - o] : 0ull;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    if (t < 64) {
+        unsigned s = t < 16 ? sh[t] : 0u;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const unsigned y = __shfl_up(s, o);
+            if (lane >= o) s += y;
+        }
+        if (t < 16) sh[16 + t] = s;
     }
-    unsigned long long run = t ? part[t - 1] : 0ull;
-    for (int64_t i = b; i < e; i++) {
-        offsets[i] = (unsigned)run;
-        run += counts[i];
-        counts[i] = 0u;
-    }
-    if (t == 1023) offsets[n] = (unsigned)part[1023];
+    __syncthreads();
+    total = sh[16 + 15];
+    const unsigned r = x - v + (w ? sh[16 + w - 1] : 0u);
+    __syncthreads();
+    return r;
 }
 
-// Counting-sort pass 2: particle indices grouped by tile.
+static __global__ void __launch_bounds__(1024) k_scan_sums(const unsigned* __restrict__ counts, int64_t n, unsigned* __restrict__ segsum) {
+    __shared__ unsigned sh[32];
+    const int64_t b0 = blockIdx.x * (int64_t)kScanSeg;
+    unsigned v = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int64_t j = b0 + i * 1024 + threadIdx.x;
+        if (j < n) v += counts[j];
+    }
+    unsigned tot;
+    block_exclusive_scan_1024(v, sh, tot);
+    if (threadIdx.x == 0) segsum[blockIdx.x] = tot;
+}
+
+static __global__ void __launch_bounds__(1024) k_scan_apply(const unsigned* __restrict__ counts, int64_t n, const unsigned* __restrict__ segsum,
+                                                          unsigned* __restrict__ offsets) {
+    __shared__ unsigned sh[32];
+    __shared__ unsigned seg[kScanSeg];
+    __shared__ unsigned before;
+    const int64_t b0 = blockIdx.x * (int64_t)kScanSeg;
+    if (threadIdx.x < 64) {  // sum of the earlier segments' totals
+        unsigned s = 0;
+        for (unsigned i = threadIdx.x; i < blockIdx.x; i += 64) s += segsum[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (threadIdx.x == 0) before = s;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {  // coalesced load of the segment
+        const int64_t j = b0 + i * 1024 + threadIdx.x;
+        seg[i * 1024 + threadIdx.x] = j < n ? counts[j] : 0u;
+    }
+    __syncthreads();
+    unsigned c[8], v = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {  // thread t scans counts 8t..8t+7 of the segment
+        c[i] = seg[8 * threadIdx.x + i];
+        v += c[i];
+    }
+    unsigned tot;
+    unsigned run = block_exclusive_scan_1024(v, sh, tot) + before;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        seg[8 * threadIdx.x + i] = run;
+        run += c[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int64_t j = b0 + i * 1024 + threadIdx.x;
+        if (j < n) offsets[j] = seg[i * 1024 + threadIdx.x];
+    }
+    if (b0 + kScanSeg >= n && threadIdx.x == 0) offsets[n] = before + tot;
+}
+
+// Counting-sort pass 2: particle indices grouped by key.
 static __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned* __restrict__ key, const unsigned* __restrict__ slot,
                                                      const unsigned* __restrict__ offsets, unsigned* __restrict__ sorted) {
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x)
         sorted[offsets[key[p]] + slot[p]] = (unsigned)p;
 }
 
-// Position of core cell (a, b, c) with a == 0, b == 0 or c == 0 in a tile's
-// low-face record (ex, ey, ez: the tile's extents); inverse: low_cell.
-__device__ __forceinline__ int low_index(int a, int b, int c, int ex, int ey, int ez) {
-    if (a == 0) return b * ez + c;
-    if (b == 0) return ey * ez + (a - 1) * ez + c;
-    return ey * ez + (ex - 1) * ez + (a - 1) * (ey - 1) + (b - 1);
-}
-__device__ __forceinline__ void low_cell(int i, int ex, int ey, int ez, int& a, int& b, int& c) {
-    if (i < ey * ez) { a = 0; b = i / ez; c = i - b * ez; return; }
-    i -= ey * ez;
-    if (i < (ex - 1) * ez) { a = 1 + i / ez; b = 0; c = i - (a - 1) * ez; return; }
-    i -= (ex - 1) * ez;
-    a = 1 + i / (ey - 1);
-    b = 1 + (i - (a - 1) * (ey - 1));
-    c = 0;
-}
-
-// One workgroup per tile: LDS fixed-point deposit.  Interior cells (no
-// neighbour contributes) are final and written as FP values; low-face cells
-// go to the ``low`` record and the +1 faces to the ``halo`` record, both as
-// integers, for k_tiles_fixup.
-template <typename T, int NC, int TS, int TB = 256>
+// One workgroup per tile: the tile's cells are summed completely in LDS and
+// written once.  The workgroup reads its own particles (all 8 face classes)
+// and, from the 7 lower neighbour tiles, the particles whose stencil reaches
+// into it (the face classes holding those neighbours' upper faces); every
+// particle deposits only its corners that lie in the tile.  A particle read by
+// several tiles computes the same 8 integer deposits in each.
+template <typename T, int NC, int TS, int TB>
 __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ mass,
                                                      T mscalar, Mesh m, TileGeom tg, const unsigned* __restrict__ offsets,
-                                                     const unsigned* __restrict__ sorted, unsigned long long* __restrict__ low,
-                                                     unsigned long long* __restrict__ halo, const unsigned long long* __restrict__ hdr,
+                                                     const unsigned* __restrict__ sorted, const unsigned long long* __restrict__ hdr,
                                                      double scale, T* __restrict__ out) {
-    constexpr int B = TS + 1, B3 = B * B * B;
+    constexpr int B3 = TS * TS * TS;
+    constexpr int NR = 20;  // 1 own range + 12 + 6 + 1 neighbour ranges
     __shared__ unsigned long long box[NC * B3];
+    __shared__ unsigned rstart[NR], rpre[NR + 1];
+    __shared__ unsigned char rcode[NR];
     // XCD-contiguous tile order: the tiles an XCD runs together are z-neighbours,
     // whose particles share cache lines of the Lagrangian-ordered arrays
     const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
@@ -614,6 +683,43 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     const int64_t lo[3] = {tx * TS, ty * TS, tz * TS};
     const int ex = (int)tg.extent(0, tx, m), ey = (int)tg.extent(1, ty, m), ez = (int)tg.extent(2, tz, m);
     for (int i = threadIdx.x; i < NC * B3; i += TB) box[i] = 0ull;
+    if (threadIdx.x < 64) {
+        // range r (lane r < NR): r = 0 is the own tile (all face classes); the
+        // others are (dir, fc) pairs -- bit d of dir: the tile one lower along
+        // axis d (periodic) -- with (fc & dir) == dir, packed 6 bits per range
+        // as dir | fc << 3
+        constexpr unsigned long long kLo = 0x6fac9a4b9a59240ull, kHi = 0xffedbdb7cd2c93bull;
+        const int r = threadIdx.x;
+        unsigned len = 0, start = 0, dir = 0;
+        if (r < NR) {
+            const unsigned v = (unsigned)((r < 10 ? kLo >> (6 * r) : kHi >> (6 * (r - 10))) & 63ull);
+            dir = v & 7u;
+            const unsigned fc = v >> 3;
+            if (r == 0) {
+                start = offsets[8 * t];
+                len = offsets[8 * t + 8] - start;
+            } else {
+                const int64_t nx = (dir & 1) ? (tx == 0 ? tg.nt[0] - 1 : tx - 1) : tx;
+                const int64_t ny = (dir & 2) ? (ty == 0 ? tg.nt[1] - 1 : ty - 1) : ty;
+                const int64_t nz = (dir & 4) ? (tz == 0 ? tg.nt[2] - 1 : tz - 1) : tz;
+                const int64_t k = 8 * ((nx * tg.nt[1] + ny) * tg.nt[2] + nz) + fc;
+                start = offsets[k];
+                len = offsets[k + 1] - start;
+            }
+        }
+        unsigned inc = len;  // inclusive scan of the range lengths over the wave
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const unsigned y = __shfl_up(inc, o);
+            if (r >= o) inc += y;
+        }
+        if (r < NR) {
+            rstart[r] = start;
+            rcode[r] = (unsigned char)dir;
+            rpre[r] = inc - len;
+            if (r == NR - 1) rpre[NR] = inc;
+        }
+    }
     int s[NC];
     double fs[NC];
 #pragma unroll
@@ -622,16 +728,29 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
         fs[c] = ldexp(scale, -s[c]);
     }
     __syncthreads();
-    const unsigned j0 = offsets[t], j1 = offsets[t + 1];
+    const unsigned total = rpre[NR];
     // KB particles per thread per round; all their loads are issued before any
     // deposit so the dependent sorted[j] -> pos[p] chains overlap
     constexpr int KB = 4;
-    for (unsigned jb = j0; jb < j1; jb += TB * KB) {
+    for (unsigned jb = 0; jb < total; jb += TB * KB) {
         unsigned pi[KB];
+        int code[KB];
 #pragma unroll
         for (int k = 0; k < KB; k++) {
-            const unsigned j = jb + k * TB + threadIdx.x;
-            pi[k] = sorted[j < j1 ? j : j1 - 1];
+            unsigned j = jb + k * TB + threadIdx.x;
+            j = j < total ? j : total - 1;
+            int r = 0;
+            if (j >= rpre[1]) {  // binary search over the neighbour ranges 1..NR-1
+                int a = 1, b = NR - 1;
+                while (a < b) {
+                    const int mid = (a + b + 1) >> 1;
+                    if (rpre[mid] <= j) a = mid;
+                    else b = mid - 1;
+                }
+                r = a;
+            }
+            pi[k] = sorted[rstart[r] + (j - rpre[r])];
+            code[k] = rcode[r];
         }
         T xs[KB][3];
         T ms[KB][NC];
@@ -646,90 +765,42 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
         }
 #pragma unroll
         for (int k = 0; k < KB; k++) {
-        if (jb + k * TB + threadIdx.x >= j1) continue;
-        const int64_t p = pi[k];
-        int64_t idx[3];
-        T f[3];
-        pd.decode_xyz(xs[k][0], xs[k][1], xs[k][2], p, idx, f);
-        const int a = (int)(idx[0] - lo[0]), b = (int)(idx[1] - lo[1]), cz = (int)(idx[2] - lo[2]);
-        const int base = (a * B + b) * B + cz;
+            if (jb + k * TB + threadIdx.x >= total) continue;
+            const int64_t p = pi[k];
+            int64_t idx[3];
+            T f[3];
+            pd.decode_xyz(xs[k][0], xs[k][1], xs[k][2], p, idx, f);
+            // local lower corner: -1 along the axes the particle was read from a
+            // lower neighbour, else its offset in this tile
+            const int a = (code[k] & 1) ? -1 : (int)(idx[0] - lo[0]);
+            const int b = (code[k] & 2) ? -1 : (int)(idx[1] - lo[1]);
+            const int cz = (code[k] & 4) ? -1 : (int)(idx[2] - lo[2]);
+            const bool in_x[2] = {a >= 0, a + 1 < ex};
+            const bool in_y[2] = {b >= 0, b + 1 < ey};
+            const bool in_z[2] = {cz >= 0, cz + 1 < ez};
+            const int base = (a * TS + b) * TS + cz;
 #pragma unroll
-        for (int c = 0; c < NC; c++) {
-            long long q[8];
-            deposits<T>(f, (double)ms[k][c], s[c], q);
+            for (int c = 0; c < NC; c++) {
+                long long q[8];
+                deposits<T>(f, (double)ms[k][c], s[c], q);
 #pragma unroll
-            for (int qq = 0; qq < 8; qq++)
-                if (q[qq] != 0) atomicAdd(&box[c * B3 + base + (qq >> 2) * B * B + ((qq >> 1) & 1) * B + (qq & 1)], (unsigned long long)q[qq]);
-        }
+                for (int qq = 0; qq < 8; qq++) {
+                    const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
+                    if (in_x[dx] && in_y[dy] && in_z[dz] && q[qq] != 0)
+                        atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (unsigned long long)q[qq]);
+                }
+            }
         }
     }
     __syncthreads();
     const int64_t cells = m.cells();
-    const int64_t lstride = tg.lstride, hstride = tg.hstride;
-    const int e1z = ez + 1, e1y = ey + 1;
-    const int nbox = (ex + 1) * e1y * e1z;
+    const int nbox = ex * ey * ez;
     for (int i = threadIdx.x; i < nbox; i += TB) {
-        const int cc = i % e1z, bb = (i / e1z) % e1y, aa = i / (e1z * e1y);
-        const int li = (aa * B + bb) * B + cc;
-        if (aa < ex && bb < ey && cc < ez) {
-            if (aa && bb && cc) {
-                const int64_t g = ((lo[0] + aa) * m.n1 + (lo[1] + bb)) * m.n2 + (lo[2] + cc);
+        const int cc = i % ez, bb = (i / ez) % ey, aa = i / (ez * ey);
+        const int li = (aa * TS + bb) * TS + cc;
+        const int64_t g = ((lo[0] + aa) * m.n1 + (lo[1] + bb)) * m.n2 + (lo[2] + cc);
 #pragma unroll
-                for (int c = 0; c < NC; c++) out[c * cells + g] = (T)((double)(long long)box[c * B3 + li] * fs[c]);
-            } else {
-                const int64_t h = t * lstride + low_index(aa, bb, cc, ex, ey, ez);
-#pragma unroll
-                for (int c = 0; c < NC; c++) low[c * tg.ntiles * lstride + h] = box[c * B3 + li];
-            }
-        } else {
-            const int64_t h = t * hstride + halo_index(aa, bb, cc, ex, ey, ez);
-#pragma unroll
-            for (int c = 0; c < NC; c++) halo[c * tg.ntiles * hstride + h] = box[c * B3 + li];
-        }
-    }
-}
-
-// Low-face cells: own value + the +1 faces of the lower neighbour tiles,
-// then the FP conversion.  One thread per low-face cell.
-template <typename T, int NC>
-__global__ void __launch_bounds__(256) k_tiles_fixup(const unsigned long long* __restrict__ low,
-                                                     const unsigned long long* __restrict__ halo,
-                                                     const unsigned long long* __restrict__ hdr, int64_t np, Mesh m,
-                                                     TileGeom tg, double scale, T* __restrict__ out) {
-    double f[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) f[c] = ldexp(scale, -fixed_shift<T>(hdr[c], np));
-    const int64_t cells = m.cells();
-    const unsigned total = (unsigned)(tg.ntiles * tg.lstride);  // < 2^31 for meshes up to 2048^3
-    const unsigned ls = (unsigned)tg.lstride, nt1 = (unsigned)tg.nt[1], nt2 = (unsigned)tg.nt[2];
-    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        const unsigned t = i / ls;
-        const int li = (int)(i - t * ls);
-        const unsigned tq = t / nt2;
-        const int64_t tz = t - tq * nt2, ty = tq % nt1, tx = tq / nt1;
-        const int ex = (int)tg.extent(0, tx, m), ey = (int)tg.extent(1, ty, m), ez = (int)tg.extent(2, tz, m);
-        if (li >= ex * ey * ez - (ex - 1) * (ey - 1) * (ez - 1)) continue;  // partial tile: shorter record
-        int a, b, c;
-        low_cell(li, ex, ey, ez, a, b, c);
-        long long v[NC];
-#pragma unroll
-        for (int k = 0; k < NC; k++) v[k] = (long long)low[k * tg.ntiles * tg.lstride + i];
-#pragma unroll
-        for (int q = 1; q < 8; q++) {
-            const int sx = q >> 2, sy = (q >> 1) & 1, sz = q & 1;
-            if ((sx && a) || (sy && b) || (sz && c)) continue;
-            const int64_t nx = sx ? (tx == 0 ? tg.nt[0] - 1 : tx - 1) : tx;
-            const int64_t ny = sy ? (ty == 0 ? tg.nt[1] - 1 : ty - 1) : ty;
-            const int64_t nz = sz ? (tz == 0 ? tg.nt[2] - 1 : tz - 1) : tz;
-            const int nex = (int)tg.extent(0, nx, m), ney = (int)tg.extent(1, ny, m), nez = (int)tg.extent(2, nz, m);
-            const int64_t nt = (nx * tg.nt[1] + ny) * tg.nt[2] + nz;
-            const int64_t h = nt * tg.hstride + halo_index(sx ? nex : a, sy ? ney : b, sz ? nez : c, nex, ney, nez);
-#pragma unroll
-            for (int k = 0; k < NC; k++) v[k] += (long long)halo[k * tg.ntiles * tg.hstride + h];
-        }
-        const int64_t g = ((tx * tg.ts + a) * m.n1 + (ty * tg.ts + b)) * m.n2 + (tz * tg.ts + c);
-#pragma unroll
-        for (int k = 0; k < NC; k++) out[k * cells + g] = (T)((double)v[k] * f[k]);
+        for (int c = 0; c < NC; c++) out[c * cells + g] = (T)((double)(long long)box[c * B3 + li] * fs[c]);
     }
 }
 
@@ -739,9 +810,9 @@ template <typename T>
 int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,
                const int64_t nmesh[3], double out_scale, int32_t flags, T* out, void* ws, size_t ws_bytes,
                hipStream_t st, unsigned* bins_out = nullptr, const unsigned* bins_in = nullptr);
-// Particle sort by 16^3-cell tile that a 1-component paint can hand to a later
-// 3-component paint of the same positions: offsets (ntiles + 1) then sorted
-// particle indices (np), uint32.
+// Particle sort by 16^3-cell tile and face class that a 1-component paint can
+// hand to a later 3-component paint of the same positions: offsets
+// (8 ntiles + 1) then sorted particle indices (np), uint32.
 size_t paint_bins_bytes(const Mesh& m, int64_t np);
 template <typename T>
 int readout_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* field, const int64_t nmesh[3],

@@ -18,26 +18,27 @@ namespace ldl {
 static constexpr int kBlock = 256;
 static constexpr int64_t kHdrBytes = 256;
 
-static constexpr int kTileSide1 = 16;  // one-component paint: 17^3 x 8 B = 39 KB of LDS
-static constexpr int kTileSide3 = 8;   // three components: 3 x 9^3 x 8 B = 17.5 KB of LDS
+// Tile side of the tiled paint: 16^3 cells = 32 KB of 64-bit LDS accumulators
+// per component (one component: several workgroups per CU; three: 96 KB, one
+// workgroup of 1024 threads per CU).
+static constexpr int kTileSide = 16;
 
 static inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Workspace: header | direct-mode mesh or tile low faces (nc) | tile halos (nc) |
-// counts | offsets | key | slot | sorted
+// Workspace: header | direct-mode mesh (nc) | counts | offsets | segment sums |
+// key | slot | sorted
 struct PaintWs {
-    size_t acc, halo, counts, offsets, key, slot, sorted, total;
+    size_t acc, counts, offsets, segsum, key, slot, sorted, total;
 };
 static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
-    // the smaller tile side has the larger face records and tile count
-    const TileGeom tg = make_tiles(m, nc == 1 ? kTileSide1 : kTileSide3);
-    const size_t low = (size_t)nc * tg.ntiles * tg.lstride * 8, direct = (size_t)nc * m.cells() * 8;
+    const TileGeom tg = make_tiles(m, kTileSide);
+    const int64_t nk = tg.nkeys();
     PaintWs w;
     w.acc = al256((size_t)kHdrBytes);
-    w.halo = w.acc + al256(low > direct ? low : direct);
-    w.counts = w.halo + al256((size_t)nc * tg.ntiles * tg.hstride * 8);
-    w.offsets = w.counts + al256((size_t)(tg.ntiles + 1) * 4);
-    w.key = w.offsets + al256((size_t)(tg.ntiles + 1) * 4);
+    w.counts = w.acc + al256((size_t)nc * m.cells() * 8);
+    w.offsets = w.counts + al256((size_t)(nk + 1) * 4);
+    w.segsum = w.offsets + al256((size_t)(nk + 1) * 4);
+    w.key = w.segsum + al256((size_t)(nk / kScanSeg + 1) * 4);
     w.slot = w.key + al256((size_t)np * 4);
     w.sorted = w.slot + al256((size_t)np * 4);
     w.total = w.sorted + al256((size_t)np * 4);
@@ -47,8 +48,8 @@ static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
 size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np) { return paint_ws(m, nc, np < 0 ? 0 : np).total; }
 
 size_t paint_bins_bytes(const Mesh& m, int64_t np) {
-    const TileGeom tg = make_tiles(m, kTileSide1);
-    return (size_t)(tg.ntiles + 1 + (np < 0 ? 0 : np)) * 4;
+    const TileGeom tg = make_tiles(m, kTileSide);
+    return (size_t)(tg.nkeys() + 1 + (np < 0 ? 0 : np)) * 4;
 }
 
 template <typename T>
@@ -115,60 +116,52 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         LDL_LAUNCH_CHECK();
         return LDL_OK;
     }
-    // tiled path: counting sort by tile, LDS deposit, halo fold.  A 3-component
-    // paint that is handed the sort of an earlier 1-component paint (bins_in)
-    // uses that paint's 16^3 tiles and skips the sort.
-    const int ts = (nc == 1 || bins_in) ? kTileSide1 : kTileSide3;
-    const TileGeom tg = make_tiles(m, ts);
-    auto* halo = reinterpret_cast<unsigned long long*>(wb + w.halo);
+    // tiled path: counting sort by (tile, face class), then one workgroup per
+    // tile sums the tile's cells completely in LDS.  A 3-component paint that
+    // is handed the sort of an earlier 1-component paint of the same positions
+    // (bins_in) skips the sort.
+    const TileGeom tg = make_tiles(m, kTileSide);
+    const int64_t nk = tg.nkeys();
     auto* counts = reinterpret_cast<unsigned*>(wb + w.counts);
+    auto* segsum = reinterpret_cast<unsigned*>(wb + w.segsum);
     auto* key = reinterpret_cast<unsigned*>(wb + w.key);
     auto* slot = reinterpret_cast<unsigned*>(wb + w.slot);
     unsigned* offsets = reinterpret_cast<unsigned*>(wb + w.offsets);
     unsigned* sorted = reinterpret_cast<unsigned*>(wb + w.sorted);
     if (bins_out) {
         offsets = bins_out;
-        sorted = bins_out + tg.ntiles + 1;
+        sorted = bins_out + nk + 1;
     }
     if (bins_in) {
         offsets = const_cast<unsigned*>(bins_in);
-        sorted = const_cast<unsigned*>(bins_in) + tg.ntiles + 1;
+        sorted = const_cast<unsigned*>(bins_in) + nk + 1;
     } else {
-        hipLaunchKernelGGL(k_fill<unsigned>, dim3(grid_for(tg.ntiles + 1, kBlock, 1 << 16)), dim3(kBlock), 0, st, counts, tg.ntiles + 1, 0u);
+        hipLaunchKernelGGL(k_fill<unsigned>, dim3(grid_for(nk, kBlock, 1 << 16)), dim3(kBlock), 0, st, counts, nk, 0u);
         if (np > 0) {
             constexpr int PPT = 8;
             KTimer _t("paint_bin", st);
             hipLaunchKernelGGL((k_bin_keys<T, PPT>), dim3(grid_for((np + 256 * PPT - 1) / (256 * PPT), 1, 1 << 15)), dim3(256), 0, st,
                                pos, pd, np, m, tg, counts, key, slot);
         }
-        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, offsets, tg.ntiles);
+        {
+            KTimer _t("paint_scan", st);
+            const unsigned nseg = (unsigned)((nk + kScanSeg - 1) / kScanSeg);
+            hipLaunchKernelGGL(k_scan_sums, dim3(nseg), dim3(1024), 0, st, counts, nk, segsum);
+            hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(1024), 0, st, counts, nk, segsum, offsets);
+        }
         if (np > 0) {
             KTimer _t("paint_scatter", st);
             hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, st, np, key, slot, offsets, sorted);
         }
     }
-    auto* low = acc;
     if (nc == 1) {
         KTimer _t("paint_deposit1", st);
-        hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide1>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, tg,
-                           offsets, sorted, low, halo, hdr, out_scale, out);
-    } else if (ts == kTileSide3) {
-        KTimer _t("paint_deposit3", st);
-        hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide3>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, tg,
-                           offsets, sorted, low, halo, hdr, out_scale, out);
+        hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide, 256>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m,
+                           tg, offsets, sorted, hdr, out_scale, out);
     } else {
-        // 3 x 17^3 x 8 B = 118 KB of LDS: one workgroup per CU, so use 1024 threads
         KTimer _t("paint_deposit3", st);
-        hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide1, 1024>), dim3((unsigned)tg.ntiles), dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m,
-                           tg, offsets, sorted, low, halo, hdr, out_scale, out);
-    }
-    {
-        KTimer _t(nc == 1 ? "paint_fixup1" : "paint_fixup3", st);
-        const unsigned gf = grid_for(tg.ntiles * tg.lstride, kBlock, 1 << 16);
-        if (nc == 1)
-            hipLaunchKernelGGL((k_tiles_fixup<T, 1>), dim3(gf), dim3(kBlock), 0, st, low, halo, hdr, np, m, tg, out_scale, out);
-        else
-            hipLaunchKernelGGL((k_tiles_fixup<T, 3>), dim3(gf), dim3(kBlock), 0, st, low, halo, hdr, np, m, tg, out_scale, out);
+        hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, 1024>), dim3((unsigned)tg.ntiles), dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar,
+                           m, tg, offsets, sorted, hdr, out_scale, out);
     }
     LDL_LAUNCH_CHECK();
     return LDL_OK;
