@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstring>
+#include <type_traits>
 
 #include "ldl_common.h"
 
@@ -29,25 +30,31 @@ __device__ __forceinline__ int fixed_shift(unsigned long long maxbits, int64_t n
 // m at scale 2^s: q[4a+2b+c] = rint((w_x[a] w_y[b]) (w_z[c] (m 2^s))), and
 // q[7] = rint(m 2^s) - (q[0] + ... + q[6]) so the deposits sum exactly to
 // the rounded mass.  The CPU emulation (oracle/cic_oracle.c) performs the
-// same operations in the same order.
+// same operations in the same order.  FP32 path: |m 2^s| < 2^30 and every
+// weight is in [0, 1], so each deposit and every partial sum (all of the sign
+// of m, in magnitude at most |m 2^s| + 4) fit in 32 bits.
+template <typename T> using DepositT = typename std::conditional<sizeof(T) == 4, int, long long>::type;
+
 template <typename T>
-__device__ __forceinline__ void deposits(const T f[3], double m, int s, long long q[8]) {
+__device__ __forceinline__ void deposits(const T f[3], double m, int s, DepositT<T> q[8]) {
     const double wx[2] = {(double)(T(1) - f[0]), (double)f[0]};
     const double wy[2] = {(double)(T(1) - f[1]), (double)f[1]};
     const double ms = ldexp(m, s);
     const double wzm[2] = {(double)(T(1) - f[2]) * ms, (double)f[2] * ms};
     const double wxy[4] = {wx[0] * wy[0], wx[0] * wy[1], wx[1] * wy[0], wx[1] * wy[1]};
-    long long part = 0;
+    DepositT<T> part = 0;
 #pragma unroll
     for (int k = 0; k < 7; k++) {
         const double w = wxy[k >> 1] * wzm[k & 1];
-        long long v;
-        if (sizeof(T) == 4) v = (long long)__double2int_rn(w);  // |w| < 2^30
+        DepositT<T> v;
+        if constexpr (sizeof(T) == 4) v = __double2int_rn(w);
         else v = __double2ll_rn(w);
         q[k] = v;
         part += v;
     }
-    const long long tot = sizeof(T) == 4 ? (long long)__double2int_rn(ms) : __double2ll_rn(ms);
+    DepositT<T> tot;
+    if constexpr (sizeof(T) == 4) tot = __double2int_rn(ms);
+    else tot = __double2ll_rn(ms);
     q[7] = tot - part;
 }
 
@@ -156,12 +163,12 @@ __global__ void __launch_bounds__(256) k_paint_fixed(const T* __restrict__ pos, 
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             const double mm = mass ? (double)mass[p * NC + c] : (double)mscalar;
-            long long q[8];
+            DepositT<T> q[8];
             deposits<T>(f, mm, s[c], q);
             unsigned long long* a = acc + c * cells;
 #pragma unroll
             for (int k = 0; k < 8; k++)
-                if (q[k] != 0) atomicAdd(&a[(ii[k >> 2] * m.n1 + jj[(k >> 1) & 1]) * m.n2 + kk[k & 1]], (unsigned long long)q[k]);
+                if (q[k] != 0) atomicAdd(&a[(ii[k >> 2] * m.n1 + jj[(k >> 1) & 1]) * m.n2 + kk[k & 1]], (unsigned long long)(long long)q[k]);
         }
     }
 }
@@ -498,18 +505,32 @@ inline TileGeom make_tiles(const Mesh& m, int ts) {
     return g;
 }
 
-// Sort key of a particle whose CIC lower corner is cell idx: 8 * tile + face
+// Lower CIC corner (32-bit cell indices) and fractions of particle p: on a
+// power-of-two mesh (P2) with mask wraps, else with the general decode.
+template <typename T, bool P2>
+__device__ __forceinline__ void cell_of(const PosDec<T>& pd, const MeshP2& mp, T x0, T x1, T x2, int64_t p, uint32_t id[3], T f[3]) {
+    if constexpr (P2) {
+        pd.decode_p2(x0, x1, x2, p, mp.msk, id, f);
+    } else {
+        int64_t idx[3];
+        pd.decode_xyz(x0, x1, x2, p, idx, f);
+#pragma unroll
+        for (int d = 0; d < 3; d++) id[d] = (uint32_t)idx[d];
+    }
+}
+
+// Sort key of a particle whose CIC lower corner is cell id: 8 * tile + face
 // class, where face-class bit d is set when the corner lies on the tile's
 // upper face along axis d -- exactly the particles whose stencil also reaches
 // the next tile along d.
-__device__ __forceinline__ unsigned tile_key(const int64_t idx[3], const TileGeom& tg, const Mesh& m) {
+__device__ __forceinline__ unsigned tile_key(const uint32_t id[3], const TileGeom& tg, const Mesh& m) {
     unsigned t[3], fc = 0;
     const int64_t n[3] = {m.n0, m.n1, m.n2};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        t[d] = (unsigned)idx[d] >> tg.tsh;
+        t[d] = id[d] >> tg.tsh;
         const int64_t hi = ((int64_t)t[d] + 1) * tg.ts;  // one past the tile's last cell
-        if (idx[d] + 1 == (hi < n[d] ? hi : n[d])) fc |= 1u << d;
+        if ((int64_t)id[d] + 1 == (hi < n[d] ? hi : n[d])) fc |= 1u << d;
     }
     return (((t[0] * (unsigned)tg.nt[1] + t[1]) * (unsigned)tg.nt[2] + t[2]) << 3) | fc;
 }
@@ -520,8 +541,8 @@ __device__ __forceinline__ unsigned tile_key(const int64_t idx[3], const TileGeo
 // wavefront: each run's first lane reserves the run's ranks with one global
 // atomic and broadcasts the base to the run.  Rank order inside a key depends
 // on atomic order, which no paint result depends on (integer sums).
-template <typename T, int PPT>
-__global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Mesh m, TileGeom tg,
+template <typename T, int PPT, bool P2>
+__global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Mesh m, MeshP2 mp, TileGeom tg,
                                                   unsigned* __restrict__ counts, unsigned* __restrict__ key,
                                                   unsigned* __restrict__ slot) {
     constexpr int CH = 256 * PPT;
@@ -537,10 +558,10 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
             const int64_t p = ch * CH + q * 256 + threadIdx.x;
             k[q] = EMPTY;
             if (p < np) {
-                int64_t idx[3];
+                uint32_t id[3];
                 T f[3];
-                pd.decode(pos, p, idx, f);
-                k[q] = tile_key(idx, tg, m);
+                cell_of<T, P2>(pd, mp, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, id, f);
+                k[q] = tile_key(id, tg, m);
             }
         }
         // run heads, run lengths and one atomic per run (all issued before any result is used)
@@ -666,9 +687,9 @@ static __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const un
 // into it (the face classes holding those neighbours' upper faces); every
 // particle deposits only its corners that lie in the tile.  A particle read by
 // several tiles computes the same 8 integer deposits in each.
-template <typename T, int NC, int TS, int TB>
+template <typename T, int NC, int TS, int TB, bool P2>
 __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ mass,
-                                                     T mscalar, Mesh m, TileGeom tg, const unsigned* __restrict__ offsets,
+                                                     T mscalar, Mesh m, MeshP2 mp, TileGeom tg, const unsigned* __restrict__ offsets,
                                                      const unsigned* __restrict__ sorted, const unsigned long long* __restrict__ hdr,
                                                      double scale, T* __restrict__ out) {
     constexpr int B3 = TS * TS * TS;
@@ -767,37 +788,36 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
         for (int k = 0; k < KB; k++) {
             if (jb + k * TB + threadIdx.x >= total) continue;
             const int64_t p = pi[k];
-            int64_t idx[3];
+            uint32_t id[3];
             T f[3];
-            pd.decode_xyz(xs[k][0], xs[k][1], xs[k][2], p, idx, f);
+            cell_of<T, P2>(pd, mp, xs[k][0], xs[k][1], xs[k][2], p, id, f);
             // local lower corner: -1 along the axes the particle was read from a
             // lower neighbour, else its offset in this tile
-            const int a = (code[k] & 1) ? -1 : (int)(idx[0] - lo[0]);
-            const int b = (code[k] & 2) ? -1 : (int)(idx[1] - lo[1]);
-            const int cz = (code[k] & 4) ? -1 : (int)(idx[2] - lo[2]);
+            const int a = (code[k] & 1) ? -1 : (int)(id[0] - (uint32_t)lo[0]);
+            const int b = (code[k] & 2) ? -1 : (int)(id[1] - (uint32_t)lo[1]);
+            const int cz = (code[k] & 4) ? -1 : (int)(id[2] - (uint32_t)lo[2]);
             const bool in_x[2] = {a >= 0, a + 1 < ex};
             const bool in_y[2] = {b >= 0, b + 1 < ey};
             const bool in_z[2] = {cz >= 0, cz + 1 < ez};
             const int base = (a * TS + b) * TS + cz;
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                long long q[8];
+                DepositT<T> q[8];
                 deposits<T>(f, (double)ms[k][c], s[c], q);
 #pragma unroll
                 for (int qq = 0; qq < 8; qq++) {
                     const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
                     if (in_x[dx] && in_y[dy] && in_z[dz] && q[qq] != 0)
-                        atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (unsigned long long)q[qq]);
+                        atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (unsigned long long)(long long)q[qq]);
                 }
             }
         }
     }
     __syncthreads();
     const int64_t cells = m.cells();
-    const int nbox = ex * ey * ez;
-    for (int i = threadIdx.x; i < nbox; i += TB) {
-        const int cc = i % ez, bb = (i / ez) % ey, aa = i / (ez * ey);
-        const int li = (aa * TS + bb) * TS + cc;
+    for (int li = threadIdx.x; li < B3; li += TB) {
+        const int cc = li % TS, bb = (li / TS) % TS, aa = li / (TS * TS);
+        if (aa >= ex || bb >= ey || cc >= ez) continue;
         const int64_t g = ((lo[0] + aa) * m.n1 + (lo[1] + bb)) * m.n2 + (lo[2] + cc);
 #pragma unroll
         for (int c = 0; c < NC; c++) out[c * cells + g] = (T)((double)(long long)box[c * B3 + li] * fs[c]);

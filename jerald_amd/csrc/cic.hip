@@ -122,6 +122,10 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     // (bins_in) skips the sort.
     const TileGeom tg = make_tiles(m, kTileSide);
     const int64_t nk = tg.nkeys();
+    // power-of-two meshes decode with 32-bit mask wraps (bin and deposit kernels
+    // must use the same decode)
+    const bool p2 = is_pow2(m.n0) && is_pow2(m.n1) && is_pow2(m.n2);
+    const MeshP2 mp = mesh_p2(m);
     auto* counts = reinterpret_cast<unsigned*>(wb + w.counts);
     auto* segsum = reinterpret_cast<unsigned*>(wb + w.segsum);
     auto* key = reinterpret_cast<unsigned*>(wb + w.key);
@@ -140,8 +144,9 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         if (np > 0) {
             constexpr int PPT = 8;
             KTimer _t("paint_bin", st);
-            hipLaunchKernelGGL((k_bin_keys<T, PPT>), dim3(grid_for((np + 256 * PPT - 1) / (256 * PPT), 1, 1 << 15)), dim3(256), 0, st,
-                               pos, pd, np, m, tg, counts, key, slot);
+            const dim3 gb(grid_for((np + 256 * PPT - 1) / (256 * PPT), 1, 1 << 15));
+            if (p2) hipLaunchKernelGGL((k_bin_keys<T, PPT, true>), gb, dim3(256), 0, st, pos, pd, np, m, mp, tg, counts, key, slot);
+            else hipLaunchKernelGGL((k_bin_keys<T, PPT, false>), gb, dim3(256), 0, st, pos, pd, np, m, mp, tg, counts, key, slot);
         }
         {
             KTimer _t("paint_scan", st);
@@ -154,14 +159,23 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
             hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, st, np, key, slot, offsets, sorted);
         }
     }
+    const dim3 gt((unsigned)tg.ntiles);
     if (nc == 1) {
         KTimer _t("paint_deposit1", st);
-        hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide, 256>), dim3((unsigned)tg.ntiles), dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m,
-                           tg, offsets, sorted, hdr, out_scale, out);
+        if (p2)
+            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide, 256, true>), gt, dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
+                               tg, offsets, sorted, hdr, out_scale, out);
+        else
+            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide, 256, false>), gt, dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
+                               tg, offsets, sorted, hdr, out_scale, out);
     } else {
         KTimer _t("paint_deposit3", st);
-        hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, 1024>), dim3((unsigned)tg.ntiles), dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar,
-                           m, tg, offsets, sorted, hdr, out_scale, out);
+        if (p2)
+            hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, 1024, true>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
+                               tg, offsets, sorted, hdr, out_scale, out);
+        else
+            hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, 1024, false>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
+                               tg, offsets, sorted, hdr, out_scale, out);
     }
     LDL_LAUNCH_CHECK();
     return LDL_OK;
