@@ -311,15 +311,28 @@ struct KTimer {
 };
 
 // ----------------------------------------------------- elementwise math
-// Positive-base power, log and exp: the FP32 path uses the hardware
-// exp2/log2-based intrinsics (a few ulp), the FP64 path the accurate library
-// routines.
-__device__ __forceinline__ float ppow(float x, float y) { return __powf(x, y); }
+// Power, log and exp of positive arguments: the FP32 path uses the hardware
+// log2 / exp2 / reciprocal instructions (a few ulp), the FP64 path the
+// accurate library routines.
+constexpr float kLn2f = 0.693147180559945309f;
+__device__ __forceinline__ float ppow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
 __device__ __forceinline__ double ppow(double x, double y) { return pow(x, y); }
-__device__ __forceinline__ float plog(float x) { return __logf(x); }
+__device__ __forceinline__ float plog(float x) { return __builtin_amdgcn_logf(x) * kLn2f; }
 __device__ __forceinline__ double plog(double x) { return log(x); }
 __device__ __forceinline__ float pexp(float x) { return __expf(x); }
 __device__ __forceinline__ double pexp(double x) { return exp(x); }
+// x^a, x^(a-1) and ln x together (x > 0)
+__device__ __forceinline__ void pow_pair_log(float x, float a, float& p, float& p1, float& ln) {
+    const float l2 = __builtin_amdgcn_logf(x);
+    p = __builtin_amdgcn_exp2f(a * l2);
+    p1 = p * __builtin_amdgcn_rcpf(x);
+    ln = l2 * kLn2f;
+}
+__device__ __forceinline__ void pow_pair_log(double x, double a, double& p, double& p1, double& ln) {
+    ln = log(x);
+    p = pow(x, a);
+    p1 = pow(x, a - 1.0);
+}
 
 // --------------------------------------------------------- reductions
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {

@@ -45,18 +45,37 @@ __global__ void k_axis_tables(T* __restrict__ k, T* __restrict__ D, int64_t n, i
 template <typename T> struct KTabs {
     const T* k[3];
     const T* D[3];
-    __device__ __forceinline__ T knorm(int64_t ix, int64_t iy, int64_t iz) const {
+    // |k|^2 with |k| = 1e-8 at k = 0 (model.py:102)
+    __device__ __forceinline__ T knorm2(int64_t ix, int64_t iy, int64_t iz) const {
         const T a = k[0][ix], b = k[1][iy], c = k[2][iz];
-        T kn = sqrt(a * a + b * b + c * c);
-        return kn == T(0) ? T(1e-8) : kn;  // model.py:102
+        const T k2 = a * a + b * b + c * c;
+        return k2 == T(0) ? T(1e-8) * T(1e-8) : k2;
     }
 };
 
+// G(k) = -exp(-k^2/kl^2 - kh^2/k^2) |k|^nu (model.py:195), from k^2, with the
+// terms the parameter gradients reuse: 1/k^2 and ln|k|.  FP32: hardware
+// reciprocal, log2 and exp2; FP64: library routines.
 template <typename T> struct SmFilter {
     T kh2, inv_kl2, nu;
-    __device__ __forceinline__ T G(T kn) const {  // model.py:195
-        const T k2 = kn * kn;
-        return -pexp(-k2 * inv_kl2 - kh2 / k2) * ppow(kn, nu);
+    __device__ __forceinline__ void eval(T k2, T& G, T& ik2, T& lnk) const {
+        if constexpr (sizeof(T) == 4) {
+            ik2 = __builtin_amdgcn_rcpf(k2);
+            const float l2 = __builtin_amdgcn_logf(k2);  // log2 k^2
+            lnk = 0.5f * kLn2f * l2;
+            const float pw = nu == 0.0f ? 1.0f : __builtin_amdgcn_exp2f(0.5f * nu * l2);
+            G = -__expf(-k2 * inv_kl2 - kh2 * ik2) * pw;
+        } else {
+            ik2 = 1.0 / k2;
+            lnk = 0.5 * log(k2);
+            const double pw = nu == 0.0 ? 1.0 : pow(k2, 0.5 * nu);
+            G = -exp(-k2 * inv_kl2 - kh2 * ik2) * pw;
+        }
+    }
+    __device__ __forceinline__ T G(T k2) const {
+        T g, ik2, lnk;
+        eval(k2, g, ik2, lnk);
+        return g;
     }
 };
 
@@ -161,10 +180,11 @@ template <typename T> struct GammaBwdStore {
         const int64_t i = row * n2 + 2 * m;
         const c2<T> r = *reinterpret_cast<const c2<T>*>(rho + i);
         const T g0 = v.x * scale, g1 = v.y * scale;
-        const T r0 = r.x + T(1e-8), r1 = r.y + T(1e-8);
-        const T l0 = plog(r0), l1 = plog(r1);
-        acc[0] += (double)(g0 * ppow(r0, gamma)) * (double)l0 + (double)(g1 * ppow(r1, gamma)) * (double)l1;
-        *reinterpret_cast<c2<T>*>(grho + i) = c2<T>{g0 * gamma * ppow(r0, gamma - T(1)), g1 * gamma * ppow(r1, gamma - T(1))};
+        T p0, q0, l0, p1, q1, l1;
+        pow_pair_log(r.x + T(1e-8), gamma, p0, q0, l0);
+        pow_pair_log(r.y + T(1e-8), gamma, p1, q1, l1);
+        acc[0] += (double)(g0 * p0) * (double)l0 + (double)(g1 * p1) * (double)l1;
+        *reinterpret_cast<c2<T>*>(grho + i) = c2<T>{g0 * gamma * q0, g1 * gamma * q1};
     }
     __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const { part[b] = acc[0]; }
 };
@@ -200,7 +220,7 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
         pk[s] = C{T(0), T(0)};
         if (p < pv) {
             U[base + n * g.es + p] = u;
-            const T G = fl.G(kt.knorm(n, plane, col0 + p));
+            const T G = fl.G(kt.knorm2(n, plane, col0 + p));
             pk[s] = C{u.x * a1e2 * G, u.y * a1e2 * G};
         }
     }
@@ -269,17 +289,18 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
         C y = C{T(0), T(0)};
         if (p < pv) {
             const int64_t kz = col0 + p;
-            const T kn = kt.knorm(n, plane, kz);
-            const T G = fl.G(kn);
+            const T k2 = kt.knorm2(n, plane, kz);
+            T G, ik2, lnk;
+            fl.eval(k2, G, ik2, lnk);
             const C u = U[base + n * g.es + p];
             const double re = (double)q[s].x * (double)u.x + (double)q[s].y * (double)u.y;
             const double c = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
-            const double Gd = (double)G, knd = (double)kn;
+            const double Gd = (double)G;
             acc[0] += (alpha_rule ? 1.0 : c) * 1e-2 * Gd * re;
             const double gG = a1e2d * c * re * Gd;  // dL/dG * G
-            acc[1] += gG * gkh / (knd * knd);
-            acc[2] += gG * gkl * (knd * knd);
-            acc[3] += gG * log(knd);
+            acc[1] += gG * gkh * (double)ik2;
+            acc[2] += gG * gkl * (double)k2;
+            acc[3] += gG * (double)lnk;
             y = C{a1e2 * G * q[s].x, a1e2 * G * q[s].y};
         }
         lds[n * LS + p] = y;
@@ -319,7 +340,7 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
         if (p >= pv) continue;
         T w;
         if (kind == 1) {
-            w = ppow(kt.knorm(n, plane, col0 + p), -par) + T(1);
+            w = ppow(kt.knorm2(n, plane, col0 + p), T(-0.5) * par) + T(1);
         } else {
             const T a = kt.k[0][n], b = kt.k[1][plane], c = kt.k[2][col0 + p];
             w = pexp(T(-0.5) * (a * a + b * b + c * c) * par * par);
@@ -409,6 +430,16 @@ __device__ __forceinline__ void response(const RespPar<T>& rp, T rho, T f2, T& b
     Fi = Fi * f2;
 }
 
+// Same, also returning (rho+1e-8)^(mu-1) and ln(rho+1e-8) for the adjoint.
+template <typename T>
+__device__ __forceinline__ void response_adj(const RespPar<T>& rp, T rho, T f2, T& base, T& base1, T& lr, T& y, T& F, T& Fi) {
+    pow_pair_log(rho + T(1e-8), rp.mu, base, base1, lr);
+    y = rp.w * base + rp.b;
+    F = y > T(1e-12) ? y : T(1e-12);
+    Fi = rp.index == T(1) ? F : ppow(F, rp.index);
+    Fi = Fi * f2;
+}
+
 // d loss / d y, given d loss / d Fi (times field2 and the index power)
 template <typename T>
 __device__ __forceinline__ T response_bwd(const RespPar<T>& rp, T gFi, T f2, T y, T F) {
@@ -444,8 +475,8 @@ __global__ void __launch_bounds__(256) k_loss_direct(const T* __restrict__ rho, 
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const T r = rho[i];
         const T f2 = rp.field2 ? rp.field2[i] : T(1);
-        T base, y, F, Fi;
-        response(rp, r, f2, base, y, F, Fi);
+        T base, base1, lr, y, F, Fi;
+        response_adj(rp, r, f2, base, base1, lr, y, F, Fi);
         const T diff = Fi - target[i];
         const T ad = fabs(diff);
         const T mk = mask ? mask[i] : T(1);
@@ -456,11 +487,10 @@ __global__ void __launch_bounds__(256) k_loss_direct(const T* __restrict__ rho, 
             const T sgn = diff > T(0) ? T(1) : diff < T(0) ? T(-1) : T(0);
             const T gd = (T)((double)mk * inv) * (rp.l1 ? sgn : T(2) * diff);
             const T gy = response_bwd(rp, gd, f2, y, F);
-            const T lr = plog(r + T(1e-8));
             a[1] += (double)(gy * rp.w * base) * (double)lr;
             a[2] += (double)(gy * base);
             a[3] += (double)gy;
-            grho[i] = gy * rp.w * rp.mu * ppow(r + T(1e-8), rp.mu - T(1));
+            grho[i] = gy * rp.w * rp.mu * base1;
         }
     }
     block_sum<5, 256>(a);
@@ -532,13 +562,13 @@ template <typename T> struct RespBwdStore {
             const int64_t i = i0 + e;
             const T r = rho[i];
             const T f2 = rp.field2 ? rp.field2[i] : T(1);
-            T base, y, F, Fi;
-            response(rp, r, f2, base, y, F, Fi);
+            T base, base1, lr, y, F, Fi;
+            response_adj(rp, r, f2, base, base1, lr, y, F, Fi);
             const T gy = response_bwd(rp, gv[e], f2, y, F);
-            acc[0] += (double)(gy * rp.w * base) * (double)plog(r + T(1e-8));
+            acc[0] += (double)(gy * rp.w * base) * (double)lr;
             acc[1] += (double)(gy * base);
             acc[2] += (double)gy;
-            grho[i] = gy * rp.w * rp.mu * ppow(r + T(1e-8), rp.mu - T(1));
+            grho[i] = gy * rp.w * rp.mu * base1;
         }
     }
     __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const {
