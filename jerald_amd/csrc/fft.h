@@ -1,7 +1,7 @@
 // Host-side 3D FFT drivers shared by fft.hip and ldl_sm.hip.
 #pragma once
 
-#include "fft_core.h"
+#include "fft_any.h"
 
 namespace ldl {
 

@@ -1,5 +1,6 @@
 // 3D r2c / c2r entry points (FFTW conventions, lib/parfft.h:33-138) built
-// from the three LDS pencil passes of fft_core.h.
+// from the three LDS pencil passes of fft_core.h (power-of-two axes) or
+// fft_any.h (axes with factors 2, 3, 5).
 #include "fft.h"
 
 namespace ldl {
@@ -7,17 +8,10 @@ namespace ldl {
 template <typename T>
 int r2c3d(const T* in, c2<T>* spec, const Mesh& m, T scale, const c2<T>* tw, hipStream_t st) {
     const int64_t nrows = m.n0 * m.n1, nh = m.nh();
-    dispatch_h(m.n2 / 2, [&](auto H) {
-        launch_zr2c<T, decltype(H)::value>(nrows, tw, RealRowLoad<T>{in, m.n2}, HalfRowStore<T>{spec, nh, T(1)}, st);
-    });
-    dispatch_n(m.n1, [&](auto N) {
-        ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
-        launch_colpass<T, decltype(N)::value, -1>(g, tw, PlainColOp<T>{spec, spec, T(1)}, st);
-    });
-    dispatch_n(m.n0, [&](auto N) {
-        ColGeom g = col_geom<T, decltype(N)::value>(m.n1, nh, nh, m.n1 * nh);
-        launch_colpass<T, decltype(N)::value, -1>(g, tw, PlainColOp<T>{spec, spec, scale}, st);
-    });
+    int rc;
+    if ((rc = zpass_r2c<T>(nrows, m.n2, tw, RealRowLoad<T>{in, m.n2}, HalfRowStore<T>{spec, nh, T(1)}, st))) return rc;
+    if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{spec, spec, T(1)}, st))) return rc;
+    if ((rc = colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{spec, spec, scale}, st))) return rc;
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }
@@ -25,17 +19,10 @@ int r2c3d(const T* in, c2<T>* spec, const Mesh& m, T scale, const c2<T>* tw, hip
 template <typename T>
 int c2r3d(const c2<T>* spec, c2<T>* tmp, T* out, const Mesh& m, T scale, const c2<T>* tw, hipStream_t st) {
     const int64_t nrows = m.n0 * m.n1, nh = m.nh();
-    dispatch_n(m.n0, [&](auto N) {
-        ColGeom g = col_geom<T, decltype(N)::value>(m.n1, nh, nh, m.n1 * nh);
-        launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{spec, tmp, T(1)}, st);
-    });
-    dispatch_n(m.n1, [&](auto N) {
-        ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
-        launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{tmp, tmp, T(1)}, st);
-    });
-    dispatch_h(m.n2 / 2, [&](auto H) {
-        launch_zc2r<T, decltype(H)::value>(nrows, tw, HalfRowLoad<T>{tmp, nh}, RealRowStore<T>{out, m.n2, scale}, st);
-    });
+    int rc;
+    if ((rc = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{spec, tmp, T(1)}, st))) return rc;
+    if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{tmp, tmp, T(1)}, st))) return rc;
+    if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{tmp, nh}, RealRowStore<T>{out, m.n2, scale}, st, nullptr))) return rc;
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }

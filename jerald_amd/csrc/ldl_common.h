@@ -205,16 +205,26 @@ inline int check_mesh(const int64_t nmesh[3], Mesh* m) {
     return LDL_OK;
 }
 
-// FFT-capable mesh: each axis a power of two in [4, 1024], last axis >= 8.
+// FFT-capable mesh: each axis in [4, 1024] with no prime factor other than 2,
+// 3 and 5; the last axis even.  (Powers of two take the register-resident
+// kernels, other lengths the mixed-radix ones.)
+inline bool fft_len_ok(int64_t n) {
+    if (n < 4 || n > 1024) return false;
+    for (int f : {2, 3, 5})
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
 inline int check_fft_mesh(const int64_t nmesh[3], Mesh* m) {
     int rc = check_mesh(nmesh, m);
     if (rc) return rc;
     for (int d = 0; d < 3; d++)
-        if (!is_pow2(nmesh[d]) || nmesh[d] < 4 || nmesh[d] > 1024)
-            return fail(LDL_EUNSUPPORTED, "FFT mesh axes must be powers of two in [4, 1024]");
-    if (nmesh[2] < 8) return fail(LDL_EUNSUPPORTED, "FFT mesh last axis must be >= 8");
+        if (!fft_len_ok(nmesh[d]))
+            return fail(LDL_EUNSUPPORTED, "FFT mesh axes must be in [4, 1024] with no prime factor other than 2, 3 and 5");
+    if (nmesh[2] % 2) return fail(LDL_EUNSUPPORTED, "FFT mesh last axis must be even");
     return LDL_OK;
 }
+// Every axis a power of two (with a last axis >= 8): the fused LDL kernels apply.
+inline bool fft_mesh_pow2(const Mesh& m) { return is_pow2(m.n0) && is_pow2(m.n1) && is_pow2(m.n2) && m.n2 >= 8; }
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
@@ -228,6 +238,7 @@ inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {
 // Twiddle table for the current device: tw[L + m] = exp(-2 pi i m / L) for
 // every power of two L in [1, 2048], built in FP64 on the host.
 template <typename T> int twiddles(const c2<T>** out);
+template <typename T> int twiddles_n(int n, const c2<T>** out);
 
 // Bijective blockIdx -> tile map that gives each of the 8 XCDs (which the
 // hardware feeds round-robin) a contiguous range of tiles, so tiles that

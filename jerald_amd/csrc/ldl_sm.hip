@@ -354,6 +354,83 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
     tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { io[base + n * g.es + p] = v; });
 }
 
+// ------------------------------------ spectral kernels of the any-length path
+// Meshes whose axes are not all powers of two run the x passes unfused: plain
+// column passes around these elementwise kernels over the half spectrum
+// (index i -> (ix, iy, iz) with iz fastest), which do what the fused x
+// kernels do between their transforms.
+
+// out3[d] = -i D_d (1e-2 alpha G U)
+template <typename T>
+__global__ void __launch_bounds__(256) k_sm_filter_fwd(const c2<T>* __restrict__ U, c2<T>* __restrict__ out3, int64_t hcells, int64_t n1,
+                                                       int64_t nh, KTabs<T> kt, SmFilter<T> fl, T a1e2) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
+        const T G = fl.G(kt.knorm2(ix, iy, iz));
+        const c2<T> u = U[i];
+        const c2<T> p = c2<T>{u.x * a1e2 * G, u.y * a1e2 * G};
+        const T D[3] = {kt.D[0][ix], kt.D[1][iy], kt.D[2][iz]};
+#pragma unroll
+        for (int d = 0; d < 3; d++) out3[d * hcells + i] = c2<T>{D[d] * p.y, -D[d] * p.x};
+    }
+}
+
+// Q = sum_d i D_d S_d; FP64 partials (alpha, kh, kl, nu) per workgroup into
+// part[4 * block + j]; out = 1e-2 alpha G Q (out may alias S3's first spectrum).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sm_filter_bwd(const c2<T>* S3, const c2<T>* __restrict__ U, c2<T>* out, int64_t hcells, int64_t n1,
+                                                       int64_t nh, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double kh, double kl,
+                                                       double alpha, int alpha_rule, double* __restrict__ part) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double a1e2d = 1e-2 * alpha;
+    const double gkh = -2.0 * kh, gkl = 2.0 / (kl * kl * kl);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
+        const T D[3] = {kt.D[0][ix], kt.D[1][iy], kt.D[2][iz]};
+        c2<T> q = c2<T>{T(0), T(0)};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const c2<T> v = S3[d * hcells + i];
+            q = c2<T>{q.x - D[d] * v.y, q.y + D[d] * v.x};  // += i D v
+        }
+        const T k2 = kt.knorm2(ix, iy, iz);
+        T G, ik2, lnk;
+        fl.eval(k2, G, ik2, lnk);
+        const c2<T> u = U[i];
+        const double re = (double)q.x * (double)u.x + (double)q.y * (double)u.y;
+        const double c = (iz == 0 || 2 * iz == n2) ? 1.0 : 2.0;
+        const double Gd = (double)G;
+        acc[0] += (alpha_rule ? 1.0 : c) * 1e-2 * Gd * re;
+        const double gG = a1e2d * c * re * Gd;  // dL/dG * G
+        acc[1] += gG * gkh * (double)ik2;
+        acc[2] += gG * gkl * (double)k2;
+        acc[3] += gG * (double)lnk;
+        out[i] = c2<T>{a1e2 * G * q.x, a1e2 * G * q.y};
+    }
+    block_sum<4, 256>(acc);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 4; j++) part[blockIdx.x * 4 + j] = acc[j];
+}
+
+// S *= scale * W(k) (the weights of k_x_realfilter)
+template <typename T>
+__global__ void __launch_bounds__(256) k_realfilter_mult(c2<T>* __restrict__ S, int64_t hcells, int64_t n1, int64_t nh, KTabs<T> kt, int kind,
+                                                         T par, T scale) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
+        T w;
+        if (kind == 1) {
+            w = ppow(kt.knorm2(ix, iy, iz), T(-0.5) * par) + T(1);
+        } else {
+            const T a = kt.k[0][ix], b = kt.k[1][iy], c = kt.k[2][iz];
+            w = pexp(T(-0.5) * (a * a + b * b + c * c) * par * par);
+        }
+        w *= scale;
+        const c2<T> v = S[i];
+        S[i] = c2<T>{v.x * w, v.y * w};
+    }
+}
+
 // --------------------------------------------------------------- gathers
 // pos_out = pos + readout3(F) * step (step = 1/cell in mesh units, 1 in box units)
 template <typename T, typename G>
@@ -622,30 +699,29 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hc = m.hcells();
     c2<T>* U = reinterpret_cast<c2<T>*>(Ubuf);
     // 2. z pass with the gamma power, y pass
-    dispatch_h(m.n2 / 2, [&](auto H) {
-        launch_zr2c<T, decltype(H)::value>(nrows, tw, PowRowLoad<T>{rho, m.n2, (T)prm[1]}, HalfRowStore<T>{U, nh, T(1)}, st);
-    });
-    dispatch_n(m.n1, [&](auto N) {
-        ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
-        launch_colpass<T, decltype(N)::value, -1>(g, tw, PlainColOp<T>{U, U, T(1)}, st);
-    });
+    if ((rc = zpass_r2c<T>(nrows, m.n2, tw, PowRowLoad<T>{rho, m.n2, (T)prm[1]}, HalfRowStore<T>{U, nh, T(1)}, st))) return rc;
+    if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{U, U, T(1)}, st))) return rc;
     // 3. x pass + filter + three inverse x passes
     const SmFilter<T> fl{(T)(prm[2] * prm[2]), (T)(1.0 / (prm[3] * prm[3])), (T)prm[4]};
-    dispatch_n(m.n0, [&](auto N) {
-        constexpr int NN = decltype(N)::value;
-        ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
-        { KTimer _t("sm_x_fwd", st); hipLaunchKernelGGL((k_sm_x_fwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, hc, kt, fl,
-                           (T)(1e-2 * prm[0]), (T)(1.0 / (double)m.cells())); }
-    });
+    const T a1e2 = (T)(1e-2 * prm[0]), invM = (T)(1.0 / (double)m.cells());
+    if (fft_mesh_pow2(m)) {
+        dispatch_n(m.n0, [&](auto N) {
+            constexpr int NN = decltype(N)::value;
+            ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
+            KTimer _t("sm_x_fwd", st);
+            hipLaunchKernelGGL((k_sm_x_fwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, hc, kt, fl,
+                               a1e2, invM);
+        });
+    } else {
+        if ((rc = colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{U, U, invM}, st))) return rc;
+        hipLaunchKernelGGL(k_sm_filter_fwd<T>, dim3(grid_for(hc, 256, 1 << 16)), dim3(256), 0, st, U, P.spec3, hc, m.n1, nh, kt, fl, a1e2);
+        for (int d = 0; d < 3; d++)
+            if ((rc = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{P.spec3 + d * hc, P.spec3 + d * hc, T(1)}, st))) return rc;
+    }
     // 4. inverse y passes (3 spectra as 3*N0 planes) and z c2r into F
-    dispatch_n(m.n1, [&](auto N) {
-        ColGeom g = col_geom<T, decltype(N)::value>(3 * m.n0, nh, m.n1 * nh, nh);
-        launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3");
-    });
-    dispatch_h(m.n2 / 2, [&](auto H) {
-        launch_zc2r<T, decltype(H)::value>(3 * nrows, tw, HalfRowLoad<T>{P.spec3, nh}, RealRowStore<T>{F, m.n2, T(1)}, st,
-                                           "zpass_c2r3");
-    });
+    if ((rc = colpass<T, +1>(m.n1, 3 * m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
+    if ((rc = zpass_c2r<T>(3 * nrows, m.n2, tw, HalfRowLoad<T>{P.spec3, nh}, RealRowStore<T>{F, m.n2, T(1)}, st, nullptr, "zpass_c2r3")))
+        return rc;
     // 5. readout + position update
     T s[3];
     for (int d = 0; d < 3; d++) s[d] = pdesc->kind == LDL_POS_LATTICE ? pd.cellinv[d] : T(1);
@@ -686,41 +762,42 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
         return rc;
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
     // 2. unnormalised r2c of the three meshes: z and y passes
-    dispatch_h(m.n2 / 2, [&](auto H) {
-        launch_zr2c<T, decltype(H)::value>(3 * nrows, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore<T>{P.spec3, nh, T(1)}, st,
-                                           "zpass_r2c3");
-    });
-    dispatch_n(m.n1, [&](auto N) {
-        ColGeom g = col_geom<T, decltype(N)::value>(3 * m.n0, nh, m.n1 * nh, nh);
-        launch_colpass<T, decltype(N)::value, -1>(g, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3");
-    });
-    // 3. fused x pass: Q, parameter partials, inverse x of 1e-2 alpha G Q
+    if ((rc = zpass_r2c<T>(3 * nrows, m.n2, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore<T>{P.spec3, nh, T(1)}, st, "zpass_r2c3")))
+        return rc;
+    if ((rc = colpass<T, -1>(m.n1, 3 * m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
+    // 3. x pass: Q, parameter partials, inverse x of 1e-2 alpha G Q
     const SmFilter<T> fl{(T)(prm[2] * prm[2]), (T)(1.0 / (prm[3] * prm[3])), (T)prm[4]};
+    const T a1e2 = (T)(1e-2 * prm[0]);
     const c2<T>* U = reinterpret_cast<const c2<T>*>(Ubuf);
-    c2<T>* S = P.spec3;  // result reuses the first spectrum (read before written per tile)
+    c2<T>* S = P.spec3;  // result reuses the first spectrum (read before written)
     int64_t ntx = 0;
-    dispatch_n(m.n0, [&](auto N) {
-        constexpr int NN = decltype(N)::value;
-        ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
-        ntx = g.ntiles;
-        { KTimer _t("sm_x_bwd", st); hipLaunchKernelGGL((k_sm_x_bwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U, S, hc, m.n2, kt,
-                           fl, (T)(1e-2 * prm[0]), prm[2], prm[3], prm[0], (int)alpha_rule, P.part); }
-    });
+    if (fft_mesh_pow2(m)) {
+        dispatch_n(m.n0, [&](auto N) {
+            constexpr int NN = decltype(N)::value;
+            ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
+            ntx = g.ntiles;
+            KTimer _t("sm_x_bwd", st);
+            hipLaunchKernelGGL((k_sm_x_bwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U, S, hc, m.n2,
+                               kt, fl, a1e2, prm[2], prm[3], prm[0], (int)alpha_rule, P.part);
+        });
+    } else {
+        for (int d = 0; d < 3; d++)
+            if ((rc = colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{P.spec3 + d * hc, P.spec3 + d * hc, T(1)}, st))) return rc;
+        const unsigned gf = grid_for(hc, 256, 4096);
+        ntx = gf;
+        hipLaunchKernelGGL(k_sm_filter_bwd<T>, dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, m.n2, kt, fl, a1e2, prm[2], prm[3],
+                           prm[0], (int)alpha_rule, P.part);
+        if ((rc = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc;
+    }
     RedMap rx{4, {0, 2, 3, 4}};
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, ntx, 4, rx, grad5, 0);
     // 4. inverse y, z c2r with the gamma-power adjoint -> g_rho (mesh3[0])
-    dispatch_n(m.n1, [&](auto N) {
-        ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
-        launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{S, S, T(1)}, st);
-    });
+    if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc;
     T* grho = P.mesh3;
     int64_t nzb = 0;
-    dispatch_h(m.n2 / 2, [&](auto H) {
-        constexpr int HH = decltype(H)::value;
-        nzb = (nrows + FftCfg<T, HH>::P - 1) / FftCfg<T, HH>::P;
-        launch_zc2r<T, HH>(nrows, tw, HalfRowLoad<T>{S, nh},
-                           GammaBwdStore<T>{rho, grho, m.n2, (T)(1.0 / (double)cells), (T)prm[1], P.part}, st);
-    });
+    if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, nh},
+                           GammaBwdStore<T>{rho, grho, m.n2, (T)(1.0 / (double)cells), (T)prm[1], P.part}, st, &nzb)))
+        return rc;
     RedMap rz{1, {1}};
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, nzb, 1, rz, grad5, 0);
     // 5. position adjoint
@@ -778,40 +855,39 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
         T* gds = P.mesh3 + cells;
         c2<T>* S = P.spec3;
         const T invM = (T)(1.0 / (double)cells);
-        auto smooth = [&](const T* src) {  // S = W(k) r2c(src) / M, inverse x and y passes
-            dispatch_h(m.n2 / 2, [&](auto H) {
-                launch_zr2c<T, decltype(H)::value>(nrows, tw, RealRowLoad<T>{src, m.n2}, HalfRowStore<T>{S, nh, T(1)}, st);
-            });
-            dispatch_n(m.n1, [&](auto N) {
-                ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
-                launch_colpass<T, decltype(N)::value, -1>(g, tw, PlainColOp<T>{S, S, T(1)}, st);
-            });
-            dispatch_n(m.n0, [&](auto N) {
-                constexpr int NN = decltype(N)::value;
-                ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
-                { KTimer _t("x_realfilter", st); hipLaunchKernelGGL((k_x_realfilter<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, S, kt,
-                                   (int)smooth_kind, (T)smooth_par, invM); }
-            });
-            dispatch_n(m.n1, [&](auto N) {
-                ColGeom g = col_geom<T, decltype(N)::value>(m.n0, nh, m.n1 * nh, nh);
-                launch_colpass<T, decltype(N)::value, +1>(g, tw, PlainColOp<T>{S, S, T(1)}, st);
-            });
+        auto smooth = [&](const T* src) -> int {  // S = W(k) r2c(src) / M, inverse x and y passes
+            int rc2;
+            if ((rc2 = zpass_r2c<T>(nrows, m.n2, tw, RealRowLoad<T>{src, m.n2}, HalfRowStore<T>{S, nh, T(1)}, st))) return rc2;
+            if ((rc2 = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
+            if (fft_mesh_pow2(m)) {
+                dispatch_n(m.n0, [&](auto N) {
+                    constexpr int NN = decltype(N)::value;
+                    ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
+                    KTimer _t("x_realfilter", st);
+                    hipLaunchKernelGGL((k_x_realfilter<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, S, kt,
+                                       (int)smooth_kind, (T)smooth_par, invM);
+                });
+            } else {
+                const int64_t hc = m.hcells();
+                if ((rc2 = colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
+                hipLaunchKernelGGL(k_realfilter_mult<T>, dim3(grid_for(hc, 256, 1 << 16)), dim3(256), 0, st, S, hc, m.n1, nh, kt, (int)smooth_kind,
+                                   (T)smooth_par, invM);
+                if ((rc2 = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
+            }
+            return colpass<T, +1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{S, S, T(1)}, st);
         };
-        int64_t nzb = 0;
-        dispatch_h(m.n2 / 2, [&](auto H) { nzb = (nrows + FftCfg<T, decltype(H)::value>::P - 1) / FftCfg<T, decltype(H)::value>::P; });
+        const int64_t nzb = zpass_c2r_blocks<T>(nrows, m.n2);
         // every block writes the loss columns; the gradient columns stay zero without g_rho
         hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nzb * 5, 256)), dim3(256), 0, st, bpart, nzb * 5, 0.0);
         hipLaunchKernelGGL(k_loss_diff<T>, dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff);
-        smooth(diff);
-        dispatch_h(m.n2 / 2, [&](auto H) {
-            launch_zc2r<T, decltype(H)::value>(nrows, tw, HalfRowLoad<T>{S, nh},
-                                               LossStore<T>{mask, maskv, grho ? gds : nullptr, npix, m.n2, T(1), p_norm == 1, bpart}, st);
-        });
+        if ((rc = smooth(diff))) return rc;
+        if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, nh},
+                               LossStore<T>{mask, maskv, grho ? gds : nullptr, npix, m.n2, T(1), p_norm == 1, bpart}, st, nullptr)))
+            return rc;
         if (grho) {
-            smooth(gds);
-            dispatch_h(m.n2 / 2, [&](auto H) {
-                launch_zc2r<T, decltype(H)::value>(nrows, tw, HalfRowLoad<T>{S, nh}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart}, st);
-            });
+            if ((rc = smooth(gds))) return rc;
+            if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, nh}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart}, st, nullptr)))
+                return rc;
         }
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, nzb, 5, r5, sums, 1);
     }

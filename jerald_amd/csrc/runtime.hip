@@ -1,5 +1,6 @@
 // libldl runtime: error channel, version, twiddle-table cache, buildplan.
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -58,6 +59,35 @@ int twiddles(const c2<T>** out) {
 
 template int twiddles<float>(const c2<float>**);
 template int twiddles<double>(const c2<double>**);
+
+// Length-n tables tw[m] = exp(-2 pi i m / n), m < n, for transform lengths
+// that are not powers of two; one per (device, precision, n).
+static std::map<std::tuple<int, int, int>, void*> g_tw_n;
+
+template <typename T>
+int twiddles_n(int n, const c2<T>** out) {
+    int dev = 0;
+    LDL_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    const auto key = std::make_tuple(dev, (int)sizeof(T), n);
+    auto it = g_tw_n.find(key);
+    if (it == g_tw_n.end()) {
+        std::vector<c2<T>> h(n);
+        for (int m = 0; m < n; m++) {
+            const double a = -2.0 * M_PI * (double)m / (double)n;
+            h[m] = c2<T>{(T)std::cos(a), (T)std::sin(a)};
+        }
+        void* d = nullptr;
+        LDL_HIP_TRY(hipMalloc(&d, h.size() * sizeof(c2<T>)));
+        LDL_HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(c2<T>), hipMemcpyHostToDevice));
+        it = g_tw_n.emplace(key, d).first;
+    }
+    *out = reinterpret_cast<const c2<T>*>(it->second);
+    return LDL_OK;
+}
+
+template int twiddles_n<float>(int, const c2<float>**);
+template int twiddles_n<double>(int, const c2<double>**);
 
 // ------------------------------------------------- kernel timing (bench)
 // When enabled, instrumented launches are bracketed by HIP events recorded on

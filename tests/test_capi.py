@@ -39,8 +39,9 @@ def test_workspace_queries():
     assert lib.ldl_sm_workspace_bytes(nm, 1, 1000) > lib.ldl_sm_workspace_bytes(nm, 0, 1000) > 0
 
 
-@pytest.mark.parametrize("shape", [(12, 8, 8), (8, 8, 4), (2048, 8, 8), (8, 8, 6)])
+@pytest.mark.parametrize("shape", [(14, 8, 8), (8, 8, 7), (8, 8, 15), (2048, 8, 8), (8, 8, 22), (3, 8, 8), (8, 1100, 8)])
 def test_buildplan_rejects_unsupported_meshes(shape):
+    """FFT axes: [4, 1024], prime factors 2, 3, 5 only, last axis even."""
     with pytest.raises(L.LdlError, match="FFT mesh"):
         L.call("ldl_buildplan", L.i64x3(shape), None)
 

@@ -16,9 +16,11 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda"
 
 SHAPES = [(8, 8, 8), (4, 16, 8), (32, 16, 64), (64, 64, 64), (128, 32, 16), (16, 8, 256), (8, 4, 1024), (512, 4, 8)]
+# axes with factors 3 and 5 take the mixed-radix kernels (and (8, 8, 4) its 2-point z pass)
+MIXED = [(20, 20, 20), (12, 10, 6), (6, 24, 30), (40, 12, 18), (30, 45, 60), (8, 8, 4), (1000, 4, 10), (4, 6, 1000), (96, 80, 50)]
 
 
-@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("shape", SHAPES + MIXED)
 @pytest.mark.parametrize("dt", [torch.float32, torch.float64])
 def test_pfft_pifft_match_numpy(shape, dt):
     rng = np.random.default_rng(sum(shape))
@@ -119,6 +121,26 @@ def test_ldl_blocked_sweep_lattice(lat):
     loss, grad = m.loss_and_grad(torch.tensor(p))
     assert abs(loss.item() - loss0) <= 1e-10 * abs(loss0)
     np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
+
+
+@pytest.mark.parametrize("N,lattice", [(20, None), (12, (12, 12, 12)), (24, None)])
+@pytest.mark.parametrize("smoothing,L1", [(None, True), (("gaussian", 4.0), False), ("reference", True)])
+def test_ldl_mixed_radix_mesh_matches_oracle(N, lattice, smoothing, L1):
+    """Meshes whose axes are not powers of two run the unfused x passes
+    around the elementwise filter kernels: same loss and gradients."""
+    L, nstep = 4.0 * N, 2
+    X, tgt, mk, p, _ = _problem(N, nstep, L, smoothing=None, L1=L1, mask=True)
+    ref = smoothing == "reference"  # k^-n + 1 with n = 1 (the model's n argument)
+    orc = O.SMModelOracle(N, L, nstep, tgt, L1=L1, n=1.0 if ref else 0.0, smoothing=None if ref else smoothing, mask=mk)
+    loss0, g0 = orc.loss_and_grad(p, X)
+    kw = dict(n=1.0) if ref else dict(smoothing=smoothing)
+    m = _model(X, tgt, mk, N, L, nstep, torch.float64, L1=L1, lattice=lattice, **kw)
+    loss, grad = m.loss_and_grad(torch.tensor(p))
+    assert abs(loss.item() - loss0) <= 1e-9 * abs(loss0)
+    # with the reference weight W(0) = 1e8 + 1 the layer gradients carry ~1e-8
+    # relative rounding from either side (see test_lossv_and_reference_smoothing)
+    rtol = 1e-5 if ref else 1e-8
+    np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=rtol, atol=1e-12 * np.abs(g0).max())
 
 
 def test_ldl_fp32_within_1e5_of_fp64():
