@@ -77,8 +77,10 @@ int ldl_timer_read(const char* name, double* total_ms, int64_t* count);
 
 /* ------------------------------------------------------------------ plan */
 /* Replaces buildplan_f32/_f64 (lib/cpu.cc:140-167, lib/parfft.h:33-76).
- * Validates the mesh (each axis a power of two in [4, 1024]; the last one
- * also >= 8) and prepares the twiddle tables for the current device.
+ * Validates the mesh (each axis in [4, 1024] with no prime factor other than
+ * 2, 3 and 5; the last axis even) and prepares the twiddle tables for the
+ * current device.  Power-of-two axes run the register-resident FFT kernels,
+ * other lengths the mixed-radix ones.
  * out[0] = local x extent (= N0), out[1] = local x start (= 0). */
 int ldl_buildplan(const int64_t nmesh[3], int32_t out[2]);
 
