@@ -221,6 +221,40 @@ int ldl_sm_loss_f64(const double* rho, const double* target, const double* mask,
                     double smooth_par, double* out5, double* g_rho,
                     void* workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------- dm-kind LDL layer, loss
+ * Replace JERALDModel.potential_gradient_dm (model.py:136-183) and the dm
+ * branch of evaluate (model.py:346-347).  Same arguments, workspace and
+ * saved tensors as the sm entry points; the layer's kernel is the
+ * non-squared G = -exp(-|k|/kl - kh/|k|) |k|^nu, and the response is the
+ * density itself (F = rho, no ReLU): ldl_dm_loss takes no {mu, w, b} and
+ * writes out5 = {loss, 0, 0, 0, loss_valid}. */
+int ldl_dm_layer_fwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[5], double fact, int32_t paint_flags,
+                         float* pos_out, float* rho, float* U, float* F, void* bins,
+                         void* workspace, size_t workspace_bytes, void* stream);
+int ldl_dm_layer_fwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[5], double fact, int32_t paint_flags,
+                         double* pos_out, double* rho, double* U, double* F, void* bins,
+                         void* workspace, size_t workspace_bytes, void* stream);
+int ldl_dm_layer_bwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[5], double fact, int32_t alpha_rule,
+                         int32_t paint_flags, const float* rho, const float* U, const float* F,
+                         const void* bins, const float* g_out, float* g_in, double* grad5,
+                         void* workspace, size_t workspace_bytes, void* stream);
+int ldl_dm_layer_bwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[5], double fact, int32_t alpha_rule,
+                         int32_t paint_flags, const double* rho, const double* U, const double* F,
+                         const void* bins, const double* g_out, double* g_in, double* grad5,
+                         void* workspace, size_t workspace_bytes, void* stream);
+int ldl_dm_loss_f32(const float* rho, const float* target, const float* mask, const float* mask_valid,
+                    const float* field2, const int64_t nmesh[3], const double box[3], double index,
+                    int32_t p_norm, int32_t smooth_kind, double smooth_par, double* out5, float* g_rho,
+                    void* workspace, size_t workspace_bytes, void* stream);
+int ldl_dm_loss_f64(const double* rho, const double* target, const double* mask, const double* mask_valid,
+                    const double* field2, const int64_t nmesh[3], const double box[3], double index,
+                    int32_t p_norm, int32_t smooth_kind, double smooth_par, double* out5, double* g_rho,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
 /* Final-layer adjoint gather (fused readout-gradient): g (Np,3) =
  * fact * grad(interp g_rho) at pos  (the ppaint VJP with unit mass,
  * jaxpops_bind.py:242-247). */

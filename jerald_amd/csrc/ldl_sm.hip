@@ -53,31 +53,60 @@ template <typename T> struct KTabs {
     }
 };
 
-// G(k) = -exp(-k^2/kl^2 - kh^2/k^2) |k|^nu (model.py:195), from k^2, with the
-// terms the parameter gradients reuse: 1/k^2 and ln|k|.  FP32: hardware
-// reciprocal, log2 and exp2; FP64: library routines.
+// Green's kernel of a layer, from k^2 (|k| = 1e-8 at k = 0):
+//   sm (model.py:195): G = -exp(-k^2/kl^2 - kh^2/k^2) |k|^nu
+//   dm (model.py:156): G = -exp(-|k|/kl  - kh/|k|)    |k|^nu
+// with the terms its parameter gradients reuse: th, tl with
+// dG/dkh = G gkh th and dG/dkl = G gkl tl (sm: th = 1/k^2, tl = k^2; dm:
+// th = 1/|k|, tl = |k|; gkh, gkl from the host), and ln|k|.  FP32: hardware
+// reciprocal, square root, log2 and exp2; FP64: library routines.
 template <typename T> struct SmFilter {
-    T kh2, inv_kl2, nu;
-    __device__ __forceinline__ void eval(T k2, T& G, T& ik2, T& lnk) const {
+    T kh, inv_kl, nu;  // sm: kh^2, 1/kl^2; dm: kh, 1/kl
+    int dm;
+    __device__ __forceinline__ void eval(T k2, T& G, T& th, T& tl, T& lnk) const {
         if constexpr (sizeof(T) == 4) {
-            ik2 = __builtin_amdgcn_rcpf(k2);
             const float l2 = __builtin_amdgcn_logf(k2);  // log2 k^2
             lnk = 0.5f * kLn2f * l2;
             const float pw = nu == 0.0f ? 1.0f : __builtin_amdgcn_exp2f(0.5f * nu * l2);
-            G = -__expf(-k2 * inv_kl2 - kh2 * ik2) * pw;
+            if (dm) {
+                tl = __builtin_amdgcn_sqrtf(k2);
+                th = __builtin_amdgcn_rcpf(tl);
+            } else {
+                tl = k2;
+                th = __builtin_amdgcn_rcpf(k2);
+            }
+            G = -__expf(-tl * inv_kl - kh * th) * pw;
         } else {
-            ik2 = 1.0 / k2;
             lnk = 0.5 * log(k2);
             const double pw = nu == 0.0 ? 1.0 : pow(k2, 0.5 * nu);
-            G = -exp(-k2 * inv_kl2 - kh2 * ik2) * pw;
+            tl = dm ? sqrt(k2) : k2;
+            th = 1.0 / tl;
+            G = -exp(-tl * inv_kl - kh * th) * pw;
         }
     }
     __device__ __forceinline__ T G(T k2) const {
-        T g, ik2, lnk;
-        eval(k2, g, ik2, lnk);
+        T g, th, tl, lnk;
+        eval(k2, g, th, tl, lnk);
         return g;
     }
 };
+
+// Filter parameters and gradient coefficients of one layer from prm =
+// [alpha, gamma, kh, kl, nu].
+template <typename T>
+inline SmFilter<T> layer_filter(const double* prm, int dm) {
+    return dm ? SmFilter<T>{(T)prm[2], (T)(1.0 / prm[3]), (T)prm[4], 1}
+              : SmFilter<T>{(T)(prm[2] * prm[2]), (T)(1.0 / (prm[3] * prm[3])), (T)prm[4], 0};
+}
+inline void filter_grad_coeffs(const double* prm, int dm, double* gkh, double* gkl) {
+    if (dm) {
+        *gkh = -1.0;
+        *gkl = 1.0 / (prm[3] * prm[3]);
+    } else {
+        *gkh = -2.0 * prm[2];
+        *gkl = 2.0 / (prm[3] * prm[3] * prm[3]);
+    }
+}
 
 // Workspace carve-up of ldl_sm_* calls.
 struct SmWs {
@@ -247,8 +276,8 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
 template <typename T, int N>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
 k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3, const c2<T>* __restrict__ U,
-           c2<T>* __restrict__ out, int64_t hcells, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double kh,
-           double kl, double alpha, int alpha_rule, double* __restrict__ part) {
+           c2<T>* __restrict__ out, int64_t hcells, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh,
+           double gkl, double alpha, int alpha_rule, double* __restrict__ part) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
@@ -281,7 +310,6 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
     }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const double a1e2d = 1e-2 * alpha;
-    const double gkh = -2.0 * kh, gkl = 2.0 / (kl * kl * kl);
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
         const int i = threadIdx.x + s * TH;
@@ -290,16 +318,16 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
         if (p < pv) {
             const int64_t kz = col0 + p;
             const T k2 = kt.knorm2(n, plane, kz);
-            T G, ik2, lnk;
-            fl.eval(k2, G, ik2, lnk);
+            T G, th, tl, lnk;
+            fl.eval(k2, G, th, tl, lnk);
             const C u = U[base + n * g.es + p];
             const double re = (double)q[s].x * (double)u.x + (double)q[s].y * (double)u.y;
             const double c = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
             const double Gd = (double)G;
             acc[0] += (alpha_rule ? 1.0 : c) * 1e-2 * Gd * re;
             const double gG = a1e2d * c * re * Gd;  // dL/dG * G
-            acc[1] += gG * gkh * (double)ik2;
-            acc[2] += gG * gkl * (double)k2;
+            acc[1] += gG * gkh * (double)th;
+            acc[2] += gG * gkl * (double)tl;
             acc[3] += gG * (double)lnk;
             y = C{a1e2 * G * q[s].x, a1e2 * G * q[s].y};
         }
@@ -379,11 +407,10 @@ __global__ void __launch_bounds__(256) k_sm_filter_fwd(const c2<T>* __restrict__
 // part[4 * block + j]; out = 1e-2 alpha G Q (out may alias S3's first spectrum).
 template <typename T>
 __global__ void __launch_bounds__(256) k_sm_filter_bwd(const c2<T>* S3, const c2<T>* __restrict__ U, c2<T>* out, int64_t hcells, int64_t n1,
-                                                       int64_t nh, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double kh, double kl,
+                                                       int64_t nh, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh, double gkl,
                                                        double alpha, int alpha_rule, double* __restrict__ part) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const double a1e2d = 1e-2 * alpha;
-    const double gkh = -2.0 * kh, gkl = 2.0 / (kl * kl * kl);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
         const T D[3] = {kt.D[0][ix], kt.D[1][iy], kt.D[2][iz]};
@@ -394,16 +421,16 @@ __global__ void __launch_bounds__(256) k_sm_filter_bwd(const c2<T>* S3, const c2
             q = c2<T>{q.x - D[d] * v.y, q.y + D[d] * v.x};  // += i D v
         }
         const T k2 = kt.knorm2(ix, iy, iz);
-        T G, ik2, lnk;
-        fl.eval(k2, G, ik2, lnk);
+        T G, th, tl, lnk;
+        fl.eval(k2, G, th, tl, lnk);
         const c2<T> u = U[i];
         const double re = (double)q.x * (double)u.x + (double)q.y * (double)u.y;
         const double c = (iz == 0 || 2 * iz == n2) ? 1.0 : 2.0;
         const double Gd = (double)G;
         acc[0] += (alpha_rule ? 1.0 : c) * 1e-2 * Gd * re;
         const double gG = a1e2d * c * re * Gd;  // dL/dG * G
-        acc[1] += gG * gkh * (double)ik2;
-        acc[2] += gG * gkl * (double)k2;
+        acc[1] += gG * gkh * (double)th;
+        acc[2] += gG * gkl * (double)tl;
         acc[3] += gG * (double)lnk;
         out[i] = c2<T>{a1e2 * G * q.x, a1e2 * G * q.y};
     }
@@ -491,18 +518,25 @@ __global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos
 }
 
 // ------------------------------------------------------------------ loss
+// Response of the final density.  sm (model.py:338-344): F = ReLU(w (rho +
+// 1e-8)^mu + b); dm (``ident``, model.py:346-347): F = rho.
 template <typename T> struct RespPar {
     T mu, w, b, index;
     int l1;
     const T* field2;
+    int ident;
 };
 
 // Forward response value at a cell: y, F, Fi (model.py:344, :358-362)
 template <typename T>
 __device__ __forceinline__ void response(const RespPar<T>& rp, T rho, T f2, T& base, T& y, T& F, T& Fi) {
-    base = ppow(rho + T(1e-8), rp.mu);
-    y = rp.w * base + rp.b;
-    F = y > T(1e-12) ? y : T(1e-12);
+    if (rp.ident) {
+        base = y = F = rho;
+    } else {
+        base = ppow(rho + T(1e-8), rp.mu);
+        y = rp.w * base + rp.b;
+        F = y > T(1e-12) ? y : T(1e-12);
+    }
     Fi = rp.index == T(1) ? F : ppow(F, rp.index);
     Fi = Fi * f2;
 }
@@ -510,9 +544,15 @@ __device__ __forceinline__ void response(const RespPar<T>& rp, T rho, T f2, T& b
 // Same, also returning (rho+1e-8)^(mu-1) and ln(rho+1e-8) for the adjoint.
 template <typename T>
 __device__ __forceinline__ void response_adj(const RespPar<T>& rp, T rho, T f2, T& base, T& base1, T& lr, T& y, T& F, T& Fi) {
-    pow_pair_log(rho + T(1e-8), rp.mu, base, base1, lr);
-    y = rp.w * base + rp.b;
-    F = y > T(1e-12) ? y : T(1e-12);
+    if (rp.ident) {
+        base = y = F = rho;
+        base1 = T(1);
+        lr = T(0);
+    } else {
+        pow_pair_log(rho + T(1e-8), rp.mu, base, base1, lr);
+        y = rp.w * base + rp.b;
+        F = y > T(1e-12) ? y : T(1e-12);
+    }
     Fi = rp.index == T(1) ? F : ppow(F, rp.index);
     Fi = Fi * f2;
 }
@@ -522,7 +562,13 @@ template <typename T>
 __device__ __forceinline__ T response_bwd(const RespPar<T>& rp, T gFi, T f2, T y, T F) {
     T g = gFi * f2;
     if (rp.index != T(1)) g = g * rp.index * ppow(F, rp.index - T(1));
-    return y > T(1e-12) ? g : T(0);
+    return (rp.ident || y > T(1e-12)) ? g : T(0);
+}
+
+// d loss / d rho from d loss / d y (the mu, w, b partials are zero for ident)
+template <typename T>
+__device__ __forceinline__ T response_grho(const RespPar<T>& rp, T gy, T base1) {
+    return rp.ident ? gy : gy * rp.w * rp.mu * base1;
 }
 
 
@@ -564,10 +610,12 @@ __global__ void __launch_bounds__(256) k_loss_direct(const T* __restrict__ rho, 
             const T sgn = diff > T(0) ? T(1) : diff < T(0) ? T(-1) : T(0);
             const T gd = (T)((double)mk * inv) * (rp.l1 ? sgn : T(2) * diff);
             const T gy = response_bwd(rp, gd, f2, y, F);
-            a[1] += (double)(gy * rp.w * base) * (double)lr;
-            a[2] += (double)(gy * base);
-            a[3] += (double)gy;
-            grho[i] = gy * rp.w * rp.mu * base1;
+            if (!rp.ident) {
+                a[1] += (double)(gy * rp.w * base) * (double)lr;
+                a[2] += (double)(gy * base);
+                a[3] += (double)gy;
+            }
+            grho[i] = response_grho(rp, gy, base1);
         }
     }
     block_sum<5, 256>(a);
@@ -642,10 +690,12 @@ template <typename T> struct RespBwdStore {
             T base, base1, lr, y, F, Fi;
             response_adj(rp, r, f2, base, base1, lr, y, F, Fi);
             const T gy = response_bwd(rp, gv[e], f2, y, F);
-            acc[0] += (double)(gy * rp.w * base) * (double)lr;
-            acc[1] += (double)(gy * base);
-            acc[2] += (double)gy;
-            grho[i] = gy * rp.w * rp.mu * base1;
+            if (!rp.ident) {
+                acc[0] += (double)(gy * rp.w * base) * (double)lr;
+                acc[1] += (double)(gy * base);
+                acc[2] += (double)gy;
+            }
+            grho[i] = response_grho(rp, gy, base1);
         }
     }
     __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const {
@@ -676,8 +726,8 @@ static int check_prm(const double* prm, int n) {
 }
 
 template <typename T>
-static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double prm[5], double fact,
-                        int32_t pflags, T* pos_out, T* rho, T* Ubuf, T* F, void* bins, void* ws, size_t wsb, hipStream_t st) {
+static int sm_layer_fwd(int dm, const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double prm[5],
+                        double fact, int32_t pflags, T* pos_out, T* rho, T* Ubuf, T* F, void* bins, void* ws, size_t wsb, hipStream_t st) {
     Mesh m;
     int rc = check_fft_mesh(nmesh, &m);
     if (rc) return rc;
@@ -702,7 +752,7 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
     if ((rc = zpass_r2c<T>(nrows, m.n2, tw, PowRowLoad<T>{rho, m.n2, (T)prm[1]}, HalfRowStore<T>{U, nh, T(1)}, st))) return rc;
     if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{U, U, T(1)}, st))) return rc;
     // 3. x pass + filter + three inverse x passes
-    const SmFilter<T> fl{(T)(prm[2] * prm[2]), (T)(1.0 / (prm[3] * prm[3])), (T)prm[4]};
+    const SmFilter<T> fl = layer_filter<T>(prm, dm);
     const T a1e2 = (T)(1e-2 * prm[0]), invM = (T)(1.0 / (double)m.cells());
     if (fft_mesh_pow2(m)) {
         dispatch_n(m.n0, [&](auto N) {
@@ -738,8 +788,8 @@ static int sm_layer_fwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
 }
 
 template <typename T>
-static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double prm[5], double fact,
-                        int32_t alpha_rule, int32_t pflags, const T* rho, const T* Ubuf, const T* F, const void* bins,
+static int sm_layer_bwd(int dm, const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double prm[5],
+                        double fact, int32_t alpha_rule, int32_t pflags, const T* rho, const T* Ubuf, const T* F, const void* bins,
                         const T* gout, T* gin, double* grad5, void* ws, size_t wsb, hipStream_t st) {
     Mesh m;
     int rc = check_fft_mesh(nmesh, &m);
@@ -766,7 +816,9 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
         return rc;
     if ((rc = colpass<T, -1>(m.n1, 3 * m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
     // 3. x pass: Q, parameter partials, inverse x of 1e-2 alpha G Q
-    const SmFilter<T> fl{(T)(prm[2] * prm[2]), (T)(1.0 / (prm[3] * prm[3])), (T)prm[4]};
+    const SmFilter<T> fl = layer_filter<T>(prm, dm);
+    double gkh, gkl;
+    filter_grad_coeffs(prm, dm, &gkh, &gkl);
     const T a1e2 = (T)(1e-2 * prm[0]);
     const c2<T>* U = reinterpret_cast<const c2<T>*>(Ubuf);
     c2<T>* S = P.spec3;  // result reuses the first spectrum (read before written)
@@ -778,14 +830,14 @@ static int sm_layer_bwd(const T* pos, const ldl_pos_desc* pdesc, int64_t np, con
             ntx = g.ntiles;
             KTimer _t("sm_x_bwd", st);
             hipLaunchKernelGGL((k_sm_x_bwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U, S, hc, m.n2,
-                               kt, fl, a1e2, prm[2], prm[3], prm[0], (int)alpha_rule, P.part);
+                               kt, fl, a1e2, gkh, gkl, prm[0], (int)alpha_rule, P.part);
         });
     } else {
         for (int d = 0; d < 3; d++)
             if ((rc = colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{P.spec3 + d * hc, P.spec3 + d * hc, T(1)}, st))) return rc;
         const unsigned gf = grid_for(hc, 256, 4096);
         ntx = gf;
-        hipLaunchKernelGGL(k_sm_filter_bwd<T>, dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, m.n2, kt, fl, a1e2, prm[2], prm[3],
+        hipLaunchKernelGGL(k_sm_filter_bwd<T>, dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, m.n2, kt, fl, a1e2, gkh, gkl,
                            prm[0], (int)alpha_rule, P.part);
         if ((rc = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc;
     }
@@ -820,7 +872,7 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
     Mesh m;
     int rc = smooth_kind ? check_fft_mesh(nmesh, &m) : check_mesh(nmesh, &m);
     if (rc) return rc;
-    if ((rc = check_prm<T>(prm3, 3))) return rc;
+    if (prm3 && (rc = check_prm<T>(prm3, 3))) return rc;
     if (!rho || !target || !out5) return fail(LDL_EINVAL, "NULL array argument");
     if (p_norm != 1 && p_norm != 2) return fail(LDL_EINVAL, "p_norm must be 1 or 2");
     if (smooth_kind < 0 || smooth_kind > 2) return fail(LDL_EINVAL, "smooth_kind must be 0, 1 or 2");
@@ -841,7 +893,9 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
         const double nc = (double)cells;
         hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(1), 0, st, npix, (int64_t)1, nc);
     }
-    const RespPar<T> rp{(T)prm3[0], (T)prm3[1], (T)prm3[2], (T)index, p_norm == 1, field2};
+    // prm3 == NULL: the dm response F = rho
+    const RespPar<T> rp = prm3 ? RespPar<T>{(T)prm3[0], (T)prm3[1], (T)prm3[2], (T)index, p_norm == 1, field2, 0}
+                               : RespPar<T>{T(0), T(0), T(0), (T)index, p_norm == 1, field2, 1};
     RedMap r5{5, {0, 1, 2, 3, 4}};
     if (smooth_kind == 0) {
         { KTimer _t("loss", st); hipLaunchKernelGGL(k_loss_direct<T>, dim3(gb), dim3(256), 0, st, rho, target, mask, maskv, rp, cells, npix, grho, bpart); }
@@ -937,21 +991,40 @@ size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np
     int ldl_sm_layer_fwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
                                double fact, int32_t pflags, T* pos_out, T* rho, T* U, T* F, void* bins, void* ws, size_t wsb, \
                                void* stream) {                                                                              \
-        return sm_layer_fwd<T>(pos, pd, np, nmesh, prm, fact, pflags, pos_out, rho, U, F, bins, ws, wsb, as_stream(stream)); \
+        return sm_layer_fwd<T>(0, pos, pd, np, nmesh, prm, fact, pflags, pos_out, rho, U, F, bins, ws, wsb, as_stream(stream)); \
+    }                                                                                                                       \
+    int ldl_dm_layer_fwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
+                               double fact, int32_t pflags, T* pos_out, T* rho, T* U, T* F, void* bins, void* ws, size_t wsb, \
+                               void* stream) {                                                                              \
+        return sm_layer_fwd<T>(1, pos, pd, np, nmesh, prm, fact, pflags, pos_out, rho, U, F, bins, ws, wsb, as_stream(stream)); \
     }                                                                                                                       \
     int ldl_sm_layer_bwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
                                double fact, int32_t alpha_rule, int32_t pflags, const T* rho, const T* U, const T* F,      \
                                const void* bins, const T* g_out, T* g_in, double* grad5, void* ws, size_t wsb,             \
                                void* stream) {                                                                              \
-        return sm_layer_bwd<T>(pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, rho, U, F, bins, g_out, g_in, grad5, ws,  \
+        return sm_layer_bwd<T>(0, pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, rho, U, F, bins, g_out, g_in, grad5, ws, \
+                               wsb, as_stream(stream));                                                                     \
+    }                                                                                                                       \
+    int ldl_dm_layer_bwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
+                               double fact, int32_t alpha_rule, int32_t pflags, const T* rho, const T* U, const T* F,      \
+                               const void* bins, const T* g_out, T* g_in, double* grad5, void* ws, size_t wsb,             \
+                               void* stream) {                                                                              \
+        return sm_layer_bwd<T>(1, pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, rho, U, F, bins, g_out, g_in, grad5, ws, \
                                wsb, as_stream(stream));                                                                     \
     }                                                                                                                       \
     int ldl_sm_loss_##SUF(const T* rho, const T* target, const T* mask, const T* mask_valid, const T* field2,               \
                           const int64_t nmesh[3], const double box[3], const double prm3[3], double index, int32_t p_norm,  \
                           int32_t smooth_kind, double smooth_par, double* out5, T* g_rho, void* ws, size_t wsb,             \
                           void* stream) {                                                                                   \
+        if (!prm3) return fail(LDL_EINVAL, "prm3 is NULL");                                                                \
         return sm_loss<T>(rho, target, mask, mask_valid, field2, nmesh, box, prm3, index, p_norm, smooth_kind, smooth_par,  \
                           out5, g_rho, ws, wsb, as_stream(stream));                                                        \
+    }                                                                                                                       \
+    int ldl_dm_loss_##SUF(const T* rho, const T* target, const T* mask, const T* mask_valid, const T* field2,               \
+                          const int64_t nmesh[3], const double box[3], double index, int32_t p_norm, int32_t smooth_kind,   \
+                          double smooth_par, double* out5, T* g_rho, void* ws, size_t wsb, void* stream) {                 \
+        return sm_loss<T>(rho, target, mask, mask_valid, field2, nmesh, box, nullptr, index, p_norm, smooth_kind,          \
+                          smooth_par, out5, g_rho, ws, wsb, as_stream(stream));                                            \
     }                                                                                                                       \
     int ldl_paint_vjp_pos_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const T* g_rho, const int64_t nmesh[3],   \
                                 double fact, T* g, void* stream) {                                                          \

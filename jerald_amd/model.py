@@ -1,14 +1,16 @@
-"""JERALD's LDL model, stellar-mass ('sm') kind, on one MI355X.
+"""JERALD's LDL model, stellar-mass ('sm') and dark-matter ('dm') kinds, on one MI355X.
 
 Mirrors ``JERALDModel`` (model.py:62-479): ``displace``, ``evaluate``,
 ``loss``, ``loss_grad``, ``loss_and_grad``, ``lossv``, ``lossv_and_grad``
 with the reference's parameter layout ``[alpha, gamma, kh, kl, nu] * Nstep +
-[mu, w, b]`` (model.py:268-272, :339-341) and loss options (``n``, ``index``,
-``L1``, ``field2``, ``masktrain``, ``maskvalid``; model.py:350-468).
+[mu, w, b]`` (model.py:268-272, :339-341; dm: no response parameters, its map
+is the density, model.py:346-347) and loss options (``n``, ``index``, ``L1``,
+``field2``, ``masktrain``, ``maskvalid``; model.py:350-468).
 
 The whole loss is one ``torch.autograd.Function`` whose forward runs the fused
-layer kernels (libldl ``ldl_sm_layer_fwd_*``) and the response/loss kernel, and
-whose backward runs the hand-written adjoint kernels (``ldl_sm_layer_bwd_*``).
+layer kernels (libldl ``ldl_{sm,dm}_layer_fwd_*``) and the response/loss kernel,
+and whose backward runs the hand-written adjoint kernels
+(``ldl_{sm,dm}_layer_bwd_*``).
 Per layer the forward keeps the layer's input positions and, with
 ``save="all"``, its density, spectrum and force meshes; with
 ``save="positions"`` only the positions at layer boundaries are kept and the
@@ -23,7 +25,7 @@ Extensions over the reference, all opt-in:
     FFT VJP rules weight interior k_z modes by 1 instead of 2 in dL/dalpha
     (jaxpops_bind.py:758-759 with model.py:198); every other gradient agrees;
   * ``paint_mode``: "fixed" (64-bit fixed point, deterministic) or "float".
-Only the 'sm' kind is implemented; 'dm' and 'HI' raise NotImplementedError.
+The 'HI' kind raises NotImplementedError.
 """
 from __future__ import annotations
 
@@ -44,16 +46,17 @@ class JERALDModel:
     def __init__(self, X, target, pm, Nstep=2, kind="sm", n=0, index=1.0, L1=None, field2=None, masktrain=None,
                  maskvalid=None, starmap=None, *, smoothing=None, grad_rule="exact", paint_mode="fixed",
                  lattice=None, save="all", dtype=None, reuse_sort=True):
-        if kind in ("dm", "HI"):
-            raise NotImplementedError(f"kind '{kind}' is not implemented; only 'sm' is on this path")
-        self.kind = "sm"  # unknown kinds use the sm model (model.py:110-112)
+        if kind == "HI":
+            raise NotImplementedError("kind 'HI' is not implemented; 'sm' and 'dm' are on this path")
+        self.kind = "dm" if kind == "dm" else "sm"  # unknown kinds use the sm model (model.py:110-112)
+        self.nresp = 0 if self.kind == "dm" else 3  # response parameters after the layers
         self.pm = pm
         self.device = pm.device
         self.dtype = dtype if dtype is not None else pm.dtype
         self.Nstep = int(Nstep)
         self.n = float(n)
         self.index = float(index)
-        self.L1 = True if L1 is None else bool(L1)  # model.py:115-122 (True for non-dm kinds)
+        self.L1 = (self.kind != "dm") if L1 is None else bool(L1)  # model.py:115-122
         self.nm = tuple(int(v) for v in pm.Nmesh)
         self.box = tuple(float(v) for v in pm.BoxSize)
         self.cell = np.array(self.box) / np.array(self.nm)
@@ -135,7 +138,7 @@ class JERALDModel:
             nb = L.load().ldl_sm_bins_bytes(L.i64x3(nm), P.shape[0])
             bins = torch.empty(nb, dtype=torch.uint8, device=dev)
         ws = self._ws()
-        L.call(f"ldl_sm_layer_fwd_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], L.i64x3(nm), L.dvec(prm), float(fact),
+        L.call(f"ldl_{self.kind}_layer_fwd_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], L.i64x3(nm), L.dvec(prm), float(fact),
                self.paint_flags, L.ptr(out), L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(bins), L.ptr(ws), ws.numel(),
                L.stream_ptr(dev))
         return out, rho, U, F, bins
@@ -154,9 +157,14 @@ class JERALDModel:
         out5 = torch.zeros(5, dtype=torch.float64, device=dev)
         g_rho = torch.empty(nm, dtype=dt, device=dev) if need_grad else None
         ws = self._ws()
-        L.call(f"ldl_sm_loss_{L.sfx(dt)}", L.ptr(rho), L.ptr(self.target), L.ptr(self.masktrain), L.ptr(self.maskvalid),
-               L.ptr(self.field2), L.i64x3(nm), L.dvec(self.box), L.dvec(prm3), self.index, 1 if self.L1 else 2,
-               self.smooth_kind, self.smooth_par, L.ptr(out5), L.ptr(g_rho), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+        common = (L.ptr(rho), L.ptr(self.target), L.ptr(self.masktrain), L.ptr(self.maskvalid), L.ptr(self.field2),
+                  L.i64x3(nm), L.dvec(self.box))
+        tail = (self.index, 1 if self.L1 else 2, self.smooth_kind, self.smooth_par, L.ptr(out5), L.ptr(g_rho), L.ptr(ws),
+                ws.numel(), L.stream_ptr(dev))
+        if self.kind == "dm":
+            L.call(f"ldl_dm_loss_{L.sfx(dt)}", *common, *tail)
+        else:
+            L.call(f"ldl_sm_loss_{L.sfx(dt)}", *common, L.dvec(prm3), *tail)
         return out5, g_rho
 
     # ------------------------------------------------- model pieces
@@ -166,8 +174,8 @@ class JERALDModel:
         return p
 
     def _check(self, p, Nstep):
-        if p.shape != (5 * Nstep + 3,):
-            raise ValueError(f"expected {5 * Nstep + 3} parameters, got {p.shape}")
+        if p.shape != (5 * Nstep + self.nresp,):
+            raise ValueError(f"expected {5 * Nstep + self.nresp} parameters, got {p.shape}")
 
     def displace(self, params, X=None, Nstep=None):
         """Positions after Nstep layers (model.py:249-289), box units."""
@@ -180,7 +188,7 @@ class JERALDModel:
         return self._decode(P)
 
     def evaluate(self, params, X=None, Nstep=None):
-        """Model map ReLU(w (rho+1e-8)^mu + b) (model.py:291-347)."""
+        """Model map: sm ReLU(w (rho+1e-8)^mu + b), dm the density (model.py:291-347)."""
         Nstep = self.Nstep if Nstep is None else int(Nstep)
         p = self._params(params)
         P = self.X if X is None else self._encode(X)
@@ -188,6 +196,8 @@ class JERALDModel:
         for i in range(Nstep):
             P = self._layer_fwd(P, p[5 * i: 5 * i + 5], fact)[0]
         rho = self._paint_density(P, fact)
+        if self.kind == "dm":
+            return rho
         mu, w, b = (float(v) for v in p[5 * Nstep: 5 * Nstep + 3])
         y = w * (rho + 1e-8) ** mu + b
         return torch.where(y > 1e-12, y, torch.full_like(y, 1e-12))
@@ -225,11 +235,11 @@ class JERALDModel:
                 Pi = sv[0]
                 _, rho, U, F, bins = self._layer_fwd(Pi, prm, fact, keep_bins=self.reuse_sort)
             gp = C_ptr_offset(grads, 5 * i)
-            L.call(f"ldl_sm_layer_bwd_{L.sfx(dt)}", L.ptr(Pi), self._pd(), Pi.shape[0], L.i64x3(nm), L.dvec(prm),
+            L.call(f"ldl_{self.kind}_layer_bwd_{L.sfx(dt)}", L.ptr(Pi), self._pd(), Pi.shape[0], L.i64x3(nm), L.dvec(prm),
                    float(fact), self.alpha_rule, self.paint_flags, L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(bins),
                    L.ptr(g), L.ptr(g), gp, L.ptr(ws), ws.numel(), L.stream_ptr(dev))
             st["saved"][i] = None
-        return torch.cat([grads, st["out5"][1:4]]), g
+        return torch.cat([grads, st["out5"][1:1 + self.nresp]]), g
 
     # ------------------------------------------------------- losses
     def loss(self, params):

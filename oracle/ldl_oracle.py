@@ -16,10 +16,10 @@ Pinning (what checks each function, beyond retyping its formula):
   readout_grad .......... central finite differences of readout inside a cell,
                           golden vectors (lreadout3D vjp mode)
   pfft / pifft .......... brute-force O(N^6) DFT at N<=8, Parseval, numpy.fft
-  sm layer / loss /
-  loss_and_grad ......... central finite differences (FP64) for every parameter
+  sm and dm layer /
+  loss / loss_and_grad .. central finite differences (FP64) for every parameter
                           and for the particle positions; an independent torch
-                          float64 autograd restatement in tests
+                          float64 autograd restatement in tests (both kinds)
   reference VJP rules ... parity unpinned against JAX itself (JAX is absent);
                           the emulation follows jaxpops_bind.py:699-701,
                           :758-759 and model.py:23-29 and is checked for the
@@ -223,6 +223,11 @@ def green_sm(kn, kh, kl, nu):
     return -np.exp(-kn ** 2 / kl ** 2 - kh ** 2 / kn ** 2) * kn ** nu
 
 
+def green_dm(kn, kh, kl, nu):
+    """Dark-matter kernel -exp(-k/kl - kh/k) k^nu (model.py:156)."""
+    return -np.exp(-kn / kl - kh / kn) * kn ** nu
+
+
 def relu(x):
     """where(x > 1e-12, x, 1e-12) (model.py:54-57)."""
     return np.where(x > RELU_LOW, x, RELU_LOW)
@@ -252,14 +257,21 @@ def smoothing_kernel(nmesh, box, kind, n=0.0, sigma=None):
 # LDL stellar-mass model (model.py:185-212, :249-289, :291-347, :350-479)
 # ---------------------------------------------------------------------------
 class SMModelOracle:
-    """FP64 forward and exact hand-written backward of JERALD's 'sm' kind.
+    """FP64 forward and exact hand-written backward of JERALD's 'sm' kind, or
+    with ``kind="dm"`` of the 'dm' kind (model.py:136-183: kernel
+    -exp(-k/kl - kh/k) k^nu; the map is the density itself, model.py:346-347;
+    5 parameters per layer and none for the response).
 
-    loss options mirror JERALDModel: ``L1`` (default True for sm, model.py:115-122),
+    loss options mirror JERALDModel: ``L1`` (the caller passes it; the
+    reference defaults to True for sm and False for dm, model.py:115-122),
     ``n`` (reference smoothing exponent), ``index``, ``field2``, ``mask``.
     ``smoothing`` = None | "reference" | ("gaussian", sigma)."""
 
     def __init__(self, nmesh, box, nstep, target, L1=True, n=0.0, smoothing=None,
-                 index=1.0, field2=None, mask=None, grad_rule="exact", backend="numpy", threads=0):
+                 index=1.0, field2=None, mask=None, grad_rule="exact", backend="numpy", threads=0, kind="sm"):
+        if kind not in ("sm", "dm"):
+            raise ValueError(kind)
+        self.kind = kind
         self.nmesh = tuple(int(v) for v in np.broadcast_to(np.asarray(nmesh), (3,)))
         self.box = np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)).copy()
         self.nstep = int(nstep)
@@ -333,7 +345,7 @@ class SMModelOracle:
         rho = self._paint(x, 1.0) * fact
         u = (rho + EPS_RHO) ** gamma
         U = self._rfftn(u) / self.M
-        G = green_sm(self.kn, kh, kl, nu)
+        G = (green_dm if self.kind == "dm" else green_sm)(self.kn, kh, kl, nu)
         P = ALPHA_SCALE * alpha * U * G
         F = [self._irfftn(-1j * self.D[d] * P) * self.M for d in range(3)]
         s = np.stack([self._readout(x, F[d]) for d in range(3)], axis=-1)
@@ -356,6 +368,10 @@ class SMModelOracle:
         Xf, caches = self.displace(params, X, keep=True)
         xf = self._x(Xf)
         rho = self._paint(xf, 1.0) * fact
+        if self.kind == "dm":
+            if keep:
+                return rho, dict(caches=caches, xf=xf, rho=rho, F=rho, fact=fact)
+            return rho
         mu, w, b = params[5 * self.nstep: 5 * self.nstep + 3]
         base = (rho + EPS_RHO) ** mu
         y = w * base + b
@@ -402,13 +418,16 @@ class SMModelOracle:
             g = g * self.field2
         if self.index != 1.0:
             g = g * self.index * F ** (self.index - 1.0)
-        g_y = g * (st["y"] > RELU_LOW)
-        mu, w, b = params[5 * self.nstep: 5 * self.nstep + 3]
-        rhoe = st["rho"] + EPS_RHO
-        grad[5 * self.nstep + 0] = np.sum(g_y * w * st["base"] * np.log(rhoe))
-        grad[5 * self.nstep + 1] = np.sum(g_y * st["base"])
-        grad[5 * self.nstep + 2] = np.sum(g_y)
-        g_rho = g_y * w * mu * rhoe ** (mu - 1.0)
+        if self.kind == "dm":
+            g_rho = g
+        else:
+            g_y = g * (st["y"] > RELU_LOW)
+            mu, w, b = params[5 * self.nstep: 5 * self.nstep + 3]
+            rhoe = st["rho"] + EPS_RHO
+            grad[5 * self.nstep + 0] = np.sum(g_y * w * st["base"] * np.log(rhoe))
+            grad[5 * self.nstep + 1] = np.sum(g_y * st["base"])
+            grad[5 * self.nstep + 2] = np.sum(g_y)
+            g_rho = g_y * w * mu * rhoe ** (mu - 1.0)
         fact = st["fact"]
         gX = fact * self._readout_grad(st["xf"], g_rho)
         for i in reversed(range(self.nstep)):
@@ -429,8 +448,12 @@ class SMModelOracle:
         re = np.real(np.conj(Q) * U)
         grad[off + 0] = ALPHA_SCALE * np.sum(cw * G * re)
         gG = ALPHA_SCALE * alpha * self.c * re
-        grad[off + 2] = np.sum(gG * G * (-2.0 * kh / kn ** 2))
-        grad[off + 3] = np.sum(gG * G * (2.0 * kn ** 2 / kl ** 3))
+        if self.kind == "dm":  # d/dkh, d/dkl of -kn/kl - kh/kn
+            grad[off + 2] = np.sum(gG * G * (-1.0 / kn))
+            grad[off + 3] = np.sum(gG * G * (kn / kl ** 2))
+        else:
+            grad[off + 2] = np.sum(gG * G * (-2.0 * kh / kn ** 2))
+            grad[off + 3] = np.sum(gG * G * (2.0 * kn ** 2 / kl ** 3))
         grad[off + 4] = np.sum(gG * G * np.log(kn))
         g_u = self._irfftn(ALPHA_SCALE * alpha * G * Q)
         rhoe = cache["rho"] + EPS_RHO

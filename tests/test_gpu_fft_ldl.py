@@ -143,6 +143,26 @@ def test_ldl_mixed_radix_mesh_matches_oracle(N, lattice, smoothing, L1):
     np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=rtol, atol=1e-12 * np.abs(g0).max())
 
 
+@pytest.mark.parametrize("N,lattice,smoothing", [(16, None, None), (16, (16, 16, 16), ("gaussian", 4.0)), (12, None, ("gaussian", 4.0))])
+def test_dm_kind_matches_oracle(N, lattice, smoothing):
+    """dm kind (model.py:136-183, :346-347): kernel -exp(-k/kl - kh/k) k^nu,
+    map = density, 5 parameters per layer, L2 by default (model.py:115-122)."""
+    L, nstep = 4.0 * N, 2
+    X, tgt, mk, p, _ = _problem(N, nstep, L, mask=True)
+    p = p[:5 * nstep]
+    orc = O.SMModelOracle(N, L, nstep, tgt, L1=False, smoothing=smoothing, mask=mk, kind="dm")
+    loss0, g0 = orc.loss_and_grad(p, X)
+    pm = PMesh(N, L, device=DEV, dtype=torch.float64)
+    m = JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, kind="dm", masktrain=mk, smoothing=smoothing, lattice=lattice)
+    assert not m.L1
+    loss, grad = m.loss_and_grad(torch.tensor(p))
+    assert grad.shape == (5 * nstep,)
+    assert abs(loss.item() - loss0) <= 1e-10 * abs(loss0)
+    np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
+    rho = m.evaluate(torch.tensor(p))
+    np.testing.assert_allclose(rho.cpu().numpy(), orc.evaluate(p, X), rtol=1e-10, atol=1e-10)
+
+
 def test_ldl_fp32_within_1e5_of_fp64():
     N, L, nstep = 32, 128.0, 2
     X, tgt, mk, p, orc = _problem(N, nstep, L, smoothing=("gaussian", 4.0), L1=False, rms=2.0)

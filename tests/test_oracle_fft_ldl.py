@@ -96,6 +96,30 @@ def test_ldl_param_grad_finite_difference(smoothing, L1):
         assert abs(g[i] - fd) <= 1e-5 * abs(fd) + 1e-9, (i, g[i], fd)
 
 
+@pytest.mark.parametrize("smoothing,L1", [(None, False), ("gauss", True)])
+def test_dm_param_and_position_grad_finite_difference(smoothing, L1):
+    """dm kind: kernel -exp(-k/kl - kh/k) k^nu, map = density, 10 parameters."""
+    X, tgt, p, L = _setup()
+    p = p[:10]
+    sm = ("gaussian", L / 8) if smoothing == "gauss" else None
+    m = O.SMModelOracle(8, L, 2, tgt, L1=L1, smoothing=sm, kind="dm")
+    loss, g, gX = m.loss_and_grad(p, X, want_pos_grad=True)
+    assert g.shape == (10,)
+    for i in range(len(p)):
+        h = 1e-6 * max(abs(p[i]), 1.0)
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (m.loss(pp, X) - m.loss(pm, X)) / (2 * h)
+        assert abs(g[i] - fd) <= 1e-5 * abs(fd) + 1e-9, (i, g[i], fd)
+    for pi, d in [(5, 0), (123, 2)]:
+        Xp, Xm = X.copy(), X.copy()
+        Xp[pi, d] += 1e-6
+        Xm[pi, d] -= 1e-6
+        fd = (m.loss(p, Xp) - m.loss(p, Xm)) / 2e-6
+        assert abs(gX[pi, d] - fd) <= 1e-5 * abs(fd) + 1e-10
+
+
 def test_ldl_position_grad_finite_difference():
     X, tgt, p, L = _setup()
     m = O.SMModelOracle(8, L, 2, tgt, L1=False)
@@ -153,10 +177,13 @@ def _t_readout(x, F, nm):
     return acc
 
 
-def test_ldl_against_torch_autograd():
+@pytest.mark.parametrize("kind", ["sm", "dm"])
+def test_ldl_against_torch_autograd(kind):
     N, L = 8, 40.0
     X, tgt, p, _ = _setup(N=N, L=L)
-    m = O.SMModelOracle(N, L, 2, tgt, L1=False)
+    if kind == "dm":
+        p = p[:10]
+    m = O.SMModelOracle(N, L, 2, tgt, L1=False, kind=kind)
     loss, g = m.loss_and_grad(p, X)
     nm = (N, N, N)
     pt = torch.tensor(p, requires_grad=True)
@@ -170,14 +197,21 @@ def test_ldl_against_torch_autograd():
         x = torch.remainder(Xt, L) / cell
         rho = _t_paint(x, 1.0, nm) * fact
         U = torch.fft.rfftn((rho + 1e-8) ** gam) / N ** 3
-        G = -torch.exp(-kn ** 2 / kl ** 2 - kh ** 2 / kn ** 2) * kn ** nu
+        if kind == "dm":
+            G = -torch.exp(-kn / kl - kh / kn) * kn ** nu
+        else:
+            G = -torch.exp(-kn ** 2 / kl ** 2 - kh ** 2 / kn ** 2) * kn ** nu
         P = 1e-2 * a * U * G
         s = torch.stack([_t_readout(x, torch.fft.irfftn(-1j * D[d] * P, s=nm) * N ** 3, nm) for d in range(3)], -1)
         Xt = Xt + s
     x = torch.remainder(Xt, L) / cell
     rho = _t_paint(x, 1.0, nm) * fact
-    mu, w, b = pt[10:13]
-    F = torch.where(w * (rho + 1e-8) ** mu + b > 1e-12, w * (rho + 1e-8) ** mu + b, torch.tensor(1e-12, dtype=torch.float64))
+    if kind == "dm":
+        F = rho
+    else:
+        mu, w, b = pt[10:13]
+        F = torch.where(w * (rho + 1e-8) ** mu + b > 1e-12, w * (rho + 1e-8) ** mu + b,
+                        torch.tensor(1e-12, dtype=torch.float64))
     lt = torch.sum((F - torch.tensor(tgt)) ** 2) / N ** 3
     lt.backward()
     assert abs(lt.item() - loss) < 1e-12 * max(1, abs(loss))
