@@ -174,7 +174,9 @@ class JERALDModel:
         return p
 
     def _check(self, p, Nstep):
-        if p.shape != (5 * Nstep + self.nresp,):
+        # trailing extra entries are accepted and get a zero gradient, as in the
+        # reference (runJERALD.py:150 passes 5 Nstep + 4 for 'Mstar')
+        if p.ndim != 1 or p.shape[0] < 5 * Nstep + self.nresp:
             raise ValueError(f"expected {5 * Nstep + self.nresp} parameters, got {p.shape}")
 
     def displace(self, params, X=None, Nstep=None):
@@ -239,7 +241,8 @@ class JERALDModel:
                    float(fact), self.alpha_rule, self.paint_flags, L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(bins),
                    L.ptr(g), L.ptr(g), gp, L.ptr(ws), ws.numel(), L.stream_ptr(dev))
             st["saved"][i] = None
-        return torch.cat([grads, st["out5"][1:1 + self.nresp]]), g
+        extra = torch.zeros(len(p) - 5 * self.Nstep - self.nresp, dtype=torch.float64, device=dev)
+        return torch.cat([grads, st["out5"][1:1 + self.nresp], extra]), g
 
     # ------------------------------------------------------- losses
     def loss(self, params):

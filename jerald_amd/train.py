@@ -1,0 +1,271 @@
+"""Training around the LDL path: the reference's optimizers, bounds, masks,
+learning-rate schedule and training loop.
+
+* ``RMSProp`` and ``Adam`` (model.py:492-659), including the text state file
+  of ``Adam.save_state`` / ``Adam.import_state``;
+* ``split_masks`` (JERALD/runJERALD.py:114-127): train / validation / test
+  split of the mesh by cell index;
+* ``initial_params`` and ``bounds`` (runJERALD.py:145-166) and ``clip``, the
+  per-step clipping of runJERALD.py:190-196;
+* ``lr_schedule`` (runJERALD.py:177-180);
+* ``train`` (runJERALD.py:186-221): clip, ``lossv_and_grad``, keep the best
+  validation parameters, Adam step.
+
+Parameters and optimizer state live on the host as float64 NumPy vectors (a
+few dozen numbers); the loss and gradient come from ``JERALDModel`` on the GPU.
+
+    python -m jerald_amd.train --Nmesh 64 --Nstep 2 --target Mstar --Nopt 10
+runs the loop on a synthetic problem (seeded Zel'dovich particles and a
+target made by the model at perturbed parameters); the reference reads FastPM
+particles and bigfile maps instead.
+"""
+from __future__ import annotations
+
+import argparse
+import time
+import warnings
+
+import numpy as np
+
+
+def _lr(LR, itr):
+    return LR(itr) if callable(LR) else LR
+
+
+class RMSProp:
+    """RMSprop (model.py:492-528)."""
+
+    def __init__(self, LR=0.01, beta=0.999, eps=1e-8, eps_root=0.):
+        self.LR, self.beta, self.eps, self.eps_root = LR, beta, eps, eps_root
+
+    def init(self, params0):
+        self.v = np.zeros(len(params0))
+        self.itr = 0
+
+    def step(self, params, gradient):
+        LR = _lr(self.LR, self.itr)
+        g = np.asarray(gradient, dtype=np.float64)
+        self.v = (1 - self.beta) * g ** 2 + self.beta * self.v
+        self.itr += 1
+        vh = np.sqrt(self.v / (1. - self.beta ** self.itr) + self.eps_root)
+        return np.asarray(params, dtype=np.float64) - LR * (g / (vh + self.eps))
+
+
+class Adam:
+    """Adam (model.py:530-659) with the reference's state file layout:
+    ``mu`` / values / ``nu`` / values / ``itr`` / value / ``LR`` / value(s)
+    [/ ``params`` / values], one number per line."""
+
+    def __init__(self, LR=0.01, b1=0.9, b2=0.999, eps=1e-8, eps_root=0.):
+        self.LR, self.b1, self.b2, self.eps, self.eps_root = LR, b1, b2, eps, eps_root
+
+    def init(self, params0):
+        self.mu = np.zeros(len(params0))
+        self.nu = np.zeros(len(params0))
+        self.itr = 0
+
+    def step(self, params, gradient):
+        LR = _lr(self.LR, self.itr)
+        g = np.asarray(gradient, dtype=np.float64)
+        self.mu = self.b1 * self.mu + (1. - self.b1) * g
+        self.nu = self.b2 * self.nu + (1. - self.b2) * g ** 2
+        self.itr += 1
+        muh = self.mu / (1. - self.b1 ** self.itr)
+        nuh = self.nu / (1. - self.b2 ** self.itr)
+        return np.asarray(params, dtype=np.float64) - LR * (muh / (np.sqrt(nuh + self.eps_root) + self.eps))
+
+    def get_state(self):
+        return self.mu, self.nu, self.itr, _lr(self.LR, self.itr)
+
+    def import_state(self, filename, paramsreq=False):
+        """Reads a state file; with ``paramsreq`` returns (found, params)."""
+        mu, nu, params = [], [], []
+        itr, found, state = 0, False, 0
+        with open(filename) as f:
+            for line in f:
+                line = line.rstrip()
+                if state == 0:
+                    if line == "mu":
+                        state = 1
+                elif state == 1:
+                    if line == "nu":
+                        state = 2
+                    else:
+                        mu.append(float(line))
+                elif state == 2:
+                    if line == "itr":
+                        state = 3
+                    else:
+                        nu.append(float(line))
+                elif state == 3:
+                    itr = int(line)
+                    state = 4
+                elif state == 4:
+                    if line == "params":
+                        state = 5
+                        found = True
+                elif state == 5:
+                    params.append(float(line))
+        self.mu, self.nu, self.itr = np.array(mu), np.array(nu), itr
+        if paramsreq:
+            if found:
+                return True, np.array(params)
+            warnings.warn(f"Parameters not found in {filename}")
+            return False, 0
+
+    def save_state(self, filename, params=None):
+        LR = _lr(self.LR, self.itr)
+        with open(filename, "w") as f:
+            f.write("mu\n")
+            np.savetxt(f, self.mu)
+            f.write("nu\n")
+            np.savetxt(f, self.nu)
+            f.write("itr\n")
+            f.write(str(self.itr) + "\n")
+            f.write("LR\n")
+            if np.ndim(LR) > 0:
+                np.savetxt(f, LR)
+            else:
+                f.write(str(LR) + "\n")
+            if params is not None:
+                f.write("params\n")
+                np.savetxt(f, params)
+
+
+def lr_schedule(i, LRi=0.002, LRf=0.02, tau=100, n=1):
+    """Learning rate rising from LRi to LRf with time scale tau (runJERALD.py:177-180)."""
+    LRt = (LRf - LRi) * (2 / (np.exp(-i / tau) + 1) - 1) + LRi
+    return LRt * np.ones(n)
+
+
+def split_masks(nmesh):
+    """(train, validation, test) float masks of runJERALD.py:114-127: the
+    validation region is z < int(0.5 N); test is the cube x, y, z < int(0.43 N)
+    (taken out of validation); training is everything outside validation."""
+    N = int(nmesh)
+    i = np.arange(N)
+    x, y, z = np.meshgrid(i, i, i, indexing="ij")
+    select_validate = z < int(0.5 * N)
+    select_test = (x < int(0.43 * N)) & (y < int(0.43 * N)) & (z < int(0.43 * N))
+    mask_train = ~select_validate
+    mask_validate = select_validate & ~select_test
+    return mask_train.astype(float), mask_validate.astype(float), select_test.astype(float)
+
+
+def initial_params(target, Nstep, avgfact=1.0, zshift=0.0):
+    """params0 of runJERALD.py:145-153 ('Mstar' carries a fourth, unused response entry)."""
+    if target == "dm":
+        p = [1, 0.5, 1, 8, 0] * Nstep
+    elif target == "Mstar":
+        p = [1, 0.5, 1, 8, 0] * Nstep + [1, avgfact, 0, 0]
+    elif target == "HI":
+        p = [1, 0.5, 1, 8, 0, 1, 0.5, 1, 8, 0] * Nstep + [1, avgfact, -.1, 0.6, 1, 8, .01, 0, .1, 1, 0., zshift]
+    else:
+        raise ValueError(target)
+    return np.array(p, dtype=np.float64)
+
+
+def bounds(target, Nstep):
+    """Clipping bounds of runJERALD.py:156-166 (None = unbounded)."""
+    layer = [(None, None), (None, None), (0., None), (0.1, None), (None, None)]
+    if target == "dm":
+        return layer * Nstep
+    if target == "Mstar":
+        return layer * Nstep + [(0.1, None), (1e-5, None), (None, None)]
+    if target == "HI":
+        return (layer * 2) * Nstep + [(0.1, None), (1e-5, None), (None, None), (None, None), (0., None), (0.1, None),
+                                       (None, None), (None, None), (None, None), (None, None), (None, None), (None, None)]
+    raise ValueError(target)
+
+
+def clip(params, bnds):
+    """Clip each parameter into its bound (runJERALD.py:190-196); parameters
+    beyond the bounds list are left as they are."""
+    p = np.array(params, dtype=np.float64)
+    for j in range(min(len(p), len(bnds))):
+        lo, hi = bnds[j]
+        if lo is not None and p[j] < lo:
+            p[j] = lo
+        if hi is not None and p[j] > hi:
+            p[j] = hi
+    return p
+
+
+def train(model, params0, bnds, Nopt=10, optimizer=None, paramsave=None, optf=None, log=print):
+    """The training loop of runJERALD.py:186-221.  Returns (best params, best
+    validation loss, best step, history of (train loss, validation loss))."""
+    if optimizer is None:
+        optimizer = Adam(lambda i: lr_schedule(i, n=len(params0)))
+        optimizer.init(params0)
+    params = np.array(params0, dtype=np.float64)
+    best, bestparams, beststep, hist = np.inf, params.copy(), 0, []
+    t0 = time.time()
+    for i in range(Nopt):
+        params = clip(params, bnds)
+        st = time.time()
+        tl, vl, g = model.lossv_and_grad(params)
+        tl, vl = float(tl), float(vl)
+        g = g.detach().cpu().numpy() if hasattr(g, "detach") else np.asarray(g)
+        hist.append((tl, vl))
+        if vl < best:
+            best, bestparams, beststep = vl, params.copy(), i + 1
+            if paramsave:
+                np.savetxt(paramsave, bestparams)
+        log(f"Step {i + 1}:\nTrain loss = {tl:.5f}, validation loss = {vl:.5f}, max gradient = {np.max(np.abs(g)):.5f}, "
+            f"completed in {time.time() - st:.3f}s\nParams:\n{params}\nGrad:\n{g}")
+        params = optimizer.step(params, g)
+    log(f"Finished optimization in {time.time() - t0:.0f}s.\nBest validation loss: {best}, at step {beststep}\n"
+        f"Best parameters:\n{bestparams}")
+    if optf and hasattr(optimizer, "save_state"):
+        optimizer.save_state(optf, params)
+    return bestparams, best, beststep, hist
+
+
+def main():
+    import torch
+
+    from .model import JERALDModel
+    from .pmesh import PMesh
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--target", default="Mstar", choices=["dm", "Mstar"])
+    ap.add_argument("--Nmesh", type=int, default=64)
+    ap.add_argument("--BoxSize", type=float, default=None)
+    ap.add_argument("--Nstep", type=int, default=2)
+    ap.add_argument("--n", type=float, default=0.0)
+    ap.add_argument("--Nopt", type=int, default=10)
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--outpath", default=None)
+    a = ap.parse_args()
+    N = a.Nmesh
+    box = a.BoxSize or 0.8 * N
+    dt = torch.float32 if a.dtype == "f32" else torch.float64
+    # synthetic input: seeded lattice + Zel'dovich displacement (2 cells rms)
+    rng = np.random.default_rng(83)
+    wk = np.fft.rfftn(rng.standard_normal((N, N, N)))
+    k1 = 2 * np.pi * np.fft.fftfreq(N, box / N)
+    kz = k1[: N // 2 + 1]
+    k2 = k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kz[None, None, :] ** 2
+    dk = np.where(k2 > 0, np.sqrt(np.exp(-k2 * (2 * box / N) ** 2)) / np.where(k2 > 0, k2, 1) ** 1.5, 0) * wk
+    psi = np.stack([np.fft.irfftn(1j * kk * dk, s=(N, N, N)).ravel()
+                    for kk in (k1[:, None, None], k1[None, :, None], kz[None, None, :])], -1)
+    psi *= 2.0 * box / N / np.sqrt(np.mean(psi ** 2))
+    q = np.stack(np.meshgrid(*[np.arange(N) * box / N] * 3, indexing="ij"), -1).reshape(-1, 3)
+    X = q + psi
+    kind = "dm" if a.target == "dm" else "sm"
+    pm = PMesh(N, box, device="cuda", dtype=dt)
+    p_true = initial_params(a.target, a.Nstep)[: 5 * a.Nstep + (0 if kind == "dm" else 3)]
+    p_true[:5 * a.Nstep] *= 1.0 + 0.2 * np.cos(np.arange(5 * a.Nstep))
+    gen = JERALDModel(torch.tensor(X), None, pm, Nstep=a.Nstep, kind=kind, lattice=(N, N, N))
+    target = gen.evaluate(torch.tensor(p_true))
+    mtr, mva, _ = split_masks(N)
+    model = JERALDModel(torch.tensor(X), target, pm, Nstep=a.Nstep, kind=a.target, n=a.n, L1=(kind != "dm"),
+                        masktrain=mtr, maskvalid=mva, lattice=(N, N, N))
+    p0 = initial_params(a.target, a.Nstep, avgfact=float(target.double().mean()))
+    out = a.outpath
+    train(model, p0, bounds(a.target, a.Nstep), a.Nopt,
+          paramsave=f"{out}/{a.target}_Nmesh{N}_Nstep{a.Nstep}_n{a.n:.2f}_bestparam.txt" if out else None,
+          optf=f"{out}/{a.target}_Nmesh{N}_Nstep{a.Nstep}_n{a.n:.2f}_optstate.txt" if out else None)
+
+
+if __name__ == "__main__":
+    main()
