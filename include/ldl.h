@@ -255,6 +255,75 @@ int ldl_dm_loss_f64(const double* rho, const double* target, const double* mask,
                     int32_t p_norm, int32_t smooth_kind, double smooth_par, double* out5, double* g_rho,
                     void* workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------- HI-kind LDL layer
+ * Replaces JERALDModel.potential_gradient_HI (model.py:214-247) with
+ * prm = {alpha1, gamma1, kh1, kl1, nu1, alpha2, gamma2, kh2, kl2, nu2}
+ * (model.py:261-266, 279-285):  P = 1e-3 (alpha1 G1 U + alpha2 G2 S2) with
+ * G1 the sm kernel, G2 = +exp(-k^2/kl2^2 - kh2^2/k^2) k^nu2 and
+ * S2 = r2c(starmap^gamma2) (starmap (N0,N1,N2)).  Same workspace and saved
+ * tensors as the sm layer, plus S2 (N0,N1,N2/2+1 complex), written by the
+ * forward and read by the backward.  The backward ACCUMULATES the 10
+ * parameter gradients into grad10 (DEVICE double[10]); alpha_rule applies
+ * to alpha1 and alpha2 (both multiply outside c2f, model.py:231). */
+int ldl_hi_layer_fwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[10], double fact, int32_t paint_flags,
+                         const float* starmap, float* pos_out, float* rho, float* U, float* S2, float* F,
+                         void* bins, void* workspace, size_t workspace_bytes, void* stream);
+int ldl_hi_layer_fwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[10], double fact, int32_t paint_flags,
+                         const double* starmap, double* pos_out, double* rho, double* U, double* S2,
+                         double* F, void* bins, void* workspace, size_t workspace_bytes, void* stream);
+int ldl_hi_layer_bwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[10], double fact, int32_t alpha_rule,
+                         int32_t paint_flags, const float* starmap, const float* rho, const float* U,
+                         const float* S2, const float* F, const void* bins, const float* g_out, float* g_in,
+                         double* grad10, void* workspace, size_t workspace_bytes, void* stream);
+int ldl_hi_layer_bwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
+                         const int64_t nmesh[3], const double prm[10], double fact, int32_t alpha_rule,
+                         int32_t paint_flags, const double* starmap, const double* rho, const double* U,
+                         const double* S2, const double* F, const void* bins, const double* g_out,
+                         double* g_in, double* grad10, void* workspace, size_t workspace_bytes, void* stream);
+
+/* HI depletion map (model.py:326-332), prm8 = {beta1, khi, kli, nui, beta2,
+ * gamma3, xi, eta} (model.py:322-331):
+ *   sfield = c2r(starmapk W),  W = exp(-k/kli - khi/k) k^nui + xi,
+ *   dep    = ReLU(beta1 sfield + eta) + beta2 starmap^gamma3.
+ * starmapk = r2c(starmap) / N^3 as (N0,N1,N2/2+1) interleaved complex
+ * (PMesh.r2c, model.py:127), or NULL to compute it in the workspace.
+ * Workspace: ldl_sm_workspace_bytes(nmesh, is_f64, 0). */
+int ldl_hi_depletion_fwd_f32(const float* starmap, const float* starmapk, const int64_t nmesh[3],
+                             const double box[3], const double prm8[8], float* sfield, float* dep,
+                             void* workspace, size_t workspace_bytes, void* stream);
+int ldl_hi_depletion_fwd_f64(const double* starmap, const double* starmapk, const int64_t nmesh[3],
+                             const double box[3], const double prm8[8], double* sfield, double* dep,
+                             void* workspace, size_t workspace_bytes, void* stream);
+/* Its adjoint: given g_dep = dL/d dep, WRITES grad8 (DEVICE double[8]) =
+ * dL/d{beta1, khi, kli, nui, beta2, gamma3, xi, eta}.  The gamma3 partial
+ * takes starmap^gamma3 ln(starmap) as 0 where starmap == 0, as JAX does. */
+int ldl_hi_depletion_bwd_f32(const float* starmap, const float* starmapk, const float* sfield,
+                             const float* g_dep, const int64_t nmesh[3], const double box[3],
+                             const double prm8[8], double* grad8, void* workspace, size_t workspace_bytes,
+                             void* stream);
+int ldl_hi_depletion_bwd_f64(const double* starmap, const double* starmapk, const double* sfield,
+                             const double* g_dep, const int64_t nmesh[3], const double box[3],
+                             const double prm8[8], double* grad8, void* workspace, size_t workspace_bytes,
+                             void* stream);
+
+/* HI response + loss (model.py:313-334, :350-468): prm4 = {mu, w, b, zs},
+ * F = ReLU(w ReLU(rho + zs)^mu - dep + b), then as ldl_sm_loss.  Writes
+ * out6 (DEVICE double[6]) = {loss, dL/dmu, dL/dw, dL/db, loss_valid, dL/dzs},
+ * g_rho and g_dep = dL/d dep (both NULL: value only). */
+int ldl_hi_loss_f32(const float* rho, const float* target, const float* mask, const float* mask_valid,
+                    const float* field2, const float* dep, const int64_t nmesh[3], const double box[3],
+                    const double prm4[4], double index, int32_t p_norm, int32_t smooth_kind, double smooth_par,
+                    double* out6, float* g_rho, float* g_dep, void* workspace, size_t workspace_bytes,
+                    void* stream);
+int ldl_hi_loss_f64(const double* rho, const double* target, const double* mask, const double* mask_valid,
+                    const double* field2, const double* dep, const int64_t nmesh[3], const double box[3],
+                    const double prm4[4], double index, int32_t p_norm, int32_t smooth_kind, double smooth_par,
+                    double* out6, double* g_rho, double* g_dep, void* workspace, size_t workspace_bytes,
+                    void* stream);
+
 /* Final-layer adjoint gather (fused readout-gradient): g (Np,3) =
  * fact * grad(interp g_rho) at pos  (the ppaint VJP with unit mass,
  * jaxpops_bind.py:242-247). */

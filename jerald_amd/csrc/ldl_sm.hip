@@ -19,6 +19,17 @@
 //   g_X  = g + sum_d g_d grad F_d + fact grad g_rho  fused gather
 // Parameter reductions are per-workgroup FP64 partials summed in a fixed
 // order, so gradients are bit-reproducible run to run.
+//
+// The dm kind (model.py:136-183) swaps in its kernel and an identity response.
+// The HI kind (model.py:214-247) scales the potential by 1e-3 and adds a
+// second source, the star map:
+//   P = 1e-3 (alpha1 G1 U + alpha2 G2 S2),  S2 = r2c(starmap^gamma2) / N^3,
+//   G2 = +exp(-k^2/kl2^2 - kh2^2/k^2) k^nu2,
+// whose parameter partials the backward x pass adds against S2 and against
+// T2 = r2c(starmap^gamma2 ln starmap) / N^3 (for gamma2).  Its response
+// (model.py:313-334) is F = ReLU(w ReLU(rho + zs)^mu - dep + b) with the
+// depletion map dep = ReLU(beta1 c2r(r2c(starmap) W) + eta) + beta2
+// starmap^gamma3, W = exp(-k/kli - khi/k) k^nui + xi.
 #include "cic.h"
 #include "fft.h"
 
@@ -120,7 +131,7 @@ static SmWs sm_ws(const Mesh& m, int tsize, int64_t np) {
     w.spec3 = al((size_t)3 * m.hcells() * 2 * tsize);
     w.mesh3 = al((size_t)3 * m.cells() * tsize);
     w.tabs = al((size_t)2 * (m.n0 + m.n1 + m.nh()) * tsize);
-    w.part_doubles = 8 * (m.n0 * m.n1 + m.n1 * (m.nh() + 1) + 4096);
+    w.part_doubles = 12 * (m.n0 * m.n1 + m.n1 * (m.nh() + 1) + 4096);
     w.part = al((size_t)w.part_doubles * 8);
     w.total = w.paint + w.spec3 + w.mesh3 + w.tabs + w.part;
     return w;
@@ -166,7 +177,7 @@ static KTabs<T> build_tables(T* tabs, const Mesh& m, const double box[3], hipStr
 // (or store when ``assign``); fixed order, one workgroup.
 struct RedMap {
     int n;
-    int map[8];
+    int map[12];
 };
 __global__ void __launch_bounds__(256) k_reduce_cols(const double* __restrict__ part, int64_t nblocks, int stride, RedMap rm,
                                                      double* __restrict__ out, int assign) {
@@ -197,6 +208,23 @@ template <typename T> struct PowRowLoad {  // (rho + 1e-8)^gamma, model.py:193
     }
 };
 
+// starmap^g with 0^0 = 1, and starmap^g ln(starmap) with the value 0 at a zero
+// base: the derivative of x^g in g as JAX forms it (log of a zero base taken
+// as log 1), which the reference's gamma2 and gamma3 gradients inherit.
+template <typename T> __device__ __forceinline__ T spow(T x, T g) { return (x == T(0) && g == T(0)) ? T(1) : ppow(x, g); }
+template <typename T> __device__ __forceinline__ T spow_log(T x, T g) { return x == T(0) ? T(0) : ppow(x, g) * plog(x); }
+
+template <typename T> struct StarRowLoad {  // starmap^gamma2 (model.py:228), or times ln starmap
+    const T* star;
+    int64_t n2;
+    T gamma;
+    int with_log;
+    __device__ __forceinline__ c2<T> load2(int64_t row, int m) const {
+        const c2<T> r = *reinterpret_cast<const c2<T>*>(star + row * n2 + 2 * m);
+        return with_log ? c2<T>{spow_log(r.x, gamma), spow_log(r.y, gamma)} : c2<T>{spow(r.x, gamma), spow(r.y, gamma)};
+    }
+};
+
 // g_u -> g_rho = g_u gamma (rho+1e-8)^(gamma-1); dL/dgamma partial.
 template <typename T> struct GammaBwdStore {
     static constexpr int NR = 1;
@@ -219,12 +247,53 @@ template <typename T> struct GammaBwdStore {
 };
 
 // ------------------------------------------------------- fused x passes
+// Second potential source of the HI kind: P += a G_sm(kh2, kl2, nu2) S with
+// a = -1e-3 alpha2 (the sign turns the sm kernel into G2).  The backward adds
+// the partials of alpha2, kh2, kl2, nu2 (against S) and gamma2 (against Tl).
+template <typename T> struct Src2 {
+    const c2<T>* S;   // r2c(starmap^gamma2) / N^3
+    const c2<T>* Tl;  // r2c(starmap^gamma2 ln starmap) / N^3 (backward)
+    SmFilter<T> fl;
+    T a;
+    double ad, cs, gkh, gkl;  // ad = a in FP64, cs = -1e-3
+};
+
+// Per-cell parameter partials of the backward x pass.  acc[0..3]: alpha,
+// kh, kl, nu of the first source; with HI acc[4..8]: alpha2, kh2, kl2, nu2,
+// gamma2.  Returns G of the first source.
+template <typename T, bool HI>
+__device__ __forceinline__ T xbwd_cell(const SmFilter<T>& fl, const Src2<T>& s2, T k2, c2<T> q, c2<T> u, int64_t idx, double c,
+                                       double cpot, double apot, double gkh, double gkl, int alpha_rule, double* acc) {
+    T G, th, tl, lnk;
+    fl.eval(k2, G, th, tl, lnk);
+    const double re = (double)q.x * (double)u.x + (double)q.y * (double)u.y;
+    const double Gd = (double)G;
+    acc[0] += (alpha_rule ? 1.0 : c) * cpot * Gd * re;
+    const double gG = apot * c * re * Gd;  // dL/dG * G
+    acc[1] += gG * gkh * (double)th;
+    acc[2] += gG * gkl * (double)tl;
+    acc[3] += gG * (double)lnk;
+    if constexpr (HI) {  // both kernels are of sm form: th, tl, lnk are shared
+        const double G2 = (double)s2.fl.G(k2);
+        const c2<T> sv = s2.S[idx], tv = s2.Tl[idx];
+        const double re2 = (double)q.x * (double)sv.x + (double)q.y * (double)sv.y;
+        const double re3 = (double)q.x * (double)tv.x + (double)q.y * (double)tv.y;
+        acc[4] += (alpha_rule ? 1.0 : c) * s2.cs * G2 * re2;
+        const double gG2 = s2.ad * c * re2 * G2;
+        acc[5] += gG2 * s2.gkh * (double)th;
+        acc[6] += gG2 * s2.gkl * (double)tl;
+        acc[7] += gG2 * (double)lnk;
+        acc[8] += s2.ad * c * G2 * re3;
+    }
+    return G;
+}
+
 // Forward: X FFT, save U = . / N^3, P = 1e-2 alpha G U, three inverse X FFTs of
 // -i D_d P into out3 (3 spectra, hcells apart).
-template <typename T, int N>
+template <typename T, int N, bool HI>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
 k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>* __restrict__ out3, int64_t hcells,
-           KTabs<T> kt, SmFilter<T> fl, T a1e2, T invM) {
+           KTabs<T> kt, SmFilter<T> fl, T a1e2, T invM, Src2<T> s2) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
@@ -249,8 +318,14 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
         pk[s] = C{T(0), T(0)};
         if (p < pv) {
             U[base + n * g.es + p] = u;
-            const T G = fl.G(kt.knorm2(n, plane, col0 + p));
+            const T k2 = kt.knorm2(n, plane, col0 + p);
+            const T G = fl.G(k2);
             pk[s] = C{u.x * a1e2 * G, u.y * a1e2 * G};
+            if constexpr (HI) {
+                const C sv = s2.S[base + n * g.es + p];
+                const T G2 = s2.a * s2.fl.G(k2);
+                pk[s] = C{pk[s].x + G2 * sv.x, pk[s].y + G2 * sv.y};
+            }
         }
     }
 #pragma unroll 1
@@ -271,13 +346,13 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
 }
 
 // Backward: three forward X FFTs combined into Q = sum_d i D_d S_d, the FP64
-// parameter partials (alpha, kh, kl, nu) against the saved U, then one
-// inverse X FFT of 1e-2 alpha G Q into out.
-template <typename T, int N>
+// parameter partials (alpha, kh, kl, nu; HI also the second source's) against
+// the saved U, then one inverse X FFT of cpot alpha G Q into out.
+template <typename T, int N, bool HI>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
 k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3, const c2<T>* __restrict__ U,
            c2<T>* __restrict__ out, int64_t hcells, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh,
-           double gkl, double alpha, int alpha_rule, double* __restrict__ part) {
+           double gkl, double cpot, double alpha, int alpha_rule, Src2<T> s2, double* __restrict__ part) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
@@ -308,8 +383,9 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
         }
         __syncthreads();
     }
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const double a1e2d = 1e-2 * alpha;
+    constexpr int NP = HI ? 9 : 4;
+    double acc[NP] = {};
+    const double apot = cpot * alpha;
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
         const int i = threadIdx.x + s * TH;
@@ -317,18 +393,9 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
         C y = C{T(0), T(0)};
         if (p < pv) {
             const int64_t kz = col0 + p;
-            const T k2 = kt.knorm2(n, plane, kz);
-            T G, th, tl, lnk;
-            fl.eval(k2, G, th, tl, lnk);
-            const C u = U[base + n * g.es + p];
-            const double re = (double)q[s].x * (double)u.x + (double)q[s].y * (double)u.y;
+            const int64_t idx = base + n * g.es + p;
             const double c = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
-            const double Gd = (double)G;
-            acc[0] += (alpha_rule ? 1.0 : c) * 1e-2 * Gd * re;
-            const double gG = a1e2d * c * re * Gd;  // dL/dG * G
-            acc[1] += gG * gkh * (double)th;
-            acc[2] += gG * gkl * (double)tl;
-            acc[3] += gG * (double)lnk;
+            const T G = xbwd_cell<T, HI>(fl, s2, kt.knorm2(n, plane, kz), q[s], U[idx], idx, c, cpot, apot, gkh, gkl, alpha_rule, acc);
             y = C{a1e2 * G * q[s].x, a1e2 * G * q[s].y};
         }
         lds[n * LS + p] = y;
@@ -336,10 +403,10 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
     __syncthreads();
     lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
     tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
-    block_sum<4, TH>(acc);
+    block_sum<NP, TH>(acc);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) part[blockIdx.x * 4 + j] = acc[j];
+        for (int j = 0; j < NP; j++) part[blockIdx.x * NP + j] = acc[j];
     }
 }
 
@@ -388,29 +455,37 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
 // (index i -> (ix, iy, iz) with iz fastest), which do what the fused x
 // kernels do between their transforms.
 
-// out3[d] = -i D_d (1e-2 alpha G U)
-template <typename T>
+// out3[d] = -i D_d (cpot alpha G U [+ a G2 S2])
+template <typename T, bool HI>
 __global__ void __launch_bounds__(256) k_sm_filter_fwd(const c2<T>* __restrict__ U, c2<T>* __restrict__ out3, int64_t hcells, int64_t n1,
-                                                       int64_t nh, KTabs<T> kt, SmFilter<T> fl, T a1e2) {
+                                                       int64_t nh, KTabs<T> kt, SmFilter<T> fl, T a1e2, Src2<T> s2) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
-        const T G = fl.G(kt.knorm2(ix, iy, iz));
+        const T k2 = kt.knorm2(ix, iy, iz);
+        const T G = fl.G(k2);
         const c2<T> u = U[i];
-        const c2<T> p = c2<T>{u.x * a1e2 * G, u.y * a1e2 * G};
+        c2<T> p = c2<T>{u.x * a1e2 * G, u.y * a1e2 * G};
+        if constexpr (HI) {
+            const c2<T> sv = s2.S[i];
+            const T G2 = s2.a * s2.fl.G(k2);
+            p = c2<T>{p.x + G2 * sv.x, p.y + G2 * sv.y};
+        }
         const T D[3] = {kt.D[0][ix], kt.D[1][iy], kt.D[2][iz]};
 #pragma unroll
         for (int d = 0; d < 3; d++) out3[d * hcells + i] = c2<T>{D[d] * p.y, -D[d] * p.x};
     }
 }
 
-// Q = sum_d i D_d S_d; FP64 partials (alpha, kh, kl, nu) per workgroup into
-// part[4 * block + j]; out = 1e-2 alpha G Q (out may alias S3's first spectrum).
-template <typename T>
+// Q = sum_d i D_d S_d; FP64 partials (alpha, kh, kl, nu; HI also alpha2,
+// kh2, kl2, nu2, gamma2) per workgroup into part[NP * block + j]; out =
+// cpot alpha G Q (out may alias S3's first spectrum).
+template <typename T, bool HI>
 __global__ void __launch_bounds__(256) k_sm_filter_bwd(const c2<T>* S3, const c2<T>* __restrict__ U, c2<T>* out, int64_t hcells, int64_t n1,
                                                        int64_t nh, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh, double gkl,
-                                                       double alpha, int alpha_rule, double* __restrict__ part) {
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const double a1e2d = 1e-2 * alpha;
+                                                       double cpot, double alpha, int alpha_rule, Src2<T> s2, double* __restrict__ part) {
+    constexpr int NP = HI ? 9 : 4;
+    double acc[NP] = {};
+    const double apot = cpot * alpha;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
         const T D[3] = {kt.D[0][ix], kt.D[1][iy], kt.D[2][iz]};
@@ -420,23 +495,13 @@ __global__ void __launch_bounds__(256) k_sm_filter_bwd(const c2<T>* S3, const c2
             const c2<T> v = S3[d * hcells + i];
             q = c2<T>{q.x - D[d] * v.y, q.y + D[d] * v.x};  // += i D v
         }
-        const T k2 = kt.knorm2(ix, iy, iz);
-        T G, th, tl, lnk;
-        fl.eval(k2, G, th, tl, lnk);
-        const c2<T> u = U[i];
-        const double re = (double)q.x * (double)u.x + (double)q.y * (double)u.y;
         const double c = (iz == 0 || 2 * iz == n2) ? 1.0 : 2.0;
-        const double Gd = (double)G;
-        acc[0] += (alpha_rule ? 1.0 : c) * 1e-2 * Gd * re;
-        const double gG = a1e2d * c * re * Gd;  // dL/dG * G
-        acc[1] += gG * gkh * (double)th;
-        acc[2] += gG * gkl * (double)tl;
-        acc[3] += gG * (double)lnk;
+        const T G = xbwd_cell<T, HI>(fl, s2, kt.knorm2(ix, iy, iz), q, U[i], i, c, cpot, apot, gkh, gkl, alpha_rule, acc);
         out[i] = c2<T>{a1e2 * G * q.x, a1e2 * G * q.y};
     }
-    block_sum<4, 256>(acc);
+    block_sum<NP, 256>(acc);
     if (threadIdx.x == 0)
-        for (int j = 0; j < 4; j++) part[blockIdx.x * 4 + j] = acc[j];
+        for (int j = 0; j < NP; j++) part[blockIdx.x * NP + j] = acc[j];
 }
 
 // S *= scale * W(k) (the weights of k_x_realfilter)
@@ -519,39 +584,52 @@ __global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos
 
 // ------------------------------------------------------------------ loss
 // Response of the final density.  sm (model.py:338-344): F = ReLU(w (rho +
-// 1e-8)^mu + b); dm (``ident``, model.py:346-347): F = rho.
+// 1e-8)^mu + b); dm (``ident``, model.py:346-347): F = rho; HI
+// (model.py:313-334): F = ReLU(w ReLU(rho + zs)^mu - dep + b) with the
+// depletion map ``dep`` precomputed; ``gdep`` (optional) receives dL/d dep.
 template <typename T> struct RespPar {
     T mu, w, b, index;
     int l1;
     const T* field2;
     int ident;
+    int hi;
+    T zs;
+    const T* dep;
+    T* gdep;
 };
 
-// Forward response value at a cell: y, F, Fi (model.py:344, :358-362)
+__device__ __forceinline__ float relu12(float x) { return x > 1e-12f ? x : 1e-12f; }
+__device__ __forceinline__ double relu12(double x) { return x > 1e-12 ? x : 1e-12; }
+
+// Forward response value at cell i: y, F, Fi (model.py:344, :358-362)
 template <typename T>
-__device__ __forceinline__ void response(const RespPar<T>& rp, T rho, T f2, T& base, T& y, T& F, T& Fi) {
+__device__ __forceinline__ void response(const RespPar<T>& rp, int64_t i, T rho, T f2, T& base, T& y, T& F, T& Fi) {
     if (rp.ident) {
         base = y = F = rho;
     } else {
-        base = ppow(rho + T(1e-8), rp.mu);
+        base = ppow(rp.hi ? relu12(rho + rp.zs) : rho + T(1e-8), rp.mu);
         y = rp.w * base + rp.b;
-        F = y > T(1e-12) ? y : T(1e-12);
+        if (rp.hi) y -= rp.dep[i];
+        F = relu12(y);
     }
     Fi = rp.index == T(1) ? F : ppow(F, rp.index);
     Fi = Fi * f2;
 }
 
-// Same, also returning (rho+1e-8)^(mu-1) and ln(rho+1e-8) for the adjoint.
+// Same, also returning x^(mu-1) and ln x (x = rho+1e-8, HI ReLU(rho+zs)) for
+// the adjoint.
 template <typename T>
-__device__ __forceinline__ void response_adj(const RespPar<T>& rp, T rho, T f2, T& base, T& base1, T& lr, T& y, T& F, T& Fi) {
+__device__ __forceinline__ void response_adj(const RespPar<T>& rp, int64_t i, T rho, T f2, T& base, T& base1, T& lr, T& y, T& F,
+                                             T& Fi) {
     if (rp.ident) {
         base = y = F = rho;
         base1 = T(1);
         lr = T(0);
     } else {
-        pow_pair_log(rho + T(1e-8), rp.mu, base, base1, lr);
+        pow_pair_log(rp.hi ? relu12(rho + rp.zs) : rho + T(1e-8), rp.mu, base, base1, lr);
         y = rp.w * base + rp.b;
-        F = y > T(1e-12) ? y : T(1e-12);
+        if (rp.hi) y -= rp.dep[i];
+        F = relu12(y);
     }
     Fi = rp.index == T(1) ? F : ppow(F, rp.index);
     Fi = Fi * f2;
@@ -565,10 +643,16 @@ __device__ __forceinline__ T response_bwd(const RespPar<T>& rp, T gFi, T f2, T y
     return (rp.ident || y > T(1e-12)) ? g : T(0);
 }
 
-// d loss / d rho from d loss / d y (the mu, w, b partials are zero for ident)
+// d loss / d rho from d loss / d y (the mu, w, b partials are zero for
+// ident); HI also writes dL/d dep = -gy.
 template <typename T>
-__device__ __forceinline__ T response_grho(const RespPar<T>& rp, T gy, T base1) {
-    return rp.ident ? gy : gy * rp.w * rp.mu * base1;
+__device__ __forceinline__ T response_grho(const RespPar<T>& rp, int64_t i, T gy, T base1, T rho) {
+    if (rp.ident) return gy;
+    if (rp.hi) {
+        if (rp.gdep) rp.gdep[i] = -gy;
+        return rho + rp.zs > T(1e-12) ? gy * rp.w * rp.mu * base1 : T(0);
+    }
+    return gy * rp.w * rp.mu * base1;
 }
 
 
@@ -587,19 +671,21 @@ __global__ void __launch_bounds__(256) k_mask_sums(const T* __restrict__ mask, c
     }
 }
 
+// Loss sums per block: {loss, g_mu, g_w, g_b, loss_v, g_zs}
+static constexpr int kLossCols = 6;
+
 // Unsmoothed loss and its gradient in one pass (npix from device memory).
-// part per block: {loss, g_mu, g_w, g_b, loss_v}
 template <typename T>
 __global__ void __launch_bounds__(256) k_loss_direct(const T* __restrict__ rho, const T* __restrict__ target, const T* __restrict__ mask,
                                                      const T* __restrict__ maskv, RespPar<T> rp, int64_t n, const double* __restrict__ npix,
                                                      T* __restrict__ grho, double* __restrict__ part) {
     const double inv = 1.0 / npix[0];
-    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double a[kLossCols] = {};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const T r = rho[i];
         const T f2 = rp.field2 ? rp.field2[i] : T(1);
         T base, base1, lr, y, F, Fi;
-        response_adj(rp, r, f2, base, base1, lr, y, F, Fi);
+        response_adj(rp, i, r, f2, base, base1, lr, y, F, Fi);
         const T diff = Fi - target[i];
         const T ad = fabs(diff);
         const T mk = mask ? mask[i] : T(1);
@@ -615,12 +701,14 @@ __global__ void __launch_bounds__(256) k_loss_direct(const T* __restrict__ rho, 
                 a[2] += (double)(gy * base);
                 a[3] += (double)gy;
             }
-            grho[i] = response_grho(rp, gy, base1);
+            const T gr = response_grho(rp, i, gy, base1, r);
+            a[5] += (double)gr;
+            grho[i] = gr;
         }
     }
-    block_sum<5, 256>(a);
+    block_sum<kLossCols, 256>(a);
     if (threadIdx.x == 0)
-        for (int j = 0; j < 5; j++) part[blockIdx.x * 5 + j] = a[j];
+        for (int j = 0; j < kLossCols; j++) part[blockIdx.x * kLossCols + j] = a[j];
 }
 
 // diff = Fi - target (smoothed-loss path)
@@ -630,7 +718,7 @@ __global__ void __launch_bounds__(256) k_loss_diff(const T* __restrict__ rho, co
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const T f2 = rp.field2 ? rp.field2[i] : T(1);
         T base, y, F, Fi;
-        response(rp, rho[i], f2, base, y, F, Fi);
+        response(rp, i, rho[i], f2, base, y, F, Fi);
         diff[i] = Fi - target[i];
     }
 }
@@ -665,14 +753,14 @@ template <typename T> struct LossStore {
         }
     }
     __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const {
-        part[b * 5 + 0] = acc[0];
-        part[b * 5 + 4] = acc[1];
+        part[b * kLossCols + 0] = acc[0];
+        part[b * kLossCols + 4] = acc[1];
     }
 };
 
 // z c2r epilogue of the smoothed gradient: response backward -> g_rho.
 template <typename T> struct RespBwdStore {
-    static constexpr int NR = 3;
+    static constexpr int NR = 4;
     const T* rho;
     T* grho;
     RespPar<T> rp;
@@ -688,35 +776,146 @@ template <typename T> struct RespBwdStore {
             const T r = rho[i];
             const T f2 = rp.field2 ? rp.field2[i] : T(1);
             T base, base1, lr, y, F, Fi;
-            response_adj(rp, r, f2, base, base1, lr, y, F, Fi);
+            response_adj(rp, i, r, f2, base, base1, lr, y, F, Fi);
             const T gy = response_bwd(rp, gv[e], f2, y, F);
             if (!rp.ident) {
                 acc[0] += (double)(gy * rp.w * base) * (double)lr;
                 acc[1] += (double)(gy * base);
                 acc[2] += (double)gy;
             }
-            grho[i] = response_grho(rp, gy, base1);
+            const T gr = response_grho(rp, i, gy, base1, r);
+            acc[3] += (double)gr;
+            grho[i] = gr;
         }
     }
     __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const {
-        part[b * 5 + 1] = acc[0];
-        part[b * 5 + 2] = acc[1];
-        part[b * 5 + 3] = acc[2];
+        part[b * kLossCols + 1] = acc[0];
+        part[b * kLossCols + 2] = acc[1];
+        part[b * kLossCols + 3] = acc[2];
+        part[b * kLossCols + 5] = acc[3];
     }
 };
 
-// out5 = {loss / npix, g_mu, g_w, g_b, loss_v / npix_v}
-__global__ void k_loss_finish(const double* __restrict__ sums, const double* __restrict__ npix, double* __restrict__ out5) {
+// out = {loss / npix, g_mu, g_w, g_b, loss_v / npix_v[, g_zs]}
+__global__ void k_loss_finish(const double* __restrict__ sums, const double* __restrict__ npix, double* __restrict__ out, int nout) {
     if (threadIdx.x == 0) {
-        out5[0] = sums[0] / npix[0];
-        out5[1] = sums[1];
-        out5[2] = sums[2];
-        out5[3] = sums[3];
-        out5[4] = npix[1] > 0.0 ? sums[4] / npix[1] : 0.0;
+        out[0] = sums[0] / npix[0];
+        out[1] = sums[1];
+        out[2] = sums[2];
+        out[3] = sums[3];
+        out[4] = npix[1] > 0.0 ? sums[4] / npix[1] : 0.0;
+        if (nout > 5) out[5] = sums[5];
     }
 }
 
+// ------------------------------------------------------- HI depletion
+// dep = ReLU(beta1 S + eta) + beta2 starmap^gamma3 with S = c2r(Sk W), Sk =
+// r2c(starmap) / N^3 and W = exp(-k/kli - khi/k) k^nui + xi (model.py:326-332).
+// W is xi minus the dm-form kernel at (khi, kli, nui).
+template <typename T> struct DepPar {
+    T beta1, beta2, gamma3, eta;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_hi_wmult(const c2<T>* __restrict__ Sk, c2<T>* __restrict__ out, int64_t hcells, int64_t n1,
+                                                  int64_t nh, KTabs<T> kt, SmFilter<T> fl, T xi) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
+        const T w = xi - fl.G(kt.knorm2(ix, iy, iz));
+        const c2<T> v = Sk[i];
+        out[i] = c2<T>{v.x * w, v.y * w};
+    }
+}
+
+// z c2r epilogue: S (kept for the backward) and dep.
+template <typename T> struct DepStore {
+    static constexpr int NR = 0;
+    const T* star;
+    T* S;
+    T* dep;
+    int64_t n2;
+    DepPar<T> dp;
+    __device__ __forceinline__ void reduce_out(int64_t, const double*) const {}
+    __device__ __forceinline__ void store2(int64_t row, int m, c2<T> v, double*) const {
+        const int64_t i = row * n2 + 2 * m;
+        const c2<T> x = *reinterpret_cast<const c2<T>*>(star + i);
+        *reinterpret_cast<c2<T>*>(S + i) = v;
+        const T d0 = relu12(dp.beta1 * v.x + dp.eta) + dp.beta2 * spow(x.x, dp.gamma3);
+        const T d1 = relu12(dp.beta1 * v.y + dp.eta) + dp.beta2 * spow(x.y, dp.gamma3);
+        *reinterpret_cast<c2<T>*>(dep + i) = c2<T>{d0, d1};
+    }
+};
+
+// Depletion backward, elementwise: g_S = beta1 gdep [beta1 S + eta > 1e-12]
+// and the partials {beta1, beta2, gamma3, eta} per block.
+template <typename T>
+__global__ void __launch_bounds__(256) k_hi_dep_bwd(const T* __restrict__ star, const T* __restrict__ S, const T* __restrict__ gdep, int64_t n,
+                                                    DepPar<T> dp, T* __restrict__ gS, double* __restrict__ part) {
+    double a[4] = {};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T g = gdep[i], sv = S[i], x = star[i];
+        a[1] += (double)g * (double)spow(x, dp.gamma3);
+        a[2] += (double)g * (double)dp.beta2 * (double)spow_log(x, dp.gamma3);
+        const T gA = dp.beta1 * sv + dp.eta > T(1e-12) ? g : T(0);
+        a[3] += (double)gA;
+        a[0] += (double)gA * (double)sv;
+        gS[i] = dp.beta1 * gA;
+    }
+    block_sum<4, 256>(a);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 4; j++) part[blockIdx.x * 4 + j] = a[j];
+}
+
+// dL/d(khi, kli, nui, xi) = sum_H c Re(conj(ghat) Sk) dW/dtheta, ghat the
+// unnormalised r2c of g_S: dW/dkhi = G/k, dW/dkli = -G k/kli^2,
+// dW/dnui = -G ln k, dW/dxi = 1 (G the dm-form kernel).
+template <typename T>
+__global__ void __launch_bounds__(256) k_hi_wgrad(const c2<T>* __restrict__ ghat, const c2<T>* __restrict__ Sk, int64_t hcells, int64_t n1,
+                                                  int64_t nh, int64_t n2, KTabs<T> kt, SmFilter<T> fl, double inv_kli2,
+                                                  double* __restrict__ part) {
+    double a[4] = {};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
+        T G, th, tl, lnk;
+        fl.eval(kt.knorm2(ix, iy, iz), G, th, tl, lnk);
+        const c2<T> g = ghat[i], sk = Sk[i];
+        const double c = (iz == 0 || 2 * iz == n2) ? 1.0 : 2.0;
+        const double re = c * ((double)g.x * (double)sk.x + (double)g.y * (double)sk.y);
+        const double Gd = (double)G;
+        a[0] += re * Gd * (double)th;
+        a[1] -= re * Gd * (double)tl * inv_kli2;
+        a[2] -= re * Gd * (double)lnk;
+        a[3] += re;
+    }
+    block_sum<4, 256>(a);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 4; j++) part[blockIdx.x * 4 + j] = a[j];
+}
+
 // ------------------------------------------------------------ host side
+// out = scale r2c(rows of ``load``): z pass, y pass, x pass (plain kernels).
+template <typename T, class Load>
+static int r2c_full(const Mesh& m, const c2<T>* tw, Load load, c2<T>* out, T scale, hipStream_t st) {
+    const int64_t nh = m.nh();
+    int rc;
+    if ((rc = zpass_r2c<T>(m.n0 * m.n1, m.n2, tw, load, HalfRowStore<T>{out, nh, T(1)}, st))) return rc;
+    if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{out, out, T(1)}, st))) return rc;
+    return colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{out, out, scale}, st);
+}
+
+// Second source of an HI layer from prm[5..9] = [alpha2, gamma2, kh2, kl2, nu2].
+template <typename T>
+static Src2<T> hi_src2(const double* prm, const c2<T>* S, const c2<T>* Tl) {
+    Src2<T> s2{};
+    s2.S = S;
+    s2.Tl = Tl;
+    s2.fl = layer_filter<T>(prm + 5, 0);
+    s2.ad = -1e-3 * prm[5];
+    s2.a = (T)s2.ad;
+    s2.cs = -1e-3;
+    filter_grad_coeffs(prm + 5, 0, &s2.gkh, &s2.gkl);
+    return s2;
+}
 template <typename T>
 static int check_prm(const double* prm, int n) {
     if (!prm) return fail(LDL_EINVAL, "parameter array is NULL");
@@ -725,14 +924,20 @@ static int check_prm(const double* prm, int n) {
     return LDL_OK;
 }
 
+// Layer kinds: 0 sm, 1 dm, 2 HI (prm has 10 entries; star and S2buf used).
+enum { kKindSm = 0, kKindDm = 1, kKindHi = 2 };
+
 template <typename T>
-static int sm_layer_fwd(int dm, const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double prm[5],
-                        double fact, int32_t pflags, T* pos_out, T* rho, T* Ubuf, T* F, void* bins, void* ws, size_t wsb, hipStream_t st) {
+static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double* prm,
+                        double fact, int32_t pflags, const T* star, T* pos_out, T* rho, T* Ubuf, T* S2buf, T* F, void* bins, void* ws,
+                        size_t wsb, hipStream_t st) {
     Mesh m;
     int rc = check_fft_mesh(nmesh, &m);
     if (rc) return rc;
-    if ((rc = check_prm<T>(prm, 5))) return rc;
+    const bool hi = kind == kKindHi;
+    if ((rc = check_prm<T>(prm, hi ? 10 : 5))) return rc;
     if (!pos || !pos_out || !rho || !Ubuf || !F) return fail(LDL_EINVAL, "NULL array argument");
+    if (hi && (!star || !S2buf)) return fail(LDL_EINVAL, "NULL array argument");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
     const Sweep sw = make_sweep(pd, np);
@@ -751,20 +956,36 @@ static int sm_layer_fwd(int dm, const T* pos, const ldl_pos_desc* pdesc, int64_t
     // 2. z pass with the gamma power, y pass
     if ((rc = zpass_r2c<T>(nrows, m.n2, tw, PowRowLoad<T>{rho, m.n2, (T)prm[1]}, HalfRowStore<T>{U, nh, T(1)}, st))) return rc;
     if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{U, U, T(1)}, st))) return rc;
+    const T invM = (T)(1.0 / (double)m.cells());
+    // HI: second source S2 = r2c(starmap^gamma2) / N^3 (model.py:228)
+    Src2<T> s2{};
+    if (hi) {
+        c2<T>* S2 = reinterpret_cast<c2<T>*>(S2buf);
+        if ((rc = r2c_full<T>(m, tw, StarRowLoad<T>{star, m.n2, (T)prm[6], 0}, S2, invM, st))) return rc;
+        s2 = hi_src2<T>(prm, S2, nullptr);
+    }
     // 3. x pass + filter + three inverse x passes
-    const SmFilter<T> fl = layer_filter<T>(prm, dm);
-    const T a1e2 = (T)(1e-2 * prm[0]), invM = (T)(1.0 / (double)m.cells());
+    const SmFilter<T> fl = layer_filter<T>(prm, kind == kKindDm);
+    const T a1e2 = (T)((hi ? 1e-3 : 1e-2) * prm[0]);
     if (fft_mesh_pow2(m)) {
         dispatch_n(m.n0, [&](auto N) {
             constexpr int NN = decltype(N)::value;
             ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
             KTimer _t("sm_x_fwd", st);
-            hipLaunchKernelGGL((k_sm_x_fwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, hc, kt, fl,
-                               a1e2, invM);
+            if (hi)
+                hipLaunchKernelGGL((k_sm_x_fwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, hc,
+                                   kt, fl, a1e2, invM, s2);
+            else
+                hipLaunchKernelGGL((k_sm_x_fwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3,
+                                   hc, kt, fl, a1e2, invM, s2);
         });
     } else {
         if ((rc = colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{U, U, invM}, st))) return rc;
-        hipLaunchKernelGGL(k_sm_filter_fwd<T>, dim3(grid_for(hc, 256, 1 << 16)), dim3(256), 0, st, U, P.spec3, hc, m.n1, nh, kt, fl, a1e2);
+        const unsigned gf = grid_for(hc, 256, 1 << 16);
+        if (hi)
+            hipLaunchKernelGGL((k_sm_filter_fwd<T, true>), dim3(gf), dim3(256), 0, st, U, P.spec3, hc, m.n1, nh, kt, fl, a1e2, s2);
+        else
+            hipLaunchKernelGGL((k_sm_filter_fwd<T, false>), dim3(gf), dim3(256), 0, st, U, P.spec3, hc, m.n1, nh, kt, fl, a1e2, s2);
         for (int d = 0; d < 3; d++)
             if ((rc = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{P.spec3 + d * hc, P.spec3 + d * hc, T(1)}, st))) return rc;
     }
@@ -788,14 +1009,16 @@ static int sm_layer_fwd(int dm, const T* pos, const ldl_pos_desc* pdesc, int64_t
 }
 
 template <typename T>
-static int sm_layer_bwd(int dm, const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double prm[5],
-                        double fact, int32_t alpha_rule, int32_t pflags, const T* rho, const T* Ubuf, const T* F, const void* bins,
-                        const T* gout, T* gin, double* grad5, void* ws, size_t wsb, hipStream_t st) {
+static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double* prm,
+                        double fact, int32_t alpha_rule, int32_t pflags, const T* star, const T* rho, const T* Ubuf, const T* S2buf,
+                        const T* F, const void* bins, const T* gout, T* gin, double* grad, void* ws, size_t wsb, hipStream_t st) {
     Mesh m;
     int rc = check_fft_mesh(nmesh, &m);
     if (rc) return rc;
-    if ((rc = check_prm<T>(prm, 5))) return rc;
-    if (!pos || !rho || !Ubuf || !F || !gout || !gin || !grad5) return fail(LDL_EINVAL, "NULL array argument");
+    const bool hi = kind == kKindHi;
+    if ((rc = check_prm<T>(prm, hi ? 10 : 5))) return rc;
+    if (!pos || !rho || !Ubuf || !F || !gout || !gin || !grad) return fail(LDL_EINVAL, "NULL array argument");
+    if (hi && (!star || !S2buf)) return fail(LDL_EINVAL, "NULL array argument");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
     const Sweep sw = make_sweep(pd, np);
@@ -815,11 +1038,20 @@ static int sm_layer_bwd(int dm, const T* pos, const ldl_pos_desc* pdesc, int64_t
     if ((rc = zpass_r2c<T>(3 * nrows, m.n2, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore<T>{P.spec3, nh, T(1)}, st, "zpass_r2c3")))
         return rc;
     if ((rc = colpass<T, -1>(m.n1, 3 * m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
-    // 3. x pass: Q, parameter partials, inverse x of 1e-2 alpha G Q
-    const SmFilter<T> fl = layer_filter<T>(prm, dm);
+    // HI: T2 = r2c(starmap^gamma2 ln starmap) / N^3 for the gamma2 partial, in
+    // the mesh scratch (free after the z pass above; 2 Nh <= 3 N^3 elements)
+    Src2<T> s2{};
+    if (hi) {
+        c2<T>* T2 = reinterpret_cast<c2<T>*>(P.mesh3);
+        if ((rc = r2c_full<T>(m, tw, StarRowLoad<T>{star, m.n2, (T)prm[6], 1}, T2, (T)(1.0 / (double)cells), st))) return rc;
+        s2 = hi_src2<T>(prm, reinterpret_cast<const c2<T>*>(S2buf), T2);
+    }
+    // 3. x pass: Q, parameter partials, inverse x of cpot alpha G Q
+    const SmFilter<T> fl = layer_filter<T>(prm, kind == kKindDm);
     double gkh, gkl;
-    filter_grad_coeffs(prm, dm, &gkh, &gkl);
-    const T a1e2 = (T)(1e-2 * prm[0]);
+    filter_grad_coeffs(prm, kind == kKindDm, &gkh, &gkl);
+    const double cpot = hi ? 1e-3 : 1e-2;
+    const T a1e2 = (T)(cpot * prm[0]);
     const c2<T>* U = reinterpret_cast<const c2<T>*>(Ubuf);
     c2<T>* S = P.spec3;  // result reuses the first spectrum (read before written)
     int64_t ntx = 0;
@@ -829,20 +1061,30 @@ static int sm_layer_bwd(int dm, const T* pos, const ldl_pos_desc* pdesc, int64_t
             ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
             ntx = g.ntiles;
             KTimer _t("sm_x_bwd", st);
-            hipLaunchKernelGGL((k_sm_x_bwd<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U, S, hc, m.n2,
-                               kt, fl, a1e2, gkh, gkl, prm[0], (int)alpha_rule, P.part);
+            if (hi)
+                hipLaunchKernelGGL((k_sm_x_bwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U, S,
+                                   hc, m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
+            else
+                hipLaunchKernelGGL((k_sm_x_bwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U,
+                                   S, hc, m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
         });
     } else {
         for (int d = 0; d < 3; d++)
             if ((rc = colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{P.spec3 + d * hc, P.spec3 + d * hc, T(1)}, st))) return rc;
         const unsigned gf = grid_for(hc, 256, 4096);
         ntx = gf;
-        hipLaunchKernelGGL(k_sm_filter_bwd<T>, dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, m.n2, kt, fl, a1e2, gkh, gkl,
-                           prm[0], (int)alpha_rule, P.part);
+        if (hi)
+            hipLaunchKernelGGL((k_sm_filter_bwd<T, true>), dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, m.n2, kt, fl, a1e2, gkh,
+                               gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
+        else
+            hipLaunchKernelGGL((k_sm_filter_bwd<T, false>), dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, m.n2, kt, fl, a1e2, gkh,
+                               gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
         if ((rc = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc;
     }
-    RedMap rx{4, {0, 2, 3, 4}};
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, ntx, 4, rx, grad5, 0);
+    // partial columns -> parameter slots: alpha, kh, kl, nu (HI: then alpha2,
+    // kh2, kl2, nu2, gamma2)
+    const RedMap rx = hi ? RedMap{9, {0, 2, 3, 4, 5, 7, 8, 9, 6}} : RedMap{4, {0, 2, 3, 4}};
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, ntx, rx.n, rx, grad, 0);
     // 4. inverse y, z c2r with the gamma-power adjoint -> g_rho (mesh3[0])
     if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc;
     T* grho = P.mesh3;
@@ -851,7 +1093,7 @@ static int sm_layer_bwd(int dm, const T* pos, const ldl_pos_desc* pdesc, int64_t
                            GammaBwdStore<T>{rho, grho, m.n2, (T)(1.0 / (double)cells), (T)prm[1], P.part}, st, &nzb)))
         return rc;
     RedMap rz{1, {1}};
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, nzb, 1, rz, grad5, 0);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, nzb, 1, rz, grad, 0);
     // 5. position adjoint
     if (np > 0)
         {
@@ -865,15 +1107,18 @@ static int sm_layer_bwd(int dm, const T* pos, const ldl_pos_desc* pdesc, int64_t
     return LDL_OK;
 }
 
+// kind: 0 sm (prm = [mu, w, b]), 1 dm (prm unused), 2 HI (prm = [mu, w, b,
+// zs], dep required, gdep optional); out has 5 entries, 6 for HI (+ g_zs).
 template <typename T>
-static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv, const T* field2, const int64_t nmesh[3],
-                   const double box[3], const double prm3[3], double index, int32_t p_norm, int32_t smooth_kind, double smooth_par,
-                   double* out5, T* grho, void* ws, size_t wsb, hipStream_t st) {
+static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const T* maskv, const T* field2, const T* dep, const int64_t nmesh[3],
+                   const double box[3], const double* prm, double index, int32_t p_norm, int32_t smooth_kind, double smooth_par,
+                   double* out, T* grho, T* gdep, void* ws, size_t wsb, hipStream_t st) {
     Mesh m;
     int rc = smooth_kind ? check_fft_mesh(nmesh, &m) : check_mesh(nmesh, &m);
     if (rc) return rc;
-    if (prm3 && (rc = check_prm<T>(prm3, 3))) return rc;
-    if (!rho || !target || !out5) return fail(LDL_EINVAL, "NULL array argument");
+    if (kind != kKindDm && (rc = check_prm<T>(prm, kind == kKindHi ? 4 : 3))) return rc;
+    if (!rho || !target || !out) return fail(LDL_EINVAL, "NULL array argument");
+    if (kind == kKindHi && !dep) return fail(LDL_EINVAL, "NULL array argument");
     if (p_norm != 1 && p_norm != 2) return fail(LDL_EINVAL, "p_norm must be 1 or 2");
     if (smooth_kind < 0 || smooth_kind > 2) return fail(LDL_EINVAL, "smooth_kind must be 0, 1 or 2");
     if (smooth_kind && (!box || !(box[0] > 0 && box[1] > 0 && box[2] > 0))) return fail(LDL_EINVAL, "box must be positive");
@@ -883,7 +1128,7 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
     const int64_t cells = m.cells();
     // part layout: [0, 16) npix scratch and sums; block partials from 64
     double* npix = P.part;          // {npix, npix_v}
-    double* sums = P.part + 8;      // 5 sums
+    double* sums = P.part + 8;      // kLossCols sums
     double* bpart = P.part + 64;
     const unsigned gb = 2048;
     hipLaunchKernelGGL(k_mask_sums<T>, dim3(gb), dim3(256), 0, st, mask, maskv, cells, bpart);
@@ -893,13 +1138,26 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
         const double nc = (double)cells;
         hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(1), 0, st, npix, (int64_t)1, nc);
     }
-    // prm3 == NULL: the dm response F = rho
-    const RespPar<T> rp = prm3 ? RespPar<T>{(T)prm3[0], (T)prm3[1], (T)prm3[2], (T)index, p_norm == 1, field2, 0}
-                               : RespPar<T>{T(0), T(0), T(0), (T)index, p_norm == 1, field2, 1};
-    RedMap r5{5, {0, 1, 2, 3, 4}};
+    RespPar<T> rp{};
+    rp.index = (T)index;
+    rp.l1 = p_norm == 1;
+    rp.field2 = field2;
+    rp.ident = kind == kKindDm;
+    if (kind != kKindDm) {
+        rp.mu = (T)prm[0];
+        rp.w = (T)prm[1];
+        rp.b = (T)prm[2];
+    }
+    if (kind == kKindHi) {
+        rp.hi = 1;
+        rp.zs = (T)prm[3];
+        rp.dep = dep;
+        rp.gdep = grho ? gdep : nullptr;
+    }
+    RedMap r6{kLossCols, {0, 1, 2, 3, 4, 5}};
     if (smooth_kind == 0) {
         { KTimer _t("loss", st); hipLaunchKernelGGL(k_loss_direct<T>, dim3(gb), dim3(256), 0, st, rho, target, mask, maskv, rp, cells, npix, grho, bpart); }
-        hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, (int64_t)gb, 5, r5, sums, 1);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, (int64_t)gb, kLossCols, r6, sums, 1);
     } else {
         const c2<T>* tw;
         if ((rc = twiddles<T>(&tw))) return rc;
@@ -932,7 +1190,7 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
         };
         const int64_t nzb = zpass_c2r_blocks<T>(nrows, m.n2);
         // every block writes the loss columns; the gradient columns stay zero without g_rho
-        hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nzb * 5, 256)), dim3(256), 0, st, bpart, nzb * 5, 0.0);
+        hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nzb * kLossCols, 256)), dim3(256), 0, st, bpart, nzb * kLossCols, 0.0);
         hipLaunchKernelGGL(k_loss_diff<T>, dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff);
         if ((rc = smooth(diff))) return rc;
         if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, nh},
@@ -943,9 +1201,85 @@ static int sm_loss(const T* rho, const T* target, const T* mask, const T* maskv,
             if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, nh}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart}, st, nullptr)))
                 return rc;
         }
-        hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, nzb, 5, r5, sums, 1);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, nzb, kLossCols, r6, sums, 1);
     }
-    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(64), 0, st, sums, npix, out5);
+    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(64), 0, st, sums, npix, out, kind == kKindHi ? 6 : 5);
+    LDL_LAUNCH_CHECK();
+    return LDL_OK;
+}
+
+// prm = [beta1, khi, kli, nui, beta2, gamma3, xi, eta] (model.py:322-331)
+template <typename T>
+static int hi_dep_setup(const T* star, const c2<T>* Sk, const int64_t nmesh[3], const double box[3], const double* prm, void* ws, size_t wsb,
+                        hipStream_t st, Mesh* m, SmPtrs<T>* P, const c2<T>** tw, KTabs<T>* kt, const c2<T>** sk) {
+    int rc = check_fft_mesh(nmesh, m);
+    if (rc) return rc;
+    if ((rc = check_prm<T>(prm, 8))) return rc;
+    if (!star) return fail(LDL_EINVAL, "NULL array argument");
+    if (!box || !(box[0] > 0 && box[1] > 0 && box[2] > 0)) return fail(LDL_EINVAL, "box must be positive");
+    if (!(prm[2] != 0.0)) return fail(LDL_EINVAL, "kli must be nonzero");
+    const SmWs w = sm_ws(*m, sizeof(T), 0);
+    if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
+    *P = sm_ptrs<T>(ws, w);
+    if ((rc = twiddles<T>(tw))) return rc;
+    *kt = build_tables<T>(P->tabs, *m, box, st);
+    *sk = Sk;
+    if (!Sk) {  // Sk = r2c(starmap) / N^3 into the second spectrum slot
+        c2<T>* o = P->spec3 + m->hcells();
+        if ((rc = r2c_full<T>(*m, *tw, RealRowLoad<T>{star, m->n2}, o, (T)(1.0 / (double)m->cells()), st))) return rc;
+        *sk = o;
+    }
+    return LDL_OK;
+}
+
+template <typename T>
+static int hi_depletion_fwd(const T* star, const c2<T>* Sk, const int64_t nmesh[3], const double box[3], const double* prm, T* S, T* dep,
+                            void* ws, size_t wsb, hipStream_t st) {
+    Mesh m;
+    SmPtrs<T> P;
+    const c2<T>* tw;
+    KTabs<T> kt;
+    const c2<T>* sk;
+    int rc = hi_dep_setup<T>(star, Sk, nmesh, box, prm, ws, wsb, st, &m, &P, &tw, &kt, &sk);
+    if (rc) return rc;
+    if (!S || !dep) return fail(LDL_EINVAL, "NULL array argument");
+    const int64_t nh = m.nh(), hc = m.hcells();
+    const SmFilter<T> fl{(T)prm[1], (T)(1.0 / prm[2]), (T)prm[3], 1};
+    c2<T>* W = P.spec3;
+    hipLaunchKernelGGL(k_hi_wmult<T>, dim3(grid_for(hc, 256, 1 << 16)), dim3(256), 0, st, sk, W, hc, m.n1, nh, kt, fl, (T)prm[6]);
+    if ((rc = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{W, W, T(1)}, st))) return rc;
+    if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{W, W, T(1)}, st))) return rc;
+    const DepPar<T> dp{(T)prm[0], (T)prm[4], (T)prm[5], (T)prm[7]};
+    if ((rc = zpass_c2r<T>(m.n0 * m.n1, m.n2, tw, HalfRowLoad<T>{W, nh}, DepStore<T>{star, S, dep, m.n2, dp}, st, nullptr))) return rc;
+    LDL_LAUNCH_CHECK();
+    return LDL_OK;
+}
+
+template <typename T>
+static int hi_depletion_bwd(const T* star, const c2<T>* Sk, const T* S, const T* gdep, const int64_t nmesh[3], const double box[3],
+                            const double* prm, double* grad8, void* ws, size_t wsb, hipStream_t st) {
+    Mesh m;
+    SmPtrs<T> P;
+    const c2<T>* tw;
+    KTabs<T> kt;
+    const c2<T>* sk;
+    int rc = hi_dep_setup<T>(star, Sk, nmesh, box, prm, ws, wsb, st, &m, &P, &tw, &kt, &sk);
+    if (rc) return rc;
+    if (!S || !gdep || !grad8) return fail(LDL_EINVAL, "NULL array argument");
+    const int64_t nh = m.nh(), hc = m.hcells(), cells = m.cells();
+    const DepPar<T> dp{(T)prm[0], (T)prm[4], (T)prm[5], (T)prm[7]};
+    T* gS = P.mesh3;
+    const unsigned gb = 2048;
+    hipLaunchKernelGGL(k_hi_dep_bwd<T>, dim3(gb), dim3(256), 0, st, star, S, gdep, cells, dp, gS, P.part);
+    const RedMap re{4, {0, 4, 5, 7}};
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, (int64_t)gb, 4, re, grad8, 1);
+    c2<T>* gh = P.spec3;
+    if ((rc = r2c_full<T>(m, tw, RealRowLoad<T>{gS, m.n2}, gh, T(1), st))) return rc;
+    const SmFilter<T> fl{(T)prm[1], (T)(1.0 / prm[2]), (T)prm[3], 1};
+    const unsigned gf = grid_for(hc, 256, 4096);
+    hipLaunchKernelGGL(k_hi_wgrad<T>, dim3(gf), dim3(256), 0, st, gh, sk, hc, m.n1, nh, m.n2, kt, fl, 1.0 / (prm[2] * prm[2]), P.part);
+    const RedMap rw{4, {1, 2, 3, 6}};
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, (int64_t)gf, 4, rw, grad8, 1);
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }
@@ -991,40 +1325,74 @@ size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np
     int ldl_sm_layer_fwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
                                double fact, int32_t pflags, T* pos_out, T* rho, T* U, T* F, void* bins, void* ws, size_t wsb, \
                                void* stream) {                                                                              \
-        return sm_layer_fwd<T>(0, pos, pd, np, nmesh, prm, fact, pflags, pos_out, rho, U, F, bins, ws, wsb, as_stream(stream)); \
+        return sm_layer_fwd<T>(kKindSm, pos, pd, np, nmesh, prm, fact, pflags, nullptr, pos_out, rho, U, nullptr, F, bins, ws, \
+                               wsb, as_stream(stream));                                                                     \
     }                                                                                                                       \
     int ldl_dm_layer_fwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
                                double fact, int32_t pflags, T* pos_out, T* rho, T* U, T* F, void* bins, void* ws, size_t wsb, \
                                void* stream) {                                                                              \
-        return sm_layer_fwd<T>(1, pos, pd, np, nmesh, prm, fact, pflags, pos_out, rho, U, F, bins, ws, wsb, as_stream(stream)); \
+        return sm_layer_fwd<T>(kKindDm, pos, pd, np, nmesh, prm, fact, pflags, nullptr, pos_out, rho, U, nullptr, F, bins, ws, \
+                               wsb, as_stream(stream));                                                                     \
+    }                                                                                                                       \
+    int ldl_hi_layer_fwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[10], \
+                               double fact, int32_t pflags, const T* starmap, T* pos_out, T* rho, T* U, T* S2, T* F,        \
+                               void* bins, void* ws, size_t wsb, void* stream) {                                            \
+        return sm_layer_fwd<T>(kKindHi, pos, pd, np, nmesh, prm, fact, pflags, starmap, pos_out, rho, U, S2, F, bins, ws, wsb, \
+                               as_stream(stream));                                                                          \
     }                                                                                                                       \
     int ldl_sm_layer_bwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
                                double fact, int32_t alpha_rule, int32_t pflags, const T* rho, const T* U, const T* F,      \
                                const void* bins, const T* g_out, T* g_in, double* grad5, void* ws, size_t wsb,             \
                                void* stream) {                                                                              \
-        return sm_layer_bwd<T>(0, pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, rho, U, F, bins, g_out, g_in, grad5, ws, \
-                               wsb, as_stream(stream));                                                                     \
+        return sm_layer_bwd<T>(kKindSm, pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, nullptr, rho, U, nullptr, F, bins, \
+                               g_out, g_in, grad5, ws, wsb, as_stream(stream));                                             \
     }                                                                                                                       \
     int ldl_dm_layer_bwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
                                double fact, int32_t alpha_rule, int32_t pflags, const T* rho, const T* U, const T* F,      \
                                const void* bins, const T* g_out, T* g_in, double* grad5, void* ws, size_t wsb,             \
                                void* stream) {                                                                              \
-        return sm_layer_bwd<T>(1, pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, rho, U, F, bins, g_out, g_in, grad5, ws, \
-                               wsb, as_stream(stream));                                                                     \
+        return sm_layer_bwd<T>(kKindDm, pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, nullptr, rho, U, nullptr, F, bins, \
+                               g_out, g_in, grad5, ws, wsb, as_stream(stream));                                             \
+    }                                                                                                                       \
+    int ldl_hi_layer_bwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[10], \
+                               double fact, int32_t alpha_rule, int32_t pflags, const T* starmap, const T* rho, const T* U, \
+                               const T* S2, const T* F, const void* bins, const T* g_out, T* g_in, double* grad10,         \
+                               void* ws, size_t wsb, void* stream) {                                                        \
+        return sm_layer_bwd<T>(kKindHi, pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, starmap, rho, U, S2, F, bins,   \
+                               g_out, g_in, grad10, ws, wsb, as_stream(stream));                                            \
     }                                                                                                                       \
     int ldl_sm_loss_##SUF(const T* rho, const T* target, const T* mask, const T* mask_valid, const T* field2,               \
                           const int64_t nmesh[3], const double box[3], const double prm3[3], double index, int32_t p_norm,  \
                           int32_t smooth_kind, double smooth_par, double* out5, T* g_rho, void* ws, size_t wsb,             \
                           void* stream) {                                                                                   \
         if (!prm3) return fail(LDL_EINVAL, "prm3 is NULL");                                                                \
-        return sm_loss<T>(rho, target, mask, mask_valid, field2, nmesh, box, prm3, index, p_norm, smooth_kind, smooth_par,  \
-                          out5, g_rho, ws, wsb, as_stream(stream));                                                        \
+        return sm_loss<T>(kKindSm, rho, target, mask, mask_valid, field2, nullptr, nmesh, box, prm3, index, p_norm,        \
+                          smooth_kind, smooth_par, out5, g_rho, nullptr, ws, wsb, as_stream(stream));                       \
     }                                                                                                                       \
     int ldl_dm_loss_##SUF(const T* rho, const T* target, const T* mask, const T* mask_valid, const T* field2,               \
                           const int64_t nmesh[3], const double box[3], double index, int32_t p_norm, int32_t smooth_kind,   \
                           double smooth_par, double* out5, T* g_rho, void* ws, size_t wsb, void* stream) {                 \
-        return sm_loss<T>(rho, target, mask, mask_valid, field2, nmesh, box, nullptr, index, p_norm, smooth_kind,          \
-                          smooth_par, out5, g_rho, ws, wsb, as_stream(stream));                                            \
+        return sm_loss<T>(kKindDm, rho, target, mask, mask_valid, field2, nullptr, nmesh, box, nullptr, index, p_norm,     \
+                          smooth_kind, smooth_par, out5, g_rho, nullptr, ws, wsb, as_stream(stream));                       \
+    }                                                                                                                       \
+    int ldl_hi_loss_##SUF(const T* rho, const T* target, const T* mask, const T* mask_valid, const T* field2,               \
+                          const T* dep, const int64_t nmesh[3], const double box[3], const double prm4[4], double index,    \
+                          int32_t p_norm, int32_t smooth_kind, double smooth_par, double* out6, T* g_rho, T* g_dep,         \
+                          void* ws, size_t wsb, void* stream) {                                                             \
+        if (!prm4) return fail(LDL_EINVAL, "prm4 is NULL");                                                                \
+        return sm_loss<T>(kKindHi, rho, target, mask, mask_valid, field2, dep, nmesh, box, prm4, index, p_norm,            \
+                          smooth_kind, smooth_par, out6, g_rho, g_dep, ws, wsb, as_stream(stream));                         \
+    }                                                                                                                       \
+    int ldl_hi_depletion_fwd_##SUF(const T* starmap, const T* starmapk, const int64_t nmesh[3], const double box[3],       \
+                                   const double prm8[8], T* sfield, T* dep, void* ws, size_t wsb, void* stream) {           \
+        return hi_depletion_fwd<T>(starmap, reinterpret_cast<const c2<T>*>(starmapk), nmesh, box, prm8, sfield, dep, ws,   \
+                                   wsb, as_stream(stream));                                                                 \
+    }                                                                                                                       \
+    int ldl_hi_depletion_bwd_##SUF(const T* starmap, const T* starmapk, const T* sfield, const T* g_dep,                   \
+                                   const int64_t nmesh[3], const double box[3], const double prm8[8], double* grad8,        \
+                                   void* ws, size_t wsb, void* stream) {                                                    \
+        return hi_depletion_bwd<T>(starmap, reinterpret_cast<const c2<T>*>(starmapk), sfield, g_dep, nmesh, box, prm8,     \
+                                   grad8, ws, wsb, as_stream(stream));                                                      \
     }                                                                                                                       \
     int ldl_paint_vjp_pos_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const T* g_rho, const int64_t nmesh[3],   \
                                 double fact, T* g, void* stream) {                                                          \

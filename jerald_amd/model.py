@@ -1,16 +1,22 @@
-"""JERALD's LDL model, stellar-mass ('sm') and dark-matter ('dm') kinds, on one MI355X.
+"""JERALD's LDL model -- stellar-mass ('sm'), dark-matter ('dm') and neutral
+hydrogen ('HI') kinds -- on one MI355X.
 
 Mirrors ``JERALDModel`` (model.py:62-479): ``displace``, ``evaluate``,
 ``loss``, ``loss_grad``, ``loss_and_grad``, ``lossv``, ``lossv_and_grad``
-with the reference's parameter layout ``[alpha, gamma, kh, kl, nu] * Nstep +
-[mu, w, b]`` (model.py:268-272, :339-341; dm: no response parameters, its map
-is the density, model.py:346-347) and loss options (``n``, ``index``, ``L1``,
-``field2``, ``masktrain``, ``maskvalid``; model.py:350-468).
+with the reference's parameter layout (model.py:255-289, :313-344):
+  sm: ``[alpha, gamma, kh, kl, nu] * Nstep + [mu, w, b]``;
+  dm: ``[alpha, gamma, kh, kl, nu] * Nstep`` (the map is the density);
+  HI: ``[alpha1, gamma1, kh1, kl1, nu1, alpha2, gamma2, kh2, kl2, nu2] * Nstep
+  + [mu, w, b, beta1, khi, kli, nui, beta2, gamma3, xi, eta, zs]`` with the
+  stellar-mass map ``starmap`` as second source and depletion input;
+and loss options (``n``, ``index``, ``L1``, ``field2``, ``masktrain``,
+``maskvalid``; model.py:350-468).
 
 The whole loss is one ``torch.autograd.Function`` whose forward runs the fused
-layer kernels (libldl ``ldl_{sm,dm}_layer_fwd_*``) and the response/loss kernel,
-and whose backward runs the hand-written adjoint kernels
-(``ldl_{sm,dm}_layer_bwd_*``).
+layer kernels (libldl ``ldl_{sm,dm,hi}_layer_fwd_*``), the HI depletion map
+(``ldl_hi_depletion_fwd_*``) and the response/loss kernel, and whose backward
+runs the hand-written adjoint kernels (``ldl_{sm,dm,hi}_layer_bwd_*``,
+``ldl_hi_depletion_bwd_*``).
 Per layer the forward keeps the layer's input positions and, with
 ``save="all"``, its density, spectrum and force meshes; with
 ``save="positions"`` only the positions at layer boundaries are kept and the
@@ -25,7 +31,6 @@ Extensions over the reference, all opt-in:
     FFT VJP rules weight interior k_z modes by 1 instead of 2 in dL/dalpha
     (jaxpops_bind.py:758-759 with model.py:198); every other gradient agrees;
   * ``paint_mode``: "fixed" (64-bit fixed point, deterministic) or "float".
-The 'HI' kind raises NotImplementedError.
 """
 from __future__ import annotations
 
@@ -46,10 +51,11 @@ class JERALDModel:
     def __init__(self, X, target, pm, Nstep=2, kind="sm", n=0, index=1.0, L1=None, field2=None, masktrain=None,
                  maskvalid=None, starmap=None, *, smoothing=None, grad_rule="exact", paint_mode="fixed",
                  lattice=None, save="all", dtype=None, reuse_sort=True):
-        if kind == "HI":
-            raise NotImplementedError("kind 'HI' is not implemented; 'sm' and 'dm' are on this path")
-        self.kind = "dm" if kind == "dm" else "sm"  # unknown kinds use the sm model (model.py:110-112)
-        self.nresp = 0 if self.kind == "dm" else 3  # response parameters after the layers
+        # unknown kinds use the sm model (model.py:110-112)
+        self.kind = kind if kind in ("dm", "sm", "HI") else "sm"
+        self._k = {"sm": "sm", "dm": "dm", "HI": "hi"}[self.kind]  # libldl entry-point prefix
+        self.npot = 10 if self.kind == "HI" else 5  # parameters per layer (model.py:130-133)
+        self.nresp = {"sm": 3, "dm": 0, "HI": 12}[self.kind]  # response parameters after the layers
         self.pm = pm
         self.device = pm.device
         self.dtype = dtype if dtype is not None else pm.dtype
@@ -83,8 +89,13 @@ class JERALDModel:
         self.field2 = _as_mesh(field2, self.nm, self.dtype, self.device)
         self.masktrain = _as_mesh(masktrain, self.nm, self.dtype, self.device)
         self.maskvalid = _as_mesh(maskvalid, self.nm, self.dtype, self.device)
-        if starmap is not None:
-            raise NotImplementedError("starmap is only used by the HI kind")
+        self.starmap = _as_mesh(starmap, self.nm, self.dtype, self.device)
+        self.starmapk = None
+        if self.starmap is not None:  # r2c(starmap) (model.py:125-127), kept for the depletion filter
+            self.starmapk = pm.r2c(self.starmap).to(torch.complex128 if self.dtype == torch.float64
+                                                   else torch.complex64).contiguous()
+        elif self.kind == "HI":
+            raise ValueError("Star map required for HI target")  # model.py:128
         self.lattice = None if lattice is None else tuple(int(v) for v in lattice)
         self.X = None if X is None else self._encode(X)
 
@@ -128,6 +139,8 @@ class JERALDModel:
         return L.workspaces.get(self.device, f"sm{self.dtype}", nb)
 
     def _layer_fwd(self, P, prm, fact, keep_bins=False):
+        """One layer: (positions out, rho, U, F, bins, S2); S2 (HI second
+        source spectrum) is None for the other kinds."""
         dt, dev, nm = self.dtype, self.device, self.nm
         rho = torch.empty(nm, dtype=dt, device=dev)
         U = torch.empty(nm[0], nm[1], nm[2] // 2 + 1, 2, dtype=dt, device=dev)
@@ -138,10 +151,16 @@ class JERALDModel:
             nb = L.load().ldl_sm_bins_bytes(L.i64x3(nm), P.shape[0])
             bins = torch.empty(nb, dtype=torch.uint8, device=dev)
         ws = self._ws()
-        L.call(f"ldl_{self.kind}_layer_fwd_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], L.i64x3(nm), L.dvec(prm), float(fact),
-               self.paint_flags, L.ptr(out), L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(bins), L.ptr(ws), ws.numel(),
-               L.stream_ptr(dev))
-        return out, rho, U, F, bins
+        head = (L.ptr(P), self._pd(), P.shape[0], L.i64x3(nm), L.dvec(prm), float(fact), self.paint_flags)
+        tail = (L.ptr(F), L.ptr(bins), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+        S2 = None
+        if self.kind == "HI":
+            S2 = torch.empty_like(U)
+            L.call(f"ldl_hi_layer_fwd_{L.sfx(dt)}", *head, L.ptr(self.starmap), L.ptr(out), L.ptr(rho), L.ptr(U), L.ptr(S2),
+                   *tail)
+        else:
+            L.call(f"ldl_{self._k}_layer_fwd_{L.sfx(dt)}", *head, L.ptr(out), L.ptr(rho), L.ptr(U), *tail)
+        return out, rho, U, F, bins, S2
 
     def _paint_density(self, P, fact):
         dt, dev, nm = self.dtype, self.device, self.nm
@@ -152,20 +171,40 @@ class JERALDModel:
                self.paint_flags, L.ptr(rho), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
         return rho
 
-    def _loss_kernel(self, rho, prm3, need_grad):
+    def _depletion(self, q):
+        """HI depletion map: (sfield, dep) for q = [beta1, khi, kli, nui, beta2,
+        gamma3, xi, eta] (model.py:326-332)."""
         dt, dev, nm = self.dtype, self.device, self.nm
-        out5 = torch.zeros(5, dtype=torch.float64, device=dev)
+        sfield = torch.empty(nm, dtype=dt, device=dev)
+        dep = torch.empty(nm, dtype=dt, device=dev)
+        ws = self._ws()
+        L.call(f"ldl_hi_depletion_fwd_{L.sfx(dt)}", L.ptr(self.starmap), L.ptr(self.starmapk), L.i64x3(nm), L.dvec(self.box),
+               L.dvec(q), L.ptr(sfield), L.ptr(dep), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+        return sfield, dep
+
+    def _loss_kernel(self, rho, q, need_grad):
+        """Response + loss: out (loss at [0], response gradients, validation
+        loss at [4]), g_rho and the HI extras (sfield, g_dep)."""
+        dt, dev, nm = self.dtype, self.device, self.nm
+        out = torch.zeros(6, dtype=torch.float64, device=dev)
         g_rho = torch.empty(nm, dtype=dt, device=dev) if need_grad else None
         ws = self._ws()
-        common = (L.ptr(rho), L.ptr(self.target), L.ptr(self.masktrain), L.ptr(self.maskvalid), L.ptr(self.field2),
-                  L.i64x3(nm), L.dvec(self.box))
-        tail = (self.index, 1 if self.L1 else 2, self.smooth_kind, self.smooth_par, L.ptr(out5), L.ptr(g_rho), L.ptr(ws),
-                ws.numel(), L.stream_ptr(dev))
+        common = (L.ptr(rho), L.ptr(self.target), L.ptr(self.masktrain), L.ptr(self.maskvalid), L.ptr(self.field2))
+        geo = (L.i64x3(nm), L.dvec(self.box))
+        opts = (self.index, 1 if self.L1 else 2, self.smooth_kind, self.smooth_par, L.ptr(out), L.ptr(g_rho))
+        tail = (L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+        extra = {}
         if self.kind == "dm":
-            L.call(f"ldl_dm_loss_{L.sfx(dt)}", *common, *tail)
+            L.call(f"ldl_dm_loss_{L.sfx(dt)}", *common, *geo, *opts, *tail)
+        elif self.kind == "HI":
+            sfield, dep = self._depletion(q[3:11])
+            g_dep = torch.empty(nm, dtype=dt, device=dev) if need_grad else None
+            L.call(f"ldl_hi_loss_{L.sfx(dt)}", *common, L.ptr(dep), *geo, L.dvec([q[0], q[1], q[2], q[11]]), *opts,
+                   L.ptr(g_dep), *tail)
+            extra = dict(sfield=sfield, g_dep=g_dep)
         else:
-            L.call(f"ldl_sm_loss_{L.sfx(dt)}", *common, L.dvec(prm3), *tail)
-        return out5, g_rho
+            L.call(f"ldl_sm_loss_{L.sfx(dt)}", *common, *geo, L.dvec(q), *opts, *tail)
+        return out, g_rho, extra
 
     # ------------------------------------------------- model pieces
     @staticmethod
@@ -176,8 +215,12 @@ class JERALDModel:
     def _check(self, p, Nstep):
         # trailing extra entries are accepted and get a zero gradient, as in the
         # reference (runJERALD.py:150 passes 5 Nstep + 4 for 'Mstar')
-        if p.ndim != 1 or p.shape[0] < 5 * Nstep + self.nresp:
-            raise ValueError(f"expected {5 * Nstep + self.nresp} parameters, got {p.shape}")
+        n = self.npot * Nstep + self.nresp
+        if p.ndim != 1 or p.shape[0] < n:
+            raise ValueError(f"expected {n} parameters, got {p.shape}")
+
+    def _layer_prm(self, p, i):
+        return p[self.npot * i: self.npot * (i + 1)]
 
     def displace(self, params, X=None, Nstep=None):
         """Positions after Nstep layers (model.py:249-289), box units."""
@@ -186,22 +229,29 @@ class JERALDModel:
         P = self.X if X is None else self._encode(X)
         fact = float(np.prod(self.nm)) / P.shape[0]
         for i in range(Nstep):
-            P = self._layer_fwd(P, p[5 * i: 5 * i + 5], fact)[0]
+            P = self._layer_fwd(P, self._layer_prm(p, i), fact)[0]
         return self._decode(P)
 
     def evaluate(self, params, X=None, Nstep=None):
-        """Model map: sm ReLU(w (rho+1e-8)^mu + b), dm the density (model.py:291-347)."""
+        """Model map (model.py:291-347): sm ReLU(w (rho+1e-8)^mu + b); dm the
+        density; HI ReLU(w ReLU(rho + zs)^mu - dep + b)."""
         Nstep = self.Nstep if Nstep is None else int(Nstep)
         p = self._params(params)
         P = self.X if X is None else self._encode(X)
         fact = float(np.prod(self.nm)) / P.shape[0]
         for i in range(Nstep):
-            P = self._layer_fwd(P, p[5 * i: 5 * i + 5], fact)[0]
+            P = self._layer_fwd(P, self._layer_prm(p, i), fact)[0]
         rho = self._paint_density(P, fact)
         if self.kind == "dm":
             return rho
-        mu, w, b = (float(v) for v in p[5 * Nstep: 5 * Nstep + 3])
-        y = w * (rho + 1e-8) ** mu + b
+        q = p[self.npot * Nstep: self.npot * Nstep + self.nresp]
+        mu, w, b = (float(v) for v in q[:3])
+        if self.kind == "HI":
+            _, dep = self._depletion(q[3:11])
+            r = rho + float(q[11])
+            y = w * torch.where(r > 1e-12, r, torch.full_like(r, 1e-12)) ** mu - dep + b
+        else:
+            y = w * (rho + 1e-8) ** mu + b
         return torch.where(y > 1e-12, y, torch.full_like(y, 1e-12))
 
     def _forward(self, p, need_grad):
@@ -210,14 +260,15 @@ class JERALDModel:
         fact = float(np.prod(self.nm)) / P.shape[0]
         saved = []
         for i in range(self.Nstep):
-            out, rho, U, F, bins = self._layer_fwd(P, p[5 * i: 5 * i + 5], fact,
-                                                   keep_bins=need_grad and self.reuse_sort)
+            out, rho, U, F, bins, S2 = self._layer_fwd(P, self._layer_prm(p, i), fact,
+                                                       keep_bins=need_grad and self.reuse_sort)
             if need_grad:
-                saved.append((P, rho, U, F, bins) if self.save == "all" else (P,))
+                saved.append((P, rho, U, F, bins, S2) if self.save == "all" else (P,))
             P = out
         rho_f = self._paint_density(P, fact)
-        out5, g_rho = self._loss_kernel(rho_f, p[5 * self.Nstep: 5 * self.Nstep + 3], need_grad)
-        return out5, dict(saved=saved, P=P, g_rho=g_rho, fact=fact, out5=out5)
+        q = p[self.npot * self.Nstep: self.npot * self.Nstep + self.nresp]
+        out, g_rho, extra = self._loss_kernel(rho_f, q, need_grad)
+        return out, dict(saved=saved, P=P, g_rho=g_rho, fact=fact, out5=out, q=q, **extra)
 
     def _backward(self, p, st):
         dt, dev, nm = self.dtype, self.device, self.nm
@@ -226,23 +277,38 @@ class JERALDModel:
         g = torch.empty_like(P)
         L.call(f"ldl_paint_vjp_pos_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], L.ptr(st["g_rho"]), L.i64x3(nm),
                float(fact), L.ptr(g), L.stream_ptr(dev))
-        grads = torch.zeros(5 * self.Nstep, dtype=torch.float64, device=dev)
+        k = self.npot
+        grads = torch.zeros(k * self.Nstep, dtype=torch.float64, device=dev)
         ws = self._ws()
         for i in reversed(range(self.Nstep)):
-            prm = p[5 * i: 5 * i + 5]
+            prm = self._layer_prm(p, i)
             sv = st["saved"][i]
             if self.save == "all":
-                Pi, rho, U, F, bins = sv
+                Pi, rho, U, F, bins, S2 = sv
             else:
                 Pi = sv[0]
-                _, rho, U, F, bins = self._layer_fwd(Pi, prm, fact, keep_bins=self.reuse_sort)
-            gp = C_ptr_offset(grads, 5 * i)
-            L.call(f"ldl_{self.kind}_layer_bwd_{L.sfx(dt)}", L.ptr(Pi), self._pd(), Pi.shape[0], L.i64x3(nm), L.dvec(prm),
-                   float(fact), self.alpha_rule, self.paint_flags, L.ptr(rho), L.ptr(U), L.ptr(F), L.ptr(bins),
-                   L.ptr(g), L.ptr(g), gp, L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+                _, rho, U, F, bins, S2 = self._layer_fwd(Pi, prm, fact, keep_bins=self.reuse_sort)
+            gp = C_ptr_offset(grads, k * i)
+            head = (L.ptr(Pi), self._pd(), Pi.shape[0], L.i64x3(nm), L.dvec(prm), float(fact), self.alpha_rule,
+                    self.paint_flags)
+            tail = (L.ptr(F), L.ptr(bins), L.ptr(g), L.ptr(g), gp, L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+            if self.kind == "HI":
+                L.call(f"ldl_hi_layer_bwd_{L.sfx(dt)}", *head, L.ptr(self.starmap), L.ptr(rho), L.ptr(U), L.ptr(S2), *tail)
+            else:
+                L.call(f"ldl_{self._k}_layer_bwd_{L.sfx(dt)}", *head, L.ptr(rho), L.ptr(U), *tail)
             st["saved"][i] = None
-        extra = torch.zeros(len(p) - 5 * self.Nstep - self.nresp, dtype=torch.float64, device=dev)
-        return torch.cat([grads, st["out5"][1:1 + self.nresp], extra]), g
+        out = st["out5"]
+        if self.kind == "HI":
+            g8 = torch.empty(8, dtype=torch.float64, device=dev)
+            q = st["q"]
+            L.call(f"ldl_hi_depletion_bwd_{L.sfx(dt)}", L.ptr(self.starmap), L.ptr(self.starmapk), L.ptr(st["sfield"]),
+                   L.ptr(st["g_dep"]), L.i64x3(nm), L.dvec(self.box), L.dvec(q[3:11]), L.ptr(g8), L.ptr(ws), ws.numel(),
+                   L.stream_ptr(dev))
+            resp = torch.cat([out[1:4], g8, out[5:6]])
+        else:
+            resp = out[1:1 + self.nresp]
+        extra = torch.zeros(len(p) - k * self.Nstep - self.nresp, dtype=torch.float64, device=dev)
+        return torch.cat([grads, resp, extra]), g
 
     # ------------------------------------------------------- losses
     def loss(self, params):

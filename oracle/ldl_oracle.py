@@ -16,10 +16,11 @@ Pinning (what checks each function, beyond retyping its formula):
   readout_grad .......... central finite differences of readout inside a cell,
                           golden vectors (lreadout3D vjp mode)
   pfft / pifft .......... brute-force O(N^6) DFT at N<=8, Parseval, numpy.fft
-  sm and dm layer /
+  sm, dm and HI layer /
+  HI depletion response /
   loss / loss_and_grad .. central finite differences (FP64) for every parameter
                           and for the particle positions; an independent torch
-                          float64 autograd restatement in tests (both kinds)
+                          float64 autograd restatement in tests (all kinds)
   reference VJP rules ... parity unpinned against JAX itself (JAX is absent);
                           the emulation follows jaxpops_bind.py:699-701,
                           :758-759 and model.py:23-29 and is checked for the
@@ -45,6 +46,7 @@ import numpy as np
 EPS_RHO = 1e-8      # model.py:193, :344 ((rho + 1e-8) ** gamma)
 RELU_LOW = 1e-12    # model.py:55 (ReLU floor)
 ALPHA_SCALE = 1e-2  # model.py:198 (potk = 1e-2 * alpha * rhok * greenk)
+ALPHA_SCALE_HI = 1e-3  # model.py:231 (potk = 1e-3 * (alpha1 greenk1 rhok + alpha2 greenk2 starmapk))
 KNORM_ZERO = 1e-8   # model.py:102 (knorms == 0 replaced by 1e-8)
 
 
@@ -228,6 +230,22 @@ def green_dm(kn, kh, kl, nu):
     return -np.exp(-kn / kl - kh / kn) * kn ** nu
 
 
+def pow_log(x, p):
+    """x^p ln x with the value 0 where x == 0 -- the derivative of x^p in p as
+    JAX forms it (log of a zero base replaced by log 1), which the reference's
+    gradients of starmap^gamma2 and starmap^gamma3 inherit (model.py:228, :334)."""
+    x = np.asarray(x, dtype=np.float64)
+    safe = np.where(x == 0, 1.0, x)
+    return x ** p * np.log(safe)
+
+
+def hi_filter(kn, khi, kli, nui, xi):
+    """Depletion filter exp(-k/kli - khi/k) k^nui + xi (model.py:329), and
+    its derivatives in (khi, kli, nui, xi)."""
+    e = np.exp(-kn / kli - khi / kn) * kn ** nui
+    return e + xi, (-e / kn, e * kn / kli ** 2, e * np.log(kn), np.ones_like(kn))
+
+
 def relu(x):
     """where(x > 1e-12, x, 1e-12) (model.py:54-57)."""
     return np.where(x > RELU_LOW, x, RELU_LOW)
@@ -260,7 +278,16 @@ class SMModelOracle:
     """FP64 forward and exact hand-written backward of JERALD's 'sm' kind, or
     with ``kind="dm"`` of the 'dm' kind (model.py:136-183: kernel
     -exp(-k/kl - kh/k) k^nu; the map is the density itself, model.py:346-347;
-    5 parameters per layer and none for the response).
+    5 parameters per layer and none for the response), or with ``kind="HI"``
+    of the 'HI' kind (model.py:214-247, :313-336; needs ``starmap``):
+    10 parameters per layer [alpha1, gamma1, kh1, kl1, nu1, alpha2, gamma2,
+    kh2, kl2, nu2], potential 1e-3 (alpha1 G1 r2c((rho+1e-8)^gamma1) +
+    alpha2 G2 r2c(starmap^gamma2)) with G1 the sm kernel and G2 = -G1's form
+    at (kh2, kl2, nu2), and 12 response parameters [mu, w, b, beta1, khi, kli,
+    nui, beta2, gamma3, xi, eta, zs]:
+        dep = ReLU(beta1 c2r(r2c(starmap) W) + eta) + beta2 starmap^gamma3,
+        W   = exp(-k/kli - khi/k) k^nui + xi,
+        F   = ReLU(w ReLU(rho + zs)^mu - dep + b).
 
     loss options mirror JERALDModel: ``L1`` (the caller passes it; the
     reference defaults to True for sm and False for dm, model.py:115-122),
@@ -268,10 +295,16 @@ class SMModelOracle:
     ``smoothing`` = None | "reference" | ("gaussian", sigma)."""
 
     def __init__(self, nmesh, box, nstep, target, L1=True, n=0.0, smoothing=None,
-                 index=1.0, field2=None, mask=None, grad_rule="exact", backend="numpy", threads=0, kind="sm"):
-        if kind not in ("sm", "dm"):
+                 index=1.0, field2=None, mask=None, grad_rule="exact", backend="numpy", threads=0, kind="sm",
+                 starmap=None):
+        if kind not in ("sm", "dm", "HI"):
             raise ValueError(kind)
         self.kind = kind
+        self.npot = 10 if kind == "HI" else 5          # model.py:130-133
+        self.nresp = {"sm": 3, "dm": 0, "HI": 12}[kind]  # model.py:313-344
+        if kind == "HI" and starmap is None:
+            raise ValueError("Star map required for HI target")  # model.py:128
+        self.starmap = None if starmap is None else np.asarray(starmap, dtype=np.float64)
         self.nmesh = tuple(int(v) for v in np.broadcast_to(np.asarray(nmesh), (3,)))
         self.box = np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)).copy()
         self.nstep = int(nstep)
@@ -340,16 +373,24 @@ class SMModelOracle:
         return np.fft.irfftn(X, s=self.nmesh, axes=(0, 1, 2))
 
     def layer_forward(self, X, p, fact):
-        alpha, gamma, kh, kl, nu = p
+        alpha, gamma, kh, kl, nu = p[:5]
         x = self._x(X)
         rho = self._paint(x, 1.0) * fact
         u = (rho + EPS_RHO) ** gamma
         U = self._rfftn(u) / self.M
         G = (green_dm if self.kind == "dm" else green_sm)(self.kn, kh, kl, nu)
-        P = ALPHA_SCALE * alpha * U * G
+        cache = dict(x=x, rho=rho, u=u, U=U, G=G)
+        if self.kind == "HI":  # model.py:221-231
+            alpha2, gamma2, kh2, kl2, nu2 = p[5:10]
+            S2 = self._rfftn(self.starmap ** gamma2) / self.M
+            G2 = -green_sm(self.kn, kh2, kl2, nu2)
+            P = ALPHA_SCALE_HI * (alpha * G * U + alpha2 * G2 * S2)
+            cache.update(S2=S2, G2=G2)
+        else:
+            P = ALPHA_SCALE * alpha * U * G
         F = [self._irfftn(-1j * self.D[d] * P) * self.M for d in range(3)]
         s = np.stack([self._readout(x, F[d]) for d in range(3)], axis=-1)
-        cache = dict(x=x, rho=rho, u=u, U=U, G=G, F=F)
+        cache["F"] = F
         return X + s, cache
 
     def displace(self, params, X, keep=False):
@@ -357,8 +398,9 @@ class SMModelOracle:
         fact = self.M / X.shape[0]
         caches = []
         X = np.asarray(X, dtype=np.float64)
+        k = self.npot
         for i in range(self.nstep):
-            X, cache = self.layer_forward(X, params[5 * i: 5 * i + 5], fact)
+            X, cache = self.layer_forward(X, params[k * i: k * i + k], fact)
             caches.append(cache)
         return (X, caches) if keep else X
 
@@ -372,6 +414,10 @@ class SMModelOracle:
             if keep:
                 return rho, dict(caches=caches, xf=xf, rho=rho, F=rho, fact=fact)
             return rho
+        if self.kind == "HI":
+            r = self.hi_response(params[10 * self.nstep: 10 * self.nstep + 12], rho)
+            r.update(caches=caches, xf=xf, rho=rho, fact=fact)
+            return (r["F"], r) if keep else r["F"]
         mu, w, b = params[5 * self.nstep: 5 * self.nstep + 3]
         base = (rho + EPS_RHO) ** mu
         y = w * base + b
@@ -379,6 +425,53 @@ class SMModelOracle:
         if keep:
             return F, dict(caches=caches, xf=xf, rho=rho, base=base, y=y, F=F, fact=fact)
         return F
+
+    def hi_depletion(self, q):
+        """dep = ReLU(beta1 S + eta) + beta2 starmap^gamma3 with S = c2r(r2c(
+        starmap) W) (model.py:328-332); q = [beta1, khi, kli, nui, beta2,
+        gamma3, xi, eta]."""
+        beta1, khi, kli, nui, beta2, gamma3, xi, eta = q
+        Sk = self._rfftn(self.starmap) / self.M
+        W, dW = hi_filter(self.kn, khi, kli, nui, xi)
+        S = self._irfftn(Sk * W) * self.M
+        A = beta1 * S + eta
+        s3 = self.starmap ** gamma3
+        return dict(Sk=Sk, dW=dW, S=S, A=A, s3=s3, dep=relu(A) + beta2 * s3)
+
+    def hi_response(self, q, rho):
+        """F = ReLU(w ReLU(rho + zs)^mu - dep + b) (model.py:332-334);
+        q = [mu, w, b, beta1, khi, kli, nui, beta2, gamma3, xi, eta, zs]."""
+        mu, w, b = q[:3]
+        zs = q[11]
+        r = self.hi_depletion(q[3:11])
+        R = relu(rho + zs)
+        base = R ** mu
+        y = w * base - r["dep"] + b
+        r.update(R=R, base=base, y=y, F=relu(y))
+        return r
+
+    def hi_response_backward(self, q, st, g):
+        """Gradient of the HI response: returns (g_rho, grad of q)."""
+        mu, w, b, beta1, khi, kli, nui, beta2, gamma3, xi, eta, zs = q
+        gq = np.zeros(12)
+        g_y = g * (st["y"] > RELU_LOW)
+        R = st["R"]
+        gq[0] = np.sum(g_y * w * st["base"] * np.log(R))
+        gq[1] = np.sum(g_y * st["base"])
+        gq[2] = np.sum(g_y)
+        g_rho = g_y * w * mu * R ** (mu - 1.0) * (st["rho"] + zs > RELU_LOW)
+        gq[11] = np.sum(g_rho)
+        g_dep = -g_y
+        gq[7] = np.sum(g_dep * st["s3"])
+        gq[8] = np.sum(g_dep * beta2 * pow_log(self.starmap, gamma3))
+        g_A = g_dep * (st["A"] > RELU_LOW)
+        gq[10] = np.sum(g_A)
+        gq[3] = np.sum(g_A * st["S"])
+        # S = M irfftn(Sk W): dL/dW = c Re(conj(rfftn(g_S)) Sk) on the half spectrum
+        gW = self.c * np.real(np.conj(self._rfftn(beta1 * g_A)) * st["Sk"])
+        for j, dW in zip((4, 5, 6, 9), st["dW"]):
+            gq[j] = np.sum(gW * dW)
+        return g_rho, gq
 
     def _residual(self, F):
         Fi = F ** self.index
@@ -420,6 +513,9 @@ class SMModelOracle:
             g = g * self.index * F ** (self.index - 1.0)
         if self.kind == "dm":
             g_rho = g
+        elif self.kind == "HI":
+            g_rho, gq = self.hi_response_backward(params[10 * self.nstep: 10 * self.nstep + 12], st, g)
+            grad[10 * self.nstep: 10 * self.nstep + 12] = gq
         else:
             g_y = g * (st["y"] > RELU_LOW)
             mu, w, b = params[5 * self.nstep: 5 * self.nstep + 3]
@@ -430,12 +526,13 @@ class SMModelOracle:
             g_rho = g_y * w * mu * rhoe ** (mu - 1.0)
         fact = st["fact"]
         gX = fact * self._readout_grad(st["xf"], g_rho)
+        k = self.npot
         for i in reversed(range(self.nstep)):
-            gX = self._layer_backward(params[5 * i: 5 * i + 5], st["caches"][i], gX, fact, grad, 5 * i)
+            gX = self._layer_backward(params[k * i: k * i + k], st["caches"][i], gX, fact, grad, k * i)
         return (loss, grad, gX) if want_pos_grad else (loss, grad)
 
     def _layer_backward(self, p, cache, v, fact, grad, off):
-        alpha, gamma, kh, kl, nu = p
+        alpha, gamma, kh, kl, nu = p[:5]
         x = cache["x"]
         gX = v.copy()
         gF = [self._paint(x, np.ascontiguousarray(v[:, d])) for d in range(3)]
@@ -446,8 +543,9 @@ class SMModelOracle:
         cw = self.c if self.grad_rule == "exact" else 1.0
         kn = self.kn
         re = np.real(np.conj(Q) * U)
-        grad[off + 0] = ALPHA_SCALE * np.sum(cw * G * re)
-        gG = ALPHA_SCALE * alpha * self.c * re
+        asc = ALPHA_SCALE_HI if self.kind == "HI" else ALPHA_SCALE
+        grad[off + 0] = asc * np.sum(cw * G * re)
+        gG = asc * alpha * self.c * re
         if self.kind == "dm":  # d/dkh, d/dkl of -kn/kl - kh/kn
             grad[off + 2] = np.sum(gG * G * (-1.0 / kn))
             grad[off + 3] = np.sum(gG * G * (kn / kl ** 2))
@@ -455,7 +553,18 @@ class SMModelOracle:
             grad[off + 2] = np.sum(gG * G * (-2.0 * kh / kn ** 2))
             grad[off + 3] = np.sum(gG * G * (2.0 * kn ** 2 / kl ** 3))
         grad[off + 4] = np.sum(gG * G * np.log(kn))
-        g_u = self._irfftn(ALPHA_SCALE * alpha * G * Q)
+        if self.kind == "HI":  # second source alpha2 G2 r2c(starmap^gamma2)
+            alpha2, gamma2, kh2, kl2, nu2 = p[5:10]
+            G2, S2 = cache["G2"], cache["S2"]
+            re2 = np.real(np.conj(Q) * S2)
+            grad[off + 5] = asc * np.sum(cw * G2 * re2)
+            gG2 = asc * alpha2 * self.c * re2
+            grad[off + 7] = np.sum(gG2 * G2 * (-2.0 * kh2 / kn ** 2))
+            grad[off + 8] = np.sum(gG2 * G2 * (2.0 * kn ** 2 / kl2 ** 3))
+            grad[off + 9] = np.sum(gG2 * G2 * np.log(kn))
+            h = self._irfftn(asc * alpha2 * G2 * Q)
+            grad[off + 6] = np.sum(h * pow_log(self.starmap, gamma2))
+        g_u = self._irfftn(asc * alpha * G * Q)
         rhoe = cache["rho"] + EPS_RHO
         grad[off + 1] = np.sum(g_u * cache["u"] * np.log(rhoe))
         g_rho = g_u * gamma * rhoe ** (gamma - 1.0)

@@ -163,6 +163,65 @@ def test_dm_kind_matches_oracle(N, lattice, smoothing):
     np.testing.assert_allclose(rho.cpu().numpy(), orc.evaluate(p, X), rtol=1e-10, atol=1e-10)
 
 
+def _hi_problem(N, nstep, L, seed=83, zero_frac=0.1):
+    X, tgt, mk, _, _ = _problem(N, nstep, L, seed=seed, mask=True)
+    rng = np.random.default_rng(seed + 2)
+    star = rng.random((N, N, N)) * 1.5
+    star[rng.random((N, N, N)) < zero_frac] = 0.0  # empty cells: zero base of starmap^gamma
+    lay = [[8.0, 1.2, 0.3, 1.5, 0.2, 5.0, 0.8, 0.2, 1.1, 0.3], [5.0, 0.9, 0.2, 1.6, -0.1, 3.0, 1.1, 0.25, 1.3, 0.1]]
+    p = np.array(sum(lay[:nstep], []) + [1.1, 0.9, 0.3, 0.4, 0.05, 2.0, 0.3, 0.2, 1.3, 0.1, 0.02, 0.05])
+    return X, tgt, mk, p, star
+
+
+@pytest.mark.parametrize("N,lattice,smoothing,L1", [(16, None, None, True), (16, (16, 16, 16), ("gaussian", 4.0), False),
+                                                    (12, None, ("gaussian", 4.0), True), (20, (20, 20, 20), None, False)])
+def test_hi_kind_matches_oracle(N, lattice, smoothing, L1):
+    """HI kind (model.py:214-247, :313-336): second source starmap^gamma2 per
+    layer (10 parameters), depletion response (12), FP64 against the oracle."""
+    L, nstep = 4.0 * N, 2
+    X, tgt, mk, p, star = _hi_problem(N, nstep, L)
+    orc = O.SMModelOracle(N, L, nstep, tgt, L1=L1, smoothing=smoothing, mask=mk, kind="HI", starmap=star)
+    loss0, g0 = orc.loss_and_grad(p, X)
+    pm = PMesh(N, L, device=DEV, dtype=torch.float64)
+    m = JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, kind="HI", L1=L1, masktrain=mk, smoothing=smoothing,
+                    lattice=lattice, starmap=star)
+    loss, grad = m.loss_and_grad(torch.tensor(p))
+    assert grad.shape == (32,)
+    assert abs(loss.item() - loss0) <= 1e-10 * abs(loss0)
+    np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
+    F = m.evaluate(torch.tensor(p))
+    np.testing.assert_allclose(F.cpu().numpy(), orc.evaluate(p, X), rtol=1e-10, atol=1e-12)
+
+
+def test_hi_reference_rule_and_fp32():
+    """alpha_rule reweights only alpha1 and alpha2; FP32 loss within 1e-5 of FP64."""
+    N, L, nstep = 32, 128.0, 2
+    X, tgt, mk, p, star = _hi_problem(N, nstep, L)
+    pm = PMesh(N, L, device=DEV, dtype=torch.float64)
+    out = {}
+    for dt, rule in ((torch.float64, "exact"), (torch.float64, "reference"), (torch.float32, "exact")):
+        m = JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, kind="HI", L1=False, masktrain=mk, lattice=(N, N, N),
+                        smoothing=("gaussian", 4.0), starmap=star, grad_rule=rule, dtype=dt)
+        loss, grad = m.loss_and_grad(torch.tensor(p))
+        out[(dt, rule)] = (loss.item(), grad.cpu().numpy())
+    l64, g64 = out[(torch.float64, "exact")]
+    _, gref = out[(torch.float64, "reference")]
+    alpha = [0, 5, 10, 15]
+    other = [i for i in range(32) if i not in alpha]
+    np.testing.assert_allclose(gref[other], g64[other], rtol=1e-12)
+    assert np.all(np.abs(gref[alpha]) < np.abs(g64[alpha]))
+    orc = O.SMModelOracle(N, L, nstep, tgt, L1=False, smoothing=("gaussian", 4.0), mask=mk, kind="HI", starmap=star,
+                          grad_rule="reference")
+    _, g0r = orc.loss_and_grad(p, X)
+    np.testing.assert_allclose(gref, g0r, rtol=1e-8, atol=1e-12 * np.abs(g0r).max())
+    l32, g32 = out[(torch.float32, "exact")]
+    assert abs(l32 - l64) <= 1e-5 * abs(l64)
+    # per-layer and response gradients against the largest of their group
+    for grp in (slice(0, 20), slice(20, 32)):
+        err = np.abs(g32[grp] - g64[grp]).max() / np.abs(g64[grp]).max()
+        assert err < 1e-4, (grp, err)
+
+
 def test_ldl_fp32_within_1e5_of_fp64():
     N, L, nstep = 32, 128.0, 2
     X, tgt, mk, p, orc = _problem(N, nstep, L, smoothing=("gaussian", 4.0), L1=False, rms=2.0)

@@ -120,6 +120,46 @@ def test_dm_param_and_position_grad_finite_difference(smoothing, L1):
         assert abs(gX[pi, d] - fd) <= 1e-5 * abs(fd) + 1e-10
 
 
+def _hi_setup(N=8, nstep=2, seed=83, L=40.0):
+    X, tgt, _, L = _setup(N, nstep, seed, L)
+    rng = np.random.default_rng(seed + 2)
+    star = rng.random((N, N, N)) * 1.5
+    star[rng.random((N, N, N)) < 0.1] = 0.0  # empty cells: starmap^gamma has a zero base
+    lay = [[7.0, 1.2, 0.3, 1.4, 0.2, 5.0, 0.8, 0.2, 1.1, 0.3], [5.0, 0.9, 0.2, 1.6, -0.1, 3.0, 1.1, 0.25, 1.3, 0.1]]
+    p = np.array(sum(lay[:nstep], []) + [1.1, 0.9, 0.3, 0.4, 0.05, 2.0, 0.3, 0.2, 1.3, 0.1, 0.02, 0.05])
+    return X, tgt, p, L, star
+
+
+@pytest.mark.parametrize("smoothing,L1", [(None, True), ("gauss", False)])
+def test_hi_param_and_position_grad_finite_difference(smoothing, L1):
+    """HI kind: two sources per layer (10 parameters), depletion response (12)."""
+    X, tgt, p, L, star = _hi_setup()
+    sm = ("gaussian", L / 8) if smoothing == "gauss" else None
+    m = O.SMModelOracle(8, L, 2, tgt, L1=L1, smoothing=sm, kind="HI", starmap=star)
+    loss, g, gX = m.loss_and_grad(p, X, want_pos_grad=True)
+    assert g.shape == (32,)
+    for i in range(len(p)):
+        h = 1e-6 * max(abs(p[i]), 1.0)
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (m.loss(pp, X) - m.loss(pm, X)) / (2 * h)
+        assert abs(g[i] - fd) <= 1e-5 * abs(fd) + 1e-9, (i, g[i], fd)
+    for pi, d in [(5, 0), (123, 2), (300, 1)]:
+        Xp, Xm = X.copy(), X.copy()
+        Xp[pi, d] += 1e-6
+        Xm[pi, d] -= 1e-6
+        fd = (m.loss(p, Xp) - m.loss(p, Xm)) / 2e-6
+        assert abs(gX[pi, d] - fd) <= 1e-5 * abs(fd) + 1e-10
+
+
+def test_hi_requires_starmap_and_zero_base_convention():
+    with pytest.raises(ValueError):
+        O.SMModelOracle(8, 40.0, 1, np.zeros((8, 8, 8)), kind="HI")
+    np.testing.assert_array_equal(O.pow_log(np.array([0.0, 1.0]), 1.3), [0.0, 0.0])
+    assert O.pow_log(np.array([2.0]), 0.5)[0] == pytest.approx(np.sqrt(2) * np.log(2))
+
+
 def test_ldl_position_grad_finite_difference():
     X, tgt, p, L = _setup()
     m = O.SMModelOracle(8, L, 2, tgt, L1=False)
@@ -133,13 +173,19 @@ def test_ldl_position_grad_finite_difference():
         assert abs(gX[pi, d] - fd) <= 1e-5 * abs(fd) + 1e-10
 
 
-def test_reference_rule_changes_only_alpha():
-    X, tgt, p, L = _setup()
-    a = O.SMModelOracle(8, L, 2, tgt, L1=False)
-    b = O.SMModelOracle(8, L, 2, tgt, L1=False, grad_rule="reference")
+@pytest.mark.parametrize("kind", ["sm", "HI"])
+def test_reference_rule_changes_only_alpha(kind):
+    if kind == "HI":
+        X, tgt, p, L, star = _hi_setup()
+        alpha = [0, 5, 10, 15]  # alpha1 and alpha2 multiply outside c2f (model.py:231)
+    else:
+        X, tgt, p, L = _setup()
+        star = None
+        alpha = [0, 5]
+    a = O.SMModelOracle(8, L, 2, tgt, L1=False, kind=kind, starmap=star)
+    b = O.SMModelOracle(8, L, 2, tgt, L1=False, grad_rule="reference", kind=kind, starmap=star)
     _, ga = a.loss_and_grad(p, X)
     _, gb = b.loss_and_grad(p, X)
-    alpha = [0, 5]
     other = [i for i in range(len(p)) if i not in alpha]
     np.testing.assert_allclose(ga[other], gb[other], rtol=1e-12)
     assert np.all(np.abs(gb[alpha]) < np.abs(ga[alpha]))
@@ -177,13 +223,17 @@ def _t_readout(x, F, nm):
     return acc
 
 
-@pytest.mark.parametrize("kind", ["sm", "dm"])
+@pytest.mark.parametrize("kind", ["sm", "dm", "HI"])
 def test_ldl_against_torch_autograd(kind):
     N, L = 8, 40.0
     X, tgt, p, _ = _setup(N=N, L=L)
+    star = None
     if kind == "dm":
         p = p[:10]
-    m = O.SMModelOracle(N, L, 2, tgt, L1=False, kind=kind)
+    if kind == "HI":
+        X, tgt, p, _, star = _hi_setup(N=N, L=L)
+        st = torch.tensor(star)
+    m = O.SMModelOracle(N, L, 2, tgt, L1=False, kind=kind, starmap=star)
     loss, g = m.loss_and_grad(p, X)
     nm = (N, N, N)
     pt = torch.tensor(p, requires_grad=True)
@@ -192,8 +242,10 @@ def test_ldl_against_torch_autograd(kind):
     D = [torch.tensor(np.broadcast_to(d, (N, N, N // 2 + 1)).copy()) for d in O.fd_derivative_coeff(nm, L)]
     cell = L / N
     fact = N ** 3 / X.shape[0]
+    k = 10 if kind == "HI" else 5
+    lo = torch.tensor(1e-12, dtype=torch.float64)
     for i in range(2):
-        a, gam, kh, kl, nu = pt[5 * i: 5 * i + 5]
+        a, gam, kh, kl, nu = pt[k * i: k * i + 5]
         x = torch.remainder(Xt, L) / cell
         rho = _t_paint(x, 1.0, nm) * fact
         U = torch.fft.rfftn((rho + 1e-8) ** gam) / N ** 3
@@ -201,13 +253,29 @@ def test_ldl_against_torch_autograd(kind):
             G = -torch.exp(-kn / kl - kh / kn) * kn ** nu
         else:
             G = -torch.exp(-kn ** 2 / kl ** 2 - kh ** 2 / kn ** 2) * kn ** nu
-        P = 1e-2 * a * U * G
+        if kind == "HI":
+            a2, gam2, kh2, kl2, nu2 = pt[k * i + 5: k * i + 10]
+            G2 = torch.exp(-kn ** 2 / kl2 ** 2 - kh2 ** 2 / kn ** 2) * kn ** nu2
+            # torch's pow gradient in the exponent is 0 at a zero base, as JAX's
+            S2 = torch.fft.rfftn(st ** gam2) / N ** 3
+            P = 1e-3 * (a * G * U + a2 * G2 * S2)
+        else:
+            P = 1e-2 * a * U * G
         s = torch.stack([_t_readout(x, torch.fft.irfftn(-1j * D[d] * P, s=nm) * N ** 3, nm) for d in range(3)], -1)
         Xt = Xt + s
     x = torch.remainder(Xt, L) / cell
     rho = _t_paint(x, 1.0, nm) * fact
     if kind == "dm":
         F = rho
+    elif kind == "HI":
+        mu, w, b, beta1, khi, kli, nui, beta2, gam3, xi, eta, zs = pt[20:32]
+        W = torch.exp(-kn / kli - khi / kn) * kn ** nui + xi
+        S = torch.fft.irfftn(torch.fft.rfftn(st) * W, s=nm)
+        A = beta1 * S + eta
+        dep = torch.where(A > 1e-12, A, lo) + beta2 * st ** gam3
+        R = torch.where(rho + zs > 1e-12, rho + zs, lo)
+        y = w * R ** mu - dep + b
+        F = torch.where(y > 1e-12, y, lo)
     else:
         mu, w, b = pt[10:13]
         F = torch.where(w * (rho + 1e-8) ** mu + b > 1e-12, w * (rho + 1e-8) ** mu + b,
