@@ -8,6 +8,8 @@ learning-rate schedule and training loop.
 * ``initial_params`` and ``bounds`` (runJERALD.py:145-166) and ``clip``, the
   per-step clipping of runJERALD.py:190-196;
 * ``lr_schedule`` (runJERALD.py:177-180);
+* ``prepare_starmap`` and ``hi_target_scale`` (runJERALD.py:82-109): the HI
+  inputs' floor and normalisation;
 * ``train`` (runJERALD.py:186-221): clip, ``lossv_and_grad``, keep the best
   validation parameters, Adam step.
 
@@ -165,6 +167,22 @@ def initial_params(target, Nstep, avgfact=1.0, zshift=0.0):
     return np.array(p, dtype=np.float64)
 
 
+def prepare_starmap(starmap):
+    """Stellar-mass map for the HI kind as the driver prepares it
+    (runJERALD.py:84-87): floored at 1e-12, then divided by its mean."""
+    s = np.asarray(starmap, dtype=np.float64)
+    s = np.where(s < 1e-12, 1e-12, s)
+    return s / s.mean()
+
+
+def hi_target_scale(redshift):
+    """(divisor of the HI target map, zshift) by redshift (runJERALD.py:98-109)."""
+    table = {0: (4e-13, 0.0), 2: (4e-11, -1e-3), 5: (1e-8, -0.04)}
+    if redshift not in table:
+        raise ValueError("redshift must be 0, 2 or 5")
+    return table[redshift]
+
+
 def bounds(target, Nstep):
     """Clipping bounds of runJERALD.py:156-166 (None = unbounded)."""
     layer = [(None, None), (None, None), (0., None), (0.1, None), (None, None)]
@@ -227,7 +245,7 @@ def main():
     from .model import JERALDModel
     from .pmesh import PMesh
     ap = argparse.ArgumentParser()
-    ap.add_argument("--target", default="Mstar", choices=["dm", "Mstar"])
+    ap.add_argument("--target", default="Mstar", choices=["dm", "Mstar", "HI"])
     ap.add_argument("--Nmesh", type=int, default=64)
     ap.add_argument("--BoxSize", type=float, default=None)
     ap.add_argument("--Nstep", type=int, default=2)
@@ -251,15 +269,21 @@ def main():
     psi *= 2.0 * box / N / np.sqrt(np.mean(psi ** 2))
     q = np.stack(np.meshgrid(*[np.arange(N) * box / N] * 3, indexing="ij"), -1).reshape(-1, 3)
     X = q + psi
-    kind = "dm" if a.target == "dm" else "sm"
+    kind = {"dm": "dm", "Mstar": "sm", "HI": "HI"}[a.target]
     pm = PMesh(N, box, device="cuda", dtype=dt)
-    p_true = initial_params(a.target, a.Nstep)[: 5 * a.Nstep + (0 if kind == "dm" else 3)]
-    p_true[:5 * a.Nstep] *= 1.0 + 0.2 * np.cos(np.arange(5 * a.Nstep))
-    gen = JERALDModel(torch.tensor(X), None, pm, Nstep=a.Nstep, kind=kind, lattice=(N, N, N))
+    starmap = None
+    if kind == "HI":  # synthetic stellar-mass map: the sm model at its initial parameters
+        sm = JERALDModel(torch.tensor(X), None, pm, Nstep=a.Nstep, kind="sm", lattice=(N, N, N))
+        starmap = prepare_starmap(sm.evaluate(torch.tensor(initial_params("Mstar", a.Nstep))).cpu().numpy())
+    npar = {"dm": 5, "sm": 5, "HI": 10}[kind] * a.Nstep + {"dm": 0, "sm": 3, "HI": 12}[kind]
+    nlay = npar - {"dm": 0, "sm": 3, "HI": 12}[kind]
+    p_true = initial_params(a.target, a.Nstep)[:npar]
+    p_true[:nlay] *= 1.0 + 0.2 * np.cos(np.arange(nlay))
+    gen = JERALDModel(torch.tensor(X), None, pm, Nstep=a.Nstep, kind=kind, lattice=(N, N, N), starmap=starmap)
     target = gen.evaluate(torch.tensor(p_true))
     mtr, mva, _ = split_masks(N)
-    model = JERALDModel(torch.tensor(X), target, pm, Nstep=a.Nstep, kind=a.target, n=a.n, L1=(kind != "dm"),
-                        masktrain=mtr, maskvalid=mva, lattice=(N, N, N))
+    model = JERALDModel(torch.tensor(X), target, pm, Nstep=a.Nstep, kind=kind, n=a.n, L1=(kind != "dm"),
+                        masktrain=mtr, maskvalid=mva, lattice=(N, N, N), starmap=starmap)
     p0 = initial_params(a.target, a.Nstep, avgfact=float(target.double().mean()))
     out = a.outpath
     train(model, p0, bounds(a.target, a.Nstep), a.Nopt,

@@ -93,6 +93,16 @@ def test_initial_params_bounds_and_clip():
     np.testing.assert_array_equal(p, [1, 1, 0.0, 0.1, 1, 0.1, 1e-5, 5, 9])
 
 
+def test_prepare_starmap_and_hi_target_scale():
+    s = T.prepare_starmap(np.array([[0.0, -1.0], [2.0, 6.0]]))
+    floor = 1e-12 / ((2e-12 + 8.0) / 4)
+    np.testing.assert_allclose(s, [[floor, floor], [2.0 / 2.0000000000005, 6.0 / 2.0000000000005]], rtol=1e-12)
+    assert s.mean() == pytest.approx(1.0)
+    assert T.hi_target_scale(2) == (4e-11, -1e-3)
+    with pytest.raises(ValueError):
+        T.hi_target_scale(1)
+
+
 class _Quadratic:
     """Stand-in model: train loss |p - c|^2, validation loss |p - c'|^2."""
 
