@@ -239,25 +239,8 @@ def train(model, params0, bnds, Nopt=10, optimizer=None, paramsave=None, optf=No
     return bestparams, best, beststep, hist
 
 
-def main():
-    import torch
-
-    from .model import JERALDModel
-    from .pmesh import PMesh
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--target", default="Mstar", choices=["dm", "Mstar", "HI"])
-    ap.add_argument("--Nmesh", type=int, default=64)
-    ap.add_argument("--BoxSize", type=float, default=None)
-    ap.add_argument("--Nstep", type=int, default=2)
-    ap.add_argument("--n", type=float, default=0.0)
-    ap.add_argument("--Nopt", type=int, default=10)
-    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--outpath", default=None)
-    a = ap.parse_args()
-    N = a.Nmesh
-    box = a.BoxSize or 0.8 * N
-    dt = torch.float32 if a.dtype == "f32" else torch.float64
-    # synthetic input: seeded lattice + Zel'dovich displacement (2 cells rms)
+def _synthetic_positions(N, box):
+    """Seeded lattice + Zel'dovich displacement (2 cells rms), lattice order."""
     rng = np.random.default_rng(83)
     wk = np.fft.rfftn(rng.standard_normal((N, N, N)))
     k1 = 2 * np.pi * np.fft.fftfreq(N, box / N)
@@ -268,27 +251,73 @@ def main():
                     for kk in (k1[:, None, None], k1[None, :, None], kz[None, None, :])], -1)
     psi *= 2.0 * box / N / np.sqrt(np.mean(psi ** 2))
     q = np.stack(np.meshgrid(*[np.arange(N) * box / N] * 3, indexing="ij"), -1).reshape(-1, 3)
-    X = q + psi
+    return q + psi
+
+
+def main():
+    """runJERALD.py's train-then-evaluate flow.  With ``--Nbodypath`` (a
+    FastPM particle file, block ``Position``), ``--refpath`` (target map,
+    block ``Field``) and for HI ``--Mstarpath`` it reads the reference's
+    bigfile inputs; otherwise it builds a synthetic problem."""
+    import torch
+
+    from . import bigfile_io as bio
+    from .model import JERALDModel
+    from .pmesh import PMesh
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--target", default="Mstar", choices=["dm", "Mstar", "HI"])
+    ap.add_argument("--Nmesh", type=int, default=64)
+    ap.add_argument("--BoxSize", type=float, default=None)
+    ap.add_argument("--Nstep", type=int, default=2)
+    ap.add_argument("--n", type=float, default=0.0)
+    ap.add_argument("--Nopt", type=int, default=10)
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--Nbodypath", default=None)
+    ap.add_argument("--refpath", default=None)
+    ap.add_argument("--Mstarpath", default=None)
+    ap.add_argument("--redshift", type=int, default=0)
+    ap.add_argument("--outpath", default=None)
+    a = ap.parse_args()
+    N = a.Nmesh
+    box = a.BoxSize or 0.8 * N
+    dt = torch.float32 if a.dtype == "f32" else torch.float64
     kind = {"dm": "dm", "Mstar": "sm", "HI": "HI"}[a.target]
     pm = PMesh(N, box, device="cuda", dtype=dt)
+    if a.Nbodypath:
+        X, lattice = bio.load_positions(a.Nbodypath, device="cuda", dtype=dt), None
+    else:
+        X, lattice = torch.tensor(_synthetic_positions(N, box)), (N, N, N)
     starmap = None
-    if kind == "HI":  # synthetic stellar-mass map: the sm model at its initial parameters
-        sm = JERALDModel(torch.tensor(X), None, pm, Nstep=a.Nstep, kind="sm", lattice=(N, N, N))
-        starmap = prepare_starmap(sm.evaluate(torch.tensor(initial_params("Mstar", a.Nstep))).cpu().numpy())
-    npar = {"dm": 5, "sm": 5, "HI": 10}[kind] * a.Nstep + {"dm": 0, "sm": 3, "HI": 12}[kind]
-    nlay = npar - {"dm": 0, "sm": 3, "HI": 12}[kind]
-    p_true = initial_params(a.target, a.Nstep)[:npar]
-    p_true[:nlay] *= 1.0 + 0.2 * np.cos(np.arange(nlay))
-    gen = JERALDModel(torch.tensor(X), None, pm, Nstep=a.Nstep, kind=kind, lattice=(N, N, N), starmap=starmap)
-    target = gen.evaluate(torch.tensor(p_true))
+    if kind == "HI":
+        if a.Mstarpath:
+            starmap = prepare_starmap(bio.loadfield(a.Mstarpath)[0])
+        else:  # synthetic stellar-mass map: the sm model at its initial parameters
+            sm = JERALDModel(X, None, pm, Nstep=a.Nstep, kind="sm", lattice=lattice)
+            starmap = prepare_starmap(sm.evaluate(torch.tensor(initial_params("Mstar", a.Nstep))).cpu().numpy())
+    zshift = 0.0
+    if a.refpath:
+        target = bio.loadfield(a.refpath)[0]
+        if kind == "HI":
+            div, zshift = hi_target_scale(a.redshift)
+            target = target / div
+        target = torch.tensor(target)
+    else:  # the model itself at perturbed parameters
+        npar = {"dm": 5, "sm": 5, "HI": 10}[kind] * a.Nstep + {"dm": 0, "sm": 3, "HI": 12}[kind]
+        nlay = npar - {"dm": 0, "sm": 3, "HI": 12}[kind]
+        p_true = initial_params(a.target, a.Nstep)[:npar]
+        p_true[:nlay] *= 1.0 + 0.2 * np.cos(np.arange(nlay))
+        gen = JERALDModel(X, None, pm, Nstep=a.Nstep, kind=kind, lattice=lattice, starmap=starmap)
+        target = gen.evaluate(torch.tensor(p_true))
     mtr, mva, _ = split_masks(N)
-    model = JERALDModel(torch.tensor(X), target, pm, Nstep=a.Nstep, kind=kind, n=a.n, L1=(kind != "dm"),
-                        masktrain=mtr, maskvalid=mva, lattice=(N, N, N), starmap=starmap)
-    p0 = initial_params(a.target, a.Nstep, avgfact=float(target.double().mean()))
+    model = JERALDModel(X, target, pm, Nstep=a.Nstep, kind=kind, n=a.n, L1=(kind != "dm"),
+                        masktrain=mtr, maskvalid=mva, lattice=lattice, starmap=starmap)
+    p0 = initial_params(a.target, a.Nstep, avgfact=float(torch.as_tensor(target).double().mean()), zshift=zshift)
     out = a.outpath
-    train(model, p0, bounds(a.target, a.Nstep), a.Nopt,
-          paramsave=f"{out}/{a.target}_Nmesh{N}_Nstep{a.Nstep}_n{a.n:.2f}_bestparam.txt" if out else None,
-          optf=f"{out}/{a.target}_Nmesh{N}_Nstep{a.Nstep}_n{a.n:.2f}_optstate.txt" if out else None)
+    stem = f"{out}/{a.target}_Nmesh{N}_Nstep{a.Nstep}_n{a.n:.2f}" if out else None
+    best = train(model, p0, bounds(a.target, a.Nstep), a.Nopt, paramsave=f"{stem}_bestparam.txt" if out else None,
+                 optf=f"{stem}_optstate.txt" if out else None)[0]
+    if out:  # the evaluated map at the best parameters (runJERALD.py:251-258)
+        bio.savefield(model.evaluate(torch.tensor(best)), f"{stem}_map", pm)
 
 
 if __name__ == "__main__":
