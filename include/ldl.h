@@ -1,6 +1,7 @@
 /* libldl: MI355X (gfx950) kernels for JERALD's Lagrangian Deep Learning hot
- * path -- CIC paint/readout, 3D r2c/c2r FFTs, the sm-kind LDL displacement
- * operator, the ReLU response, the loss, and their hand-written adjoints.
+ * path -- CIC paint/readout, 3D r2c/c2r FFTs, the LDL displacement operator
+ * (sm, dm and HI kinds), the responses, the loss, and their hand-written
+ * adjoints.
  *
  * C ABI: plain pointers and sizes, no torch or HIP types.  Every entry point
  *   * takes DEVICE pointers (caller-owned, e.g. torch-allocated) for arrays,
@@ -172,7 +173,8 @@ size_t ldl_sm_bins_bytes(const int64_t nmesh[3], int64_t np);
 /* Forward of one layer.  Inputs pos (Np,3).  Writes pos_out (Np,3) = pos +
  * s / cell (same kind as pos; may alias pos), and the saved tensors the
  * backward needs: rho (N0,N1,N2), U (N0,N1,N2/2+1 complex, = r2c of
- * (rho+1e-8)^gamma), forces F (3,N0,N1,N2), and -- if ``bins`` is not NULL
+ * (rho+1e-8)^gamma), forces F (N0,N1,N2,4) interleaved per cell as
+ * (F_x, F_y, F_z, unused), and -- if ``bins`` is not NULL
  * (ldl_sm_bins_bytes) -- the particle sort of the fixed-point paint. */
 int ldl_sm_layer_fwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[5], double fact, int32_t paint_flags,

@@ -29,31 +29,32 @@ __device__ __forceinline__ int fixed_shift(unsigned long long maxbits, int64_t n
 // The 8 fixed-point CIC deposits of one particle with fractions f and mass
 // m at scale 2^s: q[4a+2b+c] = rint((w_x[a] w_y[b]) (w_z[c] (m 2^s))), and
 // q[7] = rint(m 2^s) - (q[0] + ... + q[6]) so the deposits sum exactly to
-// the rounded mass.  The CPU emulation (oracle/cic_oracle.c) performs the
-// same operations in the same order.  FP32 path: |m 2^s| < 2^30 and every
-// weight is in [0, 1], so each deposit and every partial sum (all of the sign
-// of m, in magnitude at most |m 2^s| + 4) fit in 32 bits.
+// the rounded mass.  The products are formed in the working precision T (FP32
+// deposits are int32: |m 2^s| < 2^30 and every weight is in [0, 1], so each
+// deposit and every partial sum, all of the sign of m and at most |m 2^s| + 4
+// in magnitude, fit in 32 bits; FP64 deposits are int64).  The CPU emulation
+// (oracle/cic_oracle.c) performs the same operations in the same order.
 template <typename T> using DepositT = typename std::conditional<sizeof(T) == 4, int, long long>::type;
 
 template <typename T>
-__device__ __forceinline__ void deposits(const T f[3], double m, int s, DepositT<T> q[8]) {
-    const double wx[2] = {(double)(T(1) - f[0]), (double)f[0]};
-    const double wy[2] = {(double)(T(1) - f[1]), (double)f[1]};
-    const double ms = ldexp(m, s);
-    const double wzm[2] = {(double)(T(1) - f[2]) * ms, (double)f[2] * ms};
-    const double wxy[4] = {wx[0] * wy[0], wx[0] * wy[1], wx[1] * wy[0], wx[1] * wy[1]};
+__device__ __forceinline__ void deposits(const T f[3], T m, int s, DepositT<T> q[8]) {
+    const T wx[2] = {T(1) - f[0], f[0]};
+    const T wy[2] = {T(1) - f[1], f[1]};
+    const T ms = ldexp(m, s);
+    const T wzm[2] = {(T(1) - f[2]) * ms, f[2] * ms};
+    const T wxy[4] = {wx[0] * wy[0], wx[0] * wy[1], wx[1] * wy[0], wx[1] * wy[1]};
     DepositT<T> part = 0;
 #pragma unroll
     for (int k = 0; k < 7; k++) {
-        const double w = wxy[k >> 1] * wzm[k & 1];
+        const T w = wxy[k >> 1] * wzm[k & 1];
         DepositT<T> v;
-        if constexpr (sizeof(T) == 4) v = __double2int_rn(w);
+        if constexpr (sizeof(T) == 4) v = __float2int_rn(w);
         else v = __double2ll_rn(w);
         q[k] = v;
         part += v;
     }
     DepositT<T> tot;
-    if constexpr (sizeof(T) == 4) tot = __double2int_rn(ms);
+    if constexpr (sizeof(T) == 4) tot = __float2int_rn(ms);
     else tot = __double2ll_rn(ms);
     q[7] = tot - part;
 }
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(256) k_paint_fixed(const T* __restrict__ pos, 
         const int64_t kk[2] = {idx[2], idx[2] + 1 == m.n2 ? 0 : idx[2] + 1};
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const double mm = mass ? (double)mass[p * NC + c] : (double)mscalar;
+            const T mm = mass ? mass[p * NC + c] : mscalar;
             DepositT<T> q[8];
             deposits<T>(f, mm, s[c], q);
             unsigned long long* a = acc + c * cells;
@@ -349,6 +350,20 @@ inline Fields3<T> fields3(const T* F3, int64_t cells) {
     return Fields3<T>{{F3, F3 + cells, F3 + 2 * cells}};
 }
 
+// One cell of an interleaved 4-slot field (x, y, z, unused): one 16-byte load
+// in FP32, a 16-byte and an 8-byte load in FP64.
+template <typename T> struct Cell4 {
+    T v[3];
+};
+__device__ __forceinline__ Cell4<float> load_cell4(const float* p) {
+    const float4 a = *reinterpret_cast<const float4*>(p);
+    return Cell4<float>{{a.x, a.y, a.z}};
+}
+__device__ __forceinline__ Cell4<double> load_cell4(const double* p) {
+    const double2 a = *reinterpret_cast<const double2*>(p);
+    return Cell4<double>{{a.x, a.y, p[2]}};
+}
+
 // Mesh geometry policies of the gather kernels: GeomAny works on every mesh;
 // GeomP2 on power-of-two meshes whose byte size fits 32 bits (p2_ok).
 template <typename T> struct GeomAny {
@@ -362,6 +377,19 @@ template <typename T> struct GeomAny {
 #pragma unroll
         for (int f = 0; f < NF; f++) gather8(F[f], st, m.n2, v[f]);
     }
+    // the three components of an interleaved field (cell c at F4[4c .. 4c+2])
+    __device__ __forceinline__ void gather4(const T* __restrict__ F4, const St& st, T (&v)[3][8]) const {
+        const int64_t k1 = st.k + 1 == m.n2 ? 0 : st.k + 1;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const Cell4<T> a = load_cell4(F4 + 4 * (st.row[r] + st.k)), b = load_cell4(F4 + 4 * (st.row[r] + k1));
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                v[c][2 * r] = a.v[c];
+                v[c][2 * r + 1] = b.v[c];
+            }
+        }
+    }
 };
 template <typename T> struct GeomP2 {
     using St = StencilP2<T>;
@@ -371,6 +399,26 @@ template <typename T> struct GeomP2 {
     }
     template <int NF>
     __device__ __forceinline__ void gather(const T* const (&F)[NF], const St& st, T (&v)[NF][8]) const { gather_p2<T, NF>(F, st, mp, v); }
+    // the three components of an interleaved field; byte offsets are the
+    // 1-slot row offsets times 4, so the field's bytes must fit 32 bits (p2_ok4)
+    __device__ __forceinline__ void gather4(const T* __restrict__ F4, const St& st, T (&v)[3][8]) const {
+        constexpr uint32_t cb = 4 * sizeof(T);
+        const uint32_t k0 = st.k * cb, k1 = ((st.k + 1u) & mp.msk[2]) * cb;
+        const char* b = reinterpret_cast<const char*>(F4);
+        Cell4<T> a[4], c[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            a[r] = load_cell4(reinterpret_cast<const T*>(b + ((st.row[r] << 2) + k0)));
+            c[r] = load_cell4(reinterpret_cast<const T*>(b + ((st.row[r] << 2) + k1)));
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                v[q][2 * r] = a[r].v[q];
+                v[q][2 * r + 1] = c[r].v[q];
+            }
+    }
 };
 template <typename T>
 inline bool p2_ok(const Mesh& m) {
@@ -381,6 +429,16 @@ inline bool p2_ok(const Mesh& m) {
 template <typename T, typename Fn>
 inline void with_geom(const Mesh& m, Fn&& f) {
     if (p2_ok<T>(m)) f(GeomP2<T>{mesh_p2(m)});
+    else f(GeomAny<T>{m});
+}
+// Same for kernels that also gather an interleaved 4-slot field.
+template <typename T>
+inline bool p2_ok4(const Mesh& m) {
+    return p2_ok<T>(m) && m.cells() * 4 * (int64_t)sizeof(T) <= (int64_t(1) << 32);
+}
+template <typename T, typename Fn>
+inline void with_geom4(const Mesh& m, Fn&& f) {
+    if (p2_ok4<T>(m)) f(GeomP2<T>{mesh_p2(m)});
     else f(GeomAny<T>{m});
 }
 
@@ -803,7 +861,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 DepositT<T> q[8];
-                deposits<T>(f, (double)ms[k][c], s[c], q);
+                deposits<T>(f, ms[k][c], s[c], q);
 #pragma unroll
                 for (int qq = 0; qq < 8; qq++) {
                     const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;

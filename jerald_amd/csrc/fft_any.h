@@ -317,7 +317,8 @@ inline int zpass_c2r(int64_t nrows, int64_t n2, const c2<T>* tw, const Pro& pro,
     if (pow2_len(n2 / 2, 512) && n2 % 2 == 0) {
         dispatch_h(n2 / 2, [&](auto H) {
             constexpr int HH = decltype(H)::value;
-            if (nblocks) *nblocks = (nrows + FftCfg<T, HH>::P - 1) / FftCfg<T, HH>::P;
+            constexpr int RPB = zc2r_rows_per_block<T, HH, Epi>();
+            if (nblocks) *nblocks = (nrows + RPB - 1) / RPB;
             launch_zc2r<T, HH>(nrows, tw, pro, epi, st, tname);
         });
         return LDL_OK;

@@ -275,6 +275,15 @@ __global__ void __launch_bounds__((FftCfg<T, H>::THREADS)) k_zpass_r2c(int64_t n
     }
 }
 
+// Epilogues that store rows in threads of three (Epi::TRIPLE != 0: rows 3r,
+// 3r+1, 3r+2 are the three components of output row r) get whole threes per
+// workgroup and store through Epi::store3(r, m, row 3r, row 3r+1, row 3r+2).
+template <class E, class = void> struct epi_triple : std::false_type {};
+template <class E> struct epi_triple<E, std::void_t<decltype(E::TRIPLE)>> : std::integral_constant<bool, E::TRIPLE != 0> {};
+template <typename T, int H, class Epi> constexpr int zc2r_rows_per_block() {
+    return epi_triple<Epi>::value ? FftCfg<T, H>::P / 3 * 3 : FftCfg<T, H>::P;
+}
+
 // Half spectrum rows -> real rows.  Pro::load(row, k) returns bin k in
 // [0, H]; Epi::store2(row, m, (x[2m], x[2m+1]), acc) may add to Epi::NR
 // per-thread FP64 accumulators, whose workgroup totals go to
@@ -285,9 +294,11 @@ __global__ void __launch_bounds__((FftCfg<T, H>::THREADS)) k_zpass_c2r(int64_t n
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS;
     constexpr int N2 = 2 * H;
+    constexpr int RPB = zc2r_rows_per_block<T, H, Epi>();
+    static_assert(RPB > 0, "a workgroup must hold at least one row (three for triple epilogues)");
     __shared__ C lds[(H + 1) * LS];
     __shared__ C twl[N2];
-    const int64_t row0 = xcd_tile(blockIdx.x, gridDim.x) * P;
+    const int64_t row0 = xcd_tile(blockIdx.x, gridDim.x) * RPB;
     stage_twiddles<T, N2, TH>(twl, tw);
     {
         constexpr int NL = ((H + 1) * P + TH - 1) / TH;
@@ -347,9 +358,17 @@ __global__ void __launch_bounds__((FftCfg<T, H>::THREADS)) k_zpass_c2r(int64_t n
     lds_fft<T, H, P, Cfg::TPP, +1, 2>(lds, twl);
     constexpr int NR = Epi::NR;
     double acc[NR > 0 ? NR : 1] = {};
-    for (int i = threadIdx.x; i < H * P; i += TH) {
-        const int p = i / H, m = i - p * H;
-        if (row0 + p < nrows) epi.store2(row0 + p, m, lds[m * LS + p], acc);
+    if constexpr (epi_triple<Epi>::value) {
+        for (int i = threadIdx.x; i < H * (RPB / 3); i += TH) {
+            const int q = i / H, m = i - q * H;
+            if (row0 + 3 * q < nrows)
+                epi.store3((row0 + 3 * q) / 3, m, lds[m * LS + 3 * q], lds[m * LS + 3 * q + 1], lds[m * LS + 3 * q + 2]);
+        }
+    } else {
+        for (int i = threadIdx.x; i < H * RPB; i += TH) {
+            const int p = i / H, m = i - p * H;
+            if (row0 + p < nrows) epi.store2(row0 + p, m, lds[m * LS + p], acc);
+        }
     }
     if constexpr (NR > 0) {
         block_sum<NR, TH>(acc);
@@ -431,9 +450,9 @@ inline void launch_zr2c(int64_t nrows, const c2<T>* tw, const Pro& pro, const Ep
 template <typename T, int H, class Pro, class Epi>
 inline void launch_zc2r(int64_t nrows, const c2<T>* tw, const Pro& pro, const Epi& epi, hipStream_t st,
                         const char* tname = "zpass_c2r") {
-    constexpr int P = FftCfg<T, H>::P;
+    constexpr int RPB = zc2r_rows_per_block<T, H, Epi>();
     KTimer _t(tname, st);
-    hipLaunchKernelGGL((k_zpass_c2r<T, H, Pro, Epi>), dim3((unsigned)((nrows + P - 1) / P)), dim3(FftCfg<T, H>::THREADS), 0, st,
+    hipLaunchKernelGGL((k_zpass_c2r<T, H, Pro, Epi>), dim3((unsigned)((nrows + RPB - 1) / RPB)), dim3(FftCfg<T, H>::THREADS), 0, st,
                        nrows, tw, pro, epi);
 }
 

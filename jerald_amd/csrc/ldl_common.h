@@ -299,6 +299,17 @@ inline Sweep make_sweep(const PosDec<T>& pd, int64_t np) {
 // Grid for a Sweep-ordered kernel (kChunk threads per workgroup).
 inline unsigned sweep_grid(const Sweep& w) { return grid_for(w.nchunks, 1, 1 << 22); }
 
+// Pair sweep: a workgroup takes two consecutive chunks of the Sweep order and
+// each thread one particle of each, so a thread has both particles' loads in
+// flight at once.  The body sees p0 and p1; p1 >= np when the second chunk
+// does not exist (odd chunk count).
+inline unsigned sweep2_grid(const Sweep& w) { return grid_for((w.nchunks + 1) / 2, 1, 1 << 22); }
+#define LDL_SWEEP2(sw, np, p0, p1)                                                                           \
+    for (int64_t _base = 0, _npair = ((sw).nchunks + 1) / 2; _base < _npair; _base += gridDim.x)            \
+        for (int64_t _q = _base + xcd_tile(blockIdx.x, gridDim.x), p0 = (sw).first(2 * _q) + threadIdx.x,   \
+                     p1 = 2 * _q + 1 < (sw).nchunks ? (sw).first(2 * _q + 1) + threadIdx.x : (np), _once = 1; \
+             _once && _q < _npair; _once = 0)
+
 // Loop over this workgroup's particles in Sweep order; the body sees ``p``.
 #define LDL_SWEEP(sw, np, p)                                                                  \
     for (int64_t _base = 0; _base < (sw).nchunks; _base += gridDim.x)                         \

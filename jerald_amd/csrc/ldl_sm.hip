@@ -225,6 +225,46 @@ template <typename T> struct StarRowLoad {  // starmap^gamma2 (model.py:228), or
     }
 };
 
+// Three half spectra (hcells apart) as rows taken component-fastest: row R is
+// row R / 3 of spectrum R % 3, so the rows of one cell's three components
+// are transformed by the same workgroup.
+template <typename T> struct Tri3RowLoad {
+    const c2<T>* in;
+    int64_t nh, hcells;
+    __device__ __forceinline__ c2<T> load(int64_t R, int k) const {
+        const int64_t r = R / 3;
+        return in[(R - 3 * r) * hcells + r * nh + k];
+    }
+};
+// ... and their real rows into the interleaved field F4 (x, y, z, 0): the
+// power-of-two z pass hands over whole threes of rows (store3, two cells per
+// call); the any-length pass single rows (store2).
+template <typename T> struct Tri3Store4 {
+    static constexpr int NR = 0;
+    static constexpr int TRIPLE = 1;
+    T* out;
+    int64_t n2;
+    __device__ __forceinline__ void reduce_out(int64_t, const double*) const {}
+    __device__ __forceinline__ void store2(int64_t R, int m, c2<T> v, double*) const {
+        const int64_t r = R / 3;
+        T* o = out + 4 * (r * n2 + 2 * m) + (R - 3 * r);
+        o[0] = v.x;
+        o[4] = v.y;
+    }
+    __device__ __forceinline__ void store3(int64_t r, int m, c2<T> a, c2<T> b, c2<T> c) const {
+        T* o = out + 4 * (r * n2 + 2 * m);
+        if constexpr (sizeof(T) == 4) {
+            reinterpret_cast<float4*>(o)[0] = float4{a.x, b.x, c.x, 0.0f};
+            reinterpret_cast<float4*>(o)[1] = float4{a.y, b.y, c.y, 0.0f};
+        } else {
+            reinterpret_cast<double2*>(o)[0] = double2{a.x, b.x};
+            reinterpret_cast<double2*>(o)[1] = double2{c.x, 0.0};
+            reinterpret_cast<double2*>(o)[2] = double2{a.y, b.y};
+            reinterpret_cast<double2*>(o)[3] = double2{c.y, 0.0};
+        }
+    }
+};
+
 // g_u -> g_rho = g_u gamma (rho+1e-8)^(gamma-1); dL/dgamma partial.
 template <typename T> struct GammaBwdStore {
     static constexpr int NR = 1;
@@ -524,45 +564,83 @@ __global__ void __launch_bounds__(256) k_realfilter_mult(c2<T>* __restrict__ S, 
 }
 
 // --------------------------------------------------------------- gathers
+// Pair sweep: each thread gathers for two particles, both particles' position
+// loads and then both particles' field loads issued before any is consumed.
+// Loads of a missing second particle read particle np - 1; only its stores
+// are skipped.  The forces F4 are interleaved (x, y, z, unused) per cell.
+
 // pos_out = pos + readout3(F) * step (step = 1/cell in mesh units, 1 in box units)
 template <typename T, typename G>
-__global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, Fields3<T> F3,
+__global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ F4,
                                                          G geo, T s0, T s1, T s2, T* __restrict__ out) {
-    LDL_SWEEP(sw, np, p) {
-        const T x0 = pos[3 * p], x1 = pos[3 * p + 1], x2 = pos[3 * p + 2];
-        typename G::St st;
-        geo.locate(pd, x0, x1, x2, p, st);
-        T v[3][8];
-        geo.template gather<3>(F3.f, st, v);
-        out[3 * p] = x0 + interp(v[0], st.f, -1, pd.cellinv) * s0;
-        out[3 * p + 1] = x1 + interp(v[1], st.f, -1, pd.cellinv) * s1;
-        out[3 * p + 2] = x2 + interp(v[2], st.f, -1, pd.cellinv) * s2;
+    LDL_SWEEP2(sw, np, p0, p1) {
+        const int64_t pp[2] = {p0 < np ? p0 : np - 1, p1 < np ? p1 : np - 1};
+        T x[2][3];
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) x[k][d] = pos[3 * pp[k] + d];
+        typename G::St st[2];
+        T v[2][3][8];
+#pragma unroll
+        for (int k = 0; k < 2; k++) geo.locate(pd, x[k][0], x[k][1], x[k][2], pp[k], st[k]);
+#pragma unroll
+        for (int k = 0; k < 2; k++) geo.gather4(F4, st[k], v[k]);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const T o0 = x[k][0] + interp(v[k][0], st[k].f, -1, pd.cellinv) * s0;
+            const T o1 = x[k][1] + interp(v[k][1], st[k].f, -1, pd.cellinv) * s1;
+            const T o2 = x[k][2] + interp(v[k][2], st[k].f, -1, pd.cellinv) * s2;
+            if ((k ? p1 : p0) < np) {
+                out[3 * pp[k]] = o0;
+                out[3 * pp[k] + 1] = o1;
+                out[3 * pp[k] + 2] = o2;
+            }
+        }
     }
 }
 
 // g_in = g + sum_d g_d grad F_d + fact grad g_rho  (box units)
 template <typename T, typename G>
-__global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, Fields3<T> F3,
+__global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ F4,
                                                        const T* __restrict__ grho, G geo, T fact, const T* __restrict__ gout,
                                                        T* __restrict__ gin) {
-    LDL_SWEEP(sw, np, p) {
-        typename G::St st;
-        geo.locate(pd, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, st);
-        const T* fs[4] = {F3.f[0], F3.f[1], F3.f[2], grho};
-        T v[4][8];
-        geo.template gather<4>(fs, st, v);
-        const T g[3] = {gout[3 * p], gout[3 * p + 1], gout[3 * p + 2]};
-        T acc[3] = {g[0], g[1], g[2]};
-        T val, gr[3];
+    LDL_SWEEP2(sw, np, p0, p1) {
+        const int64_t pp[2] = {p0 < np ? p0 : np - 1, p1 < np ? p1 : np - 1};
+        T x[2][3], g[2][3];
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            interp_vg(v[d], st.f, pd.cellinv, val, gr);
+        for (int k = 0; k < 2; k++)
 #pragma unroll
-            for (int e = 0; e < 3; e++) acc[e] += g[d] * gr[e];
+            for (int d = 0; d < 3; d++) {
+                x[k][d] = pos[3 * pp[k] + d];
+                g[k][d] = gout[3 * pp[k] + d];
+            }
+        typename G::St st[2];
+        T v[2][3][8], vr[2][1][8];
+        const T* fr[1] = {grho};
+#pragma unroll
+        for (int k = 0; k < 2; k++) geo.locate(pd, x[k][0], x[k][1], x[k][2], pp[k], st[k]);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            geo.gather4(F4, st[k], v[k]);
+            geo.template gather<1>(fr, st[k], vr[k]);
         }
-        interp_vg(v[3], st.f, pd.cellinv, val, gr);
 #pragma unroll
-        for (int e = 0; e < 3; e++) gin[3 * p + e] = acc[e] + fact * gr[e];
+        for (int k = 0; k < 2; k++) {
+            T acc[3] = {g[k][0], g[k][1], g[k][2]};
+            T val, gr[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                interp_vg(v[k][d], st[k].f, pd.cellinv, val, gr);
+#pragma unroll
+                for (int e = 0; e < 3; e++) acc[e] += g[k][d] * gr[e];
+            }
+            interp_vg(vr[k][0], st[k].f, pd.cellinv, val, gr);
+            if ((k ? p1 : p0) < np) {
+#pragma unroll
+                for (int e = 0; e < 3; e++) gin[3 * pp[k] + e] = acc[e] + fact * gr[e];
+            }
+        }
     }
 }
 
@@ -570,15 +648,29 @@ __global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos
 template <typename T, typename G>
 __global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ grho,
                                                        G geo, T fact, T* __restrict__ g) {
-    LDL_SWEEP(sw, np, p) {
-        typename G::St st;
-        geo.locate(pd, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, st);
-        const T* fs[1] = {grho};
-        T v[1][8], val, gr[3];
-        geo.template gather<1>(fs, st, v);
-        interp_vg(v[0], st.f, pd.cellinv, val, gr);
+    LDL_SWEEP2(sw, np, p0, p1) {
+        const int64_t pp[2] = {p0 < np ? p0 : np - 1, p1 < np ? p1 : np - 1};
+        T x[2][3];
 #pragma unroll
-        for (int e = 0; e < 3; e++) g[3 * p + e] = fact * gr[e];
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) x[k][d] = pos[3 * pp[k] + d];
+        typename G::St st[2];
+        T v[2][1][8];
+        const T* fs[1] = {grho};
+#pragma unroll
+        for (int k = 0; k < 2; k++) geo.locate(pd, x[k][0], x[k][1], x[k][2], pp[k], st[k]);
+#pragma unroll
+        for (int k = 0; k < 2; k++) geo.template gather<1>(fs, st[k], v[k]);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            T val, gr[3];
+            interp_vg(v[k][0], st[k].f, pd.cellinv, val, gr);
+            if ((k ? p1 : p0) < np) {
+#pragma unroll
+                for (int e = 0; e < 3; e++) g[3 * pp[k] + e] = fact * gr[e];
+            }
+        }
     }
 }
 
@@ -991,7 +1083,7 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     }
     // 4. inverse y passes (3 spectra as 3*N0 planes) and z c2r into F
     if ((rc = colpass<T, +1>(m.n1, 3 * m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
-    if ((rc = zpass_c2r<T>(3 * nrows, m.n2, tw, HalfRowLoad<T>{P.spec3, nh}, RealRowStore<T>{F, m.n2, T(1)}, st, nullptr, "zpass_c2r3")))
+    if ((rc = zpass_c2r<T>(3 * nrows, m.n2, tw, Tri3RowLoad<T>{P.spec3, nh, hc}, Tri3Store4<T>{F, m.n2}, st, nullptr, "zpass_c2r3")))
         return rc;
     // 5. readout + position update
     T s[3];
@@ -999,8 +1091,8 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     if (np > 0)
         {
             KTimer _t("readout3_update", st);
-            with_geom<T>(m, [&](auto geo) {
-                hipLaunchKernelGGL((k_readout3_update<T, decltype(geo)>), dim3(sweep_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, fields3(F, m.cells()), geo,
+            with_geom4<T>(m, [&](auto geo) {
+                hipLaunchKernelGGL((k_readout3_update<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, F, geo,
                                    s[0], s[1], s[2], pos_out);
             });
         }
@@ -1098,8 +1190,8 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     if (np > 0)
         {
             KTimer _t("sm_bwd_gather", st);
-            with_geom<T>(m, [&](auto geo) {
-                hipLaunchKernelGGL((k_sm_bwd_gather<T, decltype(geo)>), dim3(sweep_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, fields3(F, m.cells()), grho, geo,
+            with_geom4<T>(m, [&](auto geo) {
+                hipLaunchKernelGGL((k_sm_bwd_gather<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, F, grho, geo,
                                    (T)fact, gout, gin);
             });
         }
@@ -1297,7 +1389,7 @@ static int paint_vjp_pos(const T* pos, const ldl_pos_desc* pdesc, int64_t np, co
     const Sweep sw = make_sweep(pd, np);
     KTimer _t("paint_vjp_pos", st);
     with_geom<T>(m, [&](auto geo) {
-        hipLaunchKernelGGL((k_paint_vjp_pos<T, decltype(geo)>), dim3(sweep_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, grho, geo, (T)fact, g);
+        hipLaunchKernelGGL((k_paint_vjp_pos<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, grho, geo, (T)fact, g);
     });
     LDL_LAUNCH_CHECK();
     return LDL_OK;

@@ -144,7 +144,7 @@ class JERALDModel:
         dt, dev, nm = self.dtype, self.device, self.nm
         rho = torch.empty(nm, dtype=dt, device=dev)
         U = torch.empty(nm[0], nm[1], nm[2] // 2 + 1, 2, dtype=dt, device=dev)
-        F = torch.empty((3,) + nm, dtype=dt, device=dev)
+        F = torch.empty(nm + (4,), dtype=dt, device=dev)  # forces, interleaved (x, y, z, unused) per cell
         out = torch.empty_like(P)
         bins = None
         if keep_bins:

@@ -110,25 +110,24 @@ void oracle_mesh_units_f64(const double* pos, int64_t np_, const double* box, co
 }
 
 /* ---------------- fixed-point paint emulation ----------------
- * Per particle: CIC fractions in the working precision T, then in double
- * ms = m 2^s, wzm[c] = w_z[c] ms, wxy[2a+b] = w_x[a] w_y[b] and the deposit of
- * corner k = 4a+2b+c is rint(wxy[k>>1] * wzm[k&1]) for k < 7; corner 7 takes
- * rint(ms) minus the other seven so the mass is conserved exactly.  The
- * accumulator is summed in int64 (exact, order free).  s comes from the
- * caller (FP32: 30 - e, FP64: 61 - e - ceil(log2 Np), with max|m| < 2^e).
- * kind 0: pos physical (wrapped and divided in T);  kind 1: pos is a
- * displacement in mesh units from the lattice point idx*step.            */
-#define DEPOSIT_BODY                                                               \
-    double wx[2] = {(double)tx, (double)fx_}, wy[2] = {(double)ty, (double)fy_};   \
-    double ms = ldexp(m, s);                                                       \
-    double wzm[2] = {(double)tz * ms, (double)fz_ * ms};                           \
-    double wxy[4] = {wx[0] * wy[0], wx[0] * wy[1], wx[1] * wy[0], wx[1] * wy[1]};   \
+ * Per particle, all in the working precision T (float or double): CIC
+ * fractions, ms = m 2^s, wzm[c] = w_z[c] ms, wxy[2a+b] = w_x[a] w_y[b] and
+ * the deposit of corner k = 4a+2b+c is rint(wxy[k>>1] * wzm[k&1]) for k < 7;
+ * corner 7 takes rint(ms) minus the other seven so the mass is conserved
+ * exactly.  The accumulator is summed in int64 (exact, order free).  s comes
+ * from the caller (FP32: 30 - e, FP64: 61 - e - ceil(log2 Np), with
+ * max|m| < 2^e).  kind 0: pos physical (wrapped and divided in T);  kind 1:
+ * pos is a displacement in mesh units from the lattice point idx*step.    */
+#define DEPOSIT_BODY(T, LDEXP, RINT)                                               \
+    T ms = LDEXP(m, s);                                                            \
+    T wzm[2] = {tz * ms, fz_ * ms};                                                \
+    T wxy[4] = {tx * ty, tx * fy_, fx_ * ty, fx_ * fy_};                           \
     int64_t ii[2] = {i, (i + 1) % n0}, jj[2] = {j, (j + 1) % n1}, kk[2] = {k, (k + 1) % n2}; \
     int64_t part = 0;                                                              \
     for (int q = 0; q < 8; q++) {                                                  \
         int64_t v;                                                                 \
-        if (q == 7) v = llrint(ms) - part;                                         \
-        else { v = llrint(wxy[q >> 1] * wzm[q & 1]); part += v; }                   \
+        if (q == 7) v = (int64_t)RINT(ms) - part;                                  \
+        else { v = (int64_t)RINT(wxy[q >> 1] * wzm[q & 1]); part += v; }            \
         acc[(ii[q >> 2] * n1 + jj[(q >> 1) & 1]) * n2 + kk[q & 1]] += v;           \
     }
 
@@ -140,7 +139,7 @@ void oracle_paint_fixed_f32(const float* pos, int64_t np_, int kind, const doubl
     for (int d = 0; d < 3; d++) { box[d] = (float)boxd[d]; cell[d] = box[d] / (float)nm[d]; }
     memset(acc, 0, sizeof(int64_t) * n0 * n1 * n2);
     for (int64_t p = 0; p < np_; p++) {
-        double m = mass ? (double)mass[p] : mscalar;
+        float m = mass ? mass[p] : (float)mscalar;
         float xs[3];
         int64_t base[3] = {0, 0, 0};
         if (kind == 0) {
@@ -159,7 +158,7 @@ void oracle_paint_fixed_f32(const float* pos, int64_t np_, int kind, const doubl
         float tx = 1.0f - fx_, ty = 1.0f - fy_, tz = 1.0f - fz_;
         int64_t i = wrap64(base[0] + (int64_t)flx, n0), j = wrap64(base[1] + (int64_t)fly, n1),
                 k = wrap64(base[2] + (int64_t)flz, n2);
-        DEPOSIT_BODY
+        DEPOSIT_BODY(float, ldexpf, lrintf)
     }
 }
 
@@ -190,6 +189,6 @@ void oracle_paint_fixed_f64(const double* pos, int64_t np_, int kind, const doub
         double tx = 1.0 - fx_, ty = 1.0 - fy_, tz = 1.0 - fz_;
         int64_t i = wrap64(base[0] + (int64_t)flx, n0), j = wrap64(base[1] + (int64_t)fly, n1),
                 k = wrap64(base[2] + (int64_t)flz, n2);
-        DEPOSIT_BODY
+        DEPOSIT_BODY(double, ldexp, llrint)
     }
 }
