@@ -102,16 +102,17 @@ def kernel_bytes(name, n, b):
         "zpass_c2r": b * Nm + 2 * b * Nh,
         "zpass_c2r3": 3 * (b * Nm + 2 * b * Nh),
         "colpass": 4 * b * Nh,
-        "colpass3": 12 * b * Nh,
-        "sm_x_fwd": 10 * b * Nh,                         # in, U out, 3 spectra out
-        "sm_x_bwd": 10 * b * Nh,                         # 3 spectra + U in, 1 out
+        "colpass3": 4 * b * Nh,                          # one force spectrum in and out
+        "y_fwd_sum2": 6 * b * Nh,                        # 2 gradient spectra in, 1 out
+        "sm_x_fwd": 8 * b * Nh,                          # in, U out, 2 spectra out
+        "sm_x_bwd": 8 * b * Nh,                          # 2 spectra + U in, 1 out
         "x_realfilter": 4 * b * Nh,
     }.get(name)
 
 
 KERNELS = ("paint_absmax", "paint_bin", "paint_scan", "paint_scatter", "paint_deposit1", "paint_deposit3",
            "paint_deposit", "paint_convert", "zpass_r2c", "zpass_r2c3", "zpass_c2r", "zpass_c2r3", "colpass",
-           "colpass3", "sm_x_fwd", "sm_x_bwd", "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
+           "colpass3", "y_fwd_sum2", "sm_x_fwd", "sm_x_bwd", "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
            "loss")
 
 

@@ -70,7 +70,7 @@ int ldl_version(void);
  * launch count of one kernel name: "paint_absmax", "paint_bin", "paint_scan",
  * "paint_scatter", "paint_deposit1"/"3" (1- or 3-component tiled paint),
  * "paint_deposit", "paint_convert" (direct and float modes), "zpass_r2c"/"3",
- * "zpass_c2r"/"3", "colpass"/"3" (one or three spectra), "sm_x_fwd", "sm_x_bwd",
+ * "zpass_c2r"/"3", "colpass"/"3" (one or three spectra), "y_fwd_sum2", "sm_x_fwd", "sm_x_bwd",
  * "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
  * "readout", "loss".  Enabling clears previous records. */
 int ldl_timer_enable(int32_t on);
@@ -170,9 +170,15 @@ size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np
  * of the same positions reuse it instead of sorting again. */
 size_t ldl_sm_bins_bytes(const int64_t nmesh[3], int64_t np);
 
+/* Row pitch Hp, in complex elements, of the half spectra the layer entry
+ * points keep (U, S2): N2/2+1 rounded up to 64 bytes (complex64: a multiple
+ * of 8, complex128: of 4).  Such a buffer is (N0, N1, Hp) complex with the
+ * first N2/2+1 entries of each row used.  0 for an invalid mesh. */
+int64_t ldl_sm_spectrum_pitch(const int64_t nmesh[3], int32_t is_f64);
+
 /* Forward of one layer.  Inputs pos (Np,3).  Writes pos_out (Np,3) = pos +
  * s / cell (same kind as pos; may alias pos), and the saved tensors the
- * backward needs: rho (N0,N1,N2), U (N0,N1,N2/2+1 complex, = r2c of
+ * backward needs: rho (N0,N1,N2), U (N0,N1,Hp complex, = r2c of
  * (rho+1e-8)^gamma), forces F (N0,N1,N2,4) interleaved per cell as
  * (F_x, F_y, F_z, unused), and -- if ``bins`` is not NULL
  * (ldl_sm_bins_bytes) -- the particle sort of the fixed-point paint. */
@@ -263,8 +269,8 @@ int ldl_dm_loss_f64(const double* rho, const double* target, const double* mask,
  * (model.py:261-266, 279-285):  P = 1e-3 (alpha1 G1 U + alpha2 G2 S2) with
  * G1 the sm kernel, G2 = +exp(-k^2/kl2^2 - kh2^2/k^2) k^nu2 and
  * S2 = r2c(starmap^gamma2) (starmap (N0,N1,N2)).  Same workspace and saved
- * tensors as the sm layer, plus S2 (N0,N1,N2/2+1 complex), written by the
- * forward and read by the backward.  The backward ACCUMULATES the 10
+ * tensors as the sm layer, plus S2 (N0,N1,Hp complex, ldl_sm_spectrum_pitch),
+ * written by the forward and read by the backward.  The backward ACCUMULATES the 10
  * parameter gradients into grad10 (DEVICE double[10]); alpha_rule applies
  * to alpha1 and alpha2 (both multiply outside c2f, model.py:231). */
 int ldl_hi_layer_fwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,

@@ -54,6 +54,7 @@ _SIGS = {
     "ldl_fft_workspace_bytes": [_I64x3, _I32],
     "ldl_sm_workspace_bytes": [_I64x3, _I32, _I64],
     "ldl_sm_bins_bytes": [_I64x3, _I64],
+    "ldl_sm_spectrum_pitch": [_I64x3, _I32],
     "ldl_timer_enable": [_I32],
     "ldl_timer_read": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
 }
@@ -86,6 +87,7 @@ _RESTYPE = {
     "ldl_fft_workspace_bytes": C.c_size_t,
     "ldl_sm_workspace_bytes": C.c_size_t,
     "ldl_sm_bins_bytes": C.c_size_t,
+    "ldl_sm_spectrum_pitch": C.c_int64,
 }
 
 EXPORTED = sorted(_SIGS)

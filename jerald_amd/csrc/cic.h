@@ -810,7 +810,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     const unsigned total = rpre[NR];
     // KB particles per thread per round; all their loads are issued before any
     // deposit so the dependent sorted[j] -> pos[p] chains overlap
-    constexpr int KB = 4;
+    constexpr int KB = 6;
     for (unsigned jb = 0; jb < total; jb += TB * KB) {
         unsigned pi[KB];
         int code[KB];
@@ -858,15 +858,31 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
             const bool in_y[2] = {b >= 0, b + 1 < ey};
             const bool in_z[2] = {cz >= 0, cz + 1 < ez};
             const int base = (a * TS + b) * TS + cz;
+            // all 8 corners in the tile (face class 0 of the own tile, most
+            // particles, stored together): unconditional deposits
+            const bool all_in = in_x[0] && in_x[1] && in_y[0] && in_y[1] && in_z[0] && in_z[1];
+            if (all_in) {
 #pragma unroll
-            for (int c = 0; c < NC; c++) {
-                DepositT<T> q[8];
-                deposits<T>(f, ms[k][c], s[c], q);
+                for (int c = 0; c < NC; c++) {
+                    DepositT<T> q[8];
+                    deposits<T>(f, ms[k][c], s[c], q);
 #pragma unroll
-                for (int qq = 0; qq < 8; qq++) {
-                    const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
-                    if (in_x[dx] && in_y[dy] && in_z[dz] && q[qq] != 0)
+                    for (int qq = 0; qq < 8; qq++) {
+                        const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
                         atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (unsigned long long)(long long)q[qq]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    DepositT<T> q[8];
+                    deposits<T>(f, ms[k][c], s[c], q);
+#pragma unroll
+                    for (int qq = 0; qq < 8; qq++) {
+                        const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
+                        if (in_x[dx] && in_y[dy] && in_z[dz] && q[qq] != 0)
+                            atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (unsigned long long)(long long)q[qq]);
+                    }
                 }
             }
         }

@@ -119,6 +119,17 @@ inline void filter_grad_coeffs(const double* prm, int dm, double* gkh, double* g
     }
 }
 
+// Row pitch of the half spectra the layer, loss and depletion entry points
+// keep (U, S2 and all workspace spectra): N2/2+1 complex rounded up to 64
+// bytes.  The column passes move 16-column strips of every row; with rows
+// 64-byte aligned those are whole 64-byte sectors (an odd N2/2+1 pitch makes
+// the strided x pass about twice slower).
+template <typename T> inline int64_t spec_pitch(const Mesh& m) {
+    constexpr int64_t q = 64 / (2 * (int64_t)sizeof(T));
+    return (m.nh() + q - 1) / q * q;
+}
+template <typename T> inline int64_t spec_cells(const Mesh& m) { return m.n0 * m.n1 * spec_pitch<T>(m); }
+
 // Workspace carve-up of ldl_sm_* calls.
 struct SmWs {
     size_t paint, spec3, mesh3, tabs, part, total;
@@ -128,8 +139,10 @@ static SmWs sm_ws(const Mesh& m, int tsize, int64_t np) {
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     SmWs w;
     w.paint = al(paint_ws_bytes(m, 3, np));
-    w.spec3 = al((size_t)3 * m.hcells() * 2 * tsize);
-    w.mesh3 = al((size_t)3 * m.cells() * tsize);
+    const int64_t hq = (64 / (2 * tsize)), hcp = m.n0 * m.n1 * ((m.nh() + hq - 1) / hq * hq);  // spec_cells
+    w.spec3 = al((size_t)3 * hcp * 2 * tsize);
+    // three meshes; also holds one pitched spectrum (the HI layer's T2)
+    w.mesh3 = al((size_t)std::max<int64_t>(3 * m.cells(), 2 * hcp) * tsize);
     w.tabs = al((size_t)2 * (m.n0 + m.n1 + m.nh()) * tsize);
     w.part_doubles = 12 * (m.n0 * m.n1 + m.n1 * (m.nh() + 1) + 4096);
     w.part = al((size_t)w.part_doubles * 8);
@@ -225,20 +238,55 @@ template <typename T> struct StarRowLoad {  // starmap^gamma2 (model.py:228), or
     }
 };
 
-// Three half spectra (hcells apart) as rows taken component-fastest: row R is
-// row R / 3 of spectrum R % 3, so the rows of one cell's three components
-// are transformed by the same workgroup.
+// Three half spectra (rows of pitch nh) as rows taken component-fastest: row
+// R is row R / 3 of spectrum R % 3, so the rows of one cell's three
+// components are transformed by the same workgroup.  With Dz set, the third
+// spectrum is multiplied by -i D_z(k_z) on load.
 template <typename T> struct Tri3RowLoad {
-    const c2<T>* in;
-    int64_t nh, hcells;
+    const c2<T>* in0;
+    const c2<T>* in1;
+    const c2<T>* in2;
+    int64_t nh;
+    const T* Dz;
     __device__ __forceinline__ c2<T> load(int64_t R, int k) const {
         const int64_t r = R / 3;
-        return in[(R - 3 * r) * hcells + r * nh + k];
+        const int d = (int)(R - 3 * r);
+        const c2<T>* in = d == 0 ? in0 : d == 1 ? in1 : in2;  // (no dynamically indexed array: it would live in scratch)
+        const c2<T> v = in[r * nh + k];
+        if (d == 2 && Dz) return c2<T>{Dz[k] * v.y, -Dz[k] * v.x};
+        return v;
     }
 };
-// ... and their real rows into the interleaved field F4 (x, y, z, 0): the
-// power-of-two z pass hands over whole threes of rows (store3, two cells per
-// call); the any-length pass single rows (store2).
+
+// Rows of three stacked spectra (nrows each); with Dz set, the third
+// spectrum's bins are multiplied by i D_z(k_z) as they are stored.
+template <typename T> struct HalfRowStore3 {
+    c2<T>* out;
+    int64_t nh, nrows;
+    const T* Dz;
+    __device__ __forceinline__ void store(int64_t row, int k, c2<T> v) const {
+        if (Dz && row >= 2 * nrows) v = c2<T>{-Dz[k] * v.y, Dz[k] * v.x};
+        out[row * nh + k] = v;
+    }
+};
+
+// Inverse y pass input of F_y: -i D_y(k_y) A, with A = x^-1(P) the x pass's
+// second output (D_y and D_z are constant along x lines, so A serves F_y and
+// F_z and the x pass does two inverse transforms instead of three).
+template <typename T> struct DyColOp {
+    const c2<T>* in;
+    c2<T>* out;
+    const T* Dy;
+    __device__ __forceinline__ c2<T> load(int64_t idx, int n, int64_t plane, int64_t col) const {
+        const c2<T> v = in[idx];
+        return c2<T>{Dy[n] * v.y, -Dy[n] * v.x};
+    }
+    __device__ __forceinline__ void store(int64_t idx, int n, int64_t plane, int64_t col, c2<T> v) const { out[idx] = v; }
+};
+
+// Rows taken component-fastest (Tri3RowLoad) into the interleaved field F4
+// (x, y, z, 0): the power-of-two z pass hands over whole threes of rows
+// (store3, two cells per call); the any-length pass single rows (store2).
 template <typename T> struct Tri3Store4 {
     static constexpr int NR = 0;
     static constexpr int TRIPLE = 1;
@@ -328,8 +376,9 @@ __device__ __forceinline__ T xbwd_cell(const SmFilter<T>& fl, const Src2<T>& s2,
     return G;
 }
 
-// Forward: X FFT, save U = . / N^3, P = 1e-2 alpha G U, three inverse X FFTs of
-// -i D_d P into out3 (3 spectra, hcells apart).
+// Forward: X FFT, save U = . / N^3, P = 1e-2 alpha G U, two inverse X FFTs
+// into out3 (spectra hcells apart): of -i D_x P, and of P itself (the y and z
+// derivative factors are constant along x lines and applied later).
 template <typename T, int N, bool HI>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
 k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>* __restrict__ out3, int64_t hcells,
@@ -369,13 +418,13 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
         }
     }
 #pragma unroll 1
-    for (int d = 0; d < 3; d++) {
+    for (int d = 0; d < 2; d++) {
 #pragma unroll
         for (int s = 0; s < EPT; s++) {
             const int i = threadIdx.x + s * TH;
             const int n = i / P, p = i - n * P;
-            const T Dd = d == 0 ? kt.D[0][n] : d == 1 ? kt.D[1][plane] : (p < pv ? kt.D[2][col0 + p] : T(0));
-            lds[n * LS + p] = C{Dd * pk[s].y, -Dd * pk[s].x};  // -i D P
+            const T Dx = kt.D[0][n];
+            lds[n * LS + p] = d == 0 ? C{Dx * pk[s].y, -Dx * pk[s].x} : pk[s];  // -i D_x P, P
         }
         __syncthreads();
         lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
@@ -385,9 +434,11 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
     }
 }
 
-// Backward: three forward X FFTs combined into Q = sum_d i D_d S_d, the FP64
-// parameter partials (alpha, kh, kl, nu; HI also the second source's) against
-// the saved U, then one inverse X FFT of cpot alpha G Q into out.
+// Backward: two forward X FFTs combined into Q = i D_x S_x + T, where the y
+// pass already folded the y and z components into T = i D_y S_y + i D_z S_z;
+// the FP64 parameter partials (alpha, kh, kl, nu; HI also the second
+// source's) against the saved U, then one inverse X FFT of cpot alpha G Q
+// into out.
 template <typename T, int N, bool HI>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
 k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3, const c2<T>* __restrict__ U,
@@ -408,7 +459,7 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
 #pragma unroll
     for (int s = 0; s < EPT; s++) q[s] = C{T(0), T(0)};
 #pragma unroll 1
-    for (int d = 0; d < 3; d++) {
+    for (int d = 0; d < 2; d++) {
         const C* src = S3 + d * hcells + base;
         tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return src[n * g.es + p]; });
         __syncthreads();
@@ -417,9 +468,9 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
         for (int s = 0; s < EPT; s++) {
             const int i = threadIdx.x + s * TH;
             const int n = i / P, p = i - n * P;
-            const T Dd = d == 0 ? kt.D[0][n] : d == 1 ? kt.D[1][plane] : (p < pv ? kt.D[2][col0 + p] : T(0));
             const C v = lds[n * LS + p];
-            q[s] = C{q[s].x - Dd * v.y, q[s].y + Dd * v.x};  // += i D v
+            const T Dx = kt.D[0][n];
+            q[s] = d == 0 ? C{-Dx * v.y, Dx * v.x} : C{q[s].x + v.x, q[s].y + v.y};  // i D_x S_x, + T
         }
         __syncthreads();
     }
@@ -447,6 +498,47 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int j = 0; j < NP; j++) part[blockIdx.x * NP + j] = acc[j];
+    }
+}
+
+// Forward y pass of the backward's y and z gradient spectra into one:
+// T = i D_y(k_y) y(S_y) + y(S_z), S_z already carrying i D_z(k_z) from the z
+// pass; one tile of each, the first kept in registers.  T overwrites S_y.
+template <typename T, int N>
+__global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
+k_y_fwd_sum2(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* Sy, const c2<T>* __restrict__ Sz, const T* __restrict__ Dy) {
+    using Cfg = FftCfg<T, N>;
+    using C = c2<T>;
+    constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
+    __shared__ C lds[Cfg::LDS_ELEMS];
+    __shared__ C twl[N];
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t plane = tile / g.tiles_per_plane;
+    const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
+    const int64_t base = plane * g.plane_stride + col0;
+    const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
+    stage_twiddles<T, N, TH>(twl, tw);
+    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return Sy[base + n * g.es + p]; });
+    __syncthreads();
+    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
+    C r[EPT];
+#pragma unroll
+    for (int s = 0; s < EPT; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        const C v = lds[n * LS + p];
+        r[s] = C{-Dy[n] * v.y, Dy[n] * v.x};  // i D_y y(S_y)
+    }
+    __syncthreads();
+    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return Sz[base + n * g.es + p]; });
+    __syncthreads();
+    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
+#pragma unroll
+    for (int s = 0; s < EPT; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        const C v = lds[n * LS + p];
+        if (p < pv) Sy[base + n * g.es + p] = C{r[s].x + v.x, r[s].y + v.y};
     }
 }
 
@@ -492,15 +584,17 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
 // ------------------------------------ spectral kernels of the any-length path
 // Meshes whose axes are not all powers of two run the x passes unfused: plain
 // column passes around these elementwise kernels over the half spectrum
-// (index i -> (ix, iy, iz) with iz fastest), which do what the fused x
-// kernels do between their transforms.
+// (index i -> (ix, iy, iz) with iz fastest, rows of pitch hp; the pad columns
+// iz >= nh are skipped), which do what the fused x kernels do between their
+// transforms.
 
 // out3[d] = -i D_d (cpot alpha G U [+ a G2 S2])
 template <typename T, bool HI>
 __global__ void __launch_bounds__(256) k_sm_filter_fwd(const c2<T>* __restrict__ U, c2<T>* __restrict__ out3, int64_t hcells, int64_t n1,
-                                                       int64_t nh, KTabs<T> kt, SmFilter<T> fl, T a1e2, Src2<T> s2) {
+                                                       int64_t nh, int64_t hp, KTabs<T> kt, SmFilter<T> fl, T a1e2, Src2<T> s2) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
+        const int64_t iz = i % hp, iy = (i / hp) % n1, ix = i / (hp * n1);
+        if (iz >= nh) continue;
         const T k2 = kt.knorm2(ix, iy, iz);
         const T G = fl.G(k2);
         const c2<T> u = U[i];
@@ -521,13 +615,15 @@ __global__ void __launch_bounds__(256) k_sm_filter_fwd(const c2<T>* __restrict__
 // cpot alpha G Q (out may alias S3's first spectrum).
 template <typename T, bool HI>
 __global__ void __launch_bounds__(256) k_sm_filter_bwd(const c2<T>* S3, const c2<T>* __restrict__ U, c2<T>* out, int64_t hcells, int64_t n1,
-                                                       int64_t nh, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh, double gkl,
-                                                       double cpot, double alpha, int alpha_rule, Src2<T> s2, double* __restrict__ part) {
+                                                       int64_t nh, int64_t hp, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh,
+                                                       double gkl, double cpot, double alpha, int alpha_rule, Src2<T> s2,
+                                                       double* __restrict__ part) {
     constexpr int NP = HI ? 9 : 4;
     double acc[NP] = {};
     const double apot = cpot * alpha;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
+        const int64_t iz = i % hp, iy = (i / hp) % n1, ix = i / (hp * n1);
+        if (iz >= nh) continue;
         const T D[3] = {kt.D[0][ix], kt.D[1][iy], kt.D[2][iz]};
         c2<T> q = c2<T>{T(0), T(0)};
 #pragma unroll
@@ -546,10 +642,11 @@ __global__ void __launch_bounds__(256) k_sm_filter_bwd(const c2<T>* S3, const c2
 
 // S *= scale * W(k) (the weights of k_x_realfilter)
 template <typename T>
-__global__ void __launch_bounds__(256) k_realfilter_mult(c2<T>* __restrict__ S, int64_t hcells, int64_t n1, int64_t nh, KTabs<T> kt, int kind,
-                                                         T par, T scale) {
+__global__ void __launch_bounds__(256) k_realfilter_mult(c2<T>* __restrict__ S, int64_t hcells, int64_t n1, int64_t nh, int64_t hp, KTabs<T> kt,
+                                                         int kind, T par, T scale) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
+        const int64_t iz = i % hp, iy = (i / hp) % n1, ix = i / (hp * n1);
+        if (iz >= nh) continue;
         T w;
         if (kind == 1) {
             w = ppow(kt.knorm2(ix, iy, iz), T(-0.5) * par) + T(1);
@@ -908,14 +1005,15 @@ template <typename T> struct DepPar {
     T beta1, beta2, gamma3, eta;
 };
 
+// out (rows of pitch hp) = Sk W, Sk in the unpadded PMesh.r2c layout
 template <typename T>
 __global__ void __launch_bounds__(256) k_hi_wmult(const c2<T>* __restrict__ Sk, c2<T>* __restrict__ out, int64_t hcells, int64_t n1,
-                                                  int64_t nh, KTabs<T> kt, SmFilter<T> fl, T xi) {
+                                                  int64_t nh, int64_t hp, KTabs<T> kt, SmFilter<T> fl, T xi) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
         const T w = xi - fl.G(kt.knorm2(ix, iy, iz));
         const c2<T> v = Sk[i];
-        out[i] = c2<T>{v.x * w, v.y * w};
+        out[(ix * n1 + iy) * hp + iz] = c2<T>{v.x * w, v.y * w};
     }
 }
 
@@ -963,14 +1061,14 @@ __global__ void __launch_bounds__(256) k_hi_dep_bwd(const T* __restrict__ star, 
 // dW/dnui = -G ln k, dW/dxi = 1 (G the dm-form kernel).
 template <typename T>
 __global__ void __launch_bounds__(256) k_hi_wgrad(const c2<T>* __restrict__ ghat, const c2<T>* __restrict__ Sk, int64_t hcells, int64_t n1,
-                                                  int64_t nh, int64_t n2, KTabs<T> kt, SmFilter<T> fl, double inv_kli2,
+                                                  int64_t nh, int64_t hp, int64_t n2, KTabs<T> kt, SmFilter<T> fl, double inv_kli2,
                                                   double* __restrict__ part) {
     double a[4] = {};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
         T G, th, tl, lnk;
         fl.eval(kt.knorm2(ix, iy, iz), G, th, tl, lnk);
-        const c2<T> g = ghat[i], sk = Sk[i];
+        const c2<T> g = ghat[(ix * n1 + iy) * hp + iz], sk = Sk[i];
         const double c = (iz == 0 || 2 * iz == n2) ? 1.0 : 2.0;
         const double re = c * ((double)g.x * (double)sk.x + (double)g.y * (double)sk.y);
         const double Gd = (double)G;
@@ -985,14 +1083,15 @@ __global__ void __launch_bounds__(256) k_hi_wgrad(const c2<T>* __restrict__ ghat
 }
 
 // ------------------------------------------------------------ host side
-// out = scale r2c(rows of ``load``): z pass, y pass, x pass (plain kernels).
+// out (rows of pitch hp) = scale r2c(rows of ``load``): z pass, y pass, x
+// pass (plain kernels).
 template <typename T, class Load>
-static int r2c_full(const Mesh& m, const c2<T>* tw, Load load, c2<T>* out, T scale, hipStream_t st) {
+static int r2c_full(const Mesh& m, const c2<T>* tw, Load load, c2<T>* out, int64_t hp, T scale, hipStream_t st) {
     const int64_t nh = m.nh();
     int rc;
-    if ((rc = zpass_r2c<T>(m.n0 * m.n1, m.n2, tw, load, HalfRowStore<T>{out, nh, T(1)}, st))) return rc;
-    if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{out, out, T(1)}, st))) return rc;
-    return colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{out, out, scale}, st);
+    if ((rc = zpass_r2c<T>(m.n0 * m.n1, m.n2, tw, load, HalfRowStore<T>{out, hp, T(1)}, st))) return rc;
+    if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{out, out, T(1)}, st))) return rc;
+    return colpass<T, -1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{out, out, scale}, st);
 }
 
 // Second source of an HI layer from prm[5..9] = [alpha2, gamma2, kh2, kl2, nu2].
@@ -1043,17 +1142,18 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
                             (pflags & 1) ? nullptr : reinterpret_cast<unsigned*>(bins))))
         return rc;
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
-    const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hc = m.hcells();
+    // half spectra: nh columns in rows of pitch hp, hc apart
+    const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hp = spec_pitch<T>(m), hc = spec_cells<T>(m);
     c2<T>* U = reinterpret_cast<c2<T>*>(Ubuf);
     // 2. z pass with the gamma power, y pass
-    if ((rc = zpass_r2c<T>(nrows, m.n2, tw, PowRowLoad<T>{rho, m.n2, (T)prm[1]}, HalfRowStore<T>{U, nh, T(1)}, st))) return rc;
-    if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{U, U, T(1)}, st))) return rc;
+    if ((rc = zpass_r2c<T>(nrows, m.n2, tw, PowRowLoad<T>{rho, m.n2, (T)prm[1]}, HalfRowStore<T>{U, hp, T(1)}, st))) return rc;
+    if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{U, U, T(1)}, st))) return rc;
     const T invM = (T)(1.0 / (double)m.cells());
     // HI: second source S2 = r2c(starmap^gamma2) / N^3 (model.py:228)
     Src2<T> s2{};
     if (hi) {
         c2<T>* S2 = reinterpret_cast<c2<T>*>(S2buf);
-        if ((rc = r2c_full<T>(m, tw, StarRowLoad<T>{star, m.n2, (T)prm[6], 0}, S2, invM, st))) return rc;
+        if ((rc = r2c_full<T>(m, tw, StarRowLoad<T>{star, m.n2, (T)prm[6], 0}, S2, hp, invM, st))) return rc;
         s2 = hi_src2<T>(prm, S2, nullptr);
     }
     // 3. x pass + filter + three inverse x passes
@@ -1062,7 +1162,7 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     if (fft_mesh_pow2(m)) {
         dispatch_n(m.n0, [&](auto N) {
             constexpr int NN = decltype(N)::value;
-            ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
+            ColGeom g = col_geom<T, NN>(m.n1, nh, hp, m.n1 * hp);
             KTimer _t("sm_x_fwd", st);
             if (hi)
                 hipLaunchKernelGGL((k_sm_x_fwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, hc,
@@ -1072,19 +1172,31 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
                                    hc, kt, fl, a1e2, invM, s2);
         });
     } else {
-        if ((rc = colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{U, U, invM}, st))) return rc;
+        if ((rc = colpass<T, -1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{U, U, invM}, st))) return rc;
         const unsigned gf = grid_for(hc, 256, 1 << 16);
         if (hi)
-            hipLaunchKernelGGL((k_sm_filter_fwd<T, true>), dim3(gf), dim3(256), 0, st, U, P.spec3, hc, m.n1, nh, kt, fl, a1e2, s2);
+            hipLaunchKernelGGL((k_sm_filter_fwd<T, true>), dim3(gf), dim3(256), 0, st, U, P.spec3, hc, m.n1, nh, hp, kt, fl, a1e2, s2);
         else
-            hipLaunchKernelGGL((k_sm_filter_fwd<T, false>), dim3(gf), dim3(256), 0, st, U, P.spec3, hc, m.n1, nh, kt, fl, a1e2, s2);
+            hipLaunchKernelGGL((k_sm_filter_fwd<T, false>), dim3(gf), dim3(256), 0, st, U, P.spec3, hc, m.n1, nh, hp, kt, fl, a1e2, s2);
         for (int d = 0; d < 3; d++)
-            if ((rc = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{P.spec3 + d * hc, P.spec3 + d * hc, T(1)}, st))) return rc;
+            if ((rc = colpass<T, +1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{P.spec3 + d * hc, P.spec3 + d * hc, T(1)}, st))) return rc;
     }
-    // 4. inverse y passes (3 spectra as 3*N0 planes) and z c2r into F
-    if ((rc = colpass<T, +1>(m.n1, 3 * m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
-    if ((rc = zpass_c2r<T>(3 * nrows, m.n2, tw, Tri3RowLoad<T>{P.spec3, nh, hc}, Tri3Store4<T>{F, m.n2}, st, nullptr, "zpass_c2r3")))
-        return rc;
+    // 4. inverse y passes (3 force spectra as 3*N0 planes) and z c2r into F
+    if (fft_mesh_pow2(m)) {  // from the two x-pass outputs Bx (slot 0) and A (slot 1); F_y's spectrum goes to the free mesh scratch
+        c2<T>* Cy = reinterpret_cast<c2<T>*>(P.mesh3);
+        c2<T>* A = P.spec3 + hc;
+        if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
+        if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, DyColOp<T>{A, Cy, kt.D[1]}, st, "colpass3"))) return rc;
+        if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{A, P.spec3 + 2 * hc, T(1)}, st, "colpass3"))) return rc;
+        if ((rc = zpass_c2r<T>(3 * nrows, m.n2, tw, Tri3RowLoad<T>{P.spec3, Cy, P.spec3 + 2 * hc, hp, kt.D[2]}, Tri3Store4<T>{F, m.n2}, st,
+                               nullptr, "zpass_c2r3")))
+            return rc;
+    } else {
+        if ((rc = colpass<T, +1>(m.n1, 3 * m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
+        if ((rc = zpass_c2r<T>(3 * nrows, m.n2, tw, Tri3RowLoad<T>{P.spec3, P.spec3 + hc, P.spec3 + 2 * hc, hp, nullptr},
+                               Tri3Store4<T>{F, m.n2}, st, nullptr, "zpass_c2r3")))
+            return rc;
+    }
     // 5. readout + position update
     T s[3];
     for (int d = 0; d < 3; d++) s[d] = pdesc->kind == LDL_POS_LATTICE ? pd.cellinv[d] : T(1);
@@ -1119,7 +1231,7 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     const SmPtrs<T> P = sm_ptrs<T>(ws, w);
     const c2<T>* tw;
     if ((rc = twiddles<T>(&tw))) return rc;
-    const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hc = m.hcells(), cells = m.cells();
+    const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hp = spec_pitch<T>(m), hc = spec_cells<T>(m), cells = m.cells();
     // 1. gF_d = paint(X, g_d)  (3 meshes in mesh3)
     const bool reuse = bins && !(pflags & (LDL_PAINT_FLOAT_ATOMIC | LDL_PAINT_DIRECT));
     if ((rc = paint_impl<T>(pos, pdesc, np, gout, 0.0, 3, nmesh, 1.0, pflags, P.mesh3, P.paint, w.paint, st, nullptr,
@@ -1127,15 +1239,30 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
         return rc;
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
     // 2. unnormalised r2c of the three meshes: z and y passes
-    if ((rc = zpass_r2c<T>(3 * nrows, m.n2, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore<T>{P.spec3, nh, T(1)}, st, "zpass_r2c3")))
+    // (power-of-two meshes: the z component times i D_z(k_z) as it is stored,
+    // and the y pass folds the y and z components into one spectrum)
+    const bool p2 = fft_mesh_pow2(m);
+    if ((rc = zpass_r2c<T>(3 * nrows, m.n2, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore3<T>{P.spec3, hp, nrows, p2 ? kt.D[2] : nullptr},
+                           st, "zpass_r2c3")))
         return rc;
-    if ((rc = colpass<T, -1>(m.n1, 3 * m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
+    if (p2) {
+        if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st))) return rc;
+        dispatch_n(m.n1, [&](auto N) {
+            constexpr int NN = decltype(N)::value;
+            const ColGeom g = col_geom<T, NN>(m.n0, nh, m.n1 * hp, hp);
+            KTimer _t("y_fwd_sum2", st);
+            hipLaunchKernelGGL((k_y_fwd_sum2<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3 + hc,
+                               P.spec3 + 2 * hc, kt.D[1]);
+        });
+    } else if ((rc = colpass<T, -1>(m.n1, 3 * m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) {
+        return rc;
+    }
     // HI: T2 = r2c(starmap^gamma2 ln starmap) / N^3 for the gamma2 partial, in
-    // the mesh scratch (free after the z pass above; 2 Nh <= 3 N^3 elements)
+    // the mesh scratch (free after the z pass above; sized to hold it)
     Src2<T> s2{};
     if (hi) {
         c2<T>* T2 = reinterpret_cast<c2<T>*>(P.mesh3);
-        if ((rc = r2c_full<T>(m, tw, StarRowLoad<T>{star, m.n2, (T)prm[6], 1}, T2, (T)(1.0 / (double)cells), st))) return rc;
+        if ((rc = r2c_full<T>(m, tw, StarRowLoad<T>{star, m.n2, (T)prm[6], 1}, T2, hp, (T)(1.0 / (double)cells), st))) return rc;
         s2 = hi_src2<T>(prm, reinterpret_cast<const c2<T>*>(S2buf), T2);
     }
     // 3. x pass: Q, parameter partials, inverse x of cpot alpha G Q
@@ -1150,7 +1277,7 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     if (fft_mesh_pow2(m)) {
         dispatch_n(m.n0, [&](auto N) {
             constexpr int NN = decltype(N)::value;
-            ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
+            ColGeom g = col_geom<T, NN>(m.n1, nh, hp, m.n1 * hp);
             ntx = g.ntiles;
             KTimer _t("sm_x_bwd", st);
             if (hi)
@@ -1162,26 +1289,26 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
         });
     } else {
         for (int d = 0; d < 3; d++)
-            if ((rc = colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{P.spec3 + d * hc, P.spec3 + d * hc, T(1)}, st))) return rc;
+            if ((rc = colpass<T, -1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{P.spec3 + d * hc, P.spec3 + d * hc, T(1)}, st))) return rc;
         const unsigned gf = grid_for(hc, 256, 4096);
         ntx = gf;
         if (hi)
-            hipLaunchKernelGGL((k_sm_filter_bwd<T, true>), dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, m.n2, kt, fl, a1e2, gkh,
-                               gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
+            hipLaunchKernelGGL((k_sm_filter_bwd<T, true>), dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, hp, m.n2, kt, fl, a1e2,
+                               gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
         else
-            hipLaunchKernelGGL((k_sm_filter_bwd<T, false>), dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, m.n2, kt, fl, a1e2, gkh,
-                               gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
-        if ((rc = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc;
+            hipLaunchKernelGGL((k_sm_filter_bwd<T, false>), dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, hp, m.n2, kt, fl, a1e2,
+                               gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
+        if ((rc = colpass<T, +1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc;
     }
     // partial columns -> parameter slots: alpha, kh, kl, nu (HI: then alpha2,
     // kh2, kl2, nu2, gamma2)
     const RedMap rx = hi ? RedMap{9, {0, 2, 3, 4, 5, 7, 8, 9, 6}} : RedMap{4, {0, 2, 3, 4}};
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, ntx, rx.n, rx, grad, 0);
     // 4. inverse y, z c2r with the gamma-power adjoint -> g_rho (mesh3[0])
-    if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc;
+    if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc;
     T* grho = P.mesh3;
     int64_t nzb = 0;
-    if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, nh},
+    if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
                            GammaBwdStore<T>{rho, grho, m.n2, (T)(1.0 / (double)cells), (T)prm[1], P.part}, st, &nzb)))
         return rc;
     RedMap rz{1, {1}};
@@ -1254,43 +1381,43 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
         const c2<T>* tw;
         if ((rc = twiddles<T>(&tw))) return rc;
         const KTabs<T> kt = build_tables<T>(P.tabs, m, box, st);
-        const int64_t nrows = m.n0 * m.n1, nh = m.nh();
+        const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hp = spec_pitch<T>(m);
         T* diff = P.mesh3;
         T* gds = P.mesh3 + cells;
         c2<T>* S = P.spec3;
         const T invM = (T)(1.0 / (double)cells);
         auto smooth = [&](const T* src) -> int {  // S = W(k) r2c(src) / M, inverse x and y passes
             int rc2;
-            if ((rc2 = zpass_r2c<T>(nrows, m.n2, tw, RealRowLoad<T>{src, m.n2}, HalfRowStore<T>{S, nh, T(1)}, st))) return rc2;
-            if ((rc2 = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
+            if ((rc2 = zpass_r2c<T>(nrows, m.n2, tw, RealRowLoad<T>{src, m.n2}, HalfRowStore<T>{S, hp, T(1)}, st))) return rc2;
+            if ((rc2 = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
             if (fft_mesh_pow2(m)) {
                 dispatch_n(m.n0, [&](auto N) {
                     constexpr int NN = decltype(N)::value;
-                    ColGeom g = col_geom<T, NN>(m.n1, nh, nh, m.n1 * nh);
+                    ColGeom g = col_geom<T, NN>(m.n1, nh, hp, m.n1 * hp);
                     KTimer _t("x_realfilter", st);
                     hipLaunchKernelGGL((k_x_realfilter<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, S, kt,
                                        (int)smooth_kind, (T)smooth_par, invM);
                 });
             } else {
-                const int64_t hc = m.hcells();
-                if ((rc2 = colpass<T, -1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
-                hipLaunchKernelGGL(k_realfilter_mult<T>, dim3(grid_for(hc, 256, 1 << 16)), dim3(256), 0, st, S, hc, m.n1, nh, kt, (int)smooth_kind,
+                const int64_t hc = spec_cells<T>(m);
+                if ((rc2 = colpass<T, -1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
+                hipLaunchKernelGGL(k_realfilter_mult<T>, dim3(grid_for(hc, 256, 1 << 16)), dim3(256), 0, st, S, hc, m.n1, nh, hp, kt, (int)smooth_kind,
                                    (T)smooth_par, invM);
-                if ((rc2 = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
+                if ((rc2 = colpass<T, +1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
             }
-            return colpass<T, +1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{S, S, T(1)}, st);
+            return colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{S, S, T(1)}, st);
         };
         const int64_t nzb = zpass_c2r_blocks<T>(nrows, m.n2);
         // every block writes the loss columns; the gradient columns stay zero without g_rho
         hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nzb * kLossCols, 256)), dim3(256), 0, st, bpart, nzb * kLossCols, 0.0);
         hipLaunchKernelGGL(k_loss_diff<T>, dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff);
         if ((rc = smooth(diff))) return rc;
-        if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, nh},
+        if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
                                LossStore<T>{mask, maskv, grho ? gds : nullptr, npix, m.n2, T(1), p_norm == 1, bpart}, st, nullptr)))
             return rc;
         if (grho) {
             if ((rc = smooth(gds))) return rc;
-            if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, nh}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart}, st, nullptr)))
+            if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart}, st, nullptr)))
                 return rc;
         }
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, nzb, kLossCols, r6, sums, 1);
@@ -1316,9 +1443,9 @@ static int hi_dep_setup(const T* star, const c2<T>* Sk, const int64_t nmesh[3], 
     if ((rc = twiddles<T>(tw))) return rc;
     *kt = build_tables<T>(P->tabs, *m, box, st);
     *sk = Sk;
-    if (!Sk) {  // Sk = r2c(starmap) / N^3 into the second spectrum slot
-        c2<T>* o = P->spec3 + m->hcells();
-        if ((rc = r2c_full<T>(*m, *tw, RealRowLoad<T>{star, m->n2}, o, (T)(1.0 / (double)m->cells()), st))) return rc;
+    if (!Sk) {  // Sk = r2c(starmap) / N^3 (unpadded rows, the caller's layout) into the second spectrum slot
+        c2<T>* o = P->spec3 + spec_cells<T>(*m);
+        if ((rc = r2c_full<T>(*m, *tw, RealRowLoad<T>{star, m->n2}, o, m->nh(), (T)(1.0 / (double)m->cells()), st))) return rc;
         *sk = o;
     }
     return LDL_OK;
@@ -1335,14 +1462,14 @@ static int hi_depletion_fwd(const T* star, const c2<T>* Sk, const int64_t nmesh[
     int rc = hi_dep_setup<T>(star, Sk, nmesh, box, prm, ws, wsb, st, &m, &P, &tw, &kt, &sk);
     if (rc) return rc;
     if (!S || !dep) return fail(LDL_EINVAL, "NULL array argument");
-    const int64_t nh = m.nh(), hc = m.hcells();
+    const int64_t nh = m.nh(), hp = spec_pitch<T>(m), hc = m.hcells();
     const SmFilter<T> fl{(T)prm[1], (T)(1.0 / prm[2]), (T)prm[3], 1};
     c2<T>* W = P.spec3;
-    hipLaunchKernelGGL(k_hi_wmult<T>, dim3(grid_for(hc, 256, 1 << 16)), dim3(256), 0, st, sk, W, hc, m.n1, nh, kt, fl, (T)prm[6]);
-    if ((rc = colpass<T, +1>(m.n0, m.n1, nh, nh, m.n1 * nh, tw, PlainColOp<T>{W, W, T(1)}, st))) return rc;
-    if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * nh, nh, tw, PlainColOp<T>{W, W, T(1)}, st))) return rc;
+    hipLaunchKernelGGL(k_hi_wmult<T>, dim3(grid_for(hc, 256, 1 << 16)), dim3(256), 0, st, sk, W, hc, m.n1, nh, hp, kt, fl, (T)prm[6]);
+    if ((rc = colpass<T, +1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{W, W, T(1)}, st))) return rc;
+    if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{W, W, T(1)}, st))) return rc;
     const DepPar<T> dp{(T)prm[0], (T)prm[4], (T)prm[5], (T)prm[7]};
-    if ((rc = zpass_c2r<T>(m.n0 * m.n1, m.n2, tw, HalfRowLoad<T>{W, nh}, DepStore<T>{star, S, dep, m.n2, dp}, st, nullptr))) return rc;
+    if ((rc = zpass_c2r<T>(m.n0 * m.n1, m.n2, tw, HalfRowLoad<T>{W, hp}, DepStore<T>{star, S, dep, m.n2, dp}, st, nullptr))) return rc;
     LDL_LAUNCH_CHECK();
     return LDL_OK;
 }
@@ -1358,7 +1485,7 @@ static int hi_depletion_bwd(const T* star, const c2<T>* Sk, const T* S, const T*
     int rc = hi_dep_setup<T>(star, Sk, nmesh, box, prm, ws, wsb, st, &m, &P, &tw, &kt, &sk);
     if (rc) return rc;
     if (!S || !gdep || !grad8) return fail(LDL_EINVAL, "NULL array argument");
-    const int64_t nh = m.nh(), hc = m.hcells(), cells = m.cells();
+    const int64_t nh = m.nh(), hp = spec_pitch<T>(m), hc = m.hcells(), cells = m.cells();
     const DepPar<T> dp{(T)prm[0], (T)prm[4], (T)prm[5], (T)prm[7]};
     T* gS = P.mesh3;
     const unsigned gb = 2048;
@@ -1366,10 +1493,10 @@ static int hi_depletion_bwd(const T* star, const c2<T>* Sk, const T* S, const T*
     const RedMap re{4, {0, 4, 5, 7}};
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, (int64_t)gb, 4, re, grad8, 1);
     c2<T>* gh = P.spec3;
-    if ((rc = r2c_full<T>(m, tw, RealRowLoad<T>{gS, m.n2}, gh, T(1), st))) return rc;
+    if ((rc = r2c_full<T>(m, tw, RealRowLoad<T>{gS, m.n2}, gh, hp, T(1), st))) return rc;
     const SmFilter<T> fl{(T)prm[1], (T)(1.0 / prm[2]), (T)prm[3], 1};
     const unsigned gf = grid_for(hc, 256, 4096);
-    hipLaunchKernelGGL(k_hi_wgrad<T>, dim3(gf), dim3(256), 0, st, gh, sk, hc, m.n1, nh, m.n2, kt, fl, 1.0 / (prm[2] * prm[2]), P.part);
+    hipLaunchKernelGGL(k_hi_wgrad<T>, dim3(gf), dim3(256), 0, st, gh, sk, hc, m.n1, nh, hp, m.n2, kt, fl, 1.0 / (prm[2] * prm[2]), P.part);
     const RedMap rw{4, {1, 2, 3, 6}};
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, (int64_t)gf, 4, rw, grad8, 1);
     LDL_LAUNCH_CHECK();
@@ -1405,6 +1532,12 @@ size_t ldl_sm_bins_bytes(const int64_t nmesh[3], int64_t np) {
     Mesh m;
     if (check_mesh(nmesh, &m) || np < 0) return 0;
     return paint_bins_bytes(m, np);
+}
+
+int64_t ldl_sm_spectrum_pitch(const int64_t nmesh[3], int32_t is_f64) {
+    Mesh m;
+    if (check_fft_mesh(nmesh, &m)) return 0;
+    return is_f64 ? spec_pitch<double>(m) : spec_pitch<float>(m);
 }
 
 size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np) {

@@ -143,7 +143,8 @@ class JERALDModel:
         source spectrum) is None for the other kinds."""
         dt, dev, nm = self.dtype, self.device, self.nm
         rho = torch.empty(nm, dtype=dt, device=dev)
-        U = torch.empty(nm[0], nm[1], nm[2] // 2 + 1, 2, dtype=dt, device=dev)
+        hp = L.load().ldl_sm_spectrum_pitch(L.i64x3(nm), int(dt == torch.float64))  # 64-byte rows
+        U = torch.empty(nm[0], nm[1], hp, 2, dtype=dt, device=dev)
         F = torch.empty(nm + (4,), dtype=dt, device=dev)  # forces, interleaved (x, y, z, unused) per cell
         out = torch.empty_like(P)
         bins = None

@@ -37,6 +37,11 @@ def test_workspace_queries():
     assert lib.ldl_paint_workspace_bytes(nm, 1, -1) == 0
     assert lib.ldl_fft_workspace_bytes(nm, 0) == 64 * 64 * 33 * 8
     assert lib.ldl_sm_workspace_bytes(nm, 1, 1000) > lib.ldl_sm_workspace_bytes(nm, 0, 1000) > 0
+    # spectrum rows padded to 64 bytes: 16^3 -> 9 complex -> 16 (complex64), 12 (complex128)
+    assert lib.ldl_sm_spectrum_pitch(L.i64x3((16, 16, 16)), 0) == 16
+    assert lib.ldl_sm_spectrum_pitch(L.i64x3((16, 16, 16)), 1) == 12
+    assert lib.ldl_sm_spectrum_pitch(L.i64x3((8, 8, 512)), 0) == 264
+    assert lib.ldl_sm_spectrum_pitch(L.i64x3((8, 8, 14)), 0) == 0
 
 
 @pytest.mark.parametrize("shape", [(14, 8, 8), (8, 8, 7), (8, 8, 15), (2048, 8, 8), (8, 8, 22), (3, 8, 8), (8, 1100, 8)])
