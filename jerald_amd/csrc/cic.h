@@ -144,6 +144,25 @@ __global__ void __launch_bounds__(256) k_absmax(const T* __restrict__ mass, int6
     }
 }
 
+// max |w| * scale over a mesh into *slot as double bits: for a 3-component
+// paint of the positions whose unit-mass density (times 1/scale) is w, the
+// largest sum of CIC weights any cell receives.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mesh_absmax(const T* __restrict__ w, int64_t n, double scale, unsigned long long* __restrict__ slot) {
+    T mx = T(0);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) mx = fmax(mx, fabs(w[i]));
+    double v = (double)mx * scale;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __shared__ double wm[4];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+        if (b > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
 // Fixed-point deposit.  acc has NC consecutive meshes.
 template <typename T, int NC>
 __global__ void __launch_bounds__(256) k_paint_fixed(const T* __restrict__ pos, PosDec<T> pd, int64_t np,
@@ -745,14 +764,22 @@ static __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const un
 // into it (the face classes holding those neighbours' upper faces); every
 // particle deposits only its corners that lie in the tile.  A particle read by
 // several tiles computes the same 8 integer deposits in each.
-template <typename T, int NC, int TS, int TB, bool P2>
+//
+// ACC32 (FP32 only): 32-bit accumulators, half the LDS.  hdr[3] then holds W,
+// the largest CIC weight sum of any cell, and every shift drops by g with
+// 2^g > 2 W + 1, so a cell's |sum| stays below 2^29 plus 4 per contribution:
+// the int32 sums equal the int64 ones of the same deposits.
+template <typename T, int NC, int TS, int TB, bool P2, bool ACC32 = false>
 __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ mass,
                                                      T mscalar, Mesh m, MeshP2 mp, TileGeom tg, const unsigned* __restrict__ offsets,
                                                      const unsigned* __restrict__ sorted, const unsigned long long* __restrict__ hdr,
                                                      double scale, T* __restrict__ out) {
+    static_assert(!ACC32 || sizeof(T) == 4, "32-bit accumulation takes the 32-bit FP32 deposits");
+    using Acc = typename std::conditional<ACC32, unsigned int, unsigned long long>::type;
+    using SAcc = typename std::conditional<ACC32, int, long long>::type;
     constexpr int B3 = TS * TS * TS;
     constexpr int NR = 20;  // 1 own range + 12 + 6 + 1 neighbour ranges
-    __shared__ unsigned long long box[NC * B3];
+    __shared__ Acc box[NC * B3];
     __shared__ unsigned rstart[NR], rpre[NR + 1];
     __shared__ unsigned char rcode[NR];
     // XCD-contiguous tile order: the tiles an XCD runs together are z-neighbours,
@@ -761,7 +788,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     const int64_t tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
     const int64_t lo[3] = {tx * TS, ty * TS, tz * TS};
     const int ex = (int)tg.extent(0, tx, m), ey = (int)tg.extent(1, ty, m), ez = (int)tg.extent(2, tz, m);
-    for (int i = threadIdx.x; i < NC * B3; i += TB) box[i] = 0ull;
+    for (int i = threadIdx.x; i < NC * B3; i += TB) box[i] = Acc(0);
     if (threadIdx.x < 64) {
         // range r (lane r < NR): r = 0 is the own tile (all face classes); the
         // others are (dir, fc) pairs -- bit d of dir: the tile one lower along
@@ -799,11 +826,16 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
             if (r == NR - 1) rpre[NR] = inc;
         }
     }
+    int g = 0;
+    if constexpr (ACC32) {
+        const double W = __longlong_as_double((long long)hdr[3]);
+        g = ilogb(2.0 * W + 1.0) + 1;  // 2^g > 2 W + 1
+    }
     int s[NC];
     double fs[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-        s[c] = fixed_shift<T>(hdr[c], np);
+        s[c] = fixed_shift<T>(hdr[c], np) - g;
         fs[c] = ldexp(scale, -s[c]);
     }
     __syncthreads();
@@ -869,7 +901,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
 #pragma unroll
                     for (int qq = 0; qq < 8; qq++) {
                         const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
-                        atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (unsigned long long)(long long)q[qq]);
+                        atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (Acc)(SAcc)q[qq]);
                     }
                 }
             } else {
@@ -881,7 +913,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
                     for (int qq = 0; qq < 8; qq++) {
                         const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
                         if (in_x[dx] && in_y[dy] && in_z[dz] && q[qq] != 0)
-                            atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (unsigned long long)(long long)q[qq]);
+                            atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (Acc)(SAcc)q[qq]);
                     }
                 }
             }
@@ -892,9 +924,9 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     for (int li = threadIdx.x; li < B3; li += TB) {
         const int cc = li % TS, bb = (li / TS) % TS, aa = li / (TS * TS);
         if (aa >= ex || bb >= ey || cc >= ez) continue;
-        const int64_t g = ((lo[0] + aa) * m.n1 + (lo[1] + bb)) * m.n2 + (lo[2] + cc);
+        const int64_t gi = ((lo[0] + aa) * m.n1 + (lo[1] + bb)) * m.n2 + (lo[2] + cc);
 #pragma unroll
-        for (int c = 0; c < NC; c++) out[c * cells + g] = (T)((double)(long long)box[c * B3 + li] * fs[c]);
+        for (int c = 0; c < NC; c++) out[c * cells + gi] = (T)((double)(SAcc)box[c * B3 + li] * fs[c]);
     }
 }
 
@@ -903,7 +935,8 @@ size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np);
 template <typename T>
 int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,
                const int64_t nmesh[3], double out_scale, int32_t flags, T* out, void* ws, size_t ws_bytes,
-               hipStream_t st, unsigned* bins_out = nullptr, const unsigned* bins_in = nullptr);
+               hipStream_t st, unsigned* bins_out = nullptr, const unsigned* bins_in = nullptr, const T* wmesh = nullptr,
+               double wscale = 0.0);
 // Particle sort by 16^3-cell tile and face class that a 1-component paint can
 // hand to a later 3-component paint of the same positions: offsets
 // (8 ntiles + 1) then sorted particle indices (np), uint32.

@@ -55,7 +55,7 @@ size_t paint_bins_bytes(const Mesh& m, int64_t np) {
 template <typename T>
 int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,
                const int64_t nmesh[3], double out_scale, int32_t flags, T* out, void* ws, size_t ws_bytes,
-               hipStream_t st, unsigned* bins_out, const unsigned* bins_in) {
+               hipStream_t st, unsigned* bins_out, const unsigned* bins_in, const T* wmesh, double wscale) {
     Mesh m;
     int rc = check_mesh(nmesh, &m);
     if (rc) return rc;
@@ -168,6 +168,21 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         else
             hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide, 256, false>), gt, dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
                                tg, offsets, sorted, hdr, out_scale, out);
+    } else if (sizeof(T) == 4 && wmesh) {
+        // FP32 with the positions' unit-mass density known: 32-bit accumulators
+        {
+            KTimer _t("paint_absmax", st);
+            hipLaunchKernelGGL(k_mesh_absmax<T>, dim3(grid_for(cells, kBlock, 2048)), dim3(kBlock), 0, st, wmesh, cells, wscale, hdr + 3);
+        }
+        KTimer _t("paint_deposit3", st);
+        if constexpr (sizeof(T) == 4) {
+            if (p2)
+                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, 1024, true, true>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m,
+                                   mp, tg, offsets, sorted, hdr, out_scale, out);
+            else
+                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, 1024, false, true>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m,
+                                   mp, tg, offsets, sorted, hdr, out_scale, out);
+        }
     } else {
         KTimer _t("paint_deposit3", st);
         if (p2)
@@ -224,8 +239,10 @@ int readout_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* f
     return LDL_OK;
 }
 
-template int paint_impl<float>(const float*, const ldl_pos_desc*, int64_t, const float*, double, int, const int64_t*, double, int32_t, float*, void*, size_t, hipStream_t, unsigned*, const unsigned*);
-template int paint_impl<double>(const double*, const ldl_pos_desc*, int64_t, const double*, double, int, const int64_t*, double, int32_t, double*, void*, size_t, hipStream_t, unsigned*, const unsigned*);
+template int paint_impl<float>(const float*, const ldl_pos_desc*, int64_t, const float*, double, int, const int64_t*, double, int32_t, float*, void*,
+                               size_t, hipStream_t, unsigned*, const unsigned*, const float*, double);
+template int paint_impl<double>(const double*, const ldl_pos_desc*, int64_t, const double*, double, int, const int64_t*, double, int32_t, double*,
+                                void*, size_t, hipStream_t, unsigned*, const unsigned*, const double*, double);
 template int readout_impl<float>(const float*, const ldl_pos_desc*, int64_t, const float*, const int64_t*, int, int, const float*, float*, hipStream_t);
 template int readout_impl<double>(const double*, const ldl_pos_desc*, int64_t, const double*, const int64_t*, int, int, const double*, double*, hipStream_t);
 

@@ -1234,8 +1234,9 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     const int64_t nrows = m.n0 * m.n1, nh = m.nh(), hp = spec_pitch<T>(m), hc = spec_cells<T>(m), cells = m.cells();
     // 1. gF_d = paint(X, g_d)  (3 meshes in mesh3)
     const bool reuse = bins && !(pflags & (LDL_PAINT_FLOAT_ATOMIC | LDL_PAINT_DIRECT));
+    // (the forward density rho = fact * sum of CIC weights bounds the cells' sums)
     if ((rc = paint_impl<T>(pos, pdesc, np, gout, 0.0, 3, nmesh, 1.0, pflags, P.mesh3, P.paint, w.paint, st, nullptr,
-                            reuse ? reinterpret_cast<const unsigned*>(bins) : nullptr)))
+                            reuse ? reinterpret_cast<const unsigned*>(bins) : nullptr, rho, 1.0 / fact)))
         return rc;
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
     // 2. unnormalised r2c of the three meshes: z and y passes

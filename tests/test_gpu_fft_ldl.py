@@ -239,6 +239,29 @@ def test_ldl_fp32_within_1e5_of_fp64():
     assert abs(l64 - loss0) <= 1e-10 * abs(loss0)
 
 
+def test_fp32_clustered_particles():
+    """Most particles in a few cells (cell weight sums in the hundreds): the
+    FP32 backward paint's 32-bit accumulators, scaled by the forward density's
+    maximum, must not overflow; FP32 loss and gradients follow FP64."""
+    N, L, nstep = 16, 64.0, 2
+    rng = np.random.default_rng(5)
+    X = O.lattice_positions(N, L)
+    X[: 3 * len(X) // 4] = 20.0 + rng.random((3 * len(X) // 4, 3)) * 6.0  # 3/4 of the particles in ~2^3 cells
+    tgt = rng.random((N, N, N)) + 0.5
+    p = np.array([0.8, 1.2, 0.3, 1.5, 0.2] * nstep + [1.1, 0.9, 0.05])
+    out = {}
+    for dt in (torch.float64, torch.float32):
+        m = _model(X, tgt, None, N, L, nstep, dt, L1=False, smoothing=("gaussian", 4.0))
+        loss, grad = m.loss_and_grad(torch.tensor(p))
+        out[dt] = (loss.item(), grad.cpu().numpy())
+    (l64, g64), (l32, g32) = out[torch.float64], out[torch.float32]
+    assert abs(l32 - l64) <= 1e-5 * abs(l64)
+    assert np.all(np.abs(g32 - g64) <= 1e-4 * np.abs(g64) + 1e-6 * np.abs(g64).max()), (g32, g64)
+    orc = O.SMModelOracle(N, L, nstep, tgt, L1=False, smoothing=("gaussian", 4.0))
+    loss0, g0 = orc.loss_and_grad(p, X)
+    np.testing.assert_allclose(g64, g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
+
+
 @pytest.mark.parametrize("dt", [torch.float32, torch.float64])
 def test_sort_reuse_is_bit_identical(dt):
     """The backward paint on the forward's 16^3-tile sort gives the same
