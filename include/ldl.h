@@ -196,17 +196,21 @@ int ldl_sm_layer_fwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
  * g_in (Np,3) = dL/dX_in (may alias g_out) and ACCUMULATES dL/d{alpha,
  * gamma, kh, kl, nu} into grad5 (DEVICE double[5]).  ``alpha_rule`` 0 = exact
  * gradient, 1 = the reference's rule (interior k_z modes weighted 1 instead
- * of 2 in dL/dalpha, see jaxpops_bind.py:758-759 and model.py:198). */
+ * of 2 in dL/dalpha, see jaxpops_bind.py:758-759 and model.py:198).
+ * gmax (DEVICE double[3], or NULL): on entry max |g_out[:, c]| per component
+ * as the previous call wrote it (ldl_paint_vjp_pos or the backward of the
+ * layer above), which spares the backward's 3-component paint its own scan
+ * of g_out; on return max |g_in[:, c]|. */
 int ldl_sm_layer_bwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[5], double fact, int32_t alpha_rule,
                          int32_t paint_flags, const float* rho, const float* U, const float* F,
                          const void* bins, const float* g_out, float* g_in, double* grad5,
-                         void* workspace, size_t workspace_bytes, void* stream);
+                         double* gmax, void* workspace, size_t workspace_bytes, void* stream);
 int ldl_sm_layer_bwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[5], double fact, int32_t alpha_rule,
                          int32_t paint_flags, const double* rho, const double* U, const double* F,
                          const void* bins, const double* g_out, double* g_in, double* grad5,
-                         void* workspace, size_t workspace_bytes, void* stream);
+                         double* gmax, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Response + loss + their gradient (model.py:338-344, :350-403 / :415-468).
  * rho (N0,N1,N2) final density (already times fact); prm3 = {mu, w, b}.
@@ -248,12 +252,12 @@ int ldl_dm_layer_bwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[5], double fact, int32_t alpha_rule,
                          int32_t paint_flags, const float* rho, const float* U, const float* F,
                          const void* bins, const float* g_out, float* g_in, double* grad5,
-                         void* workspace, size_t workspace_bytes, void* stream);
+                         double* gmax, void* workspace, size_t workspace_bytes, void* stream);
 int ldl_dm_layer_bwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[5], double fact, int32_t alpha_rule,
                          int32_t paint_flags, const double* rho, const double* U, const double* F,
                          const void* bins, const double* g_out, double* g_in, double* grad5,
-                         void* workspace, size_t workspace_bytes, void* stream);
+                         double* gmax, void* workspace, size_t workspace_bytes, void* stream);
 int ldl_dm_loss_f32(const float* rho, const float* target, const float* mask, const float* mask_valid,
                     const float* field2, const int64_t nmesh[3], const double box[3], double index,
                     int32_t p_norm, int32_t smooth_kind, double smooth_par, double* out5, float* g_rho,
@@ -285,12 +289,12 @@ int ldl_hi_layer_bwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[10], double fact, int32_t alpha_rule,
                          int32_t paint_flags, const float* starmap, const float* rho, const float* U,
                          const float* S2, const float* F, const void* bins, const float* g_out, float* g_in,
-                         double* grad10, void* workspace, size_t workspace_bytes, void* stream);
+                         double* grad10, double* gmax, void* workspace, size_t workspace_bytes, void* stream);
 int ldl_hi_layer_bwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[10], double fact, int32_t alpha_rule,
                          int32_t paint_flags, const double* starmap, const double* rho, const double* U,
                          const double* S2, const double* F, const void* bins, const double* g_out,
-                         double* g_in, double* grad10, void* workspace, size_t workspace_bytes, void* stream);
+                         double* g_in, double* grad10, double* gmax, void* workspace, size_t workspace_bytes, void* stream);
 
 /* HI depletion map (model.py:326-332), prm8 = {beta1, khi, kli, nui, beta2,
  * gamma3, xi, eta} (model.py:322-331):
@@ -334,11 +338,12 @@ int ldl_hi_loss_f64(const double* rho, const double* target, const double* mask,
 
 /* Final-layer adjoint gather (fused readout-gradient): g (Np,3) =
  * fact * grad(interp g_rho) at pos  (the ppaint VJP with unit mass,
- * jaxpops_bind.py:242-247). */
+ * jaxpops_bind.py:242-247).  gmax (DEVICE double[3], or NULL) receives
+ * max |g[:, c]| per component for the next layer backward. */
 int ldl_paint_vjp_pos_f32(const float* pos, const ldl_pos_desc* pd, int64_t np, const float* g_rho,
-                          const int64_t nmesh[3], double fact, float* g, void* stream);
+                          const int64_t nmesh[3], double fact, float* g, double* gmax, void* stream);
 int ldl_paint_vjp_pos_f64(const double* pos, const ldl_pos_desc* pd, int64_t np, const double* g_rho,
-                          const int64_t nmesh[3], double fact, double* g, void* stream);
+                          const int64_t nmesh[3], double fact, double* g, double* gmax, void* stream);
 
 #ifdef __cplusplus
 }

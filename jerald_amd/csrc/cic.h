@@ -59,6 +59,10 @@ __device__ __forceinline__ void deposits(const T f[3], T m, int s, DepositT<T> q
     q[7] = tot - part;
 }
 
+static __global__ void k_copy_u64(const unsigned long long* __restrict__ src, unsigned long long* __restrict__ dst, int64_t n) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
 static __global__ void k_fill_u64(unsigned long long* __restrict__ p, int64_t n, unsigned long long v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -936,7 +940,7 @@ template <typename T>
 int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,
                const int64_t nmesh[3], double out_scale, int32_t flags, T* out, void* ws, size_t ws_bytes,
                hipStream_t st, unsigned* bins_out = nullptr, const unsigned* bins_in = nullptr, const T* wmesh = nullptr,
-               double wscale = 0.0);
+               double wscale = 0.0, const unsigned long long* absmax_in = nullptr);
 // Particle sort by 16^3-cell tile and face class that a 1-component paint can
 // hand to a later 3-component paint of the same positions: offsets
 // (8 ntiles + 1) then sorted particle indices (np), uint32.

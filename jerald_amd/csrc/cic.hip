@@ -55,7 +55,8 @@ size_t paint_bins_bytes(const Mesh& m, int64_t np) {
 template <typename T>
 int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,
                const int64_t nmesh[3], double out_scale, int32_t flags, T* out, void* ws, size_t ws_bytes,
-               hipStream_t st, unsigned* bins_out, const unsigned* bins_in, const T* wmesh, double wscale) {
+               hipStream_t st, unsigned* bins_out, const unsigned* bins_in, const T* wmesh, double wscale,
+               const unsigned long long* absmax_in) {
     Mesh m;
     int rc = check_mesh(nmesh, &m);
     if (rc) return rc;
@@ -94,7 +95,9 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         std::memcpy(&hbits, &a, 8);
     }
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, hdr, (int64_t)(kHdrBytes / 8), hbits);
-    if (mass && np > 0) {
+    if (mass && absmax_in) {  // max |m| per component handed over by the kernel that wrote the masses
+        hipLaunchKernelGGL(k_copy_u64, dim3(1), dim3(64), 0, st, absmax_in, hdr, (int64_t)nc);
+    } else if (mass && np > 0) {
         KTimer _t("paint_absmax", st);
         if (nc == 1) hipLaunchKernelGGL((k_absmax<T, 1>), dim3(grid_for(np, kBlock, 4096)), dim3(kBlock), 0, st, mass, np, hdr);
         else hipLaunchKernelGGL((k_absmax<T, 3>), dim3(grid_for(np, kBlock, 4096)), dim3(kBlock), 0, st, mass, np, hdr);
@@ -240,9 +243,9 @@ int readout_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* f
 }
 
 template int paint_impl<float>(const float*, const ldl_pos_desc*, int64_t, const float*, double, int, const int64_t*, double, int32_t, float*, void*,
-                               size_t, hipStream_t, unsigned*, const unsigned*, const float*, double);
+                               size_t, hipStream_t, unsigned*, const unsigned*, const float*, double, const unsigned long long*);
 template int paint_impl<double>(const double*, const ldl_pos_desc*, int64_t, const double*, double, int, const int64_t*, double, int32_t, double*,
-                                void*, size_t, hipStream_t, unsigned*, const unsigned*, const double*, double);
+                                void*, size_t, hipStream_t, unsigned*, const unsigned*, const double*, double, const unsigned long long*);
 template int readout_impl<float>(const float*, const ldl_pos_desc*, int64_t, const float*, const int64_t*, int, int, const float*, float*, hipStream_t);
 template int readout_impl<double>(const double*, const ldl_pos_desc*, int64_t, const double*, const int64_t*, int, int, const double*, double*, hipStream_t);
 

@@ -697,11 +697,38 @@ __global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ p
     }
 }
 
-// g_in = g + sum_d g_d grad F_d + fact grad g_rho  (box units)
+// Per-component max |v| of the values a workgroup wrote, into gmax (double
+// bits, one atomic per component and workgroup): the scale the next
+// 3-component paint of these values needs, equal to what its own max scan
+// (k_absmax) would find.
+template <typename T>
+__device__ __forceinline__ void block_absmax3(const T (&mx)[3], unsigned long long* gmax) {
+    __shared__ double wm[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double v = (double)mx[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0) wm[c][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        // the maxima of the many workgroups are alike: an atomic only when this
+        // one exceeds the current value keeps the three addresses uncontended
+        const int c = threadIdx.x;
+        const double v = fmax(fmax(wm[c][0], wm[c][1]), fmax(wm[c][2], wm[c][3]));
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        if (v > 0.0 && b > *reinterpret_cast<volatile unsigned long long*>(&gmax[c])) atomicMax(&gmax[c], b);
+    }
+}
+
+// g_in = g + sum_d g_d grad F_d + fact grad g_rho  (box units); with gmax,
+// also the per-component max |g_in|.
 template <typename T, typename G>
 __global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ F4,
                                                        const T* __restrict__ grho, G geo, T fact, const T* __restrict__ gout,
-                                                       T* __restrict__ gin) {
+                                                       T* __restrict__ gin, unsigned long long* __restrict__ gmax) {
+    T mx[3] = {T(0), T(0), T(0)};
     LDL_SWEEP2(sw, np, p0, p1) {
         const int64_t pp[2] = {p0 < np ? p0 : np - 1, p1 < np ? p1 : np - 1};
         T x[2][3], g[2][3];
@@ -735,16 +762,22 @@ __global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos
             interp_vg(vr[k][0], st[k].f, pd.cellinv, val, gr);
             if ((k ? p1 : p0) < np) {
 #pragma unroll
-                for (int e = 0; e < 3; e++) gin[3 * pp[k] + e] = acc[e] + fact * gr[e];
+                for (int e = 0; e < 3; e++) {
+                    const T o = acc[e] + fact * gr[e];
+                    gin[3 * pp[k] + e] = o;
+                    mx[e] = fmax(mx[e], fabs(o));
+                }
             }
         }
     }
+    if (gmax) block_absmax3<T>(mx, gmax);
 }
 
-// g = fact * grad(interp g_rho)
+// g = fact * grad(interp g_rho); with gmax, also the per-component max |g|.
 template <typename T, typename G>
 __global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ grho,
-                                                       G geo, T fact, T* __restrict__ g) {
+                                                       G geo, T fact, T* __restrict__ g, unsigned long long* __restrict__ gmax) {
+    T mx[3] = {T(0), T(0), T(0)};
     LDL_SWEEP2(sw, np, p0, p1) {
         const int64_t pp[2] = {p0 < np ? p0 : np - 1, p1 < np ? p1 : np - 1};
         T x[2][3];
@@ -765,10 +798,15 @@ __global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos
             interp_vg(v[k][0], st[k].f, pd.cellinv, val, gr);
             if ((k ? p1 : p0) < np) {
 #pragma unroll
-                for (int e = 0; e < 3; e++) g[3 * pp[k] + e] = fact * gr[e];
+                for (int e = 0; e < 3; e++) {
+                    const T o = fact * gr[e];
+                    g[3 * pp[k] + e] = o;
+                    mx[e] = fmax(mx[e], fabs(o));
+                }
             }
         }
     }
+    if (gmax) block_absmax3<T>(mx, gmax);
 }
 
 // ------------------------------------------------------------------ loss
@@ -1215,7 +1253,9 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
 template <typename T>
 static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64_t np, const int64_t nmesh[3], const double* prm,
                         double fact, int32_t alpha_rule, int32_t pflags, const T* star, const T* rho, const T* Ubuf, const T* S2buf,
-                        const T* F, const void* bins, const T* gout, T* gin, double* grad, void* ws, size_t wsb, hipStream_t st) {
+                        const T* F, const void* bins, const T* gout, T* gin, double* grad, double* gmaxd, void* ws, size_t wsb,
+                        hipStream_t st) {
+    auto* gmax = reinterpret_cast<unsigned long long*>(gmaxd);
     Mesh m;
     int rc = check_fft_mesh(nmesh, &m);
     if (rc) return rc;
@@ -1236,7 +1276,7 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     const bool reuse = bins && !(pflags & (LDL_PAINT_FLOAT_ATOMIC | LDL_PAINT_DIRECT));
     // (the forward density rho = fact * sum of CIC weights bounds the cells' sums)
     if ((rc = paint_impl<T>(pos, pdesc, np, gout, 0.0, 3, nmesh, 1.0, pflags, P.mesh3, P.paint, w.paint, st, nullptr,
-                            reuse ? reinterpret_cast<const unsigned*>(bins) : nullptr, rho, 1.0 / fact)))
+                            reuse ? reinterpret_cast<const unsigned*>(bins) : nullptr, rho, 1.0 / fact, gmax)))
         return rc;
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
     // 2. unnormalised r2c of the three meshes: z and y passes
@@ -1317,10 +1357,11 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     // 5. position adjoint
     if (np > 0)
         {
+            if (gmax) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, gmax, (int64_t)3, 0ull);  // the paint has read it
             KTimer _t("sm_bwd_gather", st);
             with_geom4<T>(m, [&](auto geo) {
                 hipLaunchKernelGGL((k_sm_bwd_gather<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, F, grho, geo,
-                                   (T)fact, gout, gin);
+                                   (T)fact, gout, gin, gmax);
             });
         }
     LDL_LAUNCH_CHECK();
@@ -1506,7 +1547,9 @@ static int hi_depletion_bwd(const T* star, const c2<T>* Sk, const T* S, const T*
 
 template <typename T>
 static int paint_vjp_pos(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* grho, const int64_t nmesh[3], double fact, T* g,
-                         hipStream_t st) {
+                         double* gmaxd, hipStream_t st) {
+    auto* gmax = reinterpret_cast<unsigned long long*>(gmaxd);
+    if (gmax) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, gmax, (int64_t)3, 0ull);
     Mesh m;
     int rc = check_mesh(nmesh, &m);
     if (rc) return rc;
@@ -1517,7 +1560,8 @@ static int paint_vjp_pos(const T* pos, const ldl_pos_desc* pdesc, int64_t np, co
     const Sweep sw = make_sweep(pd, np);
     KTimer _t("paint_vjp_pos", st);
     with_geom<T>(m, [&](auto geo) {
-        hipLaunchKernelGGL((k_paint_vjp_pos<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, grho, geo, (T)fact, g);
+        hipLaunchKernelGGL((k_paint_vjp_pos<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, grho, geo, (T)fact, g,
+                           gmax);
     });
     LDL_LAUNCH_CHECK();
     return LDL_OK;
@@ -1568,24 +1612,24 @@ size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np
     }                                                                                                                       \
     int ldl_sm_layer_bwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
                                double fact, int32_t alpha_rule, int32_t pflags, const T* rho, const T* U, const T* F,      \
-                               const void* bins, const T* g_out, T* g_in, double* grad5, void* ws, size_t wsb,             \
-                               void* stream) {                                                                              \
+                               const void* bins, const T* g_out, T* g_in, double* grad5, double* gmax, void* ws,           \
+                               size_t wsb, void* stream) {                                                                  \
         return sm_layer_bwd<T>(kKindSm, pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, nullptr, rho, U, nullptr, F, bins, \
-                               g_out, g_in, grad5, ws, wsb, as_stream(stream));                                             \
+                               g_out, g_in, grad5, gmax, ws, wsb, as_stream(stream));                                       \
     }                                                                                                                       \
     int ldl_dm_layer_bwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[5], \
                                double fact, int32_t alpha_rule, int32_t pflags, const T* rho, const T* U, const T* F,      \
-                               const void* bins, const T* g_out, T* g_in, double* grad5, void* ws, size_t wsb,             \
-                               void* stream) {                                                                              \
+                               const void* bins, const T* g_out, T* g_in, double* grad5, double* gmax, void* ws,           \
+                               size_t wsb, void* stream) {                                                                  \
         return sm_layer_bwd<T>(kKindDm, pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, nullptr, rho, U, nullptr, F, bins, \
-                               g_out, g_in, grad5, ws, wsb, as_stream(stream));                                             \
+                               g_out, g_in, grad5, gmax, ws, wsb, as_stream(stream));                                       \
     }                                                                                                                       \
     int ldl_hi_layer_bwd_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const int64_t nmesh[3], const double prm[10], \
                                double fact, int32_t alpha_rule, int32_t pflags, const T* starmap, const T* rho, const T* U, \
                                const T* S2, const T* F, const void* bins, const T* g_out, T* g_in, double* grad10,         \
-                               void* ws, size_t wsb, void* stream) {                                                        \
+                               double* gmax, void* ws, size_t wsb, void* stream) {                                          \
         return sm_layer_bwd<T>(kKindHi, pos, pd, np, nmesh, prm, fact, alpha_rule, pflags, starmap, rho, U, S2, F, bins,   \
-                               g_out, g_in, grad10, ws, wsb, as_stream(stream));                                            \
+                               g_out, g_in, grad10, gmax, ws, wsb, as_stream(stream));                                      \
     }                                                                                                                       \
     int ldl_sm_loss_##SUF(const T* rho, const T* target, const T* mask, const T* mask_valid, const T* field2,               \
                           const int64_t nmesh[3], const double box[3], const double prm3[3], double index, int32_t p_norm,  \
@@ -1621,8 +1665,8 @@ size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np
                                    grad8, ws, wsb, as_stream(stream));                                                      \
     }                                                                                                                       \
     int ldl_paint_vjp_pos_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const T* g_rho, const int64_t nmesh[3],   \
-                                double fact, T* g, void* stream) {                                                          \
-        return paint_vjp_pos<T>(pos, pd, np, g_rho, nmesh, fact, g, as_stream(stream));                                     \
+                                double fact, T* g, double* gmax, void* stream) {                                            \
+        return paint_vjp_pos<T>(pos, pd, np, g_rho, nmesh, fact, g, gmax, as_stream(stream));                               \
     }
 
 SM_ENTRY(f32, float)

@@ -276,8 +276,9 @@ class JERALDModel:
         P = st["P"]
         fact = st["fact"]
         g = torch.empty_like(P)
+        gmax = torch.zeros(3, dtype=torch.float64, device=dev)  # max |g| per component, handed from call to call
         L.call(f"ldl_paint_vjp_pos_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], L.ptr(st["g_rho"]), L.i64x3(nm),
-               float(fact), L.ptr(g), L.stream_ptr(dev))
+               float(fact), L.ptr(g), L.ptr(gmax), L.stream_ptr(dev))
         k = self.npot
         grads = torch.zeros(k * self.Nstep, dtype=torch.float64, device=dev)
         ws = self._ws()
@@ -292,7 +293,7 @@ class JERALDModel:
             gp = C_ptr_offset(grads, k * i)
             head = (L.ptr(Pi), self._pd(), Pi.shape[0], L.i64x3(nm), L.dvec(prm), float(fact), self.alpha_rule,
                     self.paint_flags)
-            tail = (L.ptr(F), L.ptr(bins), L.ptr(g), L.ptr(g), gp, L.ptr(ws), ws.numel(), L.stream_ptr(dev))
+            tail = (L.ptr(F), L.ptr(bins), L.ptr(g), L.ptr(g), gp, L.ptr(gmax), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
             if self.kind == "HI":
                 L.call(f"ldl_hi_layer_bwd_{L.sfx(dt)}", *head, L.ptr(self.starmap), L.ptr(rho), L.ptr(U), L.ptr(S2), *tail)
             else:
