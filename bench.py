@@ -96,7 +96,7 @@ def kernel_bytes(name, n, b):
         "paint_vjp_pos": 6 * b * Np + b * Nm,            # pos in, grad out, 1 field
         "paint_deposit1": 3 * b * Np + b * Nm,           # unit-mass paint
         "paint_deposit3": 6 * b * Np + 3 * b * Nm,       # (Np,3) masses -> 3 meshes
-        "paint_absmax": 3 * b * Np,                      # (Np,3) masses in
+        "paint_absmax": b * Nm,                          # largest cell of the density (the g maxima arrive with g)
         "zpass_r2c": b * Nm + 2 * b * Nh,
         "zpass_r2c3": 3 * (b * Nm + 2 * b * Nh),
         "zpass_c2r": b * Nm + 2 * b * Nh,
