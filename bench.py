@@ -16,7 +16,6 @@ baseline (the FP64 oracle restatement on a bounded sample).
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -50,30 +49,10 @@ def log(msg):
 
 # ------------------------------------------------------------------ inputs
 def zeldovich_gpu(n, box, rms_cells, seed, device):
-    """Lattice + Zel'dovich displacement (box units, float64), built on the
-    GPU with torch.fft (input setup only; not part of the timed path).  Same
-    recipe as oracle.ldl_oracle.zeldovich_displacement."""
-    import torch
-    g = torch.Generator(device=device).manual_seed(seed)
-    white = torch.randn((n, n, n), generator=g, device=device, dtype=torch.float64)
-    wk = torch.fft.rfftn(white)
-    del white
-    cell = box / n
-    k1 = torch.fft.fftfreq(n, d=cell, device=device, dtype=torch.float64) * 2 * math.pi
-    kz = k1[: n // 2 + 1]
-    k2 = k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kz[None, None, :] ** 2
-    k2s = torch.where(k2 > 0, k2, torch.ones_like(k2))
-    pk = torch.where(k2 > 0, torch.exp(-k2 * (2 * cell) ** 2) / k2s, torch.zeros_like(k2))
-    dk = torch.sqrt(pk) * wk / k2s
-    del wk, pk
-    psi = torch.empty((n ** 3, 3), device=device, dtype=torch.float64)
-    for d, kk in enumerate((k1[:, None, None], k1[None, :, None], kz[None, None, :])):
-        psi[:, d] = torch.fft.irfftn(1j * kk * dk, s=(n, n, n)).reshape(-1)
-    del dk, k2, k2s
-    psi *= rms_cells * cell / torch.sqrt(torch.mean(psi ** 2))
-    i = torch.arange(n, device=device, dtype=torch.float64) * cell
-    q = torch.stack(torch.meshgrid(i, i, i, indexing="ij"), dim=-1).reshape(-1, 3)
-    return q + psi
+    """Lattice + Zel'dovich displacement (box units, float64); the recipe of
+    oracle.ldl_oracle.zeldovich_displacement, built on the GPU."""
+    from jerald_amd.synthetic import zeldovich_gpu as z
+    return z(n, box, rms_cells, seed, device)
 
 
 def params0(nstep):

@@ -27,13 +27,15 @@ __device__ __forceinline__ int fixed_shift(unsigned long long maxbits, int64_t n
 }
 
 // The 8 fixed-point CIC deposits of one particle with fractions f and mass
-// m at scale 2^s: q[4a+2b+c] = rint((w_x[a] w_y[b]) (w_z[c] (m 2^s))), and
-// q[7] = rint(m 2^s) - (q[0] + ... + q[6]) so the deposits sum exactly to
-// the rounded mass.  The products are formed in the working precision T (FP32
-// deposits are int32: |m 2^s| < 2^30 and every weight is in [0, 1], so each
-// deposit and every partial sum, all of the sign of m and at most |m 2^s| + 4
-// in magnitude, fit in 32 bits; FP64 deposits are int64).  The CPU emulation
-// (oracle/cic_oracle.c) performs the same operations in the same order.
+// m at scale 2^s: q[4a+2b+c] = trunc((w_x[a] w_y[b]) (w_z[c] (m 2^s))) for the
+// first seven corners (rounded toward zero, so each has the sign of m or is
+// 0), and q[7] = rint(m 2^s) - (q[0] + ... + q[6]), which then also has the
+// sign of m: the deposits sum exactly to the rounded mass and a paint of
+// non-negative masses is non-negative in every cell.  The products are
+// formed in the working precision T (FP32 deposits are int32: |m 2^s| < 2^30
+// and every weight is in [0, 1], so each deposit and every partial sum fit in
+// 32 bits; FP64 deposits are int64).  The CPU emulation (oracle/cic_oracle.c)
+// performs the same operations in the same order.
 template <typename T> using DepositT = typename std::conditional<sizeof(T) == 4, int, long long>::type;
 
 template <typename T>
@@ -46,10 +48,7 @@ __device__ __forceinline__ void deposits(const T f[3], T m, int s, DepositT<T> q
     DepositT<T> part = 0;
 #pragma unroll
     for (int k = 0; k < 7; k++) {
-        const T w = wxy[k >> 1] * wzm[k & 1];
-        DepositT<T> v;
-        if constexpr (sizeof(T) == 4) v = __float2int_rn(w);
-        else v = __double2ll_rn(w);
+        const DepositT<T> v = (DepositT<T>)(wxy[k >> 1] * wzm[k & 1]);  // toward zero
         q[k] = v;
         part += v;
     }

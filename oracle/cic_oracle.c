@@ -112,9 +112,9 @@ void oracle_mesh_units_f64(const double* pos, int64_t np_, const double* box, co
 /* ---------------- fixed-point paint emulation ----------------
  * Per particle, all in the working precision T (float or double): CIC
  * fractions, ms = m 2^s, wzm[c] = w_z[c] ms, wxy[2a+b] = w_x[a] w_y[b] and
- * the deposit of corner k = 4a+2b+c is rint(wxy[k>>1] * wzm[k&1]) for k < 7;
- * corner 7 takes rint(ms) minus the other seven so the mass is conserved
- * exactly.  The accumulator is summed in int64 (exact, order free).  s comes
+ * the deposit of corner k = 4a+2b+c is wxy[k>>1] * wzm[k&1] rounded toward
+ * zero for k < 7; corner 7 takes rint(ms) minus the other seven so the mass
+ * is conserved exactly (and every deposit has the sign of m).  The accumulator is summed in int64 (exact, order free).  s comes
  * from the caller (FP32: 30 - e, FP64: 61 - e - ceil(log2 Np), with
  * max|m| < 2^e).  kind 0: pos physical (wrapped and divided in T);  kind 1:
  * pos is a displacement in mesh units from the lattice point idx*step.    */
@@ -127,7 +127,7 @@ void oracle_mesh_units_f64(const double* pos, int64_t np_, const double* box, co
     for (int q = 0; q < 8; q++) {                                                  \
         int64_t v;                                                                 \
         if (q == 7) v = (int64_t)RINT(ms) - part;                                  \
-        else { v = (int64_t)RINT(wxy[q >> 1] * wzm[q & 1]); part += v; }            \
+        else { v = (int64_t)(wxy[q >> 1] * wzm[q & 1]); part += v; }                \
         acc[(ii[q >> 2] * n1 + jj[(q >> 1) & 1]) * n2 + kk[q & 1]] += v;           \
     }
 

@@ -57,6 +57,32 @@ def test_paint3_fixed_point_bit_exact(dt, np_, offset):
 
 
 @pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+@pytest.mark.parametrize("nc", [1, 3])
+def test_fixed_point_paint_of_sparse_positive_masses_is_nonnegative(dt, nc):
+    """Sparse particles (most cells empty, many touched only by a corner of
+    tiny weight): every deposit carries the sign of its mass, so the paint of
+    positive masses is >= 0 everywhere and exactly 0 where no particle
+    reaches; bit-exact against the CPU emulation."""
+    rng = np.random.default_rng(11)
+    nm, box = (32, 32, 32), np.array([64.0, 64.0, 64.0])
+    centres = rng.random((12, 3)) * box
+    pos = (centres[rng.integers(0, 12, 4000)] + rng.standard_normal((4000, 3)) * 1.5) % box
+    ndt = np.float32 if dt == torch.float32 else np.float64
+    p = pos.astype(ndt)
+    m = rng.random((4000, nc)).astype(ndt) * 10.0 ** rng.integers(-3, 3, (4000, 1))
+    tp = torch.tensor(p, device=DEV)
+    got = (J.ppaint(tp, torch.tensor(m[:, 0], device=DEV), nm, box) if nc == 1
+           else J.ppaint3D(tp, torch.tensor(m, device=DEV), nm, box)).cpu().numpy().reshape(nc, *nm)
+    assert got.min() >= 0.0
+    x = O.mesh_units(p, box, nm, ndt).astype(np.float64)
+    for c in range(nc):
+        ref = O.paint(x, m[:, c].astype(np.float64), nm)
+        assert np.all(got[c][ref == 0.0] == 0.0)
+        acc, s = NV.paint_fixed(p, m[:, c], nm, box=box, dtype=ndt)
+        np.testing.assert_array_equal(got[c], NV.fixed_to_float(acc, s, ndt))
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
 def test_paint_deterministic_and_conserves_mass(dt):
     pos, nm, box = _inputs(np_=200000, nm=(32, 32, 32), box=(64.0, 64.0, 64.0))
     t = torch.tensor(pos, device=DEV, dtype=dt)

@@ -262,6 +262,30 @@ def test_fp32_clustered_particles():
     np.testing.assert_allclose(g64, g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
 
 
+def test_sparse_particles_finite_gradients():
+    """Particles in a few small clumps, most cells empty: the response's
+    (rho + 1e-8)^mu sees no negative density, gradients stay finite and FP32
+    follows FP64 and the oracle."""
+    N, L, nstep = 32, 64.0, 2
+    rng = np.random.default_rng(12)
+    centres = rng.random((6, 3)) * L
+    X = (centres[rng.integers(0, 6, 6000)] + rng.standard_normal((6000, 3)) * 2.0) % L
+    tgt = rng.random((N, N, N)) + 0.5
+    p = np.array([0.8, 1.2, 0.3, 1.5, 0.2] * nstep + [1.1, 0.9, 0.05])
+    out = {}
+    for dt in (torch.float64, torch.float32):
+        m = _model(X, tgt, None, N, L, nstep, dt, L1=False, smoothing=("gaussian", 4.0))
+        loss, grad = m.loss_and_grad(torch.tensor(p))
+        out[dt] = (loss.item(), grad.cpu().numpy())
+        assert np.all(np.isfinite(out[dt][1])), (dt, out[dt][1])
+    (l64, g64), (l32, g32) = out[torch.float64], out[torch.float32]
+    assert abs(l32 - l64) <= 1e-5 * abs(l64)
+    assert np.all(np.abs(g32 - g64) <= 1e-4 * np.abs(g64) + 1e-6 * np.abs(g64).max())
+    orc = O.SMModelOracle(N, L, nstep, tgt, L1=False, smoothing=("gaussian", 4.0))
+    loss0, g0 = orc.loss_and_grad(p, X)
+    np.testing.assert_allclose(g64, g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
+
+
 @pytest.mark.parametrize("dt", [torch.float32, torch.float64])
 def test_sort_reuse_is_bit_identical(dt):
     """The backward paint on the forward's 16^3-tile sort gives the same
