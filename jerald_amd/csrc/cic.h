@@ -10,7 +10,7 @@
 namespace ldl {
 
 // ---------------------------------------------------------------- helpers
-// Fixed-point exponent s (deposits are rint(w m 2^s)), from max |m| < 2^e:
+// Fixed-point exponent s (deposits are splits of rint(m 2^s)), from max |m| < 2^e:
 //  * FP32 path: s = 30 - e, so every deposit fits in 31 bits and converts with
 //    the native double -> int32 instruction; a cell (or the total) stays below
 //    Np 2^30 < 2^62 for Np < 2^32;
@@ -27,35 +27,45 @@ __device__ __forceinline__ int fixed_shift(unsigned long long maxbits, int64_t n
 }
 
 // The 8 fixed-point CIC deposits of one particle with fractions f and mass
-// m at scale 2^s: q[4a+2b+c] = trunc((w_x[a] w_y[b]) (w_z[c] (m 2^s))) for the
-// first seven corners (rounded toward zero, so each has the sign of m or is
-// 0), and q[7] = rint(m 2^s) - (q[0] + ... + q[6]), which then also has the
-// sign of m: the deposits sum exactly to the rounded mass and a paint of
-// non-negative masses is non-negative in every cell.  The products are
-// formed in the working precision T (FP32 deposits are int32: |m 2^s| < 2^30
-// and every weight is in [0, 1], so each deposit and every partial sum fit in
-// 32 bits; FP64 deposits are int64).  The CPU emulation (oracle/cic_oracle.c)
-// performs the same operations in the same order.
+// m at scale 2^s, split hierarchically from the rounded mass M = rint(m 2^s):
+// the x split gives X1 = trunc(f_x M) and X0 = M - X1, each X_a splits along
+// y the same way, each of those along z, and q[4a+2b+c] is the z part c of
+// (a, b).  Each split takes trunc(f V) (clamped into [0, V] or [V, 0]) and
+// V minus it, so every deposit has the sign of m (a paint of non-negative
+// masses is non-negative in every cell) and the deposits sum exactly to M.
+// The products are formed in T (their rounding moves a deposit by at most a
+// few units of 2^-s times f V 2^-24 in FP32; sign and sum are exact
+// regardless).  FP32 deposits are int32 (|m 2^s| < 2^30);
+// FP64 deposits are int64.  The CPU emulation (oracle/cic_oracle.c) performs
+// the same operations in the same order.
 template <typename T> using DepositT = typename std::conditional<sizeof(T) == 4, int, long long>::type;
+
+template <typename T, typename D>
+__device__ __forceinline__ D split_part(T fr, D V) {  // trunc(fr V), clamped between 0 and V
+    const D v = (D)(fr * (T)V);
+    return V >= 0 ? (v > V ? V : v) : (v < V ? V : v);
+}
 
 template <typename T>
 __device__ __forceinline__ void deposits(const T f[3], T m, int s, DepositT<T> q[8]) {
-    const T wx[2] = {T(1) - f[0], f[0]};
-    const T wy[2] = {T(1) - f[1], f[1]};
-    const T ms = ldexp(m, s);
-    const T wzm[2] = {(T(1) - f[2]) * ms, f[2] * ms};
-    const T wxy[4] = {wx[0] * wy[0], wx[0] * wy[1], wx[1] * wy[0], wx[1] * wy[1]};
-    DepositT<T> part = 0;
+    using D = DepositT<T>;
+    const double ms = ldexp((double)m, s);
+    D M;
+    if constexpr (sizeof(T) == 4) M = __double2int_rn(ms);
+    else M = __double2ll_rn(ms);
+    const D x1 = split_part<T, D>(f[0], M);
+    const D xa[2] = {M - x1, x1};
 #pragma unroll
-    for (int k = 0; k < 7; k++) {
-        const DepositT<T> v = (DepositT<T>)(wxy[k >> 1] * wzm[k & 1]);  // toward zero
-        q[k] = v;
-        part += v;
+    for (int a = 0; a < 2; a++) {
+        const D y1 = split_part<T, D>(f[1], xa[a]);
+        const D yb[2] = {xa[a] - y1, y1};
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const D z1 = split_part<T, D>(f[2], yb[b]);
+            q[4 * a + 2 * b] = yb[b] - z1;
+            q[4 * a + 2 * b + 1] = z1;
+        }
     }
-    DepositT<T> tot;
-    if constexpr (sizeof(T) == 4) tot = __float2int_rn(ms);
-    else tot = __double2ll_rn(ms);
-    q[7] = tot - part;
 }
 
 static __global__ void k_copy_u64(const unsigned long long* __restrict__ src, unsigned long long* __restrict__ dst, int64_t n) {

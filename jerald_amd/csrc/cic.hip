@@ -4,12 +4,13 @@
 // coordinate, i = floor(x) wrapped mod N, f = x - i; corner (i+a, j+b, k+c)
 // gets weight w_x[a] w_y[b] w_z[c] with w[0] = 1 - f, w[1] = f.
 //
-// Paint accumulates in 64-bit fixed point: each deposit is rounded once to
-// an integer multiple of 2^-s and summed with integer atomics, so the density
-// is bit-identical for any particle order or scheduling.  s is chosen per call
-// from max|m| and Np so that no cell (nor the total) can exceed 2^61 * 2^-s.
-// The eighth corner receives rint(m 2^s) minus the other seven deposits, so
-// the painted total equals the sum of the rounded masses exactly.
+// Paint accumulates in fixed point: each deposit is an integer multiple of
+// 2^-s and deposits are summed with integer adds, so the density is
+// bit-identical for any particle order or scheduling.  s is chosen per call
+// from max|m| and Np so that no cell (nor the total) can overflow.  The
+// rounded mass rint(m 2^s) is split over the corners x, then y, then z, each
+// split truncating toward zero, so the painted total equals the sum of the
+// rounded masses exactly and every deposit has the sign of its mass.
 #include "cic.h"
 
 namespace ldl {

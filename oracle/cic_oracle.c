@@ -110,25 +110,38 @@ void oracle_mesh_units_f64(const double* pos, int64_t np_, const double* box, co
 }
 
 /* ---------------- fixed-point paint emulation ----------------
- * Per particle, all in the working precision T (float or double): CIC
- * fractions, ms = m 2^s, wzm[c] = w_z[c] ms, wxy[2a+b] = w_x[a] w_y[b] and
- * the deposit of corner k = 4a+2b+c is wxy[k>>1] * wzm[k&1] rounded toward
- * zero for k < 7; corner 7 takes rint(ms) minus the other seven so the mass
- * is conserved exactly (and every deposit has the sign of m).  The accumulator is summed in int64 (exact, order free).  s comes
- * from the caller (FP32: 30 - e, FP64: 61 - e - ceil(log2 Np), with
- * max|m| < 2^e).  kind 0: pos physical (wrapped and divided in T);  kind 1:
- * pos is a displacement in mesh units from the lattice point idx*step.    */
-#define DEPOSIT_BODY(T, LDEXP, RINT)                                               \
-    T ms = LDEXP(m, s);                                                            \
-    T wzm[2] = {tz * ms, fz_ * ms};                                                \
-    T wxy[4] = {tx * ty, tx * fy_, fx_ * ty, fx_ * fy_};                           \
+ * Per particle: CIC fractions in the working precision T, then the rounded
+ * mass M = rint(m 2^s) (m 2^s in double) split hierarchically: X1 =
+ * trunc(f_x M) clamped between 0 and M, X0 = M - X1; each X_a split along y
+ * the same way; each of those along z; corner k = 4a+2b+c receives the z
+ * part c of (a, b).  Products in the working precision T.  Every deposit has the sign of m and
+ * the deposits sum exactly to M.  The accumulator is summed in int64 (exact,
+ * order free).  s comes from the caller (FP32: 30 - e, FP64: 61 - e -
+ * ceil(log2 Np), with max|m| < 2^e).  kind 0: pos physical (wrapped and
+ * divided in T);  kind 1: pos is a displacement in mesh units from the
+ * lattice point idx*step.                                                  */
+static int64_t split_part_f32(float fr, int64_t V) {
+    int64_t v = (int64_t)(fr * (float)V);
+    if (V >= 0) return v > V ? V : v;
+    return v < V ? V : v;
+}
+static int64_t split_part_f64(double fr, int64_t V) {
+    int64_t v = (int64_t)(fr * (double)V);
+    if (V >= 0) return v > V ? V : v;
+    return v < V ? V : v;
+}
+
+#define DEPOSIT_BODY(SPLIT)                                                        \
+    int64_t M = llrint(ldexp((double)m, s));                                       \
     int64_t ii[2] = {i, (i + 1) % n0}, jj[2] = {j, (j + 1) % n1}, kk[2] = {k, (k + 1) % n2}; \
-    int64_t part = 0;                                                              \
-    for (int q = 0; q < 8; q++) {                                                  \
-        int64_t v;                                                                 \
-        if (q == 7) v = (int64_t)RINT(ms) - part;                                  \
-        else { v = (int64_t)(wxy[q >> 1] * wzm[q & 1]); part += v; }                \
-        acc[(ii[q >> 2] * n1 + jj[(q >> 1) & 1]) * n2 + kk[q & 1]] += v;           \
+    int64_t x1 = SPLIT(fx_, M), xa[2] = {M - x1, x1};                              \
+    for (int a = 0; a < 2; a++) {                                                  \
+        int64_t y1 = SPLIT(fy_, xa[a]), yb[2] = {xa[a] - y1, y1};                  \
+        for (int b = 0; b < 2; b++) {                                              \
+            int64_t z1 = SPLIT(fz_, yb[b]);                                        \
+            acc[(ii[a] * n1 + jj[b]) * n2 + kk[0]] += yb[b] - z1;                  \
+            acc[(ii[a] * n1 + jj[b]) * n2 + kk[1]] += z1;                          \
+        }                                                                          \
     }
 
 void oracle_paint_fixed_f32(const float* pos, int64_t np_, int kind, const double* boxd,
@@ -155,10 +168,9 @@ void oracle_paint_fixed_f32(const float* pos, int64_t np_, int kind, const doubl
         }
         float flx = floorf(xs[0]), fly = floorf(xs[1]), flz = floorf(xs[2]);
         float fx_ = xs[0] - flx, fy_ = xs[1] - fly, fz_ = xs[2] - flz;
-        float tx = 1.0f - fx_, ty = 1.0f - fy_, tz = 1.0f - fz_;
         int64_t i = wrap64(base[0] + (int64_t)flx, n0), j = wrap64(base[1] + (int64_t)fly, n1),
                 k = wrap64(base[2] + (int64_t)flz, n2);
-        DEPOSIT_BODY(float, ldexpf, lrintf)
+        DEPOSIT_BODY(split_part_f32)
     }
 }
 
@@ -186,9 +198,8 @@ void oracle_paint_fixed_f64(const double* pos, int64_t np_, int kind, const doub
         }
         double flx = floor(xs[0]), fly = floor(xs[1]), flz = floor(xs[2]);
         double fx_ = xs[0] - flx, fy_ = xs[1] - fly, fz_ = xs[2] - flz;
-        double tx = 1.0 - fx_, ty = 1.0 - fy_, tz = 1.0 - fz_;
         int64_t i = wrap64(base[0] + (int64_t)flx, n0), j = wrap64(base[1] + (int64_t)fly, n1),
                 k = wrap64(base[2] + (int64_t)flz, n2);
-        DEPOSIT_BODY(double, ldexp, llrint)
+        DEPOSIT_BODY(split_part_f64)
     }
 }

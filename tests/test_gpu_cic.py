@@ -83,6 +83,32 @@ def test_fixed_point_paint_of_sparse_positive_masses_is_nonnegative(dt, nc):
 
 
 @pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_fixed_point_paint_small_fractions_bit_exact(dt):
+    """One particle per 2x2x2 block of cells with small fractions, where a
+    remainder corner of FP32 weight products would go negative: every cell
+    holds one deposit, >= 0, bit-exact against the CPU emulation, for the
+    one-component paint and each component of the three-component paint."""
+    rng = np.random.default_rng(21)
+    nm, g = (32, 32, 32), 16
+    idx = np.stack(np.meshgrid(*[np.arange(g)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    frac = 10.0 ** rng.uniform(-4.5, -1.0, idx.shape)
+    ndt = np.float32 if dt == torch.float32 else np.float64
+    p = (2 * idx + frac).astype(ndt)
+    box = np.array(nm, np.float64)
+    m3 = (1.0 + rng.random((len(p), 3))).astype(ndt)
+    tp = torch.tensor(p, device=DEV)
+    one = J.ppaint(tp, 1.0, nm, box).cpu().numpy()
+    acc, s = NV.paint_fixed(p, 1.0, nm, box=box, dtype=ndt)
+    assert one.min() >= 0.0
+    np.testing.assert_array_equal(one, NV.fixed_to_float(acc, s, ndt))
+    three = J.ppaint3D(tp, torch.tensor(m3, device=DEV), nm, box).cpu().numpy().reshape(3, *nm)
+    assert three.min() >= 0.0
+    for c in range(3):
+        acc, s = NV.paint_fixed(p, m3[:, c], nm, box=box, dtype=ndt)
+        np.testing.assert_array_equal(three[c], NV.fixed_to_float(acc, s, ndt))
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
 def test_paint_deterministic_and_conserves_mass(dt):
     pos, nm, box = _inputs(np_=200000, nm=(32, 32, 32), box=(64.0, 64.0, 64.0))
     t = torch.tensor(pos, device=DEV, dtype=dt)

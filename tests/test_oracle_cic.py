@@ -125,6 +125,40 @@ def test_readout_grad_finite_difference():
         np.testing.assert_allclose(g[:, d], fd, rtol=1e-6, atol=1e-8)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fixed_point_deposits_keep_sign_and_sum(dtype):
+    """One particle per 2x2x2 block of cells, so every cell holds exactly one
+    deposit.  Small fractions (the corner (1,1,1) gets a few units of 2^-s)
+    are where a remainder corner -- rint(m 2^s) minus seven rounded weight
+    products -- goes negative once the FP32 complements 1 - f sum above 1;
+    the split deposits must stay >= 0, sum to rint(m 2^s) per particle and
+    sit within rounding of w m 2^s."""
+    rng = np.random.default_rng(21)
+    nm, g = (32, 32, 32), 16
+    idx = np.stack(np.meshgrid(*[np.arange(g)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    frac = 10.0 ** rng.uniform(-4.5, -1.0, idx.shape)
+    frac[rng.random(idx.shape) < 0.3] = 0.0
+    pos = (2 * idx + frac).astype(dtype)
+    f = (pos - np.floor(pos)).astype(np.float64)  # the fractions the paint sees (exact)
+    acc, s = NV.paint_fixed(pos, 1.0, nm, box=np.array(nm, np.float64), dtype=dtype)
+    M = round(2.0 ** s)
+    assert acc.min() >= 0
+    blocks = acc.reshape(g, 2, g, 2, g, 2).transpose(0, 2, 4, 1, 3, 5).reshape(-1, 8)
+    assert np.all(blocks.sum(1) == M)
+    w = np.ones((len(f), 8))
+    for k in range(8):
+        for d, bit in enumerate(((k >> 2) & 1, (k >> 1) & 1, k & 1)):
+            w[:, k] *= f[:, d] if bit else 1.0 - f[:, d]
+    tol = M * 2.0 ** -21 + 4 if dtype == np.float32 else 4
+    assert np.max(np.abs(blocks - w * M)) <= tol
+    if dtype == np.float32:
+        # the set exercises the failure of the remainder form (FP32 weights)
+        t = np.stack([(1 - f).astype(np.float32), f.astype(np.float32)], -1).astype(np.float64)
+        wk = np.stack([t[:, 0, k >> 2 & 1] * t[:, 1, k >> 1 & 1] * t[:, 2, k & 1] for k in range(8)], -1)
+        rem = M - np.trunc(wk[:, :7] * M).sum(1)
+        assert rem.min() < 0
+
+
 def test_fixed_point_emulation_close_to_f64():
     rng = np.random.default_rng(8)
     nm = (8, 8, 8)
