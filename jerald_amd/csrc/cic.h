@@ -854,8 +854,9 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     __syncthreads();
     const unsigned total = rpre[NR];
     // KB particles per thread per round; all their loads are issued before any
-    // deposit so the dependent sorted[j] -> pos[p] chains overlap
-    constexpr int KB = 6;
+    // deposit so the dependent sorted[j] -> pos[p] chains overlap (4 keeps the
+    // 32-bit three-component paint at 74 VGPRs, 6 waves per SIMD)
+    constexpr int KB = ACC32 ? 4 : 6;
     for (unsigned jb = 0; jb < total; jb += TB * KB) {
         unsigned pi[KB];
         int code[KB];

@@ -21,8 +21,12 @@ static constexpr int64_t kHdrBytes = 256;
 
 // Tile side of the tiled paint: 16^3 cells = 32 KB of 64-bit LDS accumulators
 // per component (one component: several workgroups per CU; three: 96 KB, one
-// workgroup of 1024 threads per CU).
+// workgroup of 1024 threads per CU).  The 32-bit three-component paint holds
+// 48 KB: three workgroups of 512 threads per CU (6 waves per SIMD), so one
+// tile's LDS clear and write-out overlap the others' deposits (measured at
+// 512^3: 1.85 ms per launch against 2.49 ms for 1024 threads).
 static constexpr int kTileSide = 16;
+static constexpr int kAcc32Block = 512;
 
 static inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -181,10 +185,10 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         KTimer _t("paint_deposit3", st);
         if constexpr (sizeof(T) == 4) {
             if (p2)
-                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, 1024, true, true>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m,
+                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, kAcc32Block, true, true>), gt, dim3(kAcc32Block), 0, st, pos, pd, np, mass, (T)mscalar, m,
                                    mp, tg, offsets, sorted, hdr, out_scale, out);
             else
-                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, 1024, false, true>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m,
+                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, kAcc32Block, false, true>), gt, dim3(kAcc32Block), 0, st, pos, pd, np, mass, (T)mscalar, m,
                                    mp, tg, offsets, sorted, hdr, out_scale, out);
         }
     } else {
