@@ -272,7 +272,8 @@ template <typename T> struct HalfRowStore3 {
 
 // Inverse y pass input of F_y: -i D_y(k_y) A, with A = x^-1(P) the x pass's
 // second output (D_y and D_z are constant along x lines, so A serves F_y and
-// F_z and the x pass does two inverse transforms instead of three).
+// F_z, and -i D_x is a central difference along x of A: the x pass does one
+// inverse transform instead of three).
 template <typename T> struct DyColOp {
     const c2<T>* in;
     c2<T>* out;
@@ -376,13 +377,32 @@ __device__ __forceinline__ T xbwd_cell(const SmFilter<T>& fl, const Src2<T>& s2,
     return G;
 }
 
-// Forward: X FFT, save U = . / N^3, P = 1e-2 alpha G U, two inverse X FFTs
-// into out3 (spectra hcells apart): of -i D_x P, and of P itself (the y and z
-// derivative factors are constant along x lines and applied later).
+// Central difference of an x line held in LDS (element n of pencil p, row
+// stride LS, circular in x): 45 (a[n+1] - a[n-1]) - 9 (a[n+2] - a[n-2]) +
+// (a[n+3] - a[n-3]).  Divided by 60 h it is the line whose transform is
+// i D_x times the line's transform (forward direction; the inverse
+// direction takes -i D_x), for the derivative symbol
+// D_x = (45 sin t - 9 sin 2t + sin 3t) / (30 h), t = 2 pi n / N, of
+// k_axis_tables: a circular shift by s multiplies transform bin n by
+// exp(+-2 pi i n s / N).
+template <typename T, int N, int LS>
+__device__ __forceinline__ c2<T> x_central_diff(const c2<T>* lds, int n, int p) {
+    using C = c2<T>;
+    auto at = [&](int o) { return lds[((n + o) & (N - 1)) * LS + p]; };
+    const C a1 = at(1), b1 = at(-1), a2 = at(2), b2 = at(-2), a3 = at(3), b3 = at(-3);
+    return C{T(45) * (a1.x - b1.x) - T(9) * (a2.x - b2.x) + (a3.x - b3.x),
+             T(45) * (a1.y - b1.y) - T(9) * (a2.y - b2.y) + (a3.y - b3.y)};
+}
+
+// Forward: X FFT, save U = . / N^3, P = 1e-2 alpha G U, one inverse X FFT
+// into out3 slot 1 (spectra hcells apart); slot 0, the inverse transform of
+// -i D_x P, is minus the central difference of that line over 60 h (inv60h),
+// read from LDS.  The y and z derivative factors are constant along x lines
+// and applied later.
 template <typename T, int N, bool HI>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
 k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>* __restrict__ out3, int64_t hcells,
-           KTabs<T> kt, SmFilter<T> fl, T a1e2, T invM, Src2<T> s2) {
+           KTabs<T> kt, SmFilter<T> fl, T a1e2, T invM, Src2<T> s2, T inv60h) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
@@ -397,53 +417,53 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
     tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return U[base + n * g.es + p]; });
     __syncthreads();
     lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
-    C pk[EPT];
+    // each thread rewrites only its own elements: no barrier between
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
         const int i = threadIdx.x + s * TH;
         const int n = i / P, p = i - n * P;
         C u = lds[n * LS + p];
         u = C{u.x * invM, u.y * invM};
-        pk[s] = C{T(0), T(0)};
+        C pk = C{T(0), T(0)};
         if (p < pv) {
             U[base + n * g.es + p] = u;
             const T k2 = kt.knorm2(n, plane, col0 + p);
             const T G = fl.G(k2);
-            pk[s] = C{u.x * a1e2 * G, u.y * a1e2 * G};
+            pk = C{u.x * a1e2 * G, u.y * a1e2 * G};
             if constexpr (HI) {
                 const C sv = s2.S[base + n * g.es + p];
                 const T G2 = s2.a * s2.fl.G(k2);
-                pk[s] = C{pk[s].x + G2 * sv.x, pk[s].y + G2 * sv.y};
+                pk = C{pk.x + G2 * sv.x, pk.y + G2 * sv.y};
             }
         }
+        lds[n * LS + p] = pk;
     }
-#pragma unroll 1
-    for (int d = 0; d < 2; d++) {
+    __syncthreads();
+    lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
+    C* o0 = out3 + base;
+    C* o1 = out3 + hcells + base;
 #pragma unroll
-        for (int s = 0; s < EPT; s++) {
-            const int i = threadIdx.x + s * TH;
-            const int n = i / P, p = i - n * P;
-            const T Dx = kt.D[0][n];
-            lds[n * LS + p] = d == 0 ? C{Dx * pk[s].y, -Dx * pk[s].x} : pk[s];  // -i D_x P, P
+    for (int s = 0; s < EPT; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        if (p < pv) {
+            o1[n * g.es + p] = lds[n * LS + p];
+            const C d = x_central_diff<T, N, LS>(lds, n, p);
+            o0[n * g.es + p] = C{-inv60h * d.x, -inv60h * d.y};
         }
-        __syncthreads();
-        lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
-        C* o = out3 + d * hcells + base;
-        tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { o[n * g.es + p] = v; });
-        __syncthreads();
     }
 }
 
-// Backward: two forward X FFTs combined into Q = i D_x S_x + T, where the y
-// pass already folded the y and z components into T = i D_y S_y + i D_z S_z;
-// the FP64 parameter partials (alpha, kh, kl, nu; HI also the second
-// source's) against the saved U, then one inverse X FFT of cpot alpha G Q
-// into out.
+// Backward: Q = i D_x S_x + T, where the y pass already folded the y and z
+// components into T = i D_y S_y + i D_z S_z, in one forward X FFT of the
+// central difference of S_x over 60 h (inv60h) plus T; the FP64 parameter
+// partials (alpha, kh, kl, nu; HI also the second source's) against the
+// saved U, then one inverse X FFT of cpot alpha G Q into out.
 template <typename T, int N, bool HI>
-__global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
+__global__ void __launch_bounds__((FftCfg<T, N>::THREADS), (sizeof(T) == 4 ? 4 : 1))
 k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3, const c2<T>* __restrict__ U,
            c2<T>* __restrict__ out, int64_t hcells, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh,
-           double gkl, double cpot, double alpha, int alpha_rule, Src2<T> s2, double* __restrict__ part) {
+           double gkl, double cpot, double alpha, int alpha_rule, Src2<T> s2, double* __restrict__ part, T inv60h) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
@@ -455,39 +475,49 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
     stage_twiddles<T, N, TH>(twl, tw);
-    C q[EPT];
+    const C* sx = S3 + base;
+    const C* st = S3 + hcells + base;
+    C w[EPT];
 #pragma unroll
-    for (int s = 0; s < EPT; s++) q[s] = C{T(0), T(0)};
-#pragma unroll 1
-    for (int d = 0; d < 2; d++) {
-        const C* src = S3 + d * hcells + base;
-        tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return src[n * g.es + p]; });
-        __syncthreads();
-        lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
-#pragma unroll
-        for (int s = 0; s < EPT; s++) {
-            const int i = threadIdx.x + s * TH;
-            const int n = i / P, p = i - n * P;
-            const C v = lds[n * LS + p];
-            const T Dx = kt.D[0][n];
-            q[s] = d == 0 ? C{-Dx * v.y, Dx * v.x} : C{q[s].x + v.x, q[s].y + v.y};  // i D_x S_x, + T
-        }
-        __syncthreads();
+    for (int s = 0; s < EPT; s++) {  // T, issued before the S_x tile
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        w[s] = st[n * g.es + (p < pv ? p : pv - 1)];
     }
-    constexpr int NP = HI ? 9 : 4;
-    double acc[NP] = {};
-    const double apot = cpot * alpha;
+    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return sx[n * g.es + p]; });
+    __syncthreads();
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
         const int i = threadIdx.x + s * TH;
         const int n = i / P, p = i - n * P;
+        const C d = x_central_diff<T, N, LS>(lds, n, p);
+        w[s] = p < pv ? C{inv60h * d.x + w[s].x, inv60h * d.y + w[s].y} : C{T(0), T(0)};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < EPT; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        lds[n * LS + p] = w[s];
+    }
+    __syncthreads();
+    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
+    constexpr int NP = HI ? 9 : 4;
+    double acc[NP] = {};
+    const double apot = cpot * alpha;
+    // each thread rewrites only its own elements: no barrier between
+#pragma unroll
+    for (int s = 0; s < EPT; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        const C q = lds[n * LS + p];
         C y = C{T(0), T(0)};
         if (p < pv) {
             const int64_t kz = col0 + p;
             const int64_t idx = base + n * g.es + p;
             const double c = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
-            const T G = xbwd_cell<T, HI>(fl, s2, kt.knorm2(n, plane, kz), q[s], U[idx], idx, c, cpot, apot, gkh, gkl, alpha_rule, acc);
-            y = C{a1e2 * G * q[s].x, a1e2 * G * q[s].y};
+            const T G = xbwd_cell<T, HI>(fl, s2, kt.knorm2(n, plane, kz), q, U[idx], idx, c, cpot, apot, gkh, gkl, alpha_rule, acc);
+            y = C{a1e2 * G * q.x, a1e2 * G * q.y};
         }
         lds[n * LS + p] = y;
     }
@@ -1194,20 +1224,21 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
         if ((rc = r2c_full<T>(m, tw, StarRowLoad<T>{star, m.n2, (T)prm[6], 0}, S2, hp, invM, st))) return rc;
         s2 = hi_src2<T>(prm, S2, nullptr);
     }
-    // 3. x pass + filter + three inverse x passes
+    // 3. x pass + filter + inverse x pass (and the x difference)
     const SmFilter<T> fl = layer_filter<T>(prm, kind == kKindDm);
     const T a1e2 = (T)((hi ? 1e-3 : 1e-2) * prm[0]);
     if (fft_mesh_pow2(m)) {
         dispatch_n(m.n0, [&](auto N) {
             constexpr int NN = decltype(N)::value;
             ColGeom g = col_geom<T, NN>(m.n1, nh, hp, m.n1 * hp);
+            const T inv60h = (T)((double)m.n0 / (60.0 * pdesc->box[0]));
             KTimer _t("sm_x_fwd", st);
             if (hi)
                 hipLaunchKernelGGL((k_sm_x_fwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, hc,
-                                   kt, fl, a1e2, invM, s2);
+                                   kt, fl, a1e2, invM, s2, inv60h);
             else
                 hipLaunchKernelGGL((k_sm_x_fwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3,
-                                   hc, kt, fl, a1e2, invM, s2);
+                                   hc, kt, fl, a1e2, invM, s2, inv60h);
         });
     } else {
         if ((rc = colpass<T, -1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{U, U, invM}, st))) return rc;
@@ -1320,13 +1351,14 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
             constexpr int NN = decltype(N)::value;
             ColGeom g = col_geom<T, NN>(m.n1, nh, hp, m.n1 * hp);
             ntx = g.ntiles;
+            const T inv60h = (T)((double)m.n0 / (60.0 * pdesc->box[0]));
             KTimer _t("sm_x_bwd", st);
             if (hi)
                 hipLaunchKernelGGL((k_sm_x_bwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U, S,
-                                   hc, m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
+                                   hc, m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part, inv60h);
             else
                 hipLaunchKernelGGL((k_sm_x_bwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U,
-                                   S, hc, m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
+                                   S, hc, m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part, inv60h);
         });
     } else {
         for (int d = 0; d < 3; d++)
