@@ -82,16 +82,17 @@ def kernel_bytes(name, n, b):
         "zpass_c2r3": 3 * (b * Nm + 2 * b * Nh),
         "colpass": 4 * b * Nh,
         "colpass3": 4 * b * Nh,                          # one force spectrum in and out
-        "y_fwd_sum2": 6 * b * Nh,                        # 2 gradient spectra in, 1 out
-        "sm_x_fwd": 8 * b * Nh,                          # in, U out, 2 spectra out
-        "sm_x_bwd": 8 * b * Nh,                          # 2 spectra + U in, 1 out
+        "sm_x_fwd": 6 * b * Nh,                          # spectrum in, U out, P's x-inverse out
+        "sm_x_bwd": 6 * b * Nh,                          # spectrum + U in, 1 out
+        "grad6": 5 * b * Nm,                             # potential in, interleaved forces (4 per cell) out
+        "div6": 4 * b * Nm,                              # 3 gradient meshes in, divergence out
         "x_realfilter": 4 * b * Nh,
     }.get(name)
 
 
 KERNELS = ("paint_absmax", "paint_bin", "paint_scan", "paint_scatter", "paint_deposit1", "paint_deposit3",
            "paint_deposit", "paint_convert", "zpass_r2c", "zpass_r2c3", "zpass_c2r", "zpass_c2r3", "colpass",
-           "colpass3", "y_fwd_sum2", "sm_x_fwd", "sm_x_bwd", "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
+           "colpass3", "grad6", "div6", "sm_x_fwd", "sm_x_bwd", "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
            "loss")
 
 

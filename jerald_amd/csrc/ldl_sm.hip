@@ -247,42 +247,19 @@ template <typename T> struct Tri3RowLoad {
     const c2<T>* in1;
     const c2<T>* in2;
     int64_t nh;
-    const T* Dz;
     __device__ __forceinline__ c2<T> load(int64_t R, int k) const {
         const int64_t r = R / 3;
         const int d = (int)(R - 3 * r);
         const c2<T>* in = d == 0 ? in0 : d == 1 ? in1 : in2;  // (no dynamically indexed array: it would live in scratch)
-        const c2<T> v = in[r * nh + k];
-        if (d == 2 && Dz) return c2<T>{Dz[k] * v.y, -Dz[k] * v.x};
-        return v;
+        return in[r * nh + k];
     }
 };
 
-// Rows of three stacked spectra (nrows each); with Dz set, the third
-// spectrum's bins are multiplied by i D_z(k_z) as they are stored.
+// Rows of three stacked spectra (nrows each, row pitch nh).
 template <typename T> struct HalfRowStore3 {
     c2<T>* out;
     int64_t nh, nrows;
-    const T* Dz;
-    __device__ __forceinline__ void store(int64_t row, int k, c2<T> v) const {
-        if (Dz && row >= 2 * nrows) v = c2<T>{-Dz[k] * v.y, Dz[k] * v.x};
-        out[row * nh + k] = v;
-    }
-};
-
-// Inverse y pass input of F_y: -i D_y(k_y) A, with A = x^-1(P) the x pass's
-// second output (D_y and D_z are constant along x lines, so A serves F_y and
-// F_z, and -i D_x is a central difference along x of A: the x pass does one
-// inverse transform instead of three).
-template <typename T> struct DyColOp {
-    const c2<T>* in;
-    c2<T>* out;
-    const T* Dy;
-    __device__ __forceinline__ c2<T> load(int64_t idx, int n, int64_t plane, int64_t col) const {
-        const c2<T> v = in[idx];
-        return c2<T>{Dy[n] * v.y, -Dy[n] * v.x};
-    }
-    __device__ __forceinline__ void store(int64_t idx, int n, int64_t plane, int64_t col, c2<T> v) const { out[idx] = v; }
+    __device__ __forceinline__ void store(int64_t row, int k, c2<T> v) const { out[row * nh + k] = v; }
 };
 
 // Rows taken component-fastest (Tri3RowLoad) into the interleaved field F4
@@ -377,32 +354,13 @@ __device__ __forceinline__ T xbwd_cell(const SmFilter<T>& fl, const Src2<T>& s2,
     return G;
 }
 
-// Central difference of an x line held in LDS (element n of pencil p, row
-// stride LS, circular in x): 45 (a[n+1] - a[n-1]) - 9 (a[n+2] - a[n-2]) +
-// (a[n+3] - a[n-3]).  Divided by 60 h it is the line whose transform is
-// i D_x times the line's transform (forward direction; the inverse
-// direction takes -i D_x), for the derivative symbol
-// D_x = (45 sin t - 9 sin 2t + sin 3t) / (30 h), t = 2 pi n / N, of
-// k_axis_tables: a circular shift by s multiplies transform bin n by
-// exp(+-2 pi i n s / N).
-template <typename T, int N, int LS>
-__device__ __forceinline__ c2<T> x_central_diff(const c2<T>* lds, int n, int p) {
-    using C = c2<T>;
-    auto at = [&](int o) { return lds[((n + o) & (N - 1)) * LS + p]; };
-    const C a1 = at(1), b1 = at(-1), a2 = at(2), b2 = at(-2), a3 = at(3), b3 = at(-3);
-    return C{T(45) * (a1.x - b1.x) - T(9) * (a2.x - b2.x) + (a3.x - b3.x),
-             T(45) * (a1.y - b1.y) - T(9) * (a2.y - b2.y) + (a3.y - b3.y)};
-}
-
-// Forward: X FFT, save U = . / N^3, P = 1e-2 alpha G U, one inverse X FFT
-// into out3 slot 1 (spectra hcells apart); slot 0, the inverse transform of
-// -i D_x P, is minus the central difference of that line over 60 h (inv60h),
-// read from LDS.  The y and z derivative factors are constant along x lines
-// and applied later.
+// Forward: X FFT, save U = . / N^3, P = 1e-2 alpha G U, inverse X FFT of P
+// into out.  The derivative symbols are applied in real space afterwards
+// (k_grad6).
 template <typename T, int N, bool HI>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
-k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>* __restrict__ out3, int64_t hcells,
-           KTabs<T> kt, SmFilter<T> fl, T a1e2, T invM, Src2<T> s2, T inv60h) {
+k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>* __restrict__ out, KTabs<T> kt,
+           SmFilter<T> fl, T a1e2, T invM, Src2<T> s2) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
@@ -440,30 +398,20 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
     }
     __syncthreads();
     lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
-    C* o0 = out3 + base;
-    C* o1 = out3 + hcells + base;
-#pragma unroll
-    for (int s = 0; s < EPT; s++) {
-        const int i = threadIdx.x + s * TH;
-        const int n = i / P, p = i - n * P;
-        if (p < pv) {
-            o1[n * g.es + p] = lds[n * LS + p];
-            const C d = x_central_diff<T, N, LS>(lds, n, p);
-            o0[n * g.es + p] = C{-inv60h * d.x, -inv60h * d.y};
-        }
-    }
+    tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
 }
 
-// Backward: Q = i D_x S_x + T, where the y pass already folded the y and z
-// components into T = i D_y S_y + i D_z S_z, in one forward X FFT of the
-// central difference of S_x over 60 h (inv60h) plus T; the FP64 parameter
-// partials (alpha, kh, kl, nu; HI also the second source's) against the
-// saved U, then one inverse X FFT of cpot alpha G Q into out.
+// Backward: X FFT of Q's y,z-transformed spectrum (Q = sum_d i D_d times
+// the transforms of the painted gradient meshes, formed as the transform of
+// their real-space divergence, k_div6); the FP64 parameter partials (alpha,
+// kh, kl, nu; HI also the second source's) against the saved U, then one
+// inverse X FFT of cpot alpha G Q into out (may alias Sq: each workgroup
+// reads its tile whole before it writes).
 template <typename T, int N, bool HI>
-__global__ void __launch_bounds__((FftCfg<T, N>::THREADS), (sizeof(T) == 4 ? 4 : 1))
-k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3, const c2<T>* __restrict__ U,
-           c2<T>* __restrict__ out, int64_t hcells, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh,
-           double gkl, double cpot, double alpha, int alpha_rule, Src2<T> s2, double* __restrict__ part, T inv60h) {
+__global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
+k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ U, c2<T>* out,
+           int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh, double gkl, double cpot, double alpha,
+           int alpha_rule, Src2<T> s2, double* __restrict__ part) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
@@ -475,31 +423,7 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
     stage_twiddles<T, N, TH>(twl, tw);
-    const C* sx = S3 + base;
-    const C* st = S3 + hcells + base;
-    C w[EPT];
-#pragma unroll
-    for (int s = 0; s < EPT; s++) {  // T, issued before the S_x tile
-        const int i = threadIdx.x + s * TH;
-        const int n = i / P, p = i - n * P;
-        w[s] = st[n * g.es + (p < pv ? p : pv - 1)];
-    }
-    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return sx[n * g.es + p]; });
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < EPT; s++) {
-        const int i = threadIdx.x + s * TH;
-        const int n = i / P, p = i - n * P;
-        const C d = x_central_diff<T, N, LS>(lds, n, p);
-        w[s] = p < pv ? C{inv60h * d.x + w[s].x, inv60h * d.y + w[s].y} : C{T(0), T(0)};
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < EPT; s++) {
-        const int i = threadIdx.x + s * TH;
-        const int n = i / P, p = i - n * P;
-        lds[n * LS + p] = w[s];
-    }
+    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return Sq[base + n * g.es + p]; });
     __syncthreads();
     lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
     constexpr int NP = HI ? 9 : 4;
@@ -531,45 +455,219 @@ k_sm_x_bwd(ColGeom g, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ S3
     }
 }
 
-// Forward y pass of the backward's y and z gradient spectra into one:
-// T = i D_y(k_y) y(S_y) + y(S_z), S_z already carrying i D_z(k_z) from the z
-// pass; one tile of each, the first kept in registers.  T overwrites S_y.
-template <typename T, int N>
-__global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
-k_y_fwd_sum2(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* Sy, const c2<T>* __restrict__ Sz, const T* __restrict__ Dy) {
-    using Cfg = FftCfg<T, N>;
-    using C = c2<T>;
-    constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
-    __shared__ C lds[Cfg::LDS_ELEMS];
-    __shared__ C twl[N];
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t plane = tile / g.tiles_per_plane;
-    const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
-    const int64_t base = plane * g.plane_stride + col0;
-    const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
-    stage_twiddles<T, N, TH>(twl, tw);
-    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return Sy[base + n * g.es + p]; });
-    __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
-    C r[EPT];
+// ------------------------------------------- real-space derivative stencils
+// The derivative symbol of k_axis_tables, D_d = (45 sin t - 9 sin 2t +
+// sin 3t) / (30 h_d) with t = 2 pi n / N_d, is that of the 6th-order central
+// difference
+//   diff_d(a)[x] = 45 (a[x+1] - a[x-1]) - 9 (a[x+2] - a[x-2]) + (a[x+3] - a[x-3])
+// along axis d (circular): a circular shift by s multiplies transform bin n
+// by exp(+-2 pi i n s / N), so the transform of diff_d(a) / (60 h_d) is
+// i D_d times the transform of a, and the inverse transform of -i D_d P is
+// -diff_d(phi) / (60 h_d) with phi the inverse transform of P.  The layer
+// therefore transforms one mesh each way: k_grad6 turns the potential phi
+// into the forces F_d = -diff_d(phi) / (60 h_d), and k_div6 turns the
+// painted gradient meshes g_d into sum_d diff_d(g_d) / (60 h_d), whose
+// transform is the sum_d i D_d of theirs.
+//
+// A workgroup of 256 threads covers kStTY (y) x kStTZ (z) cells (each thread
+// 4 cells along y at one z) and marches through kStXC x planes: the x
+// difference comes from a 7-plane register queue per cell, the y and z
+// differences from the current plane staged in LDS with 3-cell halos.  All
+// indices are wrapped, so a tile larger than the mesh reads valid periodic
+// values and only its stores are bounded.
+constexpr int kStTZ = 64, kStTY = 16, kStXC = 32;
+
+struct StGeom {
+    int n0, n1, n2, tiles_y, tiles_z, chunks;
+};
+
+__device__ __forceinline__ int st_wrap(int v, int n) { return ((v % n) + n) % n; }
+
+template <typename T>
+__device__ __forceinline__ T st_diff(T m3, T m2, T m1, T p1, T p2, T p3) {
+    return T(45) * (p1 - m1) - T(9) * (p2 - m2) + (p3 - m3);
+}
+
+struct StTile {
+    int x0, x1, y0, z0, tz, yg;
+    int zw, yw[4];
+    bool zok, yok[4];
+    // halo cells this thread stages: y-halo rows hy[0..1] (row index of the
+    // (kStTY + 6)-row plane, -1 none) at its own z; one z-halo cell (zrow, zcol)
+    int hy[2], hyw[2], zrow, zcol, zyw, zzw;
+    __device__ __forceinline__ StTile(const StGeom& g) {
+        const int t = (int)xcd_tile(blockIdx.x, gridDim.x);
+        const int tzt = t % g.tiles_z, tyt = (t / g.tiles_z) % g.tiles_y, xc = t / (g.tiles_z * g.tiles_y);
+        x0 = xc * kStXC;
+        x1 = x0 + kStXC < g.n0 ? x0 + kStXC : g.n0;
+        y0 = tyt * kStTY;
+        z0 = tzt * kStTZ;
+        tz = threadIdx.x & 63;
+        yg = threadIdx.x >> 6;
+        zw = st_wrap(z0 + tz, g.n2);
+        zok = z0 + tz < g.n2;
 #pragma unroll
-    for (int s = 0; s < EPT; s++) {
-        const int i = threadIdx.x + s * TH;
-        const int n = i / P, p = i - n * P;
-        const C v = lds[n * LS + p];
-        r[s] = C{-Dy[n] * v.y, Dy[n] * v.x};  // i D_y y(S_y)
-    }
-    __syncthreads();
-    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return Sz[base + n * g.es + p]; });
-    __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
+        for (int c = 0; c < 4; c++) {
+            const int y = y0 + 4 * yg + c;
+            yw[c] = st_wrap(y, g.n1);
+            yok[c] = y < g.n1;
+        }
+        // y halo: rows 0..2 and kStTY+3..kStTY+5; thread rows yg and yg + 4
 #pragma unroll
-    for (int s = 0; s < EPT; s++) {
-        const int i = threadIdx.x + s * TH;
-        const int n = i / P, p = i - n * P;
-        const C v = lds[n * LS + p];
-        if (p < pv) Sy[base + n * g.es + p] = C{r[s].x + v.x, r[s].y + v.y};
+        for (int h = 0; h < 2; h++) {
+            const int r = yg + 4 * h;
+            hy[h] = r < 6 ? (r < 3 ? r : kStTY + r) : -1;
+            hyw[h] = st_wrap(y0 + (hy[h] < 0 ? 0 : hy[h]) - 3, g.n1);
+        }
+        // z halo: kStTY rows x 6 columns (0..2, kStTZ+3..kStTZ+5), threads 0..95
+        const int tt = threadIdx.x;
+        zrow = tt < 6 * kStTY ? tt / 6 : -1;
+        const int k = tt - (tt / 6) * 6;
+        zcol = k < 3 ? k : kStTZ + k;
+        zyw = st_wrap(y0 + (zrow < 0 ? 0 : zrow), g.n1);
+        zzw = st_wrap(z0 + zcol - 3, g.n2);
     }
+};
+
+// Forces F (N0, N1, N2, 4), component 3 zero, from the potential phi.
+template <typename T>
+__global__ void __launch_bounds__(256) k_grad6(const T* __restrict__ phi, T* __restrict__ F, StGeom g, T cx, T cy, T cz) {
+    __shared__ T pl[kStTY + 6][kStTZ + 6];
+    const StTile tl(g);
+    if (tl.x0 >= g.n0) return;  // uniform over the workgroup
+    const int64_t ps = (int64_t)g.n1 * g.n2;
+    auto at = [&](int x, int y, int z) { return phi[(int64_t)st_wrap(x, g.n0) * ps + (int64_t)y * g.n2 + z]; };
+    T q[4][7];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int k = 0; k < 7; k++) q[c][k] = at(tl.x0 - 3 + k, tl.yw[c], tl.zw);
+    T hy0 = at(tl.x0, tl.hyw[0], tl.zw), hy1 = tl.hy[1] >= 0 ? at(tl.x0, tl.hyw[1], tl.zw) : T(0);
+    T hz = tl.zrow >= 0 ? at(tl.x0, tl.zyw, tl.zzw) : T(0);
+    for (int x = tl.x0; x < tl.x1; x++) {
+        __syncthreads();  // the previous plane's reads are done
+#pragma unroll
+        for (int c = 0; c < 4; c++) pl[4 * tl.yg + c + 3][tl.tz + 3] = q[c][3];
+        pl[tl.hy[0]][tl.tz + 3] = hy0;
+        if (tl.hy[1] >= 0) pl[tl.hy[1]][tl.tz + 3] = hy1;
+        if (tl.zrow >= 0) pl[tl.zrow + 3][tl.zcol] = hz;
+        __syncthreads();
+        // next plane's halo and the queue head, in flight during this plane
+        T head[4];
+        const bool more = x + 1 < tl.x1;
+        if (more) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) head[c] = at(x + 4, tl.yw[c], tl.zw);
+            hy0 = at(x + 1, tl.hyw[0], tl.zw);
+            if (tl.hy[1] >= 0) hy1 = at(x + 1, tl.hyw[1], tl.zw);
+            if (tl.zrow >= 0) hz = at(x + 1, tl.zyw, tl.zzw);
+        }
+        if (tl.zok) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                if (!tl.yok[c]) continue;
+                const int ly = 4 * tl.yg + c + 3, lz = tl.tz + 3;
+                const T dx = st_diff(q[c][0], q[c][1], q[c][2], q[c][4], q[c][5], q[c][6]);
+                const T dy = st_diff(pl[ly - 3][lz], pl[ly - 2][lz], pl[ly - 1][lz], pl[ly + 1][lz], pl[ly + 2][lz], pl[ly + 3][lz]);
+                const T dz = st_diff(pl[ly][lz - 3], pl[ly][lz - 2], pl[ly][lz - 1], pl[ly][lz + 1], pl[ly][lz + 2], pl[ly][lz + 3]);
+                T* o = F + 4 * ((int64_t)x * ps + (int64_t)(tl.y0 + 4 * tl.yg + c) * g.n2 + (tl.z0 + tl.tz));
+                if constexpr (sizeof(T) == 4) {
+                    *reinterpret_cast<float4*>(o) = float4{-cx * dx, -cy * dy, -cz * dz, 0.0f};
+                } else {
+                    reinterpret_cast<double2*>(o)[0] = double2{-cx * dx, -cy * dy};
+                    reinterpret_cast<double2*>(o)[1] = double2{-cz * dz, 0.0};
+                }
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+#pragma unroll
+                for (int k = 0; k < 6; k++) q[c][k] = q[c][k + 1];
+                q[c][6] = head[c];
+            }
+        }
+    }
+}
+
+// Divergence out = sum_d c_d diff_d(g_d) of the meshes g (3 consecutive
+// (N0, N1, N2) meshes) into a real mesh.
+template <typename T>
+__global__ void __launch_bounds__(256) k_div6(const T* __restrict__ gm, T* __restrict__ out, StGeom g, T cx, T cy, T cz) {
+    __shared__ T ly_[kStTY + 6][kStTZ];
+    __shared__ T lz_[kStTY][kStTZ + 6];
+    const StTile tl(g);
+    if (tl.x0 >= g.n0) return;  // uniform over the workgroup
+    const int64_t ps = (int64_t)g.n1 * g.n2, cells = ps * g.n0;
+    const T* gx = gm;
+    const T* gy = gm + cells;
+    const T* gz = gm + 2 * cells;
+    auto at = [&](const T* a, int x, int y, int z) { return a[(int64_t)st_wrap(x, g.n0) * ps + (int64_t)y * g.n2 + z]; };
+    T q[4][7], vy[4], vz[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+        for (int k = 0; k < 7; k++) q[c][k] = at(gx, tl.x0 - 3 + k, tl.yw[c], tl.zw);
+        vy[c] = at(gy, tl.x0, tl.yw[c], tl.zw);
+        vz[c] = at(gz, tl.x0, tl.yw[c], tl.zw);
+    }
+    T hy0 = at(gy, tl.x0, tl.hyw[0], tl.zw), hy1 = tl.hy[1] >= 0 ? at(gy, tl.x0, tl.hyw[1], tl.zw) : T(0);
+    T hz = tl.zrow >= 0 ? at(gz, tl.x0, tl.zyw, tl.zzw) : T(0);
+    for (int x = tl.x0; x < tl.x1; x++) {
+        __syncthreads();  // the previous plane's reads are done
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            ly_[4 * tl.yg + c + 3][tl.tz] = vy[c];
+            lz_[4 * tl.yg + c][tl.tz + 3] = vz[c];
+        }
+        ly_[tl.hy[0]][tl.tz] = hy0;
+        if (tl.hy[1] >= 0) ly_[tl.hy[1]][tl.tz] = hy1;
+        if (tl.zrow >= 0) lz_[tl.zrow][tl.zcol] = hz;
+        __syncthreads();
+        T head[4];
+        const bool more = x + 1 < tl.x1;
+        if (more) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                head[c] = at(gx, x + 4, tl.yw[c], tl.zw);
+                vy[c] = at(gy, x + 1, tl.yw[c], tl.zw);
+                vz[c] = at(gz, x + 1, tl.yw[c], tl.zw);
+            }
+            hy0 = at(gy, x + 1, tl.hyw[0], tl.zw);
+            if (tl.hy[1] >= 0) hy1 = at(gy, x + 1, tl.hyw[1], tl.zw);
+            if (tl.zrow >= 0) hz = at(gz, x + 1, tl.zyw, tl.zzw);
+        }
+        if (tl.zok) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                if (!tl.yok[c]) continue;
+                const int r = 4 * tl.yg + c, z = tl.tz;
+                const T dx = st_diff(q[c][0], q[c][1], q[c][2], q[c][4], q[c][5], q[c][6]);
+                const T dy = st_diff(ly_[r][z], ly_[r + 1][z], ly_[r + 2][z], ly_[r + 4][z], ly_[r + 5][z], ly_[r + 6][z]);
+                const T dz = st_diff(lz_[r][z], lz_[r][z + 1], lz_[r][z + 2], lz_[r][z + 4], lz_[r][z + 5], lz_[r][z + 6]);
+                out[(int64_t)x * ps + (int64_t)(tl.y0 + r) * g.n2 + (tl.z0 + z)] = cx * dx + cy * dy + cz * dz;
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+#pragma unroll
+                for (int k = 0; k < 6; k++) q[c][k] = q[c][k + 1];
+                q[c][6] = head[c];
+            }
+        }
+    }
+}
+
+static StGeom st_geom(const Mesh& m) {
+    StGeom g;
+    g.n0 = (int)m.n0;
+    g.n1 = (int)m.n1;
+    g.n2 = (int)m.n2;
+    g.tiles_y = (int)((m.n1 + kStTY - 1) / kStTY);
+    g.tiles_z = (int)((m.n2 + kStTZ - 1) / kStTZ);
+    g.chunks = (int)((m.n0 + kStXC - 1) / kStXC);
+    return g;
 }
 
 // X pass with a real isotropic weight: forward, times W(k) scale, inverse.
@@ -1231,14 +1329,13 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
         dispatch_n(m.n0, [&](auto N) {
             constexpr int NN = decltype(N)::value;
             ColGeom g = col_geom<T, NN>(m.n1, nh, hp, m.n1 * hp);
-            const T inv60h = (T)((double)m.n0 / (60.0 * pdesc->box[0]));
             KTimer _t("sm_x_fwd", st);
             if (hi)
-                hipLaunchKernelGGL((k_sm_x_fwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, hc,
-                                   kt, fl, a1e2, invM, s2, inv60h);
+                hipLaunchKernelGGL((k_sm_x_fwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, kt,
+                                   fl, a1e2, invM, s2);
             else
                 hipLaunchKernelGGL((k_sm_x_fwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3,
-                                   hc, kt, fl, a1e2, invM, s2, inv60h);
+                                   kt, fl, a1e2, invM, s2);
         });
     } else {
         if ((rc = colpass<T, -1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{U, U, invM}, st))) return rc;
@@ -1251,18 +1348,19 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
             if ((rc = colpass<T, +1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{P.spec3 + d * hc, P.spec3 + d * hc, T(1)}, st))) return rc;
     }
     // 4. inverse y passes (3 force spectra as 3*N0 planes) and z c2r into F
-    if (fft_mesh_pow2(m)) {  // from the two x-pass outputs Bx (slot 0) and A (slot 1); F_y's spectrum goes to the free mesh scratch
-        c2<T>* Cy = reinterpret_cast<c2<T>*>(P.mesh3);
-        c2<T>* A = P.spec3 + hc;
-        if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
-        if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, DyColOp<T>{A, Cy, kt.D[1]}, st, "colpass3"))) return rc;
-        if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{A, P.spec3 + 2 * hc, T(1)}, st, "colpass3"))) return rc;
-        if ((rc = zpass_c2r<T>(3 * nrows, m.n2, tw, Tri3RowLoad<T>{P.spec3, Cy, P.spec3 + 2 * hc, hp, kt.D[2]}, Tri3Store4<T>{F, m.n2}, st,
-                               nullptr, "zpass_c2r3")))
+    if (fft_mesh_pow2(m)) {  // phi = inverse transform of P (x pass output in slot 0), F = -grad6(phi)
+        T* phi = P.mesh3;
+        if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st))) return rc;
+        if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{P.spec3, hp}, RealRowStore<T>{phi, m.n2, T(1)}, st, nullptr)))
             return rc;
+        const StGeom sg = st_geom(m);
+        KTimer _t("grad6", st);
+        hipLaunchKernelGGL(k_grad6<T>, dim3((unsigned)(sg.chunks * sg.tiles_y * sg.tiles_z)), dim3(256), 0, st, phi, F, sg,
+                           (T)((double)m.n0 / (60.0 * pdesc->box[0])), (T)((double)m.n1 / (60.0 * pdesc->box[1])),
+                           (T)((double)m.n2 / (60.0 * pdesc->box[2])));
     } else {
         if ((rc = colpass<T, +1>(m.n1, 3 * m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
-        if ((rc = zpass_c2r<T>(3 * nrows, m.n2, tw, Tri3RowLoad<T>{P.spec3, P.spec3 + hc, P.spec3 + 2 * hc, hp, nullptr},
+        if ((rc = zpass_c2r<T>(3 * nrows, m.n2, tw, Tri3RowLoad<T>{P.spec3, P.spec3 + hc, P.spec3 + 2 * hc, hp},
                                Tri3Store4<T>{F, m.n2}, st, nullptr, "zpass_c2r3")))
             return rc;
     }
@@ -1310,24 +1408,26 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
                             reuse ? reinterpret_cast<const unsigned*>(bins) : nullptr, rho, 1.0 / fact, gmax)))
         return rc;
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
-    // 2. unnormalised r2c of the three meshes: z and y passes
-    // (power-of-two meshes: the z component times i D_z(k_z) as it is stored,
-    // and the y pass folds the y and z components into one spectrum)
+    // 2. unnormalised r2c: power-of-two meshes transform the real-space
+    // divergence of the three meshes (one spectrum, slot 0); others the three
+    // meshes (z and y passes; the filter kernel applies the i D_d)
     const bool p2 = fft_mesh_pow2(m);
-    if ((rc = zpass_r2c<T>(3 * nrows, m.n2, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore3<T>{P.spec3, hp, nrows, p2 ? kt.D[2] : nullptr},
-                           st, "zpass_r2c3")))
-        return rc;
     if (p2) {
+        T* dv = reinterpret_cast<T*>(P.spec3 + 2 * hc);  // a real mesh in the free third spectrum
+        const StGeom sg = st_geom(m);
+        {
+            KTimer _t("div6", st);
+            hipLaunchKernelGGL(k_div6<T>, dim3((unsigned)(sg.chunks * sg.tiles_y * sg.tiles_z)), dim3(256), 0, st, P.mesh3, dv, sg,
+                               (T)((double)m.n0 / (60.0 * pdesc->box[0])), (T)((double)m.n1 / (60.0 * pdesc->box[1])),
+                               (T)((double)m.n2 / (60.0 * pdesc->box[2])));
+        }
+        if ((rc = zpass_r2c<T>(nrows, m.n2, tw, RealRowLoad<T>{dv, m.n2}, HalfRowStore<T>{P.spec3, hp, T(1)}, st))) return rc;
         if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st))) return rc;
-        dispatch_n(m.n1, [&](auto N) {
-            constexpr int NN = decltype(N)::value;
-            const ColGeom g = col_geom<T, NN>(m.n0, nh, m.n1 * hp, hp);
-            KTimer _t("y_fwd_sum2", st);
-            hipLaunchKernelGGL((k_y_fwd_sum2<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3 + hc,
-                               P.spec3 + 2 * hc, kt.D[1]);
-        });
-    } else if ((rc = colpass<T, -1>(m.n1, 3 * m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) {
-        return rc;
+    } else {
+        if ((rc = zpass_r2c<T>(3 * nrows, m.n2, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore3<T>{P.spec3, hp, nrows}, st,
+                               "zpass_r2c3")))
+            return rc;
+        if ((rc = colpass<T, -1>(m.n1, 3 * m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
     }
     // HI: T2 = r2c(starmap^gamma2 ln starmap) / N^3 for the gamma2 partial, in
     // the mesh scratch (free after the z pass above; sized to hold it)
@@ -1351,14 +1451,13 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
             constexpr int NN = decltype(N)::value;
             ColGeom g = col_geom<T, NN>(m.n1, nh, hp, m.n1 * hp);
             ntx = g.ntiles;
-            const T inv60h = (T)((double)m.n0 / (60.0 * pdesc->box[0]));
             KTimer _t("sm_x_bwd", st);
             if (hi)
-                hipLaunchKernelGGL((k_sm_x_bwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U, S,
-                                   hc, m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part, inv60h);
+                hipLaunchKernelGGL((k_sm_x_bwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, P.spec3, tw, U, S,
+                                   m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
             else
-                hipLaunchKernelGGL((k_sm_x_bwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, P.spec3, U,
-                                   S, hc, m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part, inv60h);
+                hipLaunchKernelGGL((k_sm_x_bwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, P.spec3, tw, U,
+                                   S, m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
         });
     } else {
         for (int d = 0; d < 3; d++)

@@ -284,3 +284,27 @@ def test_ldl_against_torch_autograd(kind):
     lt.backward()
     assert abs(lt.item() - loss) < 1e-12 * max(1, abs(loss))
     np.testing.assert_allclose(pt.grad.numpy(), g, rtol=1e-9, atol=1e-14)
+
+
+@pytest.mark.parametrize("nmesh,box", [((8, 12, 16), (8.0, 6.0, 20.0)), ((4, 4, 6), (1.0, 2.0, 3.0))])
+def test_central_difference_is_the_derivative_symbol(nmesh, box):
+    """The layer applies the derivative symbol D_d (model.py:200-201) in real
+    space: the transform of the 6th-order central difference over 60 h_d is
+    i D_d times the transform (forward), and the inverse transform of
+    -i D_d P is minus that difference of P's inverse transform.  Checked
+    against the symbol of the oracle, circularly, also where the +-3 offsets
+    wrap more than once around a 4-cell axis."""
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal(nmesh)
+    D = O.fd_derivative_coeff(nmesh, box)
+    for d in range(3):
+        h = box[d] / nmesh[d]
+
+        def diff(v):
+            sh = lambda s: np.roll(v, -s, axis=d)  # noqa: E731  (v[x + s])
+            return (45 * (sh(1) - sh(-1)) - 9 * (sh(2) - sh(-2)) + (sh(3) - sh(-3))) / (60 * h)
+
+        np.testing.assert_allclose(np.fft.rfftn(diff(a)), 1j * D[d] * np.fft.rfftn(a), rtol=1e-12, atol=1e-12)
+        P = np.fft.rfftn(rng.standard_normal(nmesh))
+        np.testing.assert_allclose(np.fft.irfftn(-1j * D[d] * P, s=nmesh), -diff(np.fft.irfftn(P, s=nmesh)),
+                                   rtol=1e-12, atol=1e-12)
