@@ -306,5 +306,6 @@ def test_central_difference_is_the_derivative_symbol(nmesh, box):
 
         np.testing.assert_allclose(np.fft.rfftn(diff(a)), 1j * D[d] * np.fft.rfftn(a), rtol=1e-12, atol=1e-12)
         P = np.fft.rfftn(rng.standard_normal(nmesh))
-        np.testing.assert_allclose(np.fft.irfftn(-1j * D[d] * P, s=nmesh), -diff(np.fft.irfftn(P, s=nmesh)),
+        np.testing.assert_allclose(np.fft.irfftn(-1j * D[d] * P, s=nmesh, axes=(0, 1, 2)),
+                                   -diff(np.fft.irfftn(P, s=nmesh, axes=(0, 1, 2))),
                                    rtol=1e-12, atol=1e-12)
