@@ -278,7 +278,7 @@ def main():
         "config": {"workload": f"{n}^3 particles on {n}^3 mesh, {nstep} sm LDL layers, L2 loss with 1-cell "
                                f"Gaussian smoothing, loss+grad of {5 * nstep + 3} params",
                    "n": n, "nstep": nstep, "box": box, "positions": "fp32 displacements from lattice",
-                   "paint": "64-bit fixed point", "save": args.save},
+                   "paint": "fixed point (integer deposits, deterministic)", "save": args.save},
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp64_plain_gpu": fp64,

@@ -353,3 +353,35 @@ def test_lossv_and_reference_smoothing():
     # the layer gradients come from the non-DC part of a residual whose DC
     # weight is 1e8, so they carry ~1e-8 relative rounding from either side
     np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-5)
+
+
+@pytest.mark.parametrize("lattice", [False, True])
+def test_ldl_noncubic_mesh_over_several_stencil_tiles(lattice):
+    """A non-cubic power-of-two mesh larger than one tile of the real-space
+    difference kernels along every axis (x 64 > 32-plane chunks, y 32 > 16
+    rows, z 128 > 64 cells) in a non-cubic box, so halos cross tiles and
+    chunks: FP64 loss and gradients against the oracle, FP32 against FP64."""
+    nm, box, nstep = (64, 32, 128), (64.0, 40.0, 96.0), 2
+    rng = np.random.default_rng(3)
+    if lattice:
+        lat = (32, 16, 64)
+        idx = np.stack(np.meshgrid(*[np.arange(v) for v in lat], indexing="ij"), axis=-1).reshape(-1, 3)
+        X = idx * (np.array(box) / np.array(lat)) + rng.uniform(-2.0, 2.0, idx.shape)
+    else:
+        lat = None
+        X = rng.random((32768, 3)) * np.array(box)
+    tgt = rng.random(nm) + 0.5
+    p = np.array([0.8, 1.2, 0.3, 1.5, 0.2] * nstep + [1.1, 0.9, 0.05])
+    orc = O.SMModelOracle(nm, box, nstep, tgt, L1=False, smoothing=("gaussian", 2.0))
+    loss0, g0 = orc.loss_and_grad(p, X)
+    out = {}
+    for dt in (torch.float64, torch.float32):
+        pm = PMesh(nm, box, device=DEV, dtype=dt)
+        m = JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, kind="sm", L1=False, smoothing=("gaussian", 2.0), lattice=lat)
+        loss, grad = m.loss_and_grad(torch.tensor(p))
+        out[dt] = (loss.item(), grad.cpu().numpy())
+    (l64, g64), (l32, g32) = out[torch.float64], out[torch.float32]
+    assert abs(l64 - loss0) <= 1e-10 * abs(loss0)
+    np.testing.assert_allclose(g64, g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
+    assert abs(l32 - l64) <= 1e-5 * abs(l64)
+    assert np.all(np.abs(g32 - g64) <= 1e-4 * np.abs(g64) + 1e-6 * np.abs(g64).max()), (g32, g64)
