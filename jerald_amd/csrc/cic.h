@@ -571,27 +571,28 @@ __global__ void __launch_bounds__(256) k_readout_grad3(const T* __restrict__ pos
 }
 
 struct TileGeom {
-    int ts;         // tile side in cells (a power of two)
-    int tsh;        // log2(ts)
+    int sh[3];      // log2 of the tile's side along each axis (cells)
     int64_t nt[3];  // tiles per axis (the last tile of an axis may be partial)
     int64_t ntiles;
     int64_t nkeys() const { return ntiles * 8; }
     __host__ __device__ int64_t extent(int d, int64_t t, const Mesh& m) const {
         const int64_t n = d == 0 ? m.n0 : d == 1 ? m.n1 : m.n2;
+        const int64_t ts = (int64_t)1 << sh[d];
         const int64_t lo = t * ts;
         return n - lo < ts ? n - lo : ts;
     }
 };
 
-inline TileGeom make_tiles(const Mesh& m, int ts) {
+// Tiles of 2^sh[0] x 2^sh[1] x 2^sh[2] cells.
+inline TileGeom make_tiles(const Mesh& m, const int sh[3]) {
     TileGeom g;
-    g.ts = ts;
-    g.tsh = 0;
-    while ((1 << g.tsh) < ts) g.tsh++;
-    g.nt[0] = (m.n0 + ts - 1) / ts;
-    g.nt[1] = (m.n1 + ts - 1) / ts;
-    g.nt[2] = (m.n2 + ts - 1) / ts;
-    g.ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+    const int64_t n[3] = {m.n0, m.n1, m.n2};
+    g.ntiles = 1;
+    for (int d = 0; d < 3; d++) {
+        g.sh[d] = sh[d];
+        g.nt[d] = (n[d] + ((int64_t)1 << sh[d]) - 1) >> sh[d];
+        g.ntiles *= g.nt[d];
+    }
     return g;
 }
 
@@ -618,8 +619,8 @@ __device__ __forceinline__ unsigned tile_key(const uint32_t id[3], const TileGeo
     const int64_t n[3] = {m.n0, m.n1, m.n2};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        t[d] = id[d] >> tg.tsh;
-        const int64_t hi = ((int64_t)t[d] + 1) * tg.ts;  // one past the tile's last cell
+        t[d] = id[d] >> tg.sh[d];
+        const int64_t hi = ((int64_t)t[d] + 1) << tg.sh[d];  // one past the tile's last cell
         if ((int64_t)id[d] + 1 == (hi < n[d] ? hi : n[d])) fc |= 1u << d;
     }
     return (((t[0] * (unsigned)tg.nt[1] + t[1]) * (unsigned)tg.nt[2] + t[2]) << 3) | fc;
@@ -782,7 +783,7 @@ static __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const un
 // the largest CIC weight sum of any cell, and every shift drops by g with
 // 2^g > 2 W + 1, so a cell's |sum| stays below 2^29 plus 4 per contribution:
 // the int32 sums equal the int64 ones of the same deposits.
-template <typename T, int NC, int TS, int TB, bool P2, bool ACC32 = false>
+template <typename T, int NC, int TX, int TY, int TZ, int TB, bool P2, bool ACC32 = false>
 __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ mass,
                                                      T mscalar, Mesh m, MeshP2 mp, TileGeom tg, const unsigned* __restrict__ offsets,
                                                      const unsigned* __restrict__ sorted, const unsigned long long* __restrict__ hdr,
@@ -790,7 +791,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     static_assert(!ACC32 || sizeof(T) == 4, "32-bit accumulation takes the 32-bit FP32 deposits");
     using Acc = typename std::conditional<ACC32, unsigned int, unsigned long long>::type;
     using SAcc = typename std::conditional<ACC32, int, long long>::type;
-    constexpr int B3 = TS * TS * TS;
+    constexpr int B3 = TX * TY * TZ;
     constexpr int NR = 20;  // 1 own range + 12 + 6 + 1 neighbour ranges
     __shared__ Acc box[NC * B3];
     __shared__ unsigned rstart[NR], rpre[NR + 1];
@@ -799,7 +800,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     // whose particles share cache lines of the Lagrangian-ordered arrays
     const int64_t t = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t tz = t % tg.nt[2], ty = (t / tg.nt[2]) % tg.nt[1], tx = t / (tg.nt[2] * tg.nt[1]);
-    const int64_t lo[3] = {tx * TS, ty * TS, tz * TS};
+    const int64_t lo[3] = {tx * TX, ty * TY, tz * TZ};
     const int ex = (int)tg.extent(0, tx, m), ey = (int)tg.extent(1, ty, m), ez = (int)tg.extent(2, tz, m);
     for (int i = threadIdx.x; i < NC * B3; i += TB) box[i] = Acc(0);
     if (threadIdx.x < 64) {
@@ -903,7 +904,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
             const bool in_x[2] = {a >= 0, a + 1 < ex};
             const bool in_y[2] = {b >= 0, b + 1 < ey};
             const bool in_z[2] = {cz >= 0, cz + 1 < ez};
-            const int base = (a * TS + b) * TS + cz;
+            const int base = (a * TY + b) * TZ + cz;
             // all 8 corners in the tile (face class 0 of the own tile, most
             // particles, stored together): unconditional deposits
             const bool all_in = in_x[0] && in_x[1] && in_y[0] && in_y[1] && in_z[0] && in_z[1];
@@ -915,7 +916,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
 #pragma unroll
                     for (int qq = 0; qq < 8; qq++) {
                         const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
-                        atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (Acc)(SAcc)q[qq]);
+                        atomicAdd(&box[c * B3 + base + dx * TY * TZ + dy * TZ + dz], (Acc)(SAcc)q[qq]);
                     }
                 }
             } else {
@@ -927,7 +928,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
                     for (int qq = 0; qq < 8; qq++) {
                         const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
                         if (in_x[dx] && in_y[dy] && in_z[dz] && q[qq] != 0)
-                            atomicAdd(&box[c * B3 + base + dx * TS * TS + dy * TS + dz], (Acc)(SAcc)q[qq]);
+                            atomicAdd(&box[c * B3 + base + dx * TY * TZ + dy * TZ + dz], (Acc)(SAcc)q[qq]);
                     }
                 }
             }
@@ -936,7 +937,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     __syncthreads();
     const int64_t cells = m.cells();
     for (int li = threadIdx.x; li < B3; li += TB) {
-        const int cc = li % TS, bb = (li / TS) % TS, aa = li / (TS * TS);
+        const int cc = li % TZ, bb = (li / TZ) % TY, aa = li / (TZ * TY);
         if (aa >= ex || bb >= ey || cc >= ez) continue;
         const int64_t gi = ((lo[0] + aa) * m.n1 + (lo[1] + bb)) * m.n2 + (lo[2] + cc);
 #pragma unroll
@@ -951,7 +952,7 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
                const int64_t nmesh[3], double out_scale, int32_t flags, T* out, void* ws, size_t ws_bytes,
                hipStream_t st, unsigned* bins_out = nullptr, const unsigned* bins_in = nullptr, const T* wmesh = nullptr,
                double wscale = 0.0, const unsigned long long* absmax_in = nullptr);
-// Particle sort by 16^3-cell tile and face class that a 1-component paint can
+// Particle sort by tile and face class that a 1-component paint can
 // hand to a later 3-component paint of the same positions: offsets
 // (8 ntiles + 1) then sorted particle indices (np), uint32.
 size_t paint_bins_bytes(const Mesh& m, int64_t np);

@@ -19,13 +19,16 @@ namespace ldl {
 static constexpr int kBlock = 256;
 static constexpr int64_t kHdrBytes = 256;
 
-// Tile side of the tiled paint: 16^3 cells = 32 KB of 64-bit LDS accumulators
-// per component (one component: several workgroups per CU; three: 96 KB, one
-// workgroup of 1024 threads per CU).  The 32-bit three-component paint holds
-// 48 KB: three workgroups of 512 threads per CU (6 waves per SIMD), so one
-// tile's LDS clear and write-out overlap the others' deposits (measured at
-// 512^3: 1.85 ms per launch against 2.49 ms for 1024 threads).
-static constexpr int kTileSide = 16;
+// Tiles of the tiled paint: 8 x 16 x 32 cells (z fastest) = 32 KB of 64-bit
+// LDS accumulators per component (one component: several workgroups per CU;
+// three: 96 KB, one workgroup of 1024 threads per CU).  Long z runs keep the
+// particles a tile reads from the lattice-ordered arrays in runs of 32
+// consecutive rows' worth of bytes, so fewer cache lines straddle two tiles.
+// The 32-bit three-component paint holds 48 KB: three workgroups of 512
+// threads per CU (6 waves per SIMD), so one tile's LDS clear and write-out
+// overlap the others' deposits.
+static constexpr int kTileX = 8, kTileY = 16, kTileZ = 32;
+static constexpr int kTileSh[3] = {3, 4, 5};
 static constexpr int kAcc32Block = 512;
 
 static inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -36,7 +39,7 @@ struct PaintWs {
     size_t acc, counts, offsets, segsum, key, slot, sorted, total;
 };
 static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
-    const TileGeom tg = make_tiles(m, kTileSide);
+    const TileGeom tg = make_tiles(m, kTileSh);
     const int64_t nk = tg.nkeys();
     PaintWs w;
     w.acc = al256((size_t)kHdrBytes);
@@ -53,7 +56,7 @@ static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
 size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np) { return paint_ws(m, nc, np < 0 ? 0 : np).total; }
 
 size_t paint_bins_bytes(const Mesh& m, int64_t np) {
-    const TileGeom tg = make_tiles(m, kTileSide);
+    const TileGeom tg = make_tiles(m, kTileSh);
     return (size_t)(tg.nkeys() + 1 + (np < 0 ? 0 : np)) * 4;
 }
 
@@ -128,7 +131,7 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     // tile sums the tile's cells completely in LDS.  A 3-component paint that
     // is handed the sort of an earlier 1-component paint of the same positions
     // (bins_in) skips the sort.
-    const TileGeom tg = make_tiles(m, kTileSide);
+    const TileGeom tg = make_tiles(m, kTileSh);
     const int64_t nk = tg.nkeys();
     // power-of-two meshes decode with 32-bit mask wraps (bin and deposit kernels
     // must use the same decode)
@@ -171,10 +174,10 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     if (nc == 1) {
         KTimer _t("paint_deposit1", st);
         if (p2)
-            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide, 256, true>), gt, dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
+            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileX, kTileY, kTileZ, 256, true>), gt, dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
                                tg, offsets, sorted, hdr, out_scale, out);
         else
-            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileSide, 256, false>), gt, dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
+            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileX, kTileY, kTileZ, 256, false>), gt, dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
                                tg, offsets, sorted, hdr, out_scale, out);
     } else if (sizeof(T) == 4 && wmesh) {
         // FP32 with the positions' unit-mass density known: 32-bit accumulators
@@ -185,19 +188,19 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         KTimer _t("paint_deposit3", st);
         if constexpr (sizeof(T) == 4) {
             if (p2)
-                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, kAcc32Block, true, true>), gt, dim3(kAcc32Block), 0, st, pos, pd, np, mass, (T)mscalar, m,
+                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileX, kTileY, kTileZ, kAcc32Block, true, true>), gt, dim3(kAcc32Block), 0, st, pos, pd, np, mass, (T)mscalar, m,
                                    mp, tg, offsets, sorted, hdr, out_scale, out);
             else
-                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, kAcc32Block, false, true>), gt, dim3(kAcc32Block), 0, st, pos, pd, np, mass, (T)mscalar, m,
+                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileX, kTileY, kTileZ, kAcc32Block, false, true>), gt, dim3(kAcc32Block), 0, st, pos, pd, np, mass, (T)mscalar, m,
                                    mp, tg, offsets, sorted, hdr, out_scale, out);
         }
     } else {
         KTimer _t("paint_deposit3", st);
         if (p2)
-            hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, 1024, true>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
+            hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileX, kTileY, kTileZ, 1024, true>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
                                tg, offsets, sorted, hdr, out_scale, out);
         else
-            hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileSide, 1024, false>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
+            hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileX, kTileY, kTileZ, 1024, false>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
                                tg, offsets, sorted, hdr, out_scale, out);
     }
     LDL_LAUNCH_CHECK();
