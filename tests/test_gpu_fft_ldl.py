@@ -385,3 +385,29 @@ def test_ldl_noncubic_mesh_over_several_stencil_tiles(lattice):
     np.testing.assert_allclose(g64, g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
     assert abs(l32 - l64) <= 1e-5 * abs(l64)
     assert np.all(np.abs(g32 - g64) <= 1e-4 * np.abs(g64) + 1e-6 * np.abs(g64).max()), (g32, g64)
+
+
+@pytest.mark.parametrize("kind", ["dm", "HI"])
+def test_dm_and_hi_kinds_on_noncubic_multi_tile_mesh(kind):
+    """The dm and HI kinds on the non-cubic mesh that spans several tiles of
+    the real-space difference kernels along every axis: FP64 loss and
+    gradients against the oracle."""
+    nm, box, nstep = (64, 32, 128), (64.0, 40.0, 96.0), 2
+    rng = np.random.default_rng(4)
+    X = rng.random((32768, 3)) * np.array(box)
+    tgt = rng.random(nm) + 0.5
+    lay = [[8.0, 1.2, 0.3, 1.5, 0.2, 5.0, 0.8, 0.2, 1.1, 0.3], [5.0, 0.9, 0.2, 1.6, -0.1, 3.0, 1.1, 0.25, 1.3, 0.1]]
+    star = None
+    if kind == "dm":
+        p = np.array([0.8, 1.2, 0.3, 1.5, 0.2, 0.6, 0.9, 0.2, 1.6, -0.1])
+    else:
+        star = rng.random(nm) * 1.5
+        star[rng.random(nm) < 0.1] = 0.0
+        p = np.array(lay[0] + lay[1] + [1.1, 0.9, 0.3, 0.4, 0.05, 2.0, 0.3, 0.2, 1.3, 0.1, 0.02, 0.05])
+    orc = O.SMModelOracle(nm, box, nstep, tgt, L1=False, smoothing=("gaussian", 2.0), kind=kind, starmap=star)
+    loss0, g0 = orc.loss_and_grad(p, X)
+    pm = PMesh(nm, box, device=DEV, dtype=torch.float64)
+    m = JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, kind=kind, L1=False, smoothing=("gaussian", 2.0), starmap=star)
+    loss, grad = m.loss_and_grad(torch.tensor(p))
+    assert abs(loss.item() - loss0) <= 1e-10 * abs(loss0)
+    np.testing.assert_allclose(grad.cpu().numpy(), g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
