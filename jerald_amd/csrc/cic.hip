@@ -71,7 +71,7 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     if (np < 0) return fail(LDL_EINVAL, "np must be >= 0");
     if (!out) return fail(LDL_EINVAL, "out is NULL");
     if (np > 0 && !pos) return fail(LDL_EINVAL, "pos is NULL");
-    if (nc == 3 && !mass) return fail(LDL_EINVAL, "ppaint3D needs a (Np,3) mass array");
+    if (nc == 3 && !mass && np > 0) return fail(LDL_EINVAL, "ppaint3D needs a (Np,3) mass array");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
     const int64_t cells = m.cells();

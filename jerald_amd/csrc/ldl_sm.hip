@@ -255,12 +255,6 @@ template <typename T> struct Tri3RowLoad {
     }
 };
 
-// Rows of three stacked spectra (nrows each, row pitch nh).
-template <typename T> struct HalfRowStore3 {
-    c2<T>* out;
-    int64_t nh, nrows;
-    __device__ __forceinline__ void store(int64_t row, int k, c2<T> v) const { out[row * nh + k] = v; }
-};
 
 // Rows taken component-fastest (Tri3RowLoad) into the interleaved field F4
 // (x, y, z, 0): the power-of-two z pass hands over whole threes of rows
@@ -1424,7 +1418,7 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
         if ((rc = zpass_r2c<T>(nrows, m.n2, tw, RealRowLoad<T>{dv, m.n2}, HalfRowStore<T>{P.spec3, hp, T(1)}, st))) return rc;
         if ((rc = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st))) return rc;
     } else {
-        if ((rc = zpass_r2c<T>(3 * nrows, m.n2, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore3<T>{P.spec3, hp, nrows}, st,
+        if ((rc = zpass_r2c<T>(3 * nrows, m.n2, tw, RealRowLoad<T>{P.mesh3, m.n2}, HalfRowStore<T>{P.spec3, hp, T(1)}, st,
                                "zpass_r2c3")))
             return rc;
         if ((rc = colpass<T, -1>(m.n1, 3 * m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;

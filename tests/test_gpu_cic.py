@@ -238,3 +238,26 @@ def test_autograd_adjoints(dt):
     tp2.requires_grad_(True)
     Fg = torch.tensor(rng.standard_normal(nm), device=DEV, dtype=dt)
     assert torch.autograd.gradcheck(lambda p: J.preadout(p, Fg, nm, box), (tp2,), eps=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_empty_particle_set(dt):
+    """No particles: paints are zero meshes, readouts are empty, and the
+    backward passes give empty position gradients and zero mesh gradients."""
+    nm, box = (16, 8, 32), (16.0, 8.0, 32.0)
+    pos = torch.zeros((0, 3), device=DEV, dtype=dt, requires_grad=True)
+    assert torch.count_nonzero(J.ppaint(pos, 1.0, nm, box)) == 0
+    m1 = torch.zeros((0,), device=DEV, dtype=dt, requires_grad=True)
+    rho = J.ppaint(pos, m1, nm, box)
+    assert rho.shape == nm and torch.count_nonzero(rho) == 0
+    m3 = torch.zeros((0, 3), device=DEV, dtype=dt, requires_grad=True)
+    rho3 = J.ppaint3D(pos, m3, nm, box)
+    assert torch.count_nonzero(rho3) == 0
+    F = torch.randn(nm, device=DEV, dtype=dt, requires_grad=True)
+    F3 = torch.randn((3,) + nm, device=DEV, dtype=dt, requires_grad=True)
+    r = J.preadout(pos, F, nm, box)
+    r3 = J.preadout3D(pos, F3, nm, box)
+    assert r.shape == (0,) and r3.shape == (0, 3)
+    (rho.sum() + rho3.sum() + r.sum() + r3.sum()).backward()
+    assert pos.grad.shape == (0, 3) and m1.grad.shape == (0,) and m3.grad.shape == (0, 3)
+    assert torch.count_nonzero(F.grad) == 0 and torch.count_nonzero(F3.grad) == 0
