@@ -626,24 +626,20 @@ __device__ __forceinline__ unsigned tile_key(const uint32_t id[3], const TileGeo
     return (((t[0] * (unsigned)tg.nt[1] + t[1]) * (unsigned)tg.nt[2] + t[2]) << 3) | fc;
 }
 
-// Counting-sort pass 1: key of each particle and its rank among the particles
-// with the same key.  Consecutive lanes hold consecutive particles, and
-// particles stored in lattice order fall into a few runs of equal keys per
-// wavefront: each run's first lane reserves the run's ranks with one global
-// atomic and broadcasts the base to the run.  Rank order inside a key depends
-// on atomic order, which no paint result depends on (integer sums).
+// Counting-sort pass 1: key of each particle, and the particle counts per
+// key.  Consecutive lanes hold consecutive particles, and particles stored in
+// lattice order fall into a few runs of equal keys per wavefront: each run's
+// first lane adds the run's length with one global atomic.
 template <typename T, int PPT, bool P2>
 __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Mesh m, MeshP2 mp, TileGeom tg,
-                                                  unsigned* __restrict__ counts, unsigned* __restrict__ key,
-                                                  unsigned* __restrict__ slot) {
+                                                  unsigned* __restrict__ counts, unsigned* __restrict__ key) {
     constexpr int CH = 256 * PPT;
     constexpr unsigned EMPTY = 0xFFFFFFFFu;
     const int lane = threadIdx.x & 63;
     const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
     const int64_t nchunks = (np + CH - 1) / CH;
     for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-        unsigned k[PPT], base[PPT];
-        int hl[PPT];
+        unsigned k[PPT];
 #pragma unroll
         for (int q = 0; q < PPT; q++) {
             const int64_t p = ch * CH + q * 256 + threadIdx.x;
@@ -653,28 +649,17 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
                 T f[3];
                 cell_of<T, P2>(pd, mp, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, id, f);
                 k[q] = tile_key(id, tg, m);
+                key[p] = k[q];
             }
         }
-        // run heads, run lengths and one atomic per run (all issued before any result is used)
 #pragma unroll
         for (int q = 0; q < PPT; q++) {
             const unsigned prev = __shfl_up(k[q], 1);
             const bool head = lane == 0 || prev != k[q];
             const unsigned long long heads = __ballot(head);
-            hl[q] = 63 - __clzll(heads & upto);
             const unsigned long long after = heads & ~upto;
             const int next = after ? __ffsll((long long)after) - 1 : 64;
-            base[q] = 0u;
-            if (head && k[q] != EMPTY) base[q] = atomicAdd(&counts[k[q]], (unsigned)(next - lane));
-        }
-#pragma unroll
-        for (int q = 0; q < PPT; q++) {
-            const unsigned b = __shfl(base[q], hl[q]);
-            const int64_t p = ch * CH + q * 256 + threadIdx.x;
-            if (p < np) {
-                key[p] = k[q];
-                slot[p] = b + (unsigned)(lane - hl[q]);
-            }
+            if (head && k[q] != EMPTY) atomicAdd(&counts[k[q]], (unsigned)(next - lane));
         }
     }
 }
@@ -724,8 +709,10 @@ static __global__ void __launch_bounds__(1024) k_scan_sums(const unsigned* __res
     if (threadIdx.x == 0) segsum[blockIdx.x] = tot;
 }
 
-static __global__ void __launch_bounds__(1024) k_scan_apply(const unsigned* __restrict__ counts, int64_t n, const unsigned* __restrict__ segsum,
-                                                          unsigned* __restrict__ offsets) {
+// Also writes the offsets to cursor (which may alias counts: a workgroup
+// reads its segment of counts whole before it writes).
+static __global__ void __launch_bounds__(1024) k_scan_apply(const unsigned* counts, int64_t n, const unsigned* __restrict__ segsum,
+                                                          unsigned* __restrict__ offsets, unsigned* cursor) {
     __shared__ unsigned sh[32];
     __shared__ unsigned seg[kScanSeg];
     __shared__ unsigned before;
@@ -760,16 +747,38 @@ static __global__ void __launch_bounds__(1024) k_scan_apply(const unsigned* __re
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         const int64_t j = b0 + i * 1024 + threadIdx.x;
-        if (j < n) offsets[j] = seg[i * 1024 + threadIdx.x];
+        if (j < n) {
+            offsets[j] = seg[i * 1024 + threadIdx.x];
+            cursor[j] = seg[i * 1024 + threadIdx.x];
+        }
     }
     if (b0 + kScanSeg >= n && threadIdx.x == 0) offsets[n] = before + tot;
 }
 
-// Counting-sort pass 2: particle indices grouped by key.
-static __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned* __restrict__ key, const unsigned* __restrict__ slot,
-                                                     const unsigned* __restrict__ offsets, unsigned* __restrict__ sorted) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x)
-        sorted[offsets[key[p]] + slot[p]] = (unsigned)p;
+// Counting-sort pass 2: particle indices grouped by key.  cursor[k] starts
+// at the key's offset; each run of equal keys in a wavefront reserves its
+// ranks with one atomic and places its particles in lane order.  Rank order
+// inside a key depends on atomic order, which no paint result depends on
+// (integer sums).  Whole wavefronts step together (shuffles over all lanes).
+static __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned* __restrict__ key, unsigned* __restrict__ cursor,
+                                                     unsigned* __restrict__ sorted) {
+    constexpr unsigned EMPTY = 0xFFFFFFFFu;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x; b < np; b += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = b + threadIdx.x;
+        const unsigned k = p < np ? key[p] : EMPTY;
+        const unsigned prev = __shfl_up(k, 1);
+        const bool head = lane == 0 || prev != k;
+        const unsigned long long heads = __ballot(head);
+        const int hl = 63 - __clzll(heads & upto);
+        const unsigned long long after = heads & ~upto;
+        const int next = after ? __ffsll((long long)after) - 1 : 64;
+        unsigned base = 0u;
+        if (head && k != EMPTY) base = atomicAdd(&cursor[k], (unsigned)(next - lane));
+        base = __shfl(base, hl);
+        if (p < np) sorted[base + (unsigned)(lane - hl)] = (unsigned)p;
+    }
 }
 
 // One workgroup per tile: the tile's cells are summed completely in LDS and
@@ -856,8 +865,9 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     const unsigned total = rpre[NR];
     // KB particles per thread per round; all their loads are issued before any
     // deposit so the dependent sorted[j] -> pos[p] chains overlap (4 keeps the
-    // 32-bit three-component paint at 74 VGPRs, 6 waves per SIMD)
-    constexpr int KB = ACC32 ? 4 : 6;
+    // 32-bit three-component paint at 74 VGPRs, 6 waves per SIMD, and the
+    // one-component paint at 62)
+    constexpr int KB = (ACC32 || NC == 1) ? 4 : 6;
     for (unsigned jb = 0; jb < total; jb += TB * KB) {
         unsigned pi[KB];
         int code[KB];

@@ -30,13 +30,17 @@ static constexpr int64_t kHdrBytes = 256;
 static constexpr int kTileX = 8, kTileY = 16, kTileZ = 32;
 static constexpr int kTileSh[3] = {3, 4, 5};
 static constexpr int kAcc32Block = 512;
+// One component (32 KB of 64-bit accumulators): 512 threads with 4 particles
+// each per round, 62 VGPRs, four workgroups per CU (0.84 ms per 512^3 launch
+// against 0.91 for 256 threads with 6).
+static constexpr int kPaint1Block = 512;
 
 static inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Workspace: header | direct-mode mesh (nc) | counts | offsets | segment sums |
-// key | slot | sorted
+// key | sorted
 struct PaintWs {
-    size_t acc, counts, offsets, segsum, key, slot, sorted, total;
+    size_t acc, counts, offsets, segsum, key, sorted, total;
 };
 static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
     const TileGeom tg = make_tiles(m, kTileSh);
@@ -47,8 +51,7 @@ static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
     w.offsets = w.counts + al256((size_t)(nk + 1) * 4);
     w.segsum = w.offsets + al256((size_t)(nk + 1) * 4);
     w.key = w.segsum + al256((size_t)(nk / kScanSeg + 1) * 4);
-    w.slot = w.key + al256((size_t)np * 4);
-    w.sorted = w.slot + al256((size_t)np * 4);
+    w.sorted = w.key + al256((size_t)np * 4);
     w.total = w.sorted + al256((size_t)np * 4);
     return w;
 }
@@ -140,7 +143,6 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     auto* counts = reinterpret_cast<unsigned*>(wb + w.counts);
     auto* segsum = reinterpret_cast<unsigned*>(wb + w.segsum);
     auto* key = reinterpret_cast<unsigned*>(wb + w.key);
-    auto* slot = reinterpret_cast<unsigned*>(wb + w.slot);
     unsigned* offsets = reinterpret_cast<unsigned*>(wb + w.offsets);
     unsigned* sorted = reinterpret_cast<unsigned*>(wb + w.sorted);
     if (bins_out) {
@@ -156,28 +158,28 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
             constexpr int PPT = 8;
             KTimer _t("paint_bin", st);
             const dim3 gb(grid_for((np + 256 * PPT - 1) / (256 * PPT), 1, 1 << 15));
-            if (p2) hipLaunchKernelGGL((k_bin_keys<T, PPT, true>), gb, dim3(256), 0, st, pos, pd, np, m, mp, tg, counts, key, slot);
-            else hipLaunchKernelGGL((k_bin_keys<T, PPT, false>), gb, dim3(256), 0, st, pos, pd, np, m, mp, tg, counts, key, slot);
+            if (p2) hipLaunchKernelGGL((k_bin_keys<T, PPT, true>), gb, dim3(256), 0, st, pos, pd, np, m, mp, tg, counts, key);
+            else hipLaunchKernelGGL((k_bin_keys<T, PPT, false>), gb, dim3(256), 0, st, pos, pd, np, m, mp, tg, counts, key);
         }
         {
             KTimer _t("paint_scan", st);
             const unsigned nseg = (unsigned)((nk + kScanSeg - 1) / kScanSeg);
             hipLaunchKernelGGL(k_scan_sums, dim3(nseg), dim3(1024), 0, st, counts, nk, segsum);
-            hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(1024), 0, st, counts, nk, segsum, offsets);
+            hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(1024), 0, st, counts, nk, segsum, offsets, counts);  // counts become the cursors
         }
         if (np > 0) {
             KTimer _t("paint_scatter", st);
-            hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, st, np, key, slot, offsets, sorted);
+            hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, st, np, key, counts, sorted);
         }
     }
     const dim3 gt((unsigned)tg.ntiles);
     if (nc == 1) {
         KTimer _t("paint_deposit1", st);
         if (p2)
-            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileX, kTileY, kTileZ, 256, true>), gt, dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
+            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileX, kTileY, kTileZ, kPaint1Block, true>), gt, dim3(kPaint1Block), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
                                tg, offsets, sorted, hdr, out_scale, out);
         else
-            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileX, kTileY, kTileZ, 256, false>), gt, dim3(256), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
+            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileX, kTileY, kTileZ, kPaint1Block, false>), gt, dim3(kPaint1Block), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
                                tg, offsets, sorted, hdr, out_scale, out);
     } else if (sizeof(T) == 4 && wmesh) {
         // FP32 with the positions' unit-mass density known: 32-bit accumulators
