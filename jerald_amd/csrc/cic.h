@@ -757,27 +757,43 @@ static __global__ void __launch_bounds__(1024) k_scan_apply(const unsigned* coun
 
 // Counting-sort pass 2: particle indices grouped by key.  cursor[k] starts
 // at the key's offset; each run of equal keys in a wavefront reserves its
-// ranks with one atomic and places its particles in lane order.  Rank order
+// ranks with one atomic (PPT particles per thread, all atomics issued before
+// any result is used) and places its particles in lane order.  Rank order
 // inside a key depends on atomic order, which no paint result depends on
 // (integer sums).  Whole wavefronts step together (shuffles over all lanes).
-static __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned* __restrict__ key, unsigned* __restrict__ cursor,
+template <int PPT>
+__global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned* __restrict__ key, unsigned* __restrict__ cursor,
                                                      unsigned* __restrict__ sorted) {
+    constexpr int CH = 256 * PPT;
     constexpr unsigned EMPTY = 0xFFFFFFFFu;
     const int lane = threadIdx.x & 63;
     const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
-    for (int64_t b = blockIdx.x * (int64_t)blockDim.x; b < np; b += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = b + threadIdx.x;
-        const unsigned k = p < np ? key[p] : EMPTY;
-        const unsigned prev = __shfl_up(k, 1);
-        const bool head = lane == 0 || prev != k;
-        const unsigned long long heads = __ballot(head);
-        const int hl = 63 - __clzll(heads & upto);
-        const unsigned long long after = heads & ~upto;
-        const int next = after ? __ffsll((long long)after) - 1 : 64;
-        unsigned base = 0u;
-        if (head && k != EMPTY) base = atomicAdd(&cursor[k], (unsigned)(next - lane));
-        base = __shfl(base, hl);
-        if (p < np) sorted[base + (unsigned)(lane - hl)] = (unsigned)p;
+    const int64_t nchunks = (np + CH - 1) / CH;
+    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        unsigned k[PPT], base[PPT];
+        int hl[PPT];
+#pragma unroll
+        for (int q = 0; q < PPT; q++) {
+            const int64_t p = ch * CH + q * 256 + threadIdx.x;
+            k[q] = p < np ? key[p] : EMPTY;
+        }
+#pragma unroll
+        for (int q = 0; q < PPT; q++) {
+            const unsigned prev = __shfl_up(k[q], 1);
+            const bool head = lane == 0 || prev != k[q];
+            const unsigned long long heads = __ballot(head);
+            hl[q] = 63 - __clzll(heads & upto);
+            const unsigned long long after = heads & ~upto;
+            const int next = after ? __ffsll((long long)after) - 1 : 64;
+            base[q] = 0u;
+            if (head && k[q] != EMPTY) base[q] = atomicAdd(&cursor[k[q]], (unsigned)(next - lane));
+        }
+#pragma unroll
+        for (int q = 0; q < PPT; q++) {
+            const unsigned b = __shfl(base[q], hl[q]);
+            const int64_t p = ch * CH + q * 256 + threadIdx.x;
+            if (p < np) sorted[b + (unsigned)(lane - hl[q])] = (unsigned)p;
+        }
     }
 }
 

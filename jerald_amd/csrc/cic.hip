@@ -169,7 +169,9 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         }
         if (np > 0) {
             KTimer _t("paint_scatter", st);
-            hipLaunchKernelGGL(k_bin_scatter, dim3(gp), dim3(kBlock), 0, st, np, key, counts, sorted);
+            constexpr int PPT = 8;
+            const dim3 gs(grid_for((np + 256 * PPT - 1) / (256 * PPT), 1, 1 << 15));
+            hipLaunchKernelGGL(k_bin_scatter<PPT>, gs, dim3(256), 0, st, np, key, counts, sorted);
         }
     }
     const dim3 gt((unsigned)tg.ntiles);
