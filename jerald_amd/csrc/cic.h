@@ -160,10 +160,27 @@ __global__ void __launch_bounds__(256) k_absmax(const T* __restrict__ mass, int6
 // max |w| * scale over a mesh into *slot as double bits: for a 3-component
 // paint of the positions whose unit-mass density (times 1/scale) is w, the
 // largest sum of CIC weights any cell receives.
+// 16-byte loads, four in flight per thread (a 16-byte-aligned mesh; else one
+// element at a time).
 template <typename T>
 __global__ void __launch_bounds__(256) k_mesh_absmax(const T* __restrict__ w, int64_t n, double scale, unsigned long long* __restrict__ slot) {
+    using VT = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    constexpr int V = 16 / (int)sizeof(T);
+    auto vmax = [](VT a) -> T {
+        if constexpr (sizeof(T) == 4) return fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w)));
+        else return fmax(fabs(a.x), fabs(a.y));
+    };
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, nth = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nv = (reinterpret_cast<uintptr_t>(w) % 16 == 0) ? n / V : 0;
+    const VT* wv = reinterpret_cast<const VT*>(w);
     T mx = T(0);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) mx = fmax(mx, fabs(w[i]));
+    int64_t i = tid;
+    for (; i + 3 * nth < nv; i += 4 * nth) {
+        const VT a = wv[i], b = wv[i + nth], c = wv[i + 2 * nth], d = wv[i + 3 * nth];
+        mx = fmax(mx, fmax(fmax(vmax(a), vmax(b)), fmax(vmax(c), vmax(d))));
+    }
+    for (; i < nv; i += nth) mx = fmax(mx, vmax(wv[i]));
+    for (int64_t j = nv * V + tid; j < n; j += nth) mx = fmax(mx, fabs(w[j]));
     double v = (double)mx * scale;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
