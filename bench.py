@@ -87,12 +87,13 @@ def kernel_bytes(name, n, b):
         "grad6": 5 * b * Nm,                             # potential in, interleaved forces (4 per cell) out
         "div6": 4 * b * Nm,                              # 3 gradient meshes in, divergence out
         "x_realfilter": 4 * b * Nh,
+        "gauss_line": 2 * b * Nm,                        # real mesh in and out (complex pairs)
     }.get(name)
 
 
 KERNELS = ("paint_absmax", "paint_bin", "paint_scan", "paint_scatter", "paint_deposit1", "paint_deposit3",
            "paint_deposit", "paint_convert", "zpass_r2c", "zpass_r2c3", "zpass_c2r", "zpass_c2r3", "colpass",
-           "colpass3", "grad6", "div6", "sm_x_fwd", "sm_x_bwd", "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
+           "colpass3", "grad6", "div6", "sm_x_fwd", "sm_x_bwd", "x_realfilter", "gauss_line", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
            "loss")
 
 

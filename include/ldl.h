@@ -71,7 +71,7 @@ int ldl_version(void);
  * "paint_scatter", "paint_deposit1"/"3" (1- or 3-component tiled paint),
  * "paint_deposit", "paint_convert" (direct and float modes), "zpass_r2c"/"3",
  * "zpass_c2r"/"3", "colpass"/"3" (one or three spectra), "grad6", "div6", "sm_x_fwd", "sm_x_bwd",
- * "x_realfilter", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
+ * "x_realfilter", "gauss_line", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
  * "readout", "loss".  Enabling clears previous records. */
 int ldl_timer_enable(int32_t on);
 int ldl_timer_read(const char* name, double* total_ms, int64_t* count);

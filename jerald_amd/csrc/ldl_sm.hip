@@ -664,6 +664,54 @@ static StGeom st_geom(const Mesh& m) {
     return g;
 }
 
+// One axis of a separable Gaussian smoothing, W = exp(-|k|^2 sigma^2 / 2) =
+// prod_d exp(-k_d^2 sigma^2 / 2): forward FFT along the column axis, times
+// exp(-hs2 k_d^2) (hs2 = sigma^2 / 2), inverse, unnormalised (the z pass's
+// store applies 1 / (N0 N1 N2) for all three).  It runs on a real mesh
+// viewed as complex pairs (z, z+1): the weight is real and even in k_d, so
+// its kernel is real and the two lines of a pair are filtered independently.
+template <typename T, int N>
+__global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
+k_gauss_line(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, const T* __restrict__ kd, T hs2) {
+    using Cfg = FftCfg<T, N>;
+    using C = c2<T>;
+    constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS;
+    __shared__ C lds[Cfg::LDS_ELEMS];
+    __shared__ C twl[N];
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t plane = tile / g.tiles_per_plane;
+    const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
+    const int64_t base = plane * g.plane_stride + col0;
+    const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
+    stage_twiddles<T, N, TH>(twl, tw);
+    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return io[base + n * g.es + p]; });
+    __syncthreads();
+    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
+    for (int i = threadIdx.x; i < N * P; i += TH) {
+        const int n = i / P, p = i - n * P;
+        const T k = kd[n];
+        const T w = pexp(-hs2 * k * k);  // unnormalised both ways: the r2c store applies 1/M
+        const C v = lds[n * LS + p];
+        lds[n * LS + p] = C{v.x * w, v.y * w};
+    }
+    __syncthreads();
+    lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
+    tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { io[base + n * g.es + p] = v; });
+}
+
+// z axis of the separable Gaussian: half-spectrum bins times
+// exp(-hs2 k_z^2) scale as the r2c pass stores them.
+template <typename T> struct GaussHalfRowStore {
+    c2<T>* out;
+    int64_t nh;
+    const T* kz;
+    T hs2, scale;
+    __device__ __forceinline__ void store(int64_t row, int k, c2<T> v) const {
+        const T w = pexp(-hs2 * kz[k] * kz[k]) * scale;
+        out[row * nh + k] = c2<T>{v.x * w, v.y * w};
+    }
+};
+
 // X pass with a real isotropic weight: forward, times W(k) scale, inverse.
 // kind 1: W = knorm^-n + 1 (knorm(0) = 1e-8, model.py:376-381); kind 2:
 // W = exp(-k^2 sigma^2 / 2).
@@ -1553,8 +1601,29 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
         T* gds = P.mesh3 + cells;
         c2<T>* S = P.spec3;
         const T invM = (T)(1.0 / (double)cells);
-        auto smooth = [&](const T* src) -> int {  // S = W(k) r2c(src) / M, inverse x and y passes
+        auto smooth = [&](T* src) -> int {  // S = W(k) r2c(src) / M, inverse x and y passes
             int rc2;
+            if (smooth_kind == 2 && fft_mesh_pow2(m)) {
+                // separable Gaussian: y and x line filters in place on the real
+                // mesh (as complex pairs along z), then the z factor and 1/M as
+                // the r2c pass stores the half spectrum
+                c2<T>* sc = reinterpret_cast<c2<T>*>(src);
+                const int64_t n2c = m.n2 / 2;
+                const T hs2 = (T)(0.5 * smooth_par * smooth_par);
+                dispatch_n(m.n1, [&](auto N) {
+                    constexpr int NN = decltype(N)::value;
+                    const ColGeom g = col_geom<T, NN>(m.n0, n2c, m.n1 * n2c, n2c);
+                    KTimer _t("gauss_line", st);
+                    hipLaunchKernelGGL((k_gauss_line<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, sc, kt.k[1], hs2);
+                });
+                dispatch_n(m.n0, [&](auto N) {
+                    constexpr int NN = decltype(N)::value;
+                    const ColGeom g = col_geom<T, NN>(m.n1, n2c, n2c, m.n1 * n2c);
+                    KTimer _t("gauss_line", st);
+                    hipLaunchKernelGGL((k_gauss_line<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, sc, kt.k[0], hs2);
+                });
+                return zpass_r2c<T>(nrows, m.n2, tw, RealRowLoad<T>{src, m.n2}, GaussHalfRowStore<T>{S, hp, kt.k[2], hs2, invM}, st);
+            }
             if ((rc2 = zpass_r2c<T>(nrows, m.n2, tw, RealRowLoad<T>{src, m.n2}, HalfRowStore<T>{S, hp, T(1)}, st))) return rc2;
             if ((rc2 = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
             if (fft_mesh_pow2(m)) {
