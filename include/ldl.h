@@ -156,6 +156,20 @@ int ldl_pifft_f32(const float* in, float* out, const int64_t nmesh[3], double sc
 int ldl_pifft_f64(const double* in, double* out, const int64_t nmesh[3], double scale,
                   void* workspace, size_t workspace_bytes, void* stream);
 
+/* Half-spectrum z weights, the elementwise steps of the FFT and c2f VJPs:
+ * out = w(k_z) * scale * in over complex (N0,N1,N2/2+1) (out may alias in).
+ * mode 0: w = 1; 1: w = c; 2: w = 1/c, c = 1 on the k_z = 0 and N2/2 planes
+ * and 2 elsewhere (exact adjoints: r2c^T = c2r(./c), c2r^T = c r2c, in place
+ * of the reference's rules jaxpops_bind.py:699-701, :758-759); 3: w = 2 on z
+ * bins [1, int(N1/2)) clipped to the half spectrum, 1 elsewhere -- c2f's
+ * backward v + conj(v) on those bins (model.py:23-29), which for complex
+ * bins leaves (2 Re v, 0) there.  mode | 4: the bins are real numbers
+ * (N0,N1,N2/2+1) instead of interleaved complex. */
+int ldl_spectrum_zweight_f32(const float* in, float* out, const int64_t nmesh[3], int32_t mode, double scale,
+                             void* stream);
+int ldl_spectrum_zweight_f64(const double* in, double* out, const int64_t nmesh[3], int32_t mode, double scale,
+                             void* stream);
+
 /* ------------------------------------------------- fused sm-kind LDL layer
  * One displacement layer of JERALDModel.potential_gradient_sm
  * (model.py:185-212) with parameters prm = {alpha, gamma, kh, kl, nu}
@@ -311,15 +325,19 @@ int ldl_hi_depletion_fwd_f64(const double* starmap, const double* starmapk, cons
                              void* workspace, size_t workspace_bytes, void* stream);
 /* Its adjoint: given g_dep = dL/d dep, WRITES grad8 (DEVICE double[8]) =
  * dL/d{beta1, khi, kli, nui, beta2, gamma3, xi, eta}.  The gamma3 partial
- * takes starmap^gamma3 ln(starmap) as 0 where starmap == 0, as JAX does. */
+ * takes starmap^gamma3 ln(starmap) as 0 where starmap == 0, as JAX does.
+ * grad_rule 0: exact adjoint; 1: the reference's rules, under which the
+ * filter's partials (khi, kli, nui, xi) weight z bins [1, int(N1/2)) by 2
+ * and all others by 1 (c2f, model.py:23-29, :330; equal to the exact
+ * weights when N1 == N2). */
 int ldl_hi_depletion_bwd_f32(const float* starmap, const float* starmapk, const float* sfield,
                              const float* g_dep, const int64_t nmesh[3], const double box[3],
-                             const double prm8[8], double* grad8, void* workspace, size_t workspace_bytes,
-                             void* stream);
+                             const double prm8[8], int32_t grad_rule, double* grad8, void* workspace,
+                             size_t workspace_bytes, void* stream);
 int ldl_hi_depletion_bwd_f64(const double* starmap, const double* starmapk, const double* sfield,
                              const double* g_dep, const int64_t nmesh[3], const double box[3],
-                             const double prm8[8], double* grad8, void* workspace, size_t workspace_bytes,
-                             void* stream);
+                             const double prm8[8], int32_t grad_rule, double* grad8, void* workspace,
+                             size_t workspace_bytes, void* stream);
 
 /* HI response + loss (model.py:313-334, :350-468): prm4 = {mu, w, b, zs},
  * F = ReLU(w ReLU(rho + zs)^mu - dep + b), then as ldl_sm_loss.  Writes
@@ -335,6 +353,16 @@ int ldl_hi_loss_f64(const double* rho, const double* target, const double* mask,
                     const double prm4[4], double index, int32_t p_norm, int32_t smooth_kind, double smooth_par,
                     double* out6, double* g_rho, double* g_dep, void* workspace, size_t workspace_bytes,
                     void* stream);
+
+/* The model map alone (JERALDModel.evaluate, model.py:313-344):
+ * sm: out = ReLU(w (rho + 1e-8)^mu + b), prm3 = {mu, w, b};
+ * HI: out = ReLU(w ReLU(rho + zs)^mu - dep + b), prm4 = {mu, w, b, zs}. */
+int ldl_sm_response_f32(const float* rho, const int64_t nmesh[3], const double prm3[3], float* out, void* stream);
+int ldl_sm_response_f64(const double* rho, const int64_t nmesh[3], const double prm3[3], double* out, void* stream);
+int ldl_hi_response_f32(const float* rho, const float* dep, const int64_t nmesh[3], const double prm4[4], float* out,
+                        void* stream);
+int ldl_hi_response_f64(const double* rho, const double* dep, const int64_t nmesh[3], const double prm4[4],
+                        double* out, void* stream);
 
 /* Final-layer adjoint gather (fused readout-gradient): g (Np,3) =
  * fact * grad(interp g_rho) at pos  (the ppaint VJP with unit mass,

@@ -78,8 +78,11 @@ for _t in ("f32", "f64"):
                                    _P, _SZ, _P],
         f"ldl_hi_loss_{_t}": [_P, _P, _P, _P, _P, _P, _I64x3, _Dp, _Dp, _D, _I32, _I32, _D, _P, _P, _P, _P, _SZ, _P],
         f"ldl_hi_depletion_fwd_{_t}": [_P, _P, _I64x3, _Dp, _Dp, _P, _P, _P, _SZ, _P],
-        f"ldl_hi_depletion_bwd_{_t}": [_P, _P, _P, _P, _I64x3, _Dp, _Dp, _P, _P, _SZ, _P],
+        f"ldl_hi_depletion_bwd_{_t}": [_P, _P, _P, _P, _I64x3, _Dp, _Dp, _I32, _P, _P, _SZ, _P],
         f"ldl_paint_vjp_pos_{_t}": [_P, _PD, _I64, _P, _I64x3, _D, _P, _P, _P],
+        f"ldl_spectrum_zweight_{_t}": [_P, _P, _I64x3, _I32, _D, _P],
+        f"ldl_sm_response_{_t}": [_P, _I64x3, _Dp, _P, _P],
+        f"ldl_hi_response_{_t}": [_P, _P, _I64x3, _Dp, _P, _P],
     })
 _RESTYPE = {
     "ldl_last_error": C.c_char_p,

@@ -47,7 +47,7 @@ template <typename T>
 __global__ void k_axis_tables(T* __restrict__ k, T* __restrict__ D, int64_t n, int64_t count, double cell) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= count) return;
-    const int64_t m = i < n / 2 ? i : i - n;
+    const int64_t m = 2 * i < n ? i : i - n;  // fftfreq order, odd n included: 0..(n-1)/2, -(n-1)/2..-1
     const double th = 2.0 * (double)m / (double)n;  // k cell / pi
     k[i] = (T)(2.0 * M_PI * (double)m / ((double)n * cell));
     D[i] = (T)((45.0 * sinpi(th) - 9.0 * sinpi(2.0 * th) + sinpi(3.0 * th)) / (30.0 * cell));
@@ -318,18 +318,40 @@ template <typename T> struct Src2 {
     double ad, cs, gkh, gkl;  // ad = a in FP64, cs = -1e-3
 };
 
+// Half-spectrum weights of the backward x pass at z bin kz (n2 the z
+// length, zdbl = min(n1 / 2, n2 / 2 + 1)):
+//   c  = 1 on the k_z = 0, n2/2 planes, 2 elsewhere (the exact adjoint);
+//   ca = weight of the alpha partials, cg = weight of the kernel-parameter
+//        partials (kh, kl, nu).
+// grad_rule "exact": ca = cg = c.  grad_rule "reference" reproduces the
+// reference's rules: its FFT VJPs (jaxpops_bind.py:699-701, 758-759) leave
+// every half-spectrum bin at weight 1, and c2f's backward (model.py:23-29)
+// doubles the kernel's cotangent on z bins [1, int(n1/2)) -- the second axis
+// length, clipped to the half spectrum -- so ca = 1 and cg = 2 there, 1
+// elsewhere (cg = c on meshes with n1 == n2).
+struct XbwdW {
+    double c, ca, cg;
+};
+__device__ __forceinline__ XbwdW xbwd_weights(int64_t kz, int64_t n2, int64_t zdbl, int alpha_rule) {
+    const double c = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
+    if (!alpha_rule) return XbwdW{c, c, c};
+    return XbwdW{c, 1.0, (kz >= 1 && kz < zdbl) ? 2.0 : 1.0};
+}
+
 // Per-cell parameter partials of the backward x pass.  acc[0..3]: alpha,
 // kh, kl, nu of the first source; with HI acc[4..8]: alpha2, kh2, kl2, nu2,
-// gamma2.  Returns G of the first source.
+// gamma2 (gamma2 reaches the potential through r2c and the three c2r, whose
+// rule pair composes to the exact adjoint: weight c in both rules).  Returns
+// G of the first source.
 template <typename T, bool HI>
-__device__ __forceinline__ T xbwd_cell(const SmFilter<T>& fl, const Src2<T>& s2, T k2, c2<T> q, c2<T> u, int64_t idx, double c,
-                                       double cpot, double apot, double gkh, double gkl, int alpha_rule, double* acc) {
+__device__ __forceinline__ T xbwd_cell(const SmFilter<T>& fl, const Src2<T>& s2, T k2, c2<T> q, c2<T> u, int64_t idx, XbwdW w,
+                                       double cpot, double apot, double gkh, double gkl, double* acc) {
     T G, th, tl, lnk;
     fl.eval(k2, G, th, tl, lnk);
     const double re = (double)q.x * (double)u.x + (double)q.y * (double)u.y;
     const double Gd = (double)G;
-    acc[0] += (alpha_rule ? 1.0 : c) * cpot * Gd * re;
-    const double gG = apot * c * re * Gd;  // dL/dG * G
+    acc[0] += w.ca * cpot * Gd * re;
+    const double gG = apot * w.cg * re * Gd;  // dL/dG * G
     acc[1] += gG * gkh * (double)th;
     acc[2] += gG * gkl * (double)tl;
     acc[3] += gG * (double)lnk;
@@ -338,12 +360,12 @@ __device__ __forceinline__ T xbwd_cell(const SmFilter<T>& fl, const Src2<T>& s2,
         const c2<T> sv = s2.S[idx], tv = s2.Tl[idx];
         const double re2 = (double)q.x * (double)sv.x + (double)q.y * (double)sv.y;
         const double re3 = (double)q.x * (double)tv.x + (double)q.y * (double)tv.y;
-        acc[4] += (alpha_rule ? 1.0 : c) * s2.cs * G2 * re2;
-        const double gG2 = s2.ad * c * re2 * G2;
+        acc[4] += w.ca * s2.cs * G2 * re2;
+        const double gG2 = s2.ad * w.cg * re2 * G2;
         acc[5] += gG2 * s2.gkh * (double)th;
         acc[6] += gG2 * s2.gkl * (double)tl;
         acc[7] += gG2 * (double)lnk;
-        acc[8] += s2.ad * c * G2 * re3;
+        acc[8] += s2.ad * w.c * G2 * re3;
     }
     return G;
 }
@@ -404,7 +426,7 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
 template <typename T, int N, bool HI>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
 k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ U, c2<T>* out,
-           int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh, double gkl, double cpot, double alpha,
+           int64_t n2, int64_t zdbl, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh, double gkl, double cpot, double alpha,
            int alpha_rule, Src2<T> s2, double* __restrict__ part) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
@@ -433,8 +455,8 @@ k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>
         if (p < pv) {
             const int64_t kz = col0 + p;
             const int64_t idx = base + n * g.es + p;
-            const double c = (kz == 0 || 2 * kz == n2) ? 1.0 : 2.0;
-            const T G = xbwd_cell<T, HI>(fl, s2, kt.knorm2(n, plane, kz), q, U[idx], idx, c, cpot, apot, gkh, gkl, alpha_rule, acc);
+            const T G = xbwd_cell<T, HI>(fl, s2, kt.knorm2(n, plane, kz), q, U[idx], idx, xbwd_weights(kz, n2, zdbl, alpha_rule), cpot,
+                                         apot, gkh, gkl, acc);
             y = C{a1e2 * G * q.x, a1e2 * G * q.y};
         }
         lds[n * LS + p] = y;
@@ -785,7 +807,7 @@ __global__ void __launch_bounds__(256) k_sm_filter_fwd(const c2<T>* __restrict__
 // cpot alpha G Q (out may alias S3's first spectrum).
 template <typename T, bool HI>
 __global__ void __launch_bounds__(256) k_sm_filter_bwd(const c2<T>* S3, const c2<T>* __restrict__ U, c2<T>* out, int64_t hcells, int64_t n1,
-                                                       int64_t nh, int64_t hp, int64_t n2, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh,
+                                                       int64_t nh, int64_t hp, int64_t n2, int64_t zdbl, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh,
                                                        double gkl, double cpot, double alpha, int alpha_rule, Src2<T> s2,
                                                        double* __restrict__ part) {
     constexpr int NP = HI ? 9 : 4;
@@ -801,8 +823,8 @@ __global__ void __launch_bounds__(256) k_sm_filter_bwd(const c2<T>* S3, const c2
             const c2<T> v = S3[d * hcells + i];
             q = c2<T>{q.x - D[d] * v.y, q.y + D[d] * v.x};  // += i D v
         }
-        const double c = (iz == 0 || 2 * iz == n2) ? 1.0 : 2.0;
-        const T G = xbwd_cell<T, HI>(fl, s2, kt.knorm2(ix, iy, iz), q, U[i], i, c, cpot, apot, gkh, gkl, alpha_rule, acc);
+        const T G = xbwd_cell<T, HI>(fl, s2, kt.knorm2(ix, iy, iz), q, U[i], i, xbwd_weights(iz, n2, zdbl, alpha_rule), cpot, apot, gkh,
+                                     gkl, acc);
         out[i] = c2<T>{a1e2 * G * q.x, a1e2 * G * q.y};
     }
     block_sum<NP, 256>(acc);
@@ -1120,6 +1142,16 @@ __global__ void __launch_bounds__(256) k_loss_diff(const T* __restrict__ rho, co
     }
 }
 
+// The model map alone (JERALDModel.evaluate, model.py:313-344): out = F.
+template <typename T>
+__global__ void __launch_bounds__(256) k_response(const T* __restrict__ rho, RespPar<T> rp, int64_t n, T* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        T base, y, F, Fi;
+        response(rp, i, rho[i], T(1), base, y, F, Fi);
+        out[i] = F;
+    }
+}
+
 // z c2r epilogue of the smoothed residual: loss partials and g_ds.
 template <typename T> struct LossStore {
     static constexpr int NR = 2;
@@ -1266,19 +1298,20 @@ __global__ void __launch_bounds__(256) k_hi_dep_bwd(const T* __restrict__ star, 
 
 // dL/d(khi, kli, nui, xi) = sum_H c Re(conj(ghat) Sk) dW/dtheta, ghat the
 // unnormalised r2c of g_S: dW/dkhi = G/k, dW/dkli = -G k/kli^2,
-// dW/dnui = -G ln k, dW/dxi = 1 (G the dm-form kernel).
+// dW/dnui = -G ln k, dW/dxi = 1 (G the dm-form kernel).  The filter passes
+// through c2f (model.py:329-330), so the reference rule weights these bins
+// like the layer kernels' parameters (xbwd_weights' cg).
 template <typename T>
 __global__ void __launch_bounds__(256) k_hi_wgrad(const c2<T>* __restrict__ ghat, const c2<T>* __restrict__ Sk, int64_t hcells, int64_t n1,
-                                                  int64_t nh, int64_t hp, int64_t n2, KTabs<T> kt, SmFilter<T> fl, double inv_kli2,
-                                                  double* __restrict__ part) {
+                                                  int64_t nh, int64_t hp, int64_t n2, int64_t zdbl, int rule, KTabs<T> kt, SmFilter<T> fl,
+                                                  double inv_kli2, double* __restrict__ part) {
     double a[4] = {};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < hcells; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t iz = i % nh, iy = (i / nh) % n1, ix = i / (nh * n1);
         T G, th, tl, lnk;
         fl.eval(kt.knorm2(ix, iy, iz), G, th, tl, lnk);
         const c2<T> g = ghat[(ix * n1 + iy) * hp + iz], sk = Sk[i];
-        const double c = (iz == 0 || 2 * iz == n2) ? 1.0 : 2.0;
-        const double re = c * ((double)g.x * (double)sk.x + (double)g.y * (double)sk.y);
+        const double re = xbwd_weights(iz, n2, zdbl, rule).cg * ((double)g.x * (double)sk.x + (double)g.y * (double)sk.y);
         const double Gd = (double)G;
         a[0] += re * Gd * (double)th;
         a[1] -= re * Gd * (double)tl * inv_kli2;
@@ -1486,6 +1519,7 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     const double cpot = hi ? 1e-3 : 1e-2;
     const T a1e2 = (T)(cpot * prm[0]);
     const c2<T>* U = reinterpret_cast<const c2<T>*>(Ubuf);
+    const int64_t zdbl = std::min<int64_t>(m.n1 / 2, nh);  // end of c2f's doubled z bins (reference rule)
     c2<T>* S = P.spec3;  // result reuses the first spectrum (read before written)
     int64_t ntx = 0;
     if (fft_mesh_pow2(m)) {
@@ -1496,10 +1530,10 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
             KTimer _t("sm_x_bwd", st);
             if (hi)
                 hipLaunchKernelGGL((k_sm_x_bwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, P.spec3, tw, U, S,
-                                   m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
+                                   m.n2, zdbl, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
             else
                 hipLaunchKernelGGL((k_sm_x_bwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, P.spec3, tw, U,
-                                   S, m.n2, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
+                                   S, m.n2, zdbl, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
         });
     } else {
         for (int d = 0; d < 3; d++)
@@ -1507,10 +1541,10 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
         const unsigned gf = grid_for(hc, 256, 4096);
         ntx = gf;
         if (hi)
-            hipLaunchKernelGGL((k_sm_filter_bwd<T, true>), dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, hp, m.n2, kt, fl, a1e2,
+            hipLaunchKernelGGL((k_sm_filter_bwd<T, true>), dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, hp, m.n2, zdbl, kt, fl, a1e2,
                                gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
         else
-            hipLaunchKernelGGL((k_sm_filter_bwd<T, false>), dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, hp, m.n2, kt, fl, a1e2,
+            hipLaunchKernelGGL((k_sm_filter_bwd<T, false>), dim3(gf), dim3(256), 0, st, P.spec3, U, S, hc, m.n1, nh, hp, m.n2, zdbl, kt, fl, a1e2,
                                gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
         if ((rc = colpass<T, +1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc;
     }
@@ -1543,6 +1577,26 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
 
 // kind: 0 sm (prm = [mu, w, b]), 1 dm (prm unused), 2 HI (prm = [mu, w, b,
 // zs], dep required, gdep optional); out has 5 entries, 6 for HI (+ g_zs).
+template <typename T>
+static int sm_response(int kind, const T* rho, const T* dep, const int64_t nmesh[3], const double* prm, T* out, hipStream_t st) {
+    Mesh m;
+    int rc = check_mesh(nmesh, &m);
+    if (rc) return rc;
+    if ((rc = check_prm<T>(prm, kind == kKindHi ? 4 : 3))) return rc;
+    if (!rho || !out || (kind == kKindHi && !dep)) return fail(LDL_EINVAL, "NULL array argument");
+    RespPar<T> rp{};
+    rp.index = T(1);
+    rp.mu = (T)prm[0];
+    rp.w = (T)prm[1];
+    rp.b = (T)prm[2];
+    rp.hi = kind == kKindHi;
+    rp.zs = rp.hi ? (T)prm[3] : T(0);
+    rp.dep = dep;
+    hipLaunchKernelGGL(k_response<T>, dim3(grid_for(m.cells(), 256, 4096)), dim3(256), 0, st, rho, rp, m.cells(), out);
+    LDL_LAUNCH_CHECK();
+    return LDL_OK;
+}
+
 template <typename T>
 static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const T* maskv, const T* field2, const T* dep, const int64_t nmesh[3],
                    const double box[3], const double* prm, double index, int32_t p_norm, int32_t smooth_kind, double smooth_par,
@@ -1712,7 +1766,7 @@ static int hi_depletion_fwd(const T* star, const c2<T>* Sk, const int64_t nmesh[
 
 template <typename T>
 static int hi_depletion_bwd(const T* star, const c2<T>* Sk, const T* S, const T* gdep, const int64_t nmesh[3], const double box[3],
-                            const double* prm, double* grad8, void* ws, size_t wsb, hipStream_t st) {
+                            const double* prm, int32_t rule, double* grad8, void* ws, size_t wsb, hipStream_t st) {
     Mesh m;
     SmPtrs<T> P;
     const c2<T>* tw;
@@ -1732,7 +1786,9 @@ static int hi_depletion_bwd(const T* star, const c2<T>* Sk, const T* S, const T*
     if ((rc = r2c_full<T>(m, tw, RealRowLoad<T>{gS, m.n2}, gh, hp, T(1), st))) return rc;
     const SmFilter<T> fl{(T)prm[1], (T)(1.0 / prm[2]), (T)prm[3], 1};
     const unsigned gf = grid_for(hc, 256, 4096);
-    hipLaunchKernelGGL(k_hi_wgrad<T>, dim3(gf), dim3(256), 0, st, gh, sk, hc, m.n1, nh, hp, m.n2, kt, fl, 1.0 / (prm[2] * prm[2]), P.part);
+    const int64_t zdbl = std::min<int64_t>(m.n1 / 2, nh);
+    hipLaunchKernelGGL(k_hi_wgrad<T>, dim3(gf), dim3(256), 0, st, gh, sk, hc, m.n1, nh, hp, m.n2, zdbl, (int)(rule != 0), kt, fl,
+                       1.0 / (prm[2] * prm[2]), P.part);
     const RedMap rw{4, {1, 2, 3, 6}};
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, (int64_t)gf, 4, rw, grad8, 1);
     LDL_LAUNCH_CHECK();
@@ -1833,6 +1889,13 @@ size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np
         return sm_loss<T>(kKindSm, rho, target, mask, mask_valid, field2, nullptr, nmesh, box, prm3, index, p_norm,        \
                           smooth_kind, smooth_par, out5, g_rho, nullptr, ws, wsb, as_stream(stream));                       \
     }                                                                                                                       \
+    int ldl_sm_response_##SUF(const T* rho, const int64_t nmesh[3], const double prm3[3], T* out, void* stream) {         \
+        return sm_response<T>(kKindSm, rho, nullptr, nmesh, prm3, out, as_stream(stream));                                 \
+    }                                                                                                                       \
+    int ldl_hi_response_##SUF(const T* rho, const T* dep, const int64_t nmesh[3], const double prm4[4], T* out,           \
+                              void* stream) {                                                                               \
+        return sm_response<T>(kKindHi, rho, dep, nmesh, prm4, out, as_stream(stream));                                     \
+    }                                                                                                                       \
     int ldl_dm_loss_##SUF(const T* rho, const T* target, const T* mask, const T* mask_valid, const T* field2,               \
                           const int64_t nmesh[3], const double box[3], double index, int32_t p_norm, int32_t smooth_kind,   \
                           double smooth_par, double* out5, T* g_rho, void* ws, size_t wsb, void* stream) {                 \
@@ -1853,10 +1916,10 @@ size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np
                                    wsb, as_stream(stream));                                                                 \
     }                                                                                                                       \
     int ldl_hi_depletion_bwd_##SUF(const T* starmap, const T* starmapk, const T* sfield, const T* g_dep,                   \
-                                   const int64_t nmesh[3], const double box[3], const double prm8[8], double* grad8,        \
-                                   void* ws, size_t wsb, void* stream) {                                                    \
+                                   const int64_t nmesh[3], const double box[3], const double prm8[8], int32_t grad_rule,    \
+                                   double* grad8, void* ws, size_t wsb, void* stream) {                                     \
         return hi_depletion_bwd<T>(starmap, reinterpret_cast<const c2<T>*>(starmapk), sfield, g_dep, nmesh, box, prm8,     \
-                                   grad8, ws, wsb, as_stream(stream));                                                      \
+                                   grad_rule, grad8, ws, wsb, as_stream(stream));                                           \
     }                                                                                                                       \
     int ldl_paint_vjp_pos_##SUF(const T* pos, const ldl_pos_desc* pd, int64_t np, const T* g_rho, const int64_t nmesh[3],   \
                                 double fact, T* g, double* gmax, void* stream) {                                            \

@@ -201,11 +201,33 @@ def _pifft_raw(X, nm, scale=1.0):
 
 
 def hermitian_weight(n2, device, dtype):
-    """c(k_z): 1 on the k_z = 0 and N/2 planes, 2 elsewhere."""
+    """c(k_z): 1 on the k_z = 0 and N/2 planes, 2 elsewhere (a table for
+    callers such as Parseval checks; the VJPs apply it in ``_zweight``)."""
     c = torch.full((n2 // 2 + 1,), 2.0, dtype=dtype, device=device)
     c[0] = 1.0
     c[-1] = 1.0
     return c
+
+
+ZW_ONE, ZW_C, ZW_INV_C, ZW_C2F, ZW_REAL = 0, 1, 2, 3, 4
+
+
+def _zweight(v, nm, mode, scale=1.0):
+    """New tensor w(k_z) * scale * v over a half spectrum (ldl_spectrum_zweight):
+    mode ZW_C multiplies by c, ZW_INV_C divides by it, ZW_C2F doubles z bins
+    [1, int(N1/2)); ``v`` complex, or real for a real kernel's bins."""
+    L.require_gpu(v, "spectrum")
+    v = v.resolve_conj().contiguous()
+    out = torch.empty_like(v)
+    if v.is_complex():
+        rdt = torch.float32 if v.dtype == torch.complex64 else torch.float64
+        a, b = torch.view_as_real(v), torch.view_as_real(out)
+    else:
+        rdt, a, b = v.dtype, v, out
+        mode |= ZW_REAL
+    L.call(f"ldl_spectrum_zweight_{L.sfx(rdt)}", L.ptr(a), L.ptr(b), L.i64x3(nm), int(mode), float(scale),
+           L.stream_ptr(v.device))
+    return out
 
 
 # ------------------------------------------------------- autograd ops
@@ -291,17 +313,19 @@ class _PReadout3D(torch.autograd.Function):
 
 
 class _PFFT(torch.autograd.Function):
+    """scale * r2c; scale 1 is pfft, 1/N^3 PMesh.r2c (pmesh.py:112)."""
+
     @staticmethod
-    def forward(ctx, data, rule):
-        ctx.rule, ctx.nm = rule, tuple(data.shape)
-        return _pfft_raw(data)
+    def forward(ctx, data, rule, scale):
+        ctx.rule, ctx.nm, ctx.scale = rule, tuple(data.shape), scale
+        return _pfft_raw(data, scale)
 
     @staticmethod
     def backward(ctx, g):
         g = g.contiguous()
         if ctx.rule == "exact":
-            g = g / hermitian_weight(ctx.nm[2], g.device, g.real.dtype)
-        return _pifft_raw(g, ctx.nm), None
+            g = _zweight(g, ctx.nm, ZW_INV_C)
+        return _pifft_raw(g, ctx.nm, ctx.scale), None, None
 
 
 class _PIFFT(torch.autograd.Function):
@@ -314,7 +338,7 @@ class _PIFFT(torch.autograd.Function):
     def backward(ctx, g):
         G = _pfft_raw(g.contiguous())
         if ctx.rule == "exact":
-            G = G * hermitian_weight(ctx.nm[2], G.device, G.real.dtype)
+            G = _zweight(G, ctx.nm, ZW_C)
         return G, None, None
 
 
@@ -328,10 +352,11 @@ class _C2F(torch.autograd.Function):
     def backward(ctx, v):
         if ctx.rule == "exact":
             return v, None
-        sz = v.shape[1]  # model.py:27 uses the second axis length
-        v = v.clone()
-        v[:, :, 1:sz // 2] = v[:, :, 1:sz // 2] + v[:, :, 1:sz // 2].conj()
-        return v, None
+        # model.py:26-28: z bins [1, int(v.shape[1]/2)) get v + conj(v); the
+        # second axis length is v.shape[1], the z length of the half spectrum
+        # is inferred as 2 (nh - 1) (only the bin count matters for this mode)
+        d0, d1, nh = v.shape
+        return _zweight(v.resolve_conj(), (d0, d1, 2 * (nh - 1)), ZW_C2F), None
 
 
 # --------------------------------------------------------- public API
@@ -416,14 +441,15 @@ def preadout3D(pos, field, Nmesh, BoxSize, edges=None, dims=None, comm=None, lyi
     return _PReadout3D.apply(pos, field, nm, _box(BoxSize), bool(vjp))
 
 
-def pfft(data, dims=None):
+def pfft(data, dims=None, scale=1.0):
     """Unnormalised r2c (jaxpops_bind.py:647-656): real (d0, d1, d2) ->
-    complex (d0, d1, d2//2+1)."""
+    complex (d0, d1, d2//2+1), times ``scale`` inside the transform (1/N^3 is
+    PMesh.r2c, pmesh.py:112)."""
     if dims is not None:
         data = data.reshape(tuple(int(d) for d in dims))
     if data.dtype not in (torch.float32, torch.float64):
         raise NotImplementedError(f"Unsupported dtype {data.dtype}")
-    return _PFFT.apply(data, _VJP_RULE)
+    return _PFFT.apply(data, _VJP_RULE, float(scale))
 
 
 def pifft(data, dims):

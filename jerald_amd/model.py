@@ -29,7 +29,10 @@ Extensions over the reference, all opt-in:
   * ``smoothing=("gaussian", sigma)``: Gaussian loss weight exp(-k^2 sigma^2/2);
   * ``grad_rule``: "exact" (default) or "reference" -- the reference's custom
     FFT VJP rules weight interior k_z modes by 1 instead of 2 in dL/dalpha
-    (jaxpops_bind.py:758-759 with model.py:198); every other gradient agrees;
+    (jaxpops_bind.py:758-759 with model.py:198), and c2f's backward
+    (model.py:23-29) doubles the kernel parameters' z bins [1, int(N1/2)), which
+    equals the exact weighting only when N1 == N2; with N1 == N2 every gradient
+    but alpha's agrees with the exact rule;
   * ``paint_mode``: "fixed" (64-bit fixed point, deterministic) or "float".
 """
 from __future__ import annotations
@@ -246,14 +249,15 @@ class JERALDModel:
         if self.kind == "dm":
             return rho
         q = p[self.npot * Nstep: self.npot * Nstep + self.nresp]
-        mu, w, b = (float(v) for v in q[:3])
+        F = torch.empty_like(rho)
         if self.kind == "HI":
             _, dep = self._depletion(q[3:11])
-            r = rho + float(q[11])
-            y = w * torch.where(r > 1e-12, r, torch.full_like(r, 1e-12)) ** mu - dep + b
+            L.call(f"ldl_hi_response_{L.sfx(self.dtype)}", L.ptr(rho), L.ptr(dep), L.i64x3(self.nm),
+                   L.dvec([q[0], q[1], q[2], q[11]]), L.ptr(F), L.stream_ptr(self.device))
         else:
-            y = w * (rho + 1e-8) ** mu + b
-        return torch.where(y > 1e-12, y, torch.full_like(y, 1e-12))
+            L.call(f"ldl_sm_response_{L.sfx(self.dtype)}", L.ptr(rho), L.i64x3(self.nm), L.dvec(q[:3]), L.ptr(F),
+                   L.stream_ptr(self.device))
+        return F
 
     def _forward(self, p, need_grad):
         self._check(p, self.Nstep)
@@ -304,7 +308,7 @@ class JERALDModel:
             g8 = torch.empty(8, dtype=torch.float64, device=dev)
             q = st["q"]
             L.call(f"ldl_hi_depletion_bwd_{L.sfx(dt)}", L.ptr(self.starmap), L.ptr(self.starmapk), L.ptr(st["sfield"]),
-                   L.ptr(st["g_dep"]), L.i64x3(nm), L.dvec(self.box), L.dvec(q[3:11]), L.ptr(g8), L.ptr(ws), ws.numel(),
+                   L.ptr(st["g_dep"]), L.i64x3(nm), L.dvec(self.box), L.dvec(q[3:11]), self.alpha_rule, L.ptr(g8), L.ptr(ws), ws.numel(),
                    L.stream_ptr(dev))
             resp = torch.cat([out[1:4], g8, out[5:6]])
         else:

@@ -58,7 +58,7 @@ class PMesh:
 
     def r2c(self, localreal):
         """Forward-normalised r2c: pfft / N^3 (pmesh.py:112)."""
-        return jpo.pfft(localreal, self.fftdims) / float(np.prod(self.Nmesh))
+        return jpo.pfft(localreal, self.fftdims, scale=1.0 / float(np.prod(self.Nmesh)))
 
     def c2r(self, localcplx):
         """Unnormalised c2r (pmesh.py:127)."""

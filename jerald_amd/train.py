@@ -81,57 +81,54 @@ class Adam:
 
     def import_state(self, filename, paramsreq=False):
         """Reads a state file; with ``paramsreq`` returns (found, params)."""
-        mu, nu, params = [], [], []
-        itr, found, state = 0, False, 0
-        with open(filename) as f:
-            for line in f:
-                line = line.rstrip()
-                if state == 0:
-                    if line == "mu":
-                        state = 1
-                elif state == 1:
-                    if line == "nu":
-                        state = 2
-                    else:
-                        mu.append(float(line))
-                elif state == 2:
-                    if line == "itr":
-                        state = 3
-                    else:
-                        nu.append(float(line))
-                elif state == 3:
-                    itr = int(line)
-                    state = 4
-                elif state == 4:
-                    if line == "params":
-                        state = 5
-                        found = True
-                elif state == 5:
-                    params.append(float(line))
-        self.mu, self.nu, self.itr = np.array(mu), np.array(nu), itr
+        sec = read_state_sections(filename)
+        self.mu = np.array(sec.get("mu", []), dtype=np.float64)
+        self.nu = np.array(sec.get("nu", []), dtype=np.float64)
+        self.itr = int(sec["itr"][0]) if sec.get("itr") else 0
         if paramsreq:
-            if found:
-                return True, np.array(params)
+            if "params" in sec:
+                return True, np.array(sec["params"], dtype=np.float64)
             warnings.warn(f"Parameters not found in {filename}")
             return False, 0
 
     def save_state(self, filename, params=None):
-        LR = _lr(self.LR, self.itr)
-        with open(filename, "w") as f:
-            f.write("mu\n")
-            np.savetxt(f, self.mu)
-            f.write("nu\n")
-            np.savetxt(f, self.nu)
-            f.write("itr\n")
-            f.write(str(self.itr) + "\n")
-            f.write("LR\n")
-            if np.ndim(LR) > 0:
-                np.savetxt(f, LR)
-            else:
-                f.write(str(LR) + "\n")
-            if params is not None:
-                f.write("params\n")
-                np.savetxt(f, params)
+        sections = [("mu", self.mu), ("nu", self.nu), ("itr", [self.itr]), ("LR", _lr(self.LR, self.itr))]
+        if params is not None:
+            sections.append(("params", params))
+        write_state_sections(filename, sections)
+
+
+# Optimizer state files (the layout of model.py:589-659): each section is a
+# keyword line followed by its values, one number per line.
+STATE_KEYWORDS = ("mu", "nu", "itr", "LR", "params")
+
+
+def write_state_sections(filename, sections):
+    """Writes [(keyword, values)] in order; integers as integers, other
+    numbers in %.18e (scalars and arrays alike)."""
+    out = []
+    for key, vals in sections:
+        if key not in STATE_KEYWORDS:
+            raise ValueError(f"unknown state section {key!r}")
+        out.append(key)
+        for v in np.atleast_1d(np.asarray(vals)).ravel():
+            out.append(str(int(v)) if np.issubdtype(np.asarray(v).dtype, np.integer) else format(float(v), ".18e"))
+    with open(filename, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def read_state_sections(filename):
+    """{keyword: [values]} of a state file.  Lines before the first keyword
+    are ignored; numbers are floats except in the ``itr`` section."""
+    sec, cur = {}, None
+    with open(filename) as f:
+        for raw in f:
+            tok = raw.strip()
+            if tok in STATE_KEYWORDS:
+                cur = sec.setdefault(tok, [])
+            elif tok and cur is not None:
+                cur.append(float(tok))
+    return sec
 
 
 def lr_schedule(i, LRi=0.002, LRf=0.02, tau=100, n=1):

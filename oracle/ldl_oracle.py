@@ -326,6 +326,10 @@ class SMModelOracle:
         self.kn = knorm(self.nmesh, self.box)
         self.D = fd_derivative_coeff(self.nmesh, self.box)
         self.c = hermitian_weight(self.nmesh[2])[None, None, :]
+        # weight of the kernel-parameter partials: c (exact adjoint), or under
+        # the reference rules 1 everywhere times c2f's doubling of z bins
+        # [1, int(N1/2)) (model.py:23-29: sz = v.shape[1] is the SECOND axis)
+        self.cg = self.c if grad_rule == "exact" else c2f_weight(self.nmesh)[None, None, :]
         self.cellinv = np.asarray(self.nmesh, dtype=np.float64) / self.box
         if grad_rule not in ("exact", "reference"):
             raise ValueError(grad_rule)
@@ -468,7 +472,7 @@ class SMModelOracle:
         gq[10] = np.sum(g_A)
         gq[3] = np.sum(g_A * st["S"])
         # S = M irfftn(Sk W): dL/dW = c Re(conj(rfftn(g_S)) Sk) on the half spectrum
-        gW = self.c * np.real(np.conj(self._rfftn(beta1 * g_A)) * st["Sk"])
+        gW = self.cg * np.real(np.conj(self._rfftn(beta1 * g_A)) * st["Sk"])
         for j, dW in zip((4, 5, 6, 9), st["dW"]):
             gq[j] = np.sum(gW * dW)
         return g_rho, gq
@@ -545,7 +549,7 @@ class SMModelOracle:
         re = np.real(np.conj(Q) * U)
         asc = ALPHA_SCALE_HI if self.kind == "HI" else ALPHA_SCALE
         grad[off + 0] = asc * np.sum(cw * G * re)
-        gG = asc * alpha * self.c * re
+        gG = asc * alpha * self.cg * re
         if self.kind == "dm":  # d/dkh, d/dkl of -kn/kl - kh/kn
             grad[off + 2] = np.sum(gG * G * (-1.0 / kn))
             grad[off + 3] = np.sum(gG * G * (kn / kl ** 2))
@@ -558,7 +562,7 @@ class SMModelOracle:
             G2, S2 = cache["G2"], cache["S2"]
             re2 = np.real(np.conj(Q) * S2)
             grad[off + 5] = asc * np.sum(cw * G2 * re2)
-            gG2 = asc * alpha2 * self.c * re2
+            gG2 = asc * alpha2 * self.cg * re2
             grad[off + 7] = np.sum(gG2 * G2 * (-2.0 * kh2 / kn ** 2))
             grad[off + 8] = np.sum(gG2 * G2 * (2.0 * kn ** 2 / kl2 ** 3))
             grad[off + 9] = np.sum(gG2 * G2 * np.log(kn))
@@ -593,6 +597,16 @@ def ref_rule_c2f_bwd(v):
     sz = v.shape[1]
     v[:, :, 1:sz // 2] += np.conj(v[:, :, 1:sz // 2])
     return v
+
+
+def c2f_weight(nmesh):
+    """Half-spectrum weights that c2f's backward (model.py:23-29) applies to
+    the cotangent of a real kernel: 2 on z bins [1, int(N1/2)) -- N1 the second
+    mesh axis, the slice clipped to the N2/2+1 bins -- and 1 elsewhere (= the
+    Hermitian weight c when N1 == N2)."""
+    w = np.ones(int(nmesh[2]) // 2 + 1)
+    w[1:int(nmesh[1]) // 2] = 2.0
+    return w
 
 
 def exact_pfft_bwd(g, shape):

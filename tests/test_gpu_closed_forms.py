@@ -1,0 +1,206 @@
+"""GPU checks of the LDL path against closed forms and special cases the
+mathematics fixes (tests/closed_forms.py), of the reference-rule gradients
+on meshes whose second axis differs from the third, of meshes with odd first
+and second axes, and of the op-level c2f / r2c helpers.
+
+Tolerances: FP64 kernels against closed forms and the FP64 oracle to ~1e-11
+relative; FP32 kernels to 1e-5 relative of the quantity's scale."""
+import numpy as np
+import pytest
+import torch
+
+from jerald_amd import jaxpops as J
+from jerald_amd.model import JERALDModel
+from jerald_amd.pmesh import PMesh
+from oracle import ldl_oracle as O
+
+import closed_forms as CF
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+PRM = [40.0, 1.0, 0.3, 1.5, 0.4]
+
+
+def _model(X, tgt, nm, box, nstep, dt, **kw):
+    pm = PMesh(nm, box, device=DEV, dtype=dt)
+    return JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, dtype=dt, **kw)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("gamma,m", [(1.0, 1), (2.0, 2)])
+@pytest.mark.parametrize("lattice", [False, True])
+def test_layer_displacement_closed_form(axis, gamma, m, lattice):
+    N, box = 16, 24.0
+    prm = list(PRM)
+    prm[1] = gamma
+    X = CF.one_axis_positions(N, box, axis, m=m)
+    want = CF.layer_closed_form(N, box, axis, prm, m=m)
+    moved = np.abs(want - X).max()
+    p = torch.tensor(prm + [1.0, 1.0, 0.0])
+    for dt, tol in ((torch.float64, 1e-11), (torch.float32, 1e-5)):
+        mdl = _model(X, np.ones((N, N, N)), N, box, 1, dt, kind="sm", L1=False, lattice=(N, N, N) if lattice else None)
+        got = mdl.displace(p).double().cpu().numpy()
+        # compare displacements (positions wrap at the box edge)
+        d = (got - X + box / 2) % box - box / 2
+        dw = (want - X + box / 2) % box - box / 2
+        assert np.abs(d - dw).max() <= tol * moved, (dt, np.abs(d - dw).max() / moved)
+
+
+@pytest.mark.parametrize("mu", [1.0, 2.0])
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_response_and_l2_loss_closed_form(mu, dt):
+    N, box = 16, 24.0
+    w, b, t = 1.3, 0.2, 0.7
+    X = CF.one_axis_positions(N, box, 0)
+    mdl = _model(X, np.full((N, N, N), t), N, box, 0, dt, kind="sm", L1=False)
+    loss, grad = mdl.loss_and_grad(torch.tensor([mu, w, b]))
+    want, gw, gb = CF.sm_loss_closed_form(N, mu, w, b, t)
+    tol = 1e-12 if dt == torch.float64 else 1e-5
+    assert abs(loss.item() - want) <= tol * want
+    g = grad.cpu().numpy()
+    assert abs(g[1] - gw) <= tol * abs(gw) and abs(g[2] - gb) <= tol * abs(gb)
+    rho = CF.density_profile(N)[0]
+    F = mdl.evaluate(torch.tensor([mu, w, b])).double().cpu().numpy()
+    np.testing.assert_allclose(F, np.broadcast_to((w * (rho + 1e-8) ** mu + b)[:, None, None], (N, N, N)),
+                               rtol=1e-13 if dt == torch.float64 else 2e-6)
+
+
+@pytest.mark.parametrize("smooth", [("gaussian", 2.5), ("reference", 1.0), ("l1", None)])
+def test_smoothed_and_l1_losses_closed_form(smooth):
+    N, box = 16, 24.0
+    mu, w, b, t = 1.0, 1.3, 0.2, 0.7 if smooth[0] != "l1" else 0.1
+    X = CF.one_axis_positions(N, box, 2)
+    if smooth[0] == "gaussian":
+        kw = dict(L1=False, smoothing=smooth)
+        want = CF.sm_loss_closed_form(N, mu, w, b, t, smooth=smooth, box=box)[0]
+    elif smooth[0] == "reference":
+        kw = dict(L1=True, n=smooth[1])
+        want = CF.sm_loss_closed_form(N, mu, w, b, t, smooth=smooth, box=box, L1=True)[0]
+    else:
+        kw = dict(L1=True)
+        want = CF.sm_loss_closed_form(N, mu, w, b, t, L1=True)[0]
+    mdl = _model(X, np.full((N, N, N), t), N, box, 0, torch.float64, kind="sm", **kw)
+    loss, _ = mdl.lossv(torch.tensor([mu, w, b]))
+    assert abs(loss.item() - want) <= 1e-11 * want
+
+
+@pytest.mark.parametrize("axis,ratio", [(2, 0.5), (0, 1.0)])
+@pytest.mark.parametrize("lattice", [False, True])
+def test_reference_rule_alpha_special_cases(axis, ratio, lattice):
+    """Density varying along z only: spectrum on the k_x = k_y = 0 line with
+    the k_z = 0 and Nyquist bins silenced (G(0) = 0, D_z(Nyquist) = 0), so the
+    reference-rule dL/dalpha is exactly 1/2 of the exact one; along x only:
+    the k_z = 0 plane, weight 1 in both rules."""
+    N, box, nstep = 16, 32.0, 2
+    X = CF.one_axis_positions(N, box, axis)
+    tgt = np.random.default_rng(9).random((N, N, N)) + 0.5
+    p = torch.tensor([0.7, 1.2, 0.3, 1.4, 0.2] * nstep + [1.1, 0.9, 0.05])
+    res = {}
+    for rule in ("exact", "reference"):
+        mdl = _model(X, tgt, N, box, nstep, torch.float64, kind="sm", L1=False, grad_rule=rule,
+                     lattice=(N, N, N) if lattice else None)
+        res[rule] = mdl.loss_and_grad(p)[1].cpu().numpy()
+    ge, gr = res["exact"], res["reference"]
+    for a in (0, 5):
+        assert abs(ge[a]) > 1e-8 and abs(gr[a] / ge[a] - ratio) < 1e-9, (a, gr[a] / ge[a])
+    other = [i for i in range(len(ge)) if i not in (0, 5)]
+    np.testing.assert_allclose(gr[other], ge[other], rtol=1e-12)
+
+
+@pytest.mark.parametrize("nm,box", [((16, 8, 32), (32.0, 20.0, 48.0)), ((12, 20, 10), (30.0, 40.0, 24.0))])
+@pytest.mark.parametrize("kind", ["sm", "dm", "HI"])
+def test_reference_rule_on_noncubic_meshes(nm, box, kind):
+    """c2f doubles z bins [1, int(N1/2)) -- fewer than the interior planes on
+    (16, 8, 32), all of them plus the Nyquist plane on (12, 20, 10): the
+    kernel-parameter gradients under grad_rule="reference" follow the
+    oracle's rule emulation (itself pinned by autograd applying the rules)."""
+    nstep = 2
+    rng = np.random.default_rng(11)
+    X = rng.random((int(np.prod(nm)), 3)) * np.asarray(box)
+    tgt = rng.random(nm) + 0.5
+    star = None
+    if kind == "HI":
+        star = rng.random(nm) * 1.5
+        p = np.array([7.0, 1.2, 0.3, 1.4, 0.2, 5.0, 0.8, 0.2, 1.1, 0.3] * nstep
+                     + [1.1, 0.9, 0.3, 0.4, 0.05, 2.0, 0.3, 0.2, 1.3, 0.1, 0.02, 0.05])
+    else:
+        p = np.array([0.7, 1.2, 0.3, 1.4, 0.2] * nstep + ([1.1, 0.9, 0.05] if kind == "sm" else []))
+    orc = O.SMModelOracle(nm, box, nstep, tgt, L1=False, kind=kind, starmap=star, grad_rule="reference",
+                          smoothing=("gaussian", 3.0))
+    loss0, g0 = orc.loss_and_grad(p, X)
+    mdl = _model(X, tgt, nm, box, nstep, torch.float64, kind=kind, L1=False, starmap=star, grad_rule="reference",
+                 smoothing=("gaussian", 3.0))
+    loss, g = mdl.loss_and_grad(torch.tensor(p))
+    assert abs(loss.item() - loss0) <= 1e-10 * abs(loss0)
+    np.testing.assert_allclose(g.cpu().numpy(), g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
+
+
+@pytest.mark.parametrize("kind", ["sm", "HI"])
+@pytest.mark.parametrize("smoothing", [None, ("gaussian", 3.0)])
+def test_odd_first_and_second_axes(kind, smoothing):
+    """fftfreq of odd N is 0..(N-1)/2, -(N-1)/2..-1: meshes with odd first and
+    second axes put a +(N-1)/2 plane where an even-N rule would put -(N+1)/2,
+    which changes |k| there (G, the smoothing and the HI filter)."""
+    nm, box, nstep = (45, 15, 20), (90.0, 30.0, 40.0), 2
+    rng = np.random.default_rng(13)
+    X = rng.random((13500, 3)) * np.asarray(box)
+    tgt = rng.random(nm) + 0.5
+    star = None
+    if kind == "HI":
+        star = rng.random(nm) * 1.5
+        p = np.array([7.0, 1.2, 0.3, 1.4, 0.2, 5.0, 0.8, 0.2, 1.1, 0.3] * nstep
+                     + [1.1, 0.9, 0.3, 0.4, 0.05, 2.0, 0.3, 0.2, 1.3, 0.1, 0.02, 0.05])
+    else:
+        p = np.array([0.7, 1.2, 0.3, 1.4, 0.2] * nstep + [1.1, 0.9, 0.05])
+    orc = O.SMModelOracle(nm, box, nstep, tgt, L1=False, kind=kind, starmap=star, smoothing=smoothing)
+    loss0, g0 = orc.loss_and_grad(p, X)
+    mdl = _model(X, tgt, nm, box, nstep, torch.float64, kind=kind, L1=False, starmap=star, smoothing=smoothing)
+    loss, g = mdl.loss_and_grad(torch.tensor(p))
+    assert abs(loss.item() - loss0) <= 1e-10 * abs(loss0)
+    np.testing.assert_allclose(g.cpu().numpy(), g0, rtol=1e-8, atol=1e-12 * np.abs(g0).max())
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 5), (8, 4, 7), (6, 12, 5)])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_c2f_op_backward(shape, cplx):
+    """jaxpops.c2f: identity forward; backward v + conj(v) on z bins
+    [1, int(v.shape[1]/2)) under the reference rule (model.py:23-29), the
+    identity under the exact rule."""
+    rng = np.random.default_rng(sum(shape))
+    v = rng.standard_normal(shape) + (1j * rng.standard_normal(shape) if cplx else 0)
+    g = rng.standard_normal(shape) + (1j * rng.standard_normal(shape) if cplx else 0)
+    try:
+        for rule in ("reference", "exact"):
+            J.set_vjp_rule(rule)
+            x = torch.tensor(v, device=DEV, requires_grad=True)
+            y = J.c2f(x)
+            assert torch.equal(y.detach(), x.detach())
+            y.backward(torch.tensor(g, device=DEV))
+            want = O.ref_rule_c2f_bwd(g) if rule == "reference" else g
+            np.testing.assert_allclose(x.grad.cpu().numpy(), want, rtol=0, atol=0)
+    finally:
+        J.set_vjp_rule("reference")
+
+
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_pmesh_r2c_scale_and_adjoint(dt):
+    """PMesh.r2c = pfft / N^3 in one transform, and its backward under the
+    exact rule is torch's own autograd of rfftn / N^3."""
+    nm, box = (8, 12, 16), (8.0, 12.0, 16.0)
+    pm = PMesh(nm, box, device=DEV, dtype=dt)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(nm)
+    X = pm.r2c(torch.tensor(x, device=DEV, dtype=dt)).cpu().numpy()
+    rt = 1e-13 if dt == torch.float64 else 2e-6
+    np.testing.assert_allclose(X, np.fft.rfftn(x) / np.prod(nm), rtol=0, atol=rt * np.abs(np.fft.rfftn(x)).max() / np.prod(nm))
+    G = rng.standard_normal(X.shape) + 1j * rng.standard_normal(X.shape)
+    J.set_vjp_rule("exact")
+    try:
+        xt = torch.tensor(x, device=DEV, dtype=torch.float64, requires_grad=True)
+        pm64 = PMesh(nm, box, device=DEV, dtype=torch.float64)
+        pm64.r2c(xt).backward(torch.tensor(G, device=DEV))
+        xc = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+        (torch.fft.rfftn(xc) / np.prod(nm)).backward(torch.tensor(G))
+        np.testing.assert_allclose(xt.grad.cpu().numpy(), xc.grad.numpy(), rtol=1e-12, atol=1e-15)
+    finally:
+        J.set_vjp_rule("reference")

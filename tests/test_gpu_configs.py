@@ -1,0 +1,253 @@
+"""The five workload configurations of the LDL path, run on the HIP kernels:
+
+c0  64^3 particles on a 64^3 mesh, L = 100, FP64, 1 layer, rms 1.5 cells:
+    loss and all 8 gradients against the FP64 oracle (1e-10 / 1e-8), FP32
+    against it at 1e-5.
+c1  256^3 on 256^3, L = 205, FP32, 2 layers (13 parameters), rms 2 cells:
+    loss and every gradient within 1e-5 of the FP64 oracle; the FP64 HIP
+    path against the oracle at 1e-10 / 1e-8; central finite differences of
+    the FP64 HIP loss at h = 1e-3 relative for all 13 parameters.
+c2  512^3 on 512^3 (the headline size), 2 layers, per-layer position
+    recomputation: FP32 against the FP64 HIP path within 1e-5 for the loss
+    and every gradient, density within 1e-5, mass conservation, bitwise
+    identical reruns, recomputation identical to storage.
+c3  512^3 particles on a 1024^3 mesh, 40% in 2000 clumps: bitwise identical
+    loss, gradients and density over 3 reruns; FP32 against FP64 within 1e-5.
+c4  1024^3 on 1024^3, 3 layers, positions checkpointed: finite loss and
+    gradients within the HBM; the same configuration at 256^3 within 1e-5 of
+    the FP64 oracle.
+
+Every configuration uses the seeded Zel'dovich lattice (seed 83), the
+parameters [alpha, gamma, kh, kl, nu] = [0.01, 1, 0.1, 1, 0] per layer and
+[mu, w, b] = [1, 1, 0], a target from the model at seed-perturbed (+-10%,
+seed 84) parameters plus 1% Gaussian noise (seed 85), and the L2 loss of the
+residual smoothed by a 1-cell Gaussian."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from jerald_amd import _lib as L
+from jerald_amd.model import JERALDModel
+from jerald_amd.pmesh import PMesh
+from jerald_amd.synthetic import clumped_positions, zeldovich_gpu
+from oracle import ldl_oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def params0(nstep):
+    return np.array([0.01, 1.0, 0.1, 1.0, 0.0] * nstep + [1.0, 1.0, 0.0])
+
+
+def perturbed(p):
+    rng = np.random.default_rng(84)
+    return p * (1 + 0.1 * (2 * rng.random(p.shape) - 1))
+
+
+def release():
+    L.workspaces.clear()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+
+def gpu_problem(n, box, nstep, rms, dt_target=torch.float64, X=None):
+    """Positions (GPU, float64 box units) and the target map (GPU float64)
+    made by the FP64 HIP model at the perturbed parameters plus noise."""
+    if X is None:
+        X = zeldovich_gpu(n, box, rms, 83, DEV)
+    pm = PMesh(n, box, device=DEV, dtype=dt_target)
+    gen = JERALDModel(X, None, pm, Nstep=nstep, kind="sm", L1=False)
+    tgt = gen.evaluate(torch.tensor(perturbed(params0(nstep)))).double()
+    del gen
+    g = torch.Generator(device=DEV).manual_seed(85)
+    tgt += 0.01 * tgt.std() * torch.randn(tgt.shape, generator=g, device=DEV, dtype=torch.float64)
+    return X, tgt
+
+
+def model(X, tgt, n, box, nstep, dt, lattice=True, **kw):
+    pm = PMesh(n, box, device=DEV, dtype=dt)
+    kw.setdefault("smoothing", ("gaussian", box / n))
+    return JERALDModel(X, tgt, pm, Nstep=nstep, kind="sm", L1=False, dtype=dt,
+                       lattice=(n, n, n) if lattice else None, **kw)
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)) / np.abs(np.asarray(b))
+
+
+# --------------------------------------------------------------------- c0
+def test_c0_64cubed_fp64_one_layer():
+    n, box, nstep = 64, 100.0, 1
+    p = params0(nstep)
+    X = O.lattice_positions(n, box) + O.zeldovich_displacement(n, box, 1.5, 83)
+    gen = O.SMModelOracle(n, box, nstep, np.zeros((n, n, n)), L1=False)
+    tgt = gen.evaluate(perturbed(p), X)
+    tgt = tgt + 0.01 * tgt.std() * np.random.default_rng(85).standard_normal(tgt.shape)
+    orc = O.SMModelOracle(n, box, nstep, tgt, L1=False, smoothing=("gaussian", box / n))
+    loss0, g0 = orc.loss_and_grad(p, X)
+    for dt, lt, gt in ((torch.float64, 1e-10, 1e-8), (torch.float32, 1e-5, 1e-5)):
+        for lattice in (False, True):
+            m = model(torch.tensor(X), tgt, n, box, nstep, dt, lattice=lattice)
+            loss, g = m.loss_and_grad(torch.tensor(p))
+            g = g.cpu().numpy()
+            assert g.shape == (8,)
+            assert rel(loss.item(), loss0) <= lt, (dt, lattice, rel(loss.item(), loss0))
+            assert np.all(rel(g, g0) <= gt), (dt, lattice, rel(g, g0))
+    release()
+
+
+# --------------------------------------------------------------------- c1
+@pytest.fixture(scope="module")
+def c1():
+    n, nstep = 256, 2
+    box = 205.0
+    X, tgt = gpu_problem(n, box, nstep, 2.0)
+    Xc, tc = X.cpu().numpy(), tgt.cpu().numpy()
+    orc = O.SMModelOracle(n, box, nstep, tc, L1=False, smoothing=("gaussian", box / n), backend="native")
+    p = params0(nstep)
+    loss0, g0 = orc.loss_and_grad(p, Xc)
+    yield dict(n=n, box=box, nstep=nstep, X=X, tgt=tgt, p=p, loss0=loss0, g0=g0)
+    release()
+
+
+def test_c1_256cubed_fp32_within_1e5_of_oracle(c1):
+    m = model(c1["X"], c1["tgt"], c1["n"], c1["box"], c1["nstep"], torch.float32)
+    loss, g = m.loss_and_grad(torch.tensor(c1["p"]))
+    g = g.cpu().numpy()
+    assert g.shape == (13,)
+    assert rel(loss.item(), c1["loss0"]) <= 1e-5
+    assert np.all(rel(g, c1["g0"]) <= 1e-5), rel(g, c1["g0"])
+
+
+def test_c1_256cubed_fp64_matches_oracle(c1):
+    m = model(c1["X"], c1["tgt"], c1["n"], c1["box"], c1["nstep"], torch.float64, lattice=False)
+    loss, g = m.loss_and_grad(torch.tensor(c1["p"]))
+    assert rel(loss.item(), c1["loss0"]) <= 1e-10
+    np.testing.assert_allclose(g.cpu().numpy(), c1["g0"], rtol=1e-8)
+
+
+def test_c1_central_finite_differences_on_the_hip_path(c1):
+    """Central differences of the FP64 HIP loss at h = 1e-3 |p_i| (1e-3 for
+    nu = 0).  The truncation error h^2 f'''/6 measured on the oracle is at most
+    2e-5 relative (mu), hence the 5e-5 gate.  b = 0 puts the response
+    w (rho + 1e-8)^mu + b of empty cells 1e-8 above the ReLU floor, where the
+    loss has a kink; its step is 1e-9 so the difference stays on one side."""
+    m = model(c1["X"], c1["tgt"], c1["n"], c1["box"], c1["nstep"], torch.float64, lattice=False)
+    p = c1["p"]
+    loss, g = m.loss_and_grad(torch.tensor(p))
+    g = g.cpu().numpy()
+    for i in range(len(p)):
+        h = 1e-3 * abs(p[i]) if p[i] != 0 else (1e-9 if i == len(p) - 1 else 1e-3)
+        a, b = p.copy(), p.copy()
+        a[i] += h
+        b[i] -= h
+        fd = (m.lossv(torch.tensor(a))[0].item() - m.lossv(torch.tensor(b))[0].item()) / (2 * h)
+        assert abs(g[i] - fd) <= 5e-5 * abs(fd), (i, g[i], fd)
+
+
+# --------------------------------------------------------------------- c2
+def _density(m, p):
+    """Final density of the model's forward (FP64 sum of cells and the mesh)."""
+    P = m.X
+    fact = float(np.prod(m.nm)) / P.shape[0]
+    for i in range(m.Nstep):
+        P = m._layer_fwd(P, m._layer_prm(p, i), fact)[0]
+    return m._paint_density(P, fact).double()
+
+
+def test_c2_512cubed_headline_precision_and_recompute():
+    n, nstep = 512, 2
+    box = 0.8 * n
+    X, tgt = gpu_problem(n, box, nstep, 2.0, dt_target=torch.float32)
+    p = params0(nstep)
+    out = {}
+    for key, dt, save, lat in (("f32", torch.float32, "positions", True), ("f32_all", torch.float32, "all", True),
+                               ("f64", torch.float64, "positions", False)):
+        m = model(X, tgt, n, box, nstep, dt, lattice=lat, save=save, paint_mode="fixed" if dt == torch.float32 else "float")
+        loss, g = m.loss_and_grad(torch.tensor(p))
+        out[key] = (loss.item(), g.cpu().numpy())
+        if key == "f32":
+            loss2, g2 = m.loss_and_grad(torch.tensor(p))  # bitwise rerun
+            assert loss2.item() == loss.item() and np.array_equal(g2.cpu().numpy(), out[key][1])
+            rho32 = _density(m, p)
+            assert abs(rho32.sum().item() / n ** 3 - 1) < 1e-6  # mass conservation
+        if key == "f64":
+            rho64 = _density(m, p)
+        del m
+        release()
+    # recomputation of the layers gives exactly what storing them gives
+    assert out["f32"][0] == out["f32_all"][0] and np.array_equal(out["f32"][1], out["f32_all"][1])
+    l32, g32 = out["f32"]
+    l64, g64 = out["f64"]
+    assert rel(l32, l64) <= 1e-5
+    assert np.all(rel(g32, g64) <= 1e-5), rel(g32, g64)
+    drho = (rho32 - rho64).abs()
+    assert drho.max().item() <= 1e-5 * max(rho64.max().item(), 1.0), drho.max().item()
+
+
+# --------------------------------------------------------------------- c3
+def test_c3_clumps_on_1024_mesh_deterministic_and_within_1e5():
+    n, nm, nstep = 512, 1024, 2
+    box = 0.8 * nm
+    X0 = zeldovich_gpu(n, box, 2.0 * nm / n, 83, DEV)
+    X = clumped_positions(X0, box, box / nm)
+    del X0
+    _, tgt = gpu_problem(nm, box, nstep, 0.0, dt_target=torch.float32, X=X)
+    p = params0(nstep)
+    pm = PMesh(nm, box, device=DEV, dtype=torch.float32)
+    m = JERALDModel(X, tgt, pm, Nstep=nstep, kind="sm", L1=False, smoothing=("gaussian", box / nm), save="positions")
+    runs = []
+    for _ in range(3):
+        loss, g = m.loss_and_grad(torch.tensor(p))
+        runs.append((loss.item(), g.cpu().numpy(), _density(m, p).float().cpu().numpy()))
+    for r in runs[1:]:
+        assert r[0] == runs[0][0] and np.array_equal(r[1], runs[0][1]) and np.array_equal(r[2], runs[0][2])
+    del m
+    release()
+    pm64 = PMesh(nm, box, device=DEV, dtype=torch.float64)
+    m64 = JERALDModel(X, tgt, pm64, Nstep=nstep, kind="sm", L1=False, smoothing=("gaussian", box / nm),
+                      save="positions", paint_mode="float")
+    l64, g64 = m64.loss_and_grad(torch.tensor(p))
+    assert rel(runs[0][0], l64.item()) <= 1e-5
+    assert np.all(rel(runs[0][1], g64.cpu().numpy()) <= 1e-5), rel(runs[0][1], g64.cpu().numpy())
+    del m64
+    release()
+
+
+# --------------------------------------------------------------------- c4
+def test_c4_1024cubed_three_layers_positions_checkpointed():
+    n, nstep = 1024, 3
+    box = 0.8 * n
+    X, tgt = gpu_problem(n, box, nstep, 2.0, dt_target=torch.float32)
+    p = params0(nstep)
+    m = model(X, tgt, n, box, nstep, torch.float32, save="positions")
+    del X
+    torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats()
+    loss, g = m.loss_and_grad(torch.tensor(p))
+    g = g.cpu().numpy()
+    peak = torch.cuda.max_memory_allocated()
+    print(f"c4 1024^3 3 layers: peak {peak / 1e9:.1f} GB (tensors held by the step), loss {loss.item():.8g}")
+    assert g.shape == (18,) and math.isfinite(loss.item()) and np.all(np.isfinite(g))
+    assert peak < 288e9
+    # every layer's parameters carry gradient, like the smaller runs'
+    assert np.all(np.abs(g[:15]) > 0)
+    del m, tgt
+    release()
+
+
+def test_c4_256cubed_subrun_against_the_oracle():
+    n, nstep = 256, 3
+    box = 0.8 * n
+    X, tgt = gpu_problem(n, box, nstep, 2.0)
+    p = params0(nstep)
+    orc = O.SMModelOracle(n, box, nstep, tgt.cpu().numpy(), L1=False, smoothing=("gaussian", box / n), backend="native")
+    loss0, g0 = orc.loss_and_grad(p, X.cpu().numpy())
+    m = model(X, tgt, n, box, nstep, torch.float32, save="positions")
+    loss, g = m.loss_and_grad(torch.tensor(p))
+    assert rel(loss.item(), loss0) <= 1e-5
+    assert np.all(rel(g.cpu().numpy(), g0) <= 1e-5), rel(g.cpu().numpy(), g0)
+    release()

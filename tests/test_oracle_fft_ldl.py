@@ -8,6 +8,8 @@ import torch
 
 from oracle import ldl_oracle as O
 
+import torch_ldl as TL
+
 
 def brute_r2c(x):
     n0, n1, n2 = x.shape
@@ -192,37 +194,6 @@ def test_reference_rule_changes_only_alpha(kind):
 
 
 # ---------------- independent torch float64 autograd restatement ----------
-def _t_paint(x, mass, nm):
-    fl = torch.floor(x)
-    f = x - fl
-    i0 = fl.long()
-    out = torch.zeros(nm[0] * nm[1] * nm[2], dtype=x.dtype)
-    for o in [(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)]:
-        w = torch.ones_like(x[:, 0]) * mass
-        idx = []
-        for d in range(3):
-            w = w * (f[:, d] if o[d] else 1 - f[:, d])
-            idx.append((i0[:, d] + o[d]) % nm[d])
-        out = out.index_add(0, (idx[0] * nm[1] + idx[1]) * nm[2] + idx[2], w)
-    return out.reshape(nm)
-
-
-def _t_readout(x, F, nm):
-    fl = torch.floor(x)
-    f = x - fl
-    i0 = fl.long()
-    Ff = F.reshape(-1)
-    acc = torch.zeros_like(x[:, 0])
-    for o in [(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)]:
-        w = torch.ones_like(x[:, 0])
-        idx = []
-        for d in range(3):
-            w = w * (f[:, d] if o[d] else 1 - f[:, d])
-            idx.append((i0[:, d] + o[d]) % nm[d])
-        acc = acc + w * Ff[(idx[0] * nm[1] + idx[1]) * nm[2] + idx[2]]
-    return acc
-
-
 @pytest.mark.parametrize("kind", ["sm", "dm", "HI"])
 def test_ldl_against_torch_autograd(kind):
     N, L = 8, 40.0
@@ -232,55 +203,10 @@ def test_ldl_against_torch_autograd(kind):
         p = p[:10]
     if kind == "HI":
         X, tgt, p, _, star = _hi_setup(N=N, L=L)
-        st = torch.tensor(star)
     m = O.SMModelOracle(N, L, 2, tgt, L1=False, kind=kind, starmap=star)
     loss, g = m.loss_and_grad(p, X)
-    nm = (N, N, N)
     pt = torch.tensor(p, requires_grad=True)
-    Xt = torch.tensor(X)
-    kn = torch.tensor(O.knorm(nm, L))
-    D = [torch.tensor(np.broadcast_to(d, (N, N, N // 2 + 1)).copy()) for d in O.fd_derivative_coeff(nm, L)]
-    cell = L / N
-    fact = N ** 3 / X.shape[0]
-    k = 10 if kind == "HI" else 5
-    lo = torch.tensor(1e-12, dtype=torch.float64)
-    for i in range(2):
-        a, gam, kh, kl, nu = pt[k * i: k * i + 5]
-        x = torch.remainder(Xt, L) / cell
-        rho = _t_paint(x, 1.0, nm) * fact
-        U = torch.fft.rfftn((rho + 1e-8) ** gam) / N ** 3
-        if kind == "dm":
-            G = -torch.exp(-kn / kl - kh / kn) * kn ** nu
-        else:
-            G = -torch.exp(-kn ** 2 / kl ** 2 - kh ** 2 / kn ** 2) * kn ** nu
-        if kind == "HI":
-            a2, gam2, kh2, kl2, nu2 = pt[k * i + 5: k * i + 10]
-            G2 = torch.exp(-kn ** 2 / kl2 ** 2 - kh2 ** 2 / kn ** 2) * kn ** nu2
-            # torch's pow gradient in the exponent is 0 at a zero base, as JAX's
-            S2 = torch.fft.rfftn(st ** gam2) / N ** 3
-            P = 1e-3 * (a * G * U + a2 * G2 * S2)
-        else:
-            P = 1e-2 * a * U * G
-        s = torch.stack([_t_readout(x, torch.fft.irfftn(-1j * D[d] * P, s=nm) * N ** 3, nm) for d in range(3)], -1)
-        Xt = Xt + s
-    x = torch.remainder(Xt, L) / cell
-    rho = _t_paint(x, 1.0, nm) * fact
-    if kind == "dm":
-        F = rho
-    elif kind == "HI":
-        mu, w, b, beta1, khi, kli, nui, beta2, gam3, xi, eta, zs = pt[20:32]
-        W = torch.exp(-kn / kli - khi / kn) * kn ** nui + xi
-        S = torch.fft.irfftn(torch.fft.rfftn(st) * W, s=nm)
-        A = beta1 * S + eta
-        dep = torch.where(A > 1e-12, A, lo) + beta2 * st ** gam3
-        R = torch.where(rho + zs > 1e-12, rho + zs, lo)
-        y = w * R ** mu - dep + b
-        F = torch.where(y > 1e-12, y, lo)
-    else:
-        mu, w, b = pt[10:13]
-        F = torch.where(w * (rho + 1e-8) ** mu + b > 1e-12, w * (rho + 1e-8) ** mu + b,
-                        torch.tensor(1e-12, dtype=torch.float64))
-    lt = torch.sum((F - torch.tensor(tgt)) ** 2) / N ** 3
+    lt = TL.model_loss(pt, X, tgt, (N, N, N), L, 2, kind=kind, star=star)
     lt.backward()
     assert abs(lt.item() - loss) < 1e-12 * max(1, abs(loss))
     np.testing.assert_allclose(pt.grad.numpy(), g, rtol=1e-9, atol=1e-14)

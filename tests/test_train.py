@@ -53,6 +53,26 @@ def test_adam_state_file_round_trip(tmp_path):
     np.testing.assert_array_equal(b.step(pb, g), a.step(p, g))
 
 
+def test_state_file_text_is_the_savetxt_layout(tmp_path):
+    """The writer's text is the layout model.py:636-659 produces with
+    numpy.savetxt (%.18e per value, one per line) for array learning rates."""
+    import io
+    a = T.Adam(LR=lambda i: T.lr_schedule(i, n=2))
+    a.init(np.zeros(2))
+    p = a.step(np.array([0.5, -1.5]), np.array([0.2, -0.4]))
+    f = tmp_path / "s.txt"
+    a.save_state(f, p)
+    buf = io.StringIO()
+    for key, vals in (("mu", a.mu), ("nu", a.nu)):
+        buf.write(key + "\n")
+        np.savetxt(buf, vals)
+    buf.write("itr\n1\nLR\n")
+    np.savetxt(buf, a.LR(1))
+    buf.write("params\n")
+    np.savetxt(buf, p)
+    assert f.read_text() == buf.getvalue()
+
+
 def test_adam_import_without_params_warns(tmp_path):
     f = tmp_path / "opt.txt"
     f.write_text("mu\n1.0\n2.0\nnu\n3.0\n4.0\nitr\n7\nLR\n0.01\n")
