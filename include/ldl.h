@@ -86,8 +86,12 @@ int ldl_timer_read(const char* name, double* total_ms, int64_t* count);
 int ldl_buildplan(const int64_t nmesh[3], int32_t out[2]);
 
 /* ----------------------------------------------------------------- paint */
-/* Workspace for a paint of ``ncomp`` (1 or 3) components of ``np`` particles. */
-size_t ldl_paint_workspace_bytes(const int64_t nmesh[3], int32_t ncomp, int64_t np);
+/* Workspace for a paint of ``ncomp`` (1 or 3) components of ``np`` particles
+ * with paint ``flags``: the particle sort's keys and indices (12 B per
+ * particle plus 8 B per 8x16x32-cell tile); LDL_PAINT_DIRECT adds the 64-bit
+ * accumulator mesh (8 B per cell and component); LDL_PAINT_FLOAT_ATOMIC needs
+ * none (0). */
+size_t ldl_paint_workspace_bytes(const int64_t nmesh[3], int32_t ncomp, int64_t np, int32_t flags);
 
 /* Replaces ppaint_f32/_f64 (lib/cpu.cc:51-72 -> lib/parops_fast.h:483-531).
  * out (N0,N1,N2) = out_scale * CIC deposit of ``mass`` (Np,) or, if mass is
@@ -177,7 +181,7 @@ int ldl_spectrum_zweight_f64(const double* in, double* out, const int64_t nmesh[
  * P = 1e-2 alpha G(k) U with G = -exp(-k^2/kl^2 - kh^2/k^2) k^nu;
  * F_d = c2r(-i D_d(k) P) with D_d the 6th-order FD symbol; s_d = readout(X, F_d).
  * Positions are LDL_POS_LATTICE displacements (mesh units) or physical. */
-size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np);
+size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np, int32_t paint_flags);
 
 /* Size of a layer's particle sort (tile offsets + sorted indices, uint32): the
  * forward's density paint can record it and the backward's 3-component paint
@@ -195,7 +199,9 @@ int64_t ldl_sm_spectrum_pitch(const int64_t nmesh[3], int32_t is_f64);
  * backward needs: rho (N0,N1,N2), U (N0,N1,Hp complex, = r2c of
  * (rho+1e-8)^gamma), forces F (N0,N1,N2,4) interleaved per cell as
  * (F_x, F_y, F_z, unused), and -- if ``bins`` is not NULL
- * (ldl_sm_bins_bytes) -- the particle sort of the fixed-point paint. */
+ * (ldl_sm_bins_bytes) -- the particle sort of the fixed-point paint.
+ * pos_out may be NULL: the saved tensors only, no gather (a backward pass
+ * that recomputes the layer from checkpointed positions). */
 int ldl_sm_layer_fwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,
                          const int64_t nmesh[3], const double prm[5], double fact, int32_t paint_flags,
                          float* pos_out, float* rho, float* U, float* F, void* bins,
@@ -316,7 +322,8 @@ int ldl_hi_layer_bwd_f64(const double* pos, const ldl_pos_desc* pd, int64_t np,
  *   dep    = ReLU(beta1 sfield + eta) + beta2 starmap^gamma3.
  * starmapk = r2c(starmap) / N^3 as (N0,N1,N2/2+1) interleaved complex
  * (PMesh.r2c, model.py:127), or NULL to compute it in the workspace.
- * Workspace: ldl_sm_workspace_bytes(nmesh, is_f64, 0). */
+ * Workspace: ldl_sm_workspace_bytes(nmesh, is_f64, 0, LDL_PAINT_FLOAT_ATOMIC) or more
+ * (any workspace sized for a layer of this mesh will do). */
 int ldl_hi_depletion_fwd_f32(const float* starmap, const float* starmapk, const int64_t nmesh[3],
                              const double box[3], const double prm8[8], float* sfield, float* dep,
                              void* workspace, size_t workspace_bytes, void* stream);

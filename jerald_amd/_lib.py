@@ -50,9 +50,9 @@ _SIGS = {
     "ldl_last_error": [],
     "ldl_version": [],
     "ldl_buildplan": [_I64x3, C.POINTER(C.c_int32)],
-    "ldl_paint_workspace_bytes": [_I64x3, _I32, _I64],
+    "ldl_paint_workspace_bytes": [_I64x3, _I32, _I64, _I32],
     "ldl_fft_workspace_bytes": [_I64x3, _I32],
-    "ldl_sm_workspace_bytes": [_I64x3, _I32, _I64],
+    "ldl_sm_workspace_bytes": [_I64x3, _I32, _I64, _I32],
     "ldl_sm_bins_bytes": [_I64x3, _I64],
     "ldl_sm_spectrum_pitch": [_I64x3, _I32],
     "ldl_timer_enable": [_I32],

@@ -988,7 +988,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     }
 }
 
-size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np);
+size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np, int32_t flags);
 
 template <typename T>
 int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, int nc,

@@ -37,17 +37,18 @@ static constexpr int kPaint1Block = 512;
 
 static inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Workspace: header | direct-mode mesh (nc) | counts | offsets | segment sums |
-// key | sorted
+// Workspace: header | direct-mode mesh (nc, LDL_PAINT_DIRECT only) | counts |
+// offsets | segment sums | key | sorted
 struct PaintWs {
     size_t acc, counts, offsets, segsum, key, sorted, total;
 };
-static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
+static bool paint_direct(int32_t flags) { return (flags & LDL_PAINT_DIRECT) && !(flags & LDL_PAINT_FLOAT_ATOMIC); }
+static PaintWs paint_ws(const Mesh& m, int nc, int64_t np, int32_t flags) {
     const TileGeom tg = make_tiles(m, kTileSh);
     const int64_t nk = tg.nkeys();
     PaintWs w;
     w.acc = al256((size_t)kHdrBytes);
-    w.counts = w.acc + al256((size_t)nc * m.cells() * 8);
+    w.counts = w.acc + (paint_direct(flags) ? al256((size_t)nc * m.cells() * 8) : 0);
     w.offsets = w.counts + al256((size_t)(nk + 1) * 4);
     w.segsum = w.offsets + al256((size_t)(nk + 1) * 4);
     w.key = w.segsum + al256((size_t)(nk / kScanSeg + 1) * 4);
@@ -56,7 +57,10 @@ static PaintWs paint_ws(const Mesh& m, int nc, int64_t np) {
     return w;
 }
 
-size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np) { return paint_ws(m, nc, np < 0 ? 0 : np).total; }
+size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np, int32_t flags) {
+    if (flags & LDL_PAINT_FLOAT_ATOMIC) return 0;  // float atomics deposit straight into the output
+    return paint_ws(m, nc, np < 0 ? 0 : np, flags).total;
+}
 
 size_t paint_bins_bytes(const Mesh& m, int64_t np) {
     const TileGeom tg = make_tiles(m, kTileSh);
@@ -94,7 +98,7 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         return LDL_OK;
     }
     if (np > (int64_t)0xFFFFFFFFll) return fail(LDL_EUNSUPPORTED, "paint supports np < 2^32");
-    const PaintWs w = paint_ws(m, nc, np);
+    const PaintWs w = paint_ws(m, nc, np, flags);
     if (!ws || ws_bytes < w.total) return fail(LDL_EINVAL, "paint workspace too small (see ldl_paint_workspace_bytes)");
     char* wb = reinterpret_cast<char*>(ws);
     auto* hdr = reinterpret_cast<unsigned long long*>(wb);
@@ -267,10 +271,10 @@ using namespace ldl;
 
 extern "C" {
 
-size_t ldl_paint_workspace_bytes(const int64_t nmesh[3], int32_t ncomp, int64_t np) {
+size_t ldl_paint_workspace_bytes(const int64_t nmesh[3], int32_t ncomp, int64_t np, int32_t flags) {
     Mesh m;
     if (check_mesh(nmesh, &m) || (ncomp != 1 && ncomp != 3) || np < 0) return 0;
-    return paint_ws_bytes(m, ncomp, np);
+    return paint_ws_bytes(m, ncomp, np, flags);
 }
 
 #define PAINT_ENTRY(SUF, T)                                                                                      \

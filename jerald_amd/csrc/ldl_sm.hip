@@ -135,10 +135,10 @@ struct SmWs {
     size_t paint, spec3, mesh3, tabs, part, total;
     int64_t part_doubles;
 };
-static SmWs sm_ws(const Mesh& m, int tsize, int64_t np) {
+static SmWs sm_ws(const Mesh& m, int tsize, int64_t np, int32_t pflags) {
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     SmWs w;
-    w.paint = al(paint_ws_bytes(m, 3, np));
+    w.paint = al(paint_ws_bytes(m, 3, np, pflags));
     const int64_t hq = (64 / (2 * tsize)), hcp = m.n0 * m.n1 * ((m.nh() + hq - 1) / hq * hq);  // spec_cells
     w.spec3 = al((size_t)3 * hcp * 2 * tsize);
     // three meshes; also holds one pitched spectrum (the HI layer's T2)
@@ -692,9 +692,12 @@ static StGeom st_geom(const Mesh& m) {
 // store applies 1 / (N0 N1 N2) for all three).  It runs on a real mesh
 // viewed as complex pairs (z, z+1): the weight is real and even in k_d, so
 // its kernel is real and the two lines of a pair are filtered independently.
+// With ``msum`` the first pass subtracts the field's mean msum[0] * inv (see
+// sm_loss: the mean is carried in FP64 around the FP32 transforms).
 template <typename T, int N>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
-k_gauss_line(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, const T* __restrict__ kd, T hs2) {
+k_gauss_line(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, const T* __restrict__ kd, T hs2,
+             const double* __restrict__ msum, double inv) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS;
@@ -706,7 +709,11 @@ k_gauss_line(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, co
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
     stage_twiddles<T, N, TH>(twl, tw);
-    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return io[base + n * g.es + p]; });
+    const T mu = msum ? (T)(msum[0] * inv) : T(0);
+    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) {
+        const C v = io[base + n * g.es + p];
+        return C{v.x - mu, v.y - mu};
+    });
     __syncthreads();
     lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
     for (int i = threadIdx.x; i < N * P; i += TH) {
@@ -736,7 +743,10 @@ template <typename T> struct GaussHalfRowStore {
 
 // X pass with a real isotropic weight: forward, times W(k) scale, inverse.
 // kind 1: W = knorm^-n + 1 (knorm(0) = 1e-8, model.py:376-381); kind 2:
-// W = exp(-k^2 sigma^2 / 2).
+// W = exp(-k^2 sigma^2 / 2).  The k = 0 bin is set to 0: the loss carries the
+// field's mean separately in FP64 and adds W(0) times it back (sm_loss), so
+// the rounding left in this bin by the FP32 transforms of the mean-free
+// field is not amplified by W(0) (1e8^n + 1 for kind 1).
 template <typename T, int N>
 __global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
 k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, KTabs<T> kt, int kind, T par, T scale) {
@@ -764,7 +774,7 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
             const T a = kt.k[0][n], b = kt.k[1][plane], c = kt.k[2][col0 + p];
             w = pexp(T(-0.5) * (a * a + b * b + c * c) * par * par);
         }
-        w *= scale;
+        w = (n == 0 && plane == 0 && col0 + p == 0) ? T(0) : w * scale;
         const C v = lds[n * LS + p];
         lds[n * LS + p] = C{v.x * w, v.y * w};
     }
@@ -832,7 +842,7 @@ __global__ void __launch_bounds__(256) k_sm_filter_bwd(const c2<T>* S3, const c2
         for (int j = 0; j < NP; j++) part[blockIdx.x * NP + j] = acc[j];
 }
 
-// S *= scale * W(k) (the weights of k_x_realfilter)
+// S *= scale * W(k) (the weights of k_x_realfilter, k = 0 bin set to 0)
 template <typename T>
 __global__ void __launch_bounds__(256) k_realfilter_mult(c2<T>* __restrict__ S, int64_t hcells, int64_t n1, int64_t nh, int64_t hp, KTabs<T> kt,
                                                          int kind, T par, T scale) {
@@ -846,7 +856,7 @@ __global__ void __launch_bounds__(256) k_realfilter_mult(c2<T>* __restrict__ S, 
             const T a = kt.k[0][ix], b = kt.k[1][iy], c = kt.k[2][iz];
             w = pexp(T(-0.5) * (a * a + b * b + c * c) * par * par);
         }
-        w *= scale;
+        w = i == 0 ? T(0) : w * scale;
         const c2<T> v = S[i];
         S[i] = c2<T>{v.x * w, v.y * w};
     }
@@ -1090,8 +1100,8 @@ __global__ void __launch_bounds__(256) k_mask_sums(const T* __restrict__ mask, c
     }
 }
 
-// Loss sums per block: {loss, g_mu, g_w, g_b, loss_v, g_zs}
-static constexpr int kLossCols = 6;
+// Loss sums per block: {loss, g_mu, g_w, g_b, loss_v, g_zs, sum of g_ds}
+static constexpr int kLossCols = 7;
 
 // Unsmoothed loss and its gradient in one pass (npix from device memory).
 template <typename T>
@@ -1130,17 +1140,35 @@ __global__ void __launch_bounds__(256) k_loss_direct(const T* __restrict__ rho, 
         for (int j = 0; j < kLossCols; j++) part[blockIdx.x * kLossCols + j] = a[j];
 }
 
-// diff = Fi - target (smoothed-loss path)
+// diff = Fi - target (smoothed-loss path) and per-block FP64 sums of diff
 template <typename T>
 __global__ void __launch_bounds__(256) k_loss_diff(const T* __restrict__ rho, const T* __restrict__ target, RespPar<T> rp, int64_t n,
-                                                   T* __restrict__ diff) {
+                                                   T* __restrict__ diff, double* __restrict__ dpart) {
+    double a[1] = {0.0};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const T f2 = rp.field2 ? rp.field2[i] : T(1);
         T base, y, F, Fi;
         response(rp, i, rho[i], f2, base, y, F, Fi);
-        diff[i] = Fi - target[i];
+        const T d = Fi - target[i];
+        diff[i] = d;
+        a[0] += (double)d;
     }
+    block_sum<1, 256>(a);
+    if (threadIdx.x == 0) dpart[blockIdx.x] = a[0];
 }
+
+// rows of N2 reals minus their mesh's mean sum[0] * inv (FP64 sum)
+template <typename T> struct ShiftRowLoad {
+    const T* in;
+    int64_t n2;
+    const double* sum;
+    double inv;
+    __device__ __forceinline__ c2<T> load2(int64_t row, int m) const {
+        const T mu = (T)(sum[0] * inv);
+        const c2<T> v = *reinterpret_cast<const c2<T>*>(in + row * n2 + 2 * m);
+        return c2<T>{v.x - mu, v.y - mu};
+    }
+};
 
 // The model map alone (JERALDModel.evaluate, model.py:313-344): out = F.
 template <typename T>
@@ -1152,9 +1180,11 @@ __global__ void __launch_bounds__(256) k_response(const T* __restrict__ rho, Res
     }
 }
 
-// z c2r epilogue of the smoothed residual: loss partials and g_ds.
+// z c2r epilogue of the smoothed residual: loss partials and g_ds (and the
+// FP64 sum of g_ds, the mean its smoothing carries).  dc[0] * dcw is added
+// to every value: W(0) times the mean of the field the smoothing took out.
 template <typename T> struct LossStore {
-    static constexpr int NR = 2;
+    static constexpr int NR = 3;
     const T* mask;
     const T* maskv;
     T* gds;
@@ -1163,10 +1193,13 @@ template <typename T> struct LossStore {
     T scale;
     int l1;
     double* part;
+    const double* dc;
+    double dcw;
     __device__ __forceinline__ void store2(int64_t row, int m, c2<T> v, double* acc) const {
         const double inv = 1.0 / npix[0];
         const int64_t i0 = row * n2 + 2 * m;
-        const T ds[2] = {v.x * scale, v.y * scale};
+        const T sh = (T)(dc[0] * dcw);
+        const T ds[2] = {v.x * scale + sh, v.y * scale + sh};
 #pragma unroll
         for (int e = 0; e < 2; e++) {
             const int64_t i = i0 + e;
@@ -1177,13 +1210,16 @@ template <typename T> struct LossStore {
             if (maskv) acc[1] += (double)maskv[i] * l;
             if (gds) {
                 const T sgn = ds[e] > T(0) ? T(1) : ds[e] < T(0) ? T(-1) : T(0);
-                gds[i] = (T)((double)mk * inv) * (l1 ? sgn : T(2) * ds[e]);
+                const T gv = (T)((double)mk * inv) * (l1 ? sgn : T(2) * ds[e]);
+                gds[i] = gv;
+                acc[2] += (double)gv;
             }
         }
     }
     __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const {
         part[b * kLossCols + 0] = acc[0];
         part[b * kLossCols + 4] = acc[1];
+        part[b * kLossCols + 6] = acc[2];
     }
 };
 
@@ -1196,9 +1232,12 @@ template <typename T> struct RespBwdStore {
     int64_t n2;
     T scale;
     double* part;
+    const double* dc;  // as LossStore's
+    double dcw;
     __device__ __forceinline__ void store2(int64_t row, int m, c2<T> v, double* acc) const {
         const int64_t i0 = row * n2 + 2 * m;
-        const T gv[2] = {v.x * scale, v.y * scale};
+        const T sh = (T)(dc[0] * dcw);
+        const T gv[2] = {v.x * scale + sh, v.y * scale + sh};
 #pragma unroll
         for (int e = 0; e < 2; e++) {
             const int64_t i = i0 + e;
@@ -1368,12 +1407,12 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     if (rc) return rc;
     const bool hi = kind == kKindHi;
     if ((rc = check_prm<T>(prm, hi ? 10 : 5))) return rc;
-    if (!pos || !pos_out || !rho || !Ubuf || !F) return fail(LDL_EINVAL, "NULL array argument");
+    if (!pos || !rho || !Ubuf || !F) return fail(LDL_EINVAL, "NULL array argument");
     if (hi && (!star || !S2buf)) return fail(LDL_EINVAL, "NULL array argument");
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
     const Sweep sw = make_sweep(pd, np);
-    const SmWs w = sm_ws(m, sizeof(T), np);
+    const SmWs w = sm_ws(m, sizeof(T), np, pflags);
     if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
     const SmPtrs<T> P = sm_ptrs<T>(ws, w);
     const c2<T>* tw;
@@ -1439,10 +1478,11 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
                                Tri3Store4<T>{F, m.n2}, st, nullptr, "zpass_c2r3")))
             return rc;
     }
-    // 5. readout + position update
+    // 5. readout + position update (skipped without pos_out: the backward's
+    // recomputation of a layer needs only the saved tensors)
     T s[3];
     for (int d = 0; d < 3; d++) s[d] = pdesc->kind == LDL_POS_LATTICE ? pd.cellinv[d] : T(1);
-    if (np > 0)
+    if (np > 0 && pos_out)
         {
             KTimer _t("readout3_update", st);
             with_geom4<T>(m, [&](auto geo) {
@@ -1470,7 +1510,7 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     PosDec<T> pd;
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
     const Sweep sw = make_sweep(pd, np);
-    const SmWs w = sm_ws(m, sizeof(T), np);
+    const SmWs w = sm_ws(m, sizeof(T), np, pflags);
     if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
     const SmPtrs<T> P = sm_ptrs<T>(ws, w);
     const c2<T>* tw;
@@ -1610,7 +1650,7 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
     if (p_norm != 1 && p_norm != 2) return fail(LDL_EINVAL, "p_norm must be 1 or 2");
     if (smooth_kind < 0 || smooth_kind > 2) return fail(LDL_EINVAL, "smooth_kind must be 0, 1 or 2");
     if (smooth_kind && (!box || !(box[0] > 0 && box[1] > 0 && box[2] > 0))) return fail(LDL_EINVAL, "box must be positive");
-    const SmWs w = sm_ws(m, sizeof(T), 0);  // no paint in the loss
+    const SmWs w = sm_ws(m, sizeof(T), 0, LDL_PAINT_FLOAT_ATOMIC);  // no paint in the loss: no paint region
     if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
     const SmPtrs<T> P = sm_ptrs<T>(ws, w);
     const int64_t cells = m.cells();
@@ -1642,7 +1682,7 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
         rp.dep = dep;
         rp.gdep = grho ? gdep : nullptr;
     }
-    RedMap r6{kLossCols, {0, 1, 2, 3, 4, 5}};
+    RedMap r6{kLossCols, {0, 1, 2, 3, 4, 5, 6}};
     if (smooth_kind == 0) {
         { KTimer _t("loss", st); hipLaunchKernelGGL(k_loss_direct<T>, dim3(gb), dim3(256), 0, st, rho, target, mask, maskv, rp, cells, npix, grho, bpart); }
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, (int64_t)gb, kLossCols, r6, sums, 1);
@@ -1655,7 +1695,17 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
         T* gds = P.mesh3 + cells;
         c2<T>* S = P.spec3;
         const T invM = (T)(1.0 / (double)cells);
-        auto smooth = [&](T* src) -> int {  // S = W(k) r2c(src) / M, inverse x and y passes
+        // The FP32 transforms lose the mean of a field whose rms is much larger
+        // than it (a residual near convergence): its DC bin comes out with an
+        // error relative to sum |x|, not to sum x, and dL/db is that bin.  So
+        // the smoothing works on the mean-free field and the field's FP64 sum
+        // (dsum for the residual, gsum for g_ds) is added back as W(0) mean
+        // by the c2r epilogues.
+        double* dsum = P.part + 16;
+        double* gsum = P.part + 17;
+        const double invMd = 1.0 / (double)cells;
+        const double W0 = smooth_kind == 1 ? std::pow(1e-8, -smooth_par) + 1.0 : 1.0;  // W at knorm(0) = 1e-8
+        auto smooth = [&](T* src, const double* msum) -> int {  // S = W(k) r2c(src - mean) / M, inverse x and y passes
             int rc2;
             if (smooth_kind == 2 && fft_mesh_pow2(m)) {
                 // separable Gaussian: y and x line filters in place on the real
@@ -1668,17 +1718,19 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
                     constexpr int NN = decltype(N)::value;
                     const ColGeom g = col_geom<T, NN>(m.n0, n2c, m.n1 * n2c, n2c);
                     KTimer _t("gauss_line", st);
-                    hipLaunchKernelGGL((k_gauss_line<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, sc, kt.k[1], hs2);
+                    hipLaunchKernelGGL((k_gauss_line<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, sc, kt.k[1], hs2,
+                                       msum, invMd);
                 });
                 dispatch_n(m.n0, [&](auto N) {
                     constexpr int NN = decltype(N)::value;
                     const ColGeom g = col_geom<T, NN>(m.n1, n2c, n2c, m.n1 * n2c);
                     KTimer _t("gauss_line", st);
-                    hipLaunchKernelGGL((k_gauss_line<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, sc, kt.k[0], hs2);
+                    hipLaunchKernelGGL((k_gauss_line<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, sc, kt.k[0], hs2,
+                                       (const double*)nullptr, 0.0);
                 });
                 return zpass_r2c<T>(nrows, m.n2, tw, RealRowLoad<T>{src, m.n2}, GaussHalfRowStore<T>{S, hp, kt.k[2], hs2, invM}, st);
             }
-            if ((rc2 = zpass_r2c<T>(nrows, m.n2, tw, RealRowLoad<T>{src, m.n2}, HalfRowStore<T>{S, hp, T(1)}, st))) return rc2;
+            if ((rc2 = zpass_r2c<T>(nrows, m.n2, tw, ShiftRowLoad<T>{src, m.n2, msum, invMd}, HalfRowStore<T>{S, hp, T(1)}, st))) return rc2;
             if ((rc2 = colpass<T, -1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
             if (fft_mesh_pow2(m)) {
                 dispatch_n(m.n0, [&](auto N) {
@@ -1700,14 +1752,20 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
         const int64_t nzb = zpass_c2r_blocks<T>(nrows, m.n2);
         // every block writes the loss columns; the gradient columns stay zero without g_rho
         hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nzb * kLossCols, 256)), dim3(256), 0, st, bpart, nzb * kLossCols, 0.0);
-        hipLaunchKernelGGL(k_loss_diff<T>, dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff);
-        if ((rc = smooth(diff))) return rc;
+        double* dpart = bpart + nzb * kLossCols;
+        hipLaunchKernelGGL(k_loss_diff<T>, dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff, dpart);
+        const RedMap r1{1, {0}};
+        hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, dpart, (int64_t)gb, 1, r1, dsum, 1);
+        if ((rc = smooth(diff, dsum))) return rc;
         if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
-                               LossStore<T>{mask, maskv, grho ? gds : nullptr, npix, m.n2, T(1), p_norm == 1, bpart}, st, nullptr)))
+                               LossStore<T>{mask, maskv, grho ? gds : nullptr, npix, m.n2, T(1), p_norm == 1, bpart, dsum, W0 * invMd},
+                               st, nullptr)))
             return rc;
         if (grho) {
-            if ((rc = smooth(gds))) return rc;
-            if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart}, st, nullptr)))
+            hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart + 6, nzb, kLossCols, r1, gsum, 1);
+            if ((rc = smooth(gds, gsum))) return rc;
+            if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart, gsum, W0 * invMd},
+                                   st, nullptr)))
                 return rc;
         }
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, nzb, kLossCols, r6, sums, 1);
@@ -1727,7 +1785,7 @@ static int hi_dep_setup(const T* star, const c2<T>* Sk, const int64_t nmesh[3], 
     if (!star) return fail(LDL_EINVAL, "NULL array argument");
     if (!box || !(box[0] > 0 && box[1] > 0 && box[2] > 0)) return fail(LDL_EINVAL, "box must be positive");
     if (!(prm[2] != 0.0)) return fail(LDL_EINVAL, "kli must be nonzero");
-    const SmWs w = sm_ws(*m, sizeof(T), 0);
+    const SmWs w = sm_ws(*m, sizeof(T), 0, LDL_PAINT_FLOAT_ATOMIC);  // no paint region
     if (!ws || wsb < w.total) return fail(LDL_EINVAL, "sm workspace too small (see ldl_sm_workspace_bytes)");
     *P = sm_ptrs<T>(ws, w);
     if ((rc = twiddles<T>(tw))) return rc;
@@ -1835,10 +1893,10 @@ int64_t ldl_sm_spectrum_pitch(const int64_t nmesh[3], int32_t is_f64) {
     return is_f64 ? spec_pitch<double>(m) : spec_pitch<float>(m);
 }
 
-size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np) {
+size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np, int32_t paint_flags) {
     Mesh m;
     if (check_mesh(nmesh, &m) || np < 0) return 0;
-    return sm_ws(m, is_f64 ? 8 : 4, np).total;
+    return sm_ws(m, is_f64 ? 8 : 4, np, paint_flags).total;
 }
 
 #define SM_ENTRY(SUF, T)                                                                                                    \

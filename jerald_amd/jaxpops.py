@@ -131,17 +131,17 @@ def _paint_raw(pos, mass, nm, box, out_scale=1.0):
     st = L.stream_ptr(dev)
     if torch.is_tensor(mass) and mass.dim() == 2:
         out = torch.empty((3,) + nm, dtype=dt, device=dev)
-        nb = L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 3, pos.shape[0])
-        ws = L.workspaces.get(dev, "paint", nb)
         flags = _PAINT_FLAGS
+        nb = L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 3, pos.shape[0], flags)
+        ws = L.workspaces.get(dev, "paint", nb)
         m = mass.to(dt).contiguous()
         L.call(f"ldl_ppaint3D_{L.sfx(dt)}", L.ptr(pos), pd, pos.shape[0], L.ptr(m), L.i64x3(nm), float(out_scale),
                flags, L.ptr(out), L.ptr(ws), ws.numel(), st)
         return out
     out = torch.empty(nm, dtype=dt, device=dev)
-    nb = L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 1, pos.shape[0])
-    ws = L.workspaces.get(dev, "paint1", nb)
     flags = _PAINT_FLAGS
+    nb = L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 1, pos.shape[0], flags)
+    ws = L.workspaces.get(dev, "paint1", nb)
     if torch.is_tensor(mass) and mass.dim() >= 1 and mass.numel() == pos.shape[0]:
         m = mass.to(dt).reshape(-1).contiguous()
         L.call(f"ldl_ppaint_{L.sfx(dt)}", L.ptr(pos), pd, pos.shape[0], L.ptr(m), 0.0, L.i64x3(nm), float(out_scale),

@@ -20,7 +20,10 @@ runs the hand-written adjoint kernels (``ldl_{sm,dm,hi}_layer_bwd_*``,
 Per layer the forward keeps the layer's input positions and, with
 ``save="all"``, its density, spectrum and force meshes; with
 ``save="positions"`` only the positions at layer boundaries are kept and the
-backward recomputes each layer's forward (the memory-lean mode for 1024^3).
+backward recomputes each layer's density, spectrum and forces from them (the
+memory-lean mode for 1024^3) -- except the last layer's, which the backward
+consumes first, so keeping it costs no extra peak memory
+(``recompute_last=True`` recomputes it too).
 
 Extensions over the reference, all opt-in:
   * ``lattice=(l0, l1, l2)``: X is in lattice order and positions are held as
@@ -53,7 +56,7 @@ def _as_mesh(a, nm, dtype, device):
 class JERALDModel:
     def __init__(self, X, target, pm, Nstep=2, kind="sm", n=0, index=1.0, L1=None, field2=None, masktrain=None,
                  maskvalid=None, starmap=None, *, smoothing=None, grad_rule="exact", paint_mode="fixed",
-                 lattice=None, save="all", dtype=None, reuse_sort=True):
+                 lattice=None, save="all", recompute_last=False, dtype=None, reuse_sort=True):
         # unknown kinds use the sm model (model.py:110-112)
         self.kind = kind if kind in ("dm", "sm", "HI") else "sm"
         self._k = {"sm": "sm", "dm": "dm", "HI": "hi"}[self.kind]  # libldl entry-point prefix
@@ -78,6 +81,10 @@ class JERALDModel:
         if save not in ("all", "positions"):
             raise ValueError("save must be 'all' or 'positions'")
         self.save = save
+        # save="positions": the backward recomputes every layer but the last,
+        # whose state it consumes first (the same peak memory); recompute_last
+        # recomputes that one too
+        self.recompute_last = bool(recompute_last)
         self.reuse_sort = bool(reuse_sort)  # backward paint reuses the forward's particle sort
         # loss smoothing: reference k^-n + 1 when n > 0 (model.py:376-381)
         if smoothing is None:
@@ -138,18 +145,19 @@ class JERALDModel:
     # ------------------------------------------------------- kernels
     def _ws(self):
         np_ = 0 if self.X is None else self.X.shape[0]
-        nb = L.load().ldl_sm_workspace_bytes(L.i64x3(self.nm), int(self.dtype == torch.float64), np_)
+        nb = L.load().ldl_sm_workspace_bytes(L.i64x3(self.nm), int(self.dtype == torch.float64), np_, self.paint_flags)
         return L.workspaces.get(self.device, f"sm{self.dtype}", nb)
 
-    def _layer_fwd(self, P, prm, fact, keep_bins=False):
+    def _layer_fwd(self, P, prm, fact, keep_bins=False, need_out=True):
         """One layer: (positions out, rho, U, F, bins, S2); S2 (HI second
-        source spectrum) is None for the other kinds."""
+        source spectrum) is None for the other kinds; positions out is None
+        without ``need_out`` (the backward's recomputation)."""
         dt, dev, nm = self.dtype, self.device, self.nm
         rho = torch.empty(nm, dtype=dt, device=dev)
         hp = L.load().ldl_sm_spectrum_pitch(L.i64x3(nm), int(dt == torch.float64))  # 64-byte rows
         U = torch.empty(nm[0], nm[1], hp, 2, dtype=dt, device=dev)
         F = torch.empty(nm + (4,), dtype=dt, device=dev)  # forces, interleaved (x, y, z, unused) per cell
-        out = torch.empty_like(P)
+        out = torch.empty_like(P) if need_out else None
         bins = None
         if keep_bins:
             nb = L.load().ldl_sm_bins_bytes(L.i64x3(nm), P.shape[0])
@@ -169,7 +177,7 @@ class JERALDModel:
     def _paint_density(self, P, fact):
         dt, dev, nm = self.dtype, self.device, self.nm
         rho = torch.empty(nm, dtype=dt, device=dev)
-        nb = L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 1, P.shape[0])
+        nb = L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 1, P.shape[0], self.paint_flags)
         ws = L.workspaces.get(dev, f"paint1", nb)
         L.call(f"ldl_ppaint_{L.sfx(dt)}", L.ptr(P), self._pd(), P.shape[0], None, 1.0, L.i64x3(nm), float(fact),
                self.paint_flags, L.ptr(rho), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
@@ -268,7 +276,8 @@ class JERALDModel:
             out, rho, U, F, bins, S2 = self._layer_fwd(P, self._layer_prm(p, i), fact,
                                                        keep_bins=need_grad and self.reuse_sort)
             if need_grad:
-                saved.append((P, rho, U, F, bins, S2) if self.save == "all" else (P,))
+                keep = self.save == "all" or (i == self.Nstep - 1 and not self.recompute_last)
+                saved.append((P, rho, U, F, bins, S2) if keep else (P,))
             P = out
         rho_f = self._paint_density(P, fact)
         q = p[self.npot * self.Nstep: self.npot * self.Nstep + self.nresp]
@@ -289,11 +298,11 @@ class JERALDModel:
         for i in reversed(range(self.Nstep)):
             prm = self._layer_prm(p, i)
             sv = st["saved"][i]
-            if self.save == "all":
+            if len(sv) == 6:
                 Pi, rho, U, F, bins, S2 = sv
             else:
                 Pi = sv[0]
-                _, rho, U, F, bins, S2 = self._layer_fwd(Pi, prm, fact, keep_bins=self.reuse_sort)
+                _, rho, U, F, bins, S2 = self._layer_fwd(Pi, prm, fact, keep_bins=self.reuse_sort, need_out=False)
             gp = C_ptr_offset(grads, k * i)
             head = (L.ptr(Pi), self._pd(), Pi.shape[0], L.i64x3(nm), L.dvec(prm), float(fact), self.alpha_rule,
                     self.paint_flags)

@@ -30,13 +30,19 @@ def test_library_exports_every_declared_symbol():
 def test_workspace_queries():
     lib = L.load()
     nm = L.i64x3((64, 64, 64))
-    one = lib.ldl_paint_workspace_bytes(nm, 1, 1000)
-    assert one >= 64 ** 3 * 8 + 3 * 1000 * 4
-    assert lib.ldl_paint_workspace_bytes(nm, 3, 1000) > one
-    assert lib.ldl_paint_workspace_bytes(nm, 2, 1000) == 0
-    assert lib.ldl_paint_workspace_bytes(nm, 1, -1) == 0
+    one = lib.ldl_paint_workspace_bytes(nm, 1, 1000, L.LDL_PAINT_FIXED)
+    # the tiled paint: sort keys and indices, no accumulator mesh
+    assert 2 * 1000 * 4 <= one < 64 ** 3 * 8
+    # the direct paint adds one 64-bit accumulator per cell and component
+    assert lib.ldl_paint_workspace_bytes(nm, 1, 1000, L.LDL_PAINT_DIRECT) >= one + 64 ** 3 * 8
+    assert lib.ldl_paint_workspace_bytes(nm, 3, 1000, L.LDL_PAINT_DIRECT) >= one + 3 * 64 ** 3 * 8
+    assert lib.ldl_paint_workspace_bytes(nm, 1, 1000, L.LDL_PAINT_FLOAT_ATOMIC) == 0
+    assert lib.ldl_paint_workspace_bytes(nm, 2, 1000, 0) == 0
+    assert lib.ldl_paint_workspace_bytes(nm, 1, -1, 0) == 0
     assert lib.ldl_fft_workspace_bytes(nm, 0) == 64 * 64 * 33 * 8
-    assert lib.ldl_sm_workspace_bytes(nm, 1, 1000) > lib.ldl_sm_workspace_bytes(nm, 0, 1000) > 0
+    assert lib.ldl_sm_workspace_bytes(nm, 1, 1000, 0) > lib.ldl_sm_workspace_bytes(nm, 0, 1000, 0) > 0
+    assert (lib.ldl_sm_workspace_bytes(nm, 0, 1000, L.LDL_PAINT_DIRECT)
+            >= lib.ldl_sm_workspace_bytes(nm, 0, 1000, 0) + 3 * 64 ** 3 * 8)
     # spectrum rows padded to 64 bytes: 16^3 -> 9 complex -> 16 (complex64), 12 (complex128)
     assert lib.ldl_sm_spectrum_pitch(L.i64x3((16, 16, 16)), 0) == 16
     assert lib.ldl_sm_spectrum_pitch(L.i64x3((16, 16, 16)), 1) == 12

@@ -193,7 +193,7 @@ def test_lattice_positions_match_physical():
     nm = (N, N, N)
     acc, s = NV.paint_fixed(d, 1.0, nm, lattice=(n, n, n), lattice_step=(2.0, 2.0, 2.0))
     out = torch.empty(nm, device=DEV)
-    ws = torch.zeros(L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 1, n ** 3), dtype=torch.uint8, device=DEV)
+    ws = torch.zeros(L.load().ldl_paint_workspace_bytes(L.i64x3(nm), 1, n ** 3, 0), dtype=torch.uint8, device=DEV)
     pd = L.pos_desc((box,) * 3, L.LDL_POS_LATTICE, (n, n, n), (2.0, 2.0, 2.0))
     td = torch.tensor(d, device=DEV)
     L.call("ldl_ppaint_f32", L.ptr(td), pd, td.shape[0], None, 1.0, L.i64x3(nm), 1.0, 0, L.ptr(out), L.ptr(ws),
