@@ -103,6 +103,24 @@ int ldl_ppaint_f64(const double* pos, const ldl_pos_desc* pd, int64_t np, const 
                    double mass_scalar, const int64_t nmesh[3], double out_scale, int32_t flags,
                    double* out, void* workspace, size_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference: ldl_ppaint without a workspace argument,
+ * for callers that cannot allocate device memory (HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, output
+ * and bits as ldl_ppaint_*; the workspace is a fixed 64 MiB array in the
+ * library's device memory (one per GPU, held until the process ends), which
+ * covers ldl_paint_workspace_bytes(nmesh, 1, np, flags) up to 64 MiB (with
+ * the tiled paint: 8.1e6 particles on a 512^3 mesh, 6.3e6 on 1024^3);
+ * larger requests return LDL_EINVAL.  Calls that overlap on one GPU (concurrent streams or
+ * threads) share that array and race: serialise them.  LDL_PAINT_WS_ZEROED is
+ * ignored (the array keeps the previous call's contents, so it is cleared).
+ * Allocates nothing (leave ldl_timer_enable off: its timers create events). */
+int ldl_ppaint_noalloc_f32(const float* pos, const ldl_pos_desc* pd, int64_t np, const float* mass,
+                           double mass_scalar, const int64_t nmesh[3], double out_scale, int32_t flags,
+                           float* out, void* stream);
+int ldl_ppaint_noalloc_f64(const double* pos, const ldl_pos_desc* pd, int64_t np, const double* mass,
+                           double mass_scalar, const int64_t nmesh[3], double out_scale, int32_t flags,
+                           double* out, void* stream);
+
 /* Replaces ppaint3D_f32/_f64 (lib/cpu.cc:96-116 -> lib/parops_fast.h:678-725).
  * mass3 (Np,3) row-major -> out (3,N0,N1,N2). */
 int ldl_ppaint3D_f32(const float* pos, const ldl_pos_desc* pd, int64_t np, const float* mass3,

@@ -81,6 +81,7 @@ for _t in ("f32", "f64"):
         f"ldl_hi_depletion_bwd_{_t}": [_P, _P, _P, _P, _I64x3, _Dp, _Dp, _I32, _P, _P, _SZ, _P],
         f"ldl_paint_vjp_pos_{_t}": [_P, _PD, _I64, _P, _I64x3, _D, _P, _P, _P],
         f"ldl_spectrum_zweight_{_t}": [_P, _P, _I64x3, _I32, _D, _P],
+        f"ldl_ppaint_noalloc_{_t}": [_P, _PD, _I64, _P, _D, _I64x3, _D, _I32, _P, _P],
         f"ldl_sm_response_{_t}": [_P, _I64x3, _Dp, _P, _P],
         f"ldl_hi_response_{_t}": [_P, _P, _I64x3, _Dp, _P, _P],
     })

@@ -13,6 +13,9 @@
 // rounded masses exactly and every deposit has the sign of its mass.
 #include "cic.h"
 
+#include <mutex>
+#include <vector>
+
 namespace ldl {
 
 // ---------------------------------------------------------- host drivers
@@ -265,11 +268,66 @@ template int paint_impl<double>(const double*, const ldl_pos_desc*, int64_t, con
 template int readout_impl<float>(const float*, const ldl_pos_desc*, int64_t, const float*, const int64_t*, int, int, const float*, float*, hipStream_t);
 template int readout_impl<double>(const double*, const ldl_pos_desc*, int64_t, const double*, const int64_t*, int, int, const double*, double*, hipStream_t);
 
+// ------------------------------------------------ allocation-free paint
+// ldl_ppaint_noalloc_* paint with this fixed device array as their workspace,
+// for callers that cannot allocate (graph capture, serving loops).  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name is a host-side stand-in there); every
+// call on a device shares that device's copy.
+static constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mutex;
+static std::vector<void*> g_scratch_ptr;  // per device id
+
+static int noalloc_scratch(void** out) {
+    int dev = 0;
+    LDL_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(LDL_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return LDL_OK;
+}
+
+template <typename T>
+static int paint_noalloc(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mass, double mscalar, const int64_t nmesh[3],
+                         double out_scale, int32_t flags, T* out, hipStream_t st) {
+    Mesh m;
+    int rc = check_mesh(nmesh, &m);
+    if (rc) return rc;
+    if (np < 0) return fail(LDL_EINVAL, "np must be >= 0");
+    // the array holds whatever the previous call left: never skip clearing
+    flags &= ~LDL_PAINT_WS_ZEROED;
+    const size_t need = paint_ws_bytes(m, 1, np, flags);
+    if (need > kNoAllocScratchBytes)
+        return fail(LDL_EINVAL, "ldl_ppaint_noalloc: this paint needs " + std::to_string(need) + " bytes of scratch, the fixed array holds " +
+                                    std::to_string(kNoAllocScratchBytes) + " (use ldl_ppaint with a workspace)");
+    void* ws = nullptr;
+    if ((rc = noalloc_scratch(&ws))) return rc;
+    return paint_impl<T>(pos, pdesc, np, mass, mscalar, 1, nmesh, out_scale, flags, out, ws, kNoAllocScratchBytes, st);
+}
+
 }  // namespace ldl
 
 using namespace ldl;
 
 extern "C" {
+
+int ldl_ppaint_noalloc_f32(const float* pos, const ldl_pos_desc* pd, int64_t np, const float* mass, double mass_scalar,
+                           const int64_t nmesh[3], double out_scale, int32_t flags, float* out, void* stream) {
+    return paint_noalloc<float>(pos, pd, np, mass, mass_scalar, nmesh, out_scale, flags, out, as_stream(stream));
+}
+int ldl_ppaint_noalloc_f64(const double* pos, const ldl_pos_desc* pd, int64_t np, const double* mass, double mass_scalar,
+                           const int64_t nmesh[3], double out_scale, int32_t flags, double* out, void* stream) {
+    return paint_noalloc<double>(pos, pd, np, mass, mass_scalar, nmesh, out_scale, flags, out, as_stream(stream));
+}
 
 size_t ldl_paint_workspace_bytes(const int64_t nmesh[3], int32_t ncomp, int64_t np, int32_t flags) {
     Mesh m;
