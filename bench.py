@@ -34,10 +34,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=512, help="particles per side = mesh per side")
     ap.add_argument("--nstep", type=int, default=2, help="LDL layers")
+    ap.add_argument("--save", default="positions", choices=["all", "positions"],
+                    help="positions: checkpoint positions at layer boundaries and recompute the layers in the backward")
+    ap.add_argument("--variant-steps", type=int, default=5,
+                    help="timed steps of the other memory modes (save-all, every layer recomputed; 0: skip)")
     ap.add_argument("--fp64-steps", type=int, default=3, help="timed steps of the plain FP64 path (0: skip)")
-    ap.add_argument("--cpu-n", type=int, default=256, help="per-side size of the CPU baseline sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--save", default="all", choices=["all", "positions"])
+    ap.add_argument("--cpu-n", type=int, default=512, help="per-side size of the CPU baseline's one timed iteration (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: every core this process may use)")
+    ap.add_argument("--cpu1-n", type=int, default=128, help="per-side size of the 1-thread CPU sample (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the plain device copy timed during setup (its HBM counters calibrate traffic)")
     return ap.parse_args()
@@ -64,17 +68,15 @@ def params0(nstep):
 def kernel_bytes(name, n, b):
     """Algorithmic (compulsory) HBM bytes of one launch of an instrumented
     kernel at n^3 particles on an n^3 mesh, b bytes per real.  Np = Nm = n^3,
-    Nh = n^2 (n/2+1) complex values of 2b bytes.  The tile deposit kernel
-    carries the bytes of the whole paint operation (positions and masses in,
-    mesh out); its binning and face-fixup kernels are reported separately."""
+    Nh = n^2 (n/2+1) complex values of 2b bytes.  The paint's kernels (sort
+    and deposit) have no bytes of their own: the paint is measured as one
+    operation (``op_bytes``)."""
     Np = Nm = n ** 3
     Nh = n * n * (n // 2 + 1)
     return {
         "readout3_update": 6 * b * Np + 3 * b * Nm,      # pos in + out, 3 fields
         "sm_bwd_gather": 9 * b * Np + 4 * b * Nm,        # pos, g_out in, g_in out, 4 fields
         "paint_vjp_pos": 6 * b * Np + b * Nm,            # pos in, grad out, 1 field
-        "paint_deposit1": 3 * b * Np + b * Nm,           # unit-mass paint
-        "paint_deposit3": 6 * b * Np + 3 * b * Nm,       # (Np,3) masses -> 3 meshes
         "paint_absmax": b * Nm,                          # largest cell of the density (the g maxima arrive with g)
         "zpass_r2c": b * Nm + 2 * b * Nh,
         "zpass_r2c3": 3 * (b * Nm + 2 * b * Nh),
@@ -89,6 +91,19 @@ def kernel_bytes(name, n, b):
         "x_realfilter": 4 * b * Nh,
         "gauss_line": 2 * b * Nm,                        # real mesh in and out (complex pairs)
     }.get(name)
+
+
+# Operations made of several instrumented kernels: name -> (kernels, the kernel
+# counted once per operation, algorithmic bytes at b bytes per real).
+def op_bytes(n, b):
+    Np = Nm = n ** 3
+    return {
+        # unit-mass paint: positions in, mesh out; its counting sort (bin, scan,
+        # scatter) and the tile deposit
+        "paint1": (("paint_bin", "paint_scan", "paint_scatter", "paint_deposit1"), "paint_deposit1", 3 * b * Np + b * Nm),
+        # 3-component paint of the position cotangents on the forward's sort
+        "paint3": (("paint_deposit3", "paint_absmax"), "paint_deposit3", 6 * b * Np + 3 * b * Nm),
+    }
 
 
 KERNELS = ("paint_absmax", "paint_bin", "paint_scan", "paint_scatter", "paint_deposit1", "paint_deposit3",
@@ -136,7 +151,8 @@ def copy_probe(mib, dev):
     return {"bytes": by, "ms": round(ms, 4), "GBps": round(by / ms / 1e6, 1)}
 
 
-def time_steps(model, p, steps, warmup):
+def timed(model, p, steps, warmup):
+    """(seconds for ``steps`` loss+gradient steps after ``warmup``, last result)."""
     import torch
     for _ in range(warmup):
         model.loss_and_grad(p)
@@ -180,7 +196,7 @@ def main():
     target += 0.01 * target.std() * torch.randn(target.shape, generator=gnoise, device=dev, dtype=torch.float64)
     del gen
     model = JERALDModel(X, target, pm32, lattice=(n, n, n), save=args.save, **kw)
-    log("[bench] FP32 warmup + timed steps")
+    log(f"[bench] FP32 warmup + {args.steps} timed steps (save={args.save})")
     L.timer_enable(False)
     for _ in range(args.warmup):
         model.loss_and_grad(p)
@@ -227,42 +243,97 @@ def main():
         if tr is not None:
             row["traffic_bytes"] = tr
         kernel_table[k] = row
-    # plain FP64 GPU path: FP64 absolute positions, FP64 atomics, same kernels
+    ops = {}
+    for name, (members, anchor, by) in op_bytes(n, 4).items():
+        if anchor not in kern:
+            continue
+        nop = kern[anchor][1]
+        t = sum(kern[k][0] for k in members if k in kern)
+        avg = t / nop
+        ops[name] = {"kernels": list(members), "ops_per_step": nop / args.steps, "avg_ms": round(avg, 4),
+                     "algorithmic_bytes": by, "GBps_alg": round(by / (avg / 1e3) / 1e9, 1),
+                     "frac": round(by / (avg / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    # the other memory modes on the same inputs: their loss and gradients must be
+    # bitwise those of the headline mode
+    variants = {}
+    if args.variant_steps > 0:
+        for vname, vkw in (("save_all", dict(save="all")), ("recompute_every_layer", dict(save="positions", recompute_last=True))):
+            if vkw["save"] == args.save and not vkw.get("recompute_last"):
+                continue
+            log(f"[bench] FP32 variant {vname}")
+            mv = JERALDModel(X, target, pm32, lattice=(n, n, n), **vkw, **kw)
+            tv, (lv, gv) = timed(mv, p, args.variant_steps, 1)
+            variants[vname] = {"value": round(args.variant_steps / tv, 4), "unit": "it/s",
+                               "ms_per_step": round(1e3 * tv / args.variant_steps, 3),
+                               "bitwise_equal_to_headline": bool(lv.item() == loss32 and np.array_equal(gv.cpu().numpy(), grad32))}
+            del mv
+    rho32 = model.density(torch.tensor(p)).double()
+    # plain FP64 GPU path: FP64 absolute positions, FP64 atomics, same kernels, same memory mode
     fp64 = None
     if args.fp64_steps > 0:
         log("[bench] FP64 plain path")
         del model
+        L.workspaces.clear()
         torch.cuda.empty_cache()
         pm64 = PMesh(n, box, device=dev, dtype=torch.float64)
         m64 = JERALDModel(X, target, pm64, paint_mode="float", save=args.save, **kw)
-        t64, (l64, g64) = time_steps(m64, p, args.fp64_steps, 1)
+        t64, (l64, g64) = timed(m64, p, args.fp64_steps, 1)
         l64 = l64.item()
         g64 = g64.cpu().numpy()
+        rho64 = m64.density(torch.tensor(p))
+        drho = (rho32 - rho64).abs()
         rel = np.abs(grad32 - g64) / np.maximum(np.abs(g64), 1e-300)
         fp64 = {"value": round(args.fp64_steps / t64, 4), "unit": "it/s", "ms_per_step": round(1e3 * t64 / args.fp64_steps, 2),
-                "loss": l64, "loss_rel_err_fp32": abs(loss32 - l64) / abs(l64),
-                "grad_rel_err_fp32_max": float(rel.max()), "grad_rel_err_fp32": [float(v) for v in rel]}
-        del m64
+                "save": args.save, "loss": l64, "loss_rel_err_fp32": abs(loss32 - l64) / abs(l64),
+                "grad_rel_err_fp32_max": float(rel.max()), "grad_rel_err_fp32": [float(v) for v in rel],
+                "density_rel_err_fp32_max": drho.max().item() / rho64.abs().max().item(),
+                "density_rel_err_fp32_rms": (drho.pow(2).mean().sqrt() / rho64.pow(2).mean().sqrt()).item()}
+        del m64, rho64, drho
+        L.workspaces.clear()
         torch.cuda.empty_cache()
-    # CPU baseline: FP64 oracle restatement (C CIC + pocketfft) on a bounded sample
+    del rho32
+    # CPU baseline: the FP64 oracle restatement (C CIC loops with OpenMP,
+    # pocketfft) timed for one iteration at the full size on the host's cores,
+    # on the same inputs; plus a 1-thread sample
     cpu = None
     if args.cpu_n > 0:
-        log(f"[bench] CPU baseline sample n={args.cpu_n}")
         from oracle import ldl_oracle as O
+        cores = args.cpu_threads or len(os.sched_getaffinity(0))
         nc = args.cpu_n
-        boxc = 0.8 * nc
-        Xc = zeldovich_gpu(nc, boxc, 2.0, 83, dev).cpu().numpy()
-        tgtc = 1.0 + 0.1 * np.random.default_rng(86).standard_normal((nc, nc, nc))
+        if nc == n:
+            Xc, tgtc, boxc = X.cpu().numpy(), target.cpu().numpy(), box
+        else:
+            boxc = 0.8 * nc
+            Xc = zeldovich_gpu(nc, boxc, 2.0, 83, dev).cpu().numpy()
+            tgtc = 1.0 + 0.1 * np.random.default_rng(86).standard_normal((nc, nc, nc))
+        log(f"[bench] CPU baseline: one FP64 iteration at {nc}^3 on {cores} threads")
         orc = O.SMModelOracle(nc, boxc, nstep, tgtc, L1=False, smoothing=("gaussian", boxc / nc), backend="native",
-                              threads=args.cpu_threads)
+                              threads=cores)
         t0 = time.perf_counter()
-        orc.loss_and_grad(p, Xc)
+        lc, _ = orc.loss_and_grad(p, Xc)
         tc = time.perf_counter() - t0
+        del orc, Xc, tgtc
         scale = (n / nc) ** 3
-        cpu = {"value": round(1.0 / (tc * scale), 6), "unit": "it/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": f"one FP64 fwd+bwd iteration at {nc}^3 particles/mesh took {tc:.2f} s on "
-                         f"{args.cpu_threads} threads (oracle C CIC + pocketfft); value scaled by "
-                         f"({n}/{nc})^3 = {scale:.0f} to {n}^3"}
+        cpu = {"value": round(1.0 / (tc * scale), 6), "unit": "it/s", "cores": cores, "kind": "port",
+               "sample": (f"one FP64 fwd+bwd iteration of the oracle restatement (C CIC loops with OpenMP, pocketfft) at "
+                          f"{nc}^3 particles/mesh on the same inputs, measured: {tc:.2f} s on {cores} threads"
+                          + ("" if scale == 1 else f"; value scaled by ({n}/{nc})^3 = {scale:.0f}")),
+               "loss": lc}
+        if nc == n:
+            cpu["loss_rel_err_fp32"] = abs(loss32 - lc) / abs(lc)
+        if args.cpu1_n > 0:
+            n1 = args.cpu1_n
+            b1 = 0.8 * n1
+            X1 = zeldovich_gpu(n1, b1, 2.0, 83, dev).cpu().numpy()
+            t1 = 1.0 + 0.1 * np.random.default_rng(86).standard_normal((n1, n1, n1))
+            log(f"[bench] CPU 1-thread sample at {n1}^3")
+            o1 = O.SMModelOracle(n1, b1, nstep, t1, L1=False, smoothing=("gaussian", b1 / n1), backend="native", threads=1)
+            t0 = time.perf_counter()
+            o1.loss_and_grad(p, X1)
+            ts = time.perf_counter() - t0
+            s1 = (n / n1) ** 3
+            cpu["one_thread"] = {"value": round(1.0 / (ts * s1), 7), "unit": "it/s", "cores": 1,
+                                 "sample": f"one iteration at {n1}^3 on 1 thread: {ts:.2f} s; value scaled by ({n}/{n1})^3 = {s1:.0f}"}
     line = {
         "metric": "LDL fwd+bwd iters/s @512^3 FP32 (2 layers, 1 GPU)",
         "value": round(args.steps / dt32, 4),
@@ -279,12 +350,17 @@ def main():
         "config": {"workload": f"{n}^3 particles on {n}^3 mesh, {nstep} sm LDL layers, L2 loss with 1-cell "
                                f"Gaussian smoothing, loss+grad of {5 * nstep + 3} params",
                    "n": n, "nstep": nstep, "box": box, "positions": "fp32 displacements from lattice",
-                   "paint": "fixed point (integer deposits, deterministic)", "save": args.save},
+                   "paint": "fixed point (integer deposits, deterministic)", "save": args.save,
+                   "recompute": ("layers 0..Nstep-2 recomputed from their checkpointed input positions in the backward; "
+                                 "the last layer's state is consumed by the backward first"
+                                 if args.save == "positions" else "none: every layer's state stored")},
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp64_plain_gpu": fp64,
+        "memory_modes": variants,
         "copy_probe": probe,
         "loss_fp32": loss32,
+        "ops": ops,
         "kernels": kernel_table,
     }
     print(json.dumps(line), flush=True)

@@ -244,6 +244,17 @@ class JERALDModel:
             P = self._layer_fwd(P, self._layer_prm(p, i), fact)[0]
         return self._decode(P)
 
+    def density(self, params, X=None, Nstep=None):
+        """Painted density fact * paint(X_final, 1) after Nstep layers
+        (model.py:308-311; the dm kind's map)."""
+        Nstep = self.Nstep if Nstep is None else int(Nstep)
+        p = self._params(params)
+        P = self.X if X is None else self._encode(X)
+        fact = float(np.prod(self.nm)) / P.shape[0]
+        for i in range(Nstep):
+            P = self._layer_fwd(P, self._layer_prm(p, i), fact)[0]
+        return self._paint_density(P, fact)
+
     def evaluate(self, params, X=None, Nstep=None):
         """Model map (model.py:291-347): sm ReLU(w (rho+1e-8)^mu + b); dm the
         density; HI ReLU(w ReLU(rho + zs)^mu - dep + b)."""

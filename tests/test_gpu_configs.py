@@ -167,11 +167,7 @@ def test_c1_central_finite_differences_on_the_hip_path(c1):
 # its traceback into the next test.
 def _density(m, p):
     """Final density of the model's forward (float64 copy on the device)."""
-    P = m.X
-    fact = float(np.prod(m.nm)) / P.shape[0]
-    for i in range(m.Nstep):
-        P = m._layer_fwd(P, m._layer_prm(p, i), fact, need_out=True)[0]
-    return m._paint_density(P, fact).double()
+    return m.density(torch.tensor(p)).double()
 
 
 def _run_c2():
