@@ -664,9 +664,9 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
             if (p < np) {
                 uint32_t id[3];
                 T f[3];
-                cell_of<T, P2>(pd, mp, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], p, id, f);
+                cell_of<T, P2>(pd, mp, ld_stream(pos + 3 * p), ld_stream(pos + 3 * p + 1), ld_stream(pos + 3 * p + 2), p, id, f);
                 k[q] = tile_key(id, tg, m);
-                key[p] = k[q];
+                st_stream(key + p, k[q]);
             }
         }
 #pragma unroll

@@ -356,6 +356,12 @@ __device__ __forceinline__ void pow_pair_log(double x, double a, double& p, doub
     p1 = pow(x, a - 1.0);
 }
 
+// Per-particle streams of the gathers (positions, cotangents): read or
+// written once per launch, marked non-temporal so they do not evict the mesh
+// lines the gathers reuse from L2.
+template <typename T> __device__ __forceinline__ T ld_stream(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T> __device__ __forceinline__ void st_stream(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 // --------------------------------------------------------- reductions
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -878,7 +878,7 @@ __global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ p
 #pragma unroll
         for (int k = 0; k < 2; k++)
 #pragma unroll
-            for (int d = 0; d < 3; d++) x[k][d] = pos[3 * pp[k] + d];
+            for (int d = 0; d < 3; d++) x[k][d] = ld_stream(pos + 3 * pp[k] + d);
         typename G::St st[2];
         T v[2][3][8];
 #pragma unroll
@@ -891,9 +891,9 @@ __global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ p
             const T o1 = x[k][1] + interp(v[k][1], st[k].f, -1, pd.cellinv) * s1;
             const T o2 = x[k][2] + interp(v[k][2], st[k].f, -1, pd.cellinv) * s2;
             if ((k ? p1 : p0) < np) {
-                out[3 * pp[k]] = o0;
-                out[3 * pp[k] + 1] = o1;
-                out[3 * pp[k] + 2] = o2;
+                st_stream(out + 3 * pp[k], o0);
+                st_stream(out + 3 * pp[k] + 1, o1);
+                st_stream(out + 3 * pp[k] + 2, o2);
             }
         }
     }
@@ -938,8 +938,8 @@ __global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos
         for (int k = 0; k < 2; k++)
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                x[k][d] = pos[3 * pp[k] + d];
-                g[k][d] = gout[3 * pp[k] + d];
+                x[k][d] = ld_stream(pos + 3 * pp[k] + d);
+                g[k][d] = ld_stream(gout + 3 * pp[k] + d);
             }
         typename G::St st[2];
         T v[2][3][8], vr[2][1][8];
@@ -966,7 +966,7 @@ __global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos
 #pragma unroll
                 for (int e = 0; e < 3; e++) {
                     const T o = acc[e] + fact * gr[e];
-                    gin[3 * pp[k] + e] = o;
+                    st_stream(gin + 3 * pp[k] + e, o);
                     mx[e] = fmax(mx[e], fabs(o));
                 }
             }
@@ -986,7 +986,7 @@ __global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos
 #pragma unroll
         for (int k = 0; k < 2; k++)
 #pragma unroll
-            for (int d = 0; d < 3; d++) x[k][d] = pos[3 * pp[k] + d];
+            for (int d = 0; d < 3; d++) x[k][d] = ld_stream(pos + 3 * pp[k] + d);
         typename G::St st[2];
         T v[2][1][8];
         const T* fs[1] = {grho};
@@ -1002,7 +1002,7 @@ __global__ void __launch_bounds__(256) k_paint_vjp_pos(const T* __restrict__ pos
 #pragma unroll
                 for (int e = 0; e < 3; e++) {
                     const T o = fact * gr[e];
-                    g[3 * pp[k] + e] = o;
+                    st_stream(g + 3 * pp[k] + e, o);
                     mx[e] = fmax(mx[e], fabs(o));
                 }
             }
@@ -1597,7 +1597,8 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     T* grho = P.mesh3;
     int64_t nzb = 0;
     if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
-                           GammaBwdStore<T>{rho, grho, m.n2, (T)(1.0 / (double)cells), (T)prm[1], P.part}, st, &nzb)))
+                           GammaBwdStore<T>{rho, grho, m.n2, (T)(1.0 / (double)cells), (T)prm[1], P.part}, st, &nzb,
+                           "zpass_c2r_gamma")))
         return rc;
     RedMap rz{1, {1}};
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, P.part, nzb, 1, rz, grad, 0);
@@ -1759,13 +1760,13 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
         if ((rc = smooth(diff, dsum))) return rc;
         if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
                                LossStore<T>{mask, maskv, grho ? gds : nullptr, npix, m.n2, T(1), p_norm == 1, bpart, dsum, W0 * invMd},
-                               st, nullptr)))
+                               st, nullptr, "zpass_c2r_loss")))
             return rc;
         if (grho) {
             hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart + 6, nzb, kLossCols, r1, gsum, 1);
             if ((rc = smooth(gds, gsum))) return rc;
             if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart, gsum, W0 * invMd},
-                                   st, nullptr)))
+                                   st, nullptr, "zpass_c2r_resp")))
                 return rc;
         }
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, nzb, kLossCols, r6, sums, 1);
