@@ -43,16 +43,28 @@ template <typename T> using DepositT = typename std::conditional<sizeof(T) == 4,
 template <typename T, typename D>
 __device__ __forceinline__ D split_part(T fr, D V) {  // trunc(fr V), clamped between 0 and V
     const D v = (D)(fr * (T)V);
-    return V >= 0 ? (v > V ? V : v) : (v < V ? V : v);
+    if constexpr (sizeof(D) == 4) {
+        // the median of v, 0 and V (one v_med3_i32): v has the sign of V
+        // (fr >= 0), so this is the clamp below
+        return max(min(v, D(0)), min(max(v, D(0)), V));
+    } else {
+        return V >= 0 ? (v > V ? V : v) : (v < V ? V : v);
+    }
 }
 
+// The particle's rounded fixed-point mass rint(m 2^s).  FP32: m 2^s is exact
+// in FP32 (|m 2^s| < 2^30; a result below the normal range rounds to 0 either
+// way), so the rounding equals that of the FP64 product.
 template <typename T>
-__device__ __forceinline__ void deposits(const T f[3], T m, int s, DepositT<T> q[8]) {
+__device__ __forceinline__ DepositT<T> fixed_mass(T m, int s) {
+    if constexpr (sizeof(T) == 4) return (int)rintf(ldexpf(m, s));
+    else return __double2ll_rn(ldexp(m, s));
+}
+
+// The 8 corner deposits of fixed-point mass M.
+template <typename T>
+__device__ __forceinline__ void deposits_of(const T f[3], DepositT<T> M, DepositT<T> q[8]) {
     using D = DepositT<T>;
-    const double ms = ldexp((double)m, s);
-    D M;
-    if constexpr (sizeof(T) == 4) M = __double2int_rn(ms);
-    else M = __double2ll_rn(ms);
     const D x1 = split_part<T, D>(f[0], M);
     const D xa[2] = {M - x1, x1};
 #pragma unroll
@@ -66,6 +78,11 @@ __device__ __forceinline__ void deposits(const T f[3], T m, int s, DepositT<T> q
             q[4 * a + 2 * b + 1] = z1;
         }
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void deposits(const T f[3], T m, int s, DepositT<T> q[8]) {
+    deposits_of<T>(f, fixed_mass<T>(m, s), q);
 }
 
 static __global__ void k_copy_u64(const unsigned long long* __restrict__ src, unsigned long long* __restrict__ dst, int64_t n) {
@@ -591,6 +608,9 @@ struct TileGeom {
     int sh[3];      // log2 of the tile's side along each axis (cells)
     int64_t nt[3];  // tiles per axis (the last tile of an axis may be partial)
     int64_t ntiles;
+    uint32_t n32[3];  // mesh sides (each < 2^31 on the tiled path)
+    uint32_t fm[3];   // power-of-two mesh: min(tile side, n) - 1
+    int lnt[3];       // power-of-two mesh: log2 nt
     int64_t nkeys() const { return ntiles * 8; }
     __host__ __device__ int64_t extent(int d, int64_t t, const Mesh& m) const {
         const int64_t n = d == 0 ? m.n0 : d == 1 ? m.n1 : m.n2;
@@ -609,15 +629,29 @@ inline TileGeom make_tiles(const Mesh& m, const int sh[3]) {
         g.sh[d] = sh[d];
         g.nt[d] = (n[d] + ((int64_t)1 << sh[d]) - 1) >> sh[d];
         g.ntiles *= g.nt[d];
+        const int64_t ts = (int64_t)1 << sh[d];
+        g.n32[d] = (uint32_t)n[d];
+        g.fm[d] = (uint32_t)((ts < n[d] ? ts : n[d]) - 1);
+        int l = 0;
+        while (((int64_t)1 << l) < g.nt[d]) l++;
+        g.lnt[d] = l;
     }
     return g;
 }
 
+// Position decode of the paint's sort and deposit kernels, chosen on the host
+// (both kernels use the same one): any mesh; a power-of-two mesh; a
+// power-of-two mesh with lattice positions of unit steps and power-of-two
+// strides (PosDec::lattice_unit_p2), decoded without run-time branches.
+constexpr int kDecAny = 0, kDecP2 = 1, kDecLU = 2;
+
 // Lower CIC corner (32-bit cell indices) and fractions of particle p: on a
-// power-of-two mesh (P2) with mask wraps, else with the general decode.
-template <typename T, bool P2>
+// power-of-two mesh with mask wraps, else with the general decode.
+template <typename T, int DM>
 __device__ __forceinline__ void cell_of(const PosDec<T>& pd, const MeshP2& mp, T x0, T x1, T x2, int64_t p, uint32_t id[3], T f[3]) {
-    if constexpr (P2) {
+    if constexpr (DM == kDecLU) {
+        pd.decode_lu(x0, x1, x2, p, mp.msk, id, f);
+    } else if constexpr (DM == kDecP2) {
         pd.decode_p2(x0, x1, x2, p, mp.msk, id, f);
     } else {
         int64_t idx[3];
@@ -630,25 +664,33 @@ __device__ __forceinline__ void cell_of(const PosDec<T>& pd, const MeshP2& mp, T
 // Sort key of a particle whose CIC lower corner is cell id: 8 * tile + face
 // class, where face-class bit d is set when the corner lies on the tile's
 // upper face along axis d -- exactly the particles whose stencil also reaches
-// the next tile along d.
-__device__ __forceinline__ unsigned tile_key(const uint32_t id[3], const TileGeom& tg, const Mesh& m) {
+// the next tile along d.  32-bit arithmetic: sides below 2^31, keys below
+// 2^32.  On a power-of-two mesh the tile counts are powers of two and the
+// upper face is where id + 1 is a multiple of min(tile side, n).
+template <int DM>
+__device__ __forceinline__ unsigned tile_key(const uint32_t id[3], const TileGeom& tg) {
     unsigned t[3], fc = 0;
-    const int64_t n[3] = {m.n0, m.n1, m.n2};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         t[d] = id[d] >> tg.sh[d];
-        const int64_t hi = ((int64_t)t[d] + 1) << tg.sh[d];  // one past the tile's last cell
-        if ((int64_t)id[d] + 1 == (hi < n[d] ? hi : n[d])) fc |= 1u << d;
+        if constexpr (DM == kDecAny) {
+            const uint32_t hi = (t[d] + 1u) << tg.sh[d];  // one past the tile's last cell
+            if (id[d] + 1u == (hi < tg.n32[d] ? hi : tg.n32[d])) fc |= 1u << d;
+        } else {
+            if (((id[d] + 1u) & tg.fm[d]) == 0u) fc |= 1u << d;
+        }
     }
-    return (((t[0] * (unsigned)tg.nt[1] + t[1]) * (unsigned)tg.nt[2] + t[2]) << 3) | fc;
+    if constexpr (DM == kDecAny) return (((t[0] * (unsigned)tg.nt[1] + t[1]) * (unsigned)tg.nt[2] + t[2]) << 3) | fc;
+    else return (((((t[0] << tg.lnt[1]) | t[1]) << tg.lnt[2]) | t[2]) << 3) | fc;
 }
 
 // Counting-sort pass 1: key of each particle, and the particle counts per
 // key.  Consecutive lanes hold consecutive particles, and particles stored in
 // lattice order fall into a few runs of equal keys per wavefront: each run's
-// first lane adds the run's length with one global atomic.
-template <typename T, int PPT, bool P2>
-__global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Mesh m, MeshP2 mp, TileGeom tg,
+// first lane adds the run's length with one global atomic.  All PPT positions
+// of a thread are loaded before any is decoded (np >= 1).
+template <typename T, int PPT, int DM>
+__global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, PosDec<T> pd, int64_t np, MeshP2 mp, TileGeom tg,
                                                   unsigned* __restrict__ counts, unsigned* __restrict__ key) {
     constexpr int CH = 256 * PPT;
     constexpr unsigned EMPTY = 0xFFFFFFFFu;
@@ -657,6 +699,14 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
     const int64_t nchunks = (np + CH - 1) / CH;
     for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
         unsigned k[PPT];
+        T xs[PPT][3];
+#pragma unroll
+        for (int q = 0; q < PPT; q++) {
+            int64_t p = ch * CH + q * 256 + threadIdx.x;
+            p = p < np ? p : np - 1;
+#pragma unroll
+            for (int d = 0; d < 3; d++) xs[q][d] = ld_stream(pos + 3 * p + d);
+        }
 #pragma unroll
         for (int q = 0; q < PPT; q++) {
             const int64_t p = ch * CH + q * 256 + threadIdx.x;
@@ -664,8 +714,8 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
             if (p < np) {
                 uint32_t id[3];
                 T f[3];
-                cell_of<T, P2>(pd, mp, ld_stream(pos + 3 * p), ld_stream(pos + 3 * p + 1), ld_stream(pos + 3 * p + 2), p, id, f);
-                k[q] = tile_key(id, tg, m);
+                cell_of<T, DM>(pd, mp, xs[q][0], xs[q][1], xs[q][2], p, id, f);
+                k[q] = tile_key<DM>(id, tg);
                 st_stream(key + p, k[q]);
             }
         }
@@ -825,7 +875,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned*
 // the largest CIC weight sum of any cell, and every shift drops by g with
 // 2^g > 2 W + 1, so a cell's |sum| stays below 2^29 plus 4 per contribution:
 // the int32 sums equal the int64 ones of the same deposits.
-template <typename T, int NC, int TX, int TY, int TZ, int TB, bool P2, bool ACC32 = false>
+template <typename T, int NC, int TX, int TY, int TZ, int TB, int DM, bool ACC32 = false>
 __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ mass,
                                                      T mscalar, Mesh m, MeshP2 mp, TileGeom tg, const unsigned* __restrict__ offsets,
                                                      const unsigned* __restrict__ sorted, const unsigned long long* __restrict__ hdr,
@@ -889,10 +939,12 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     }
     int s[NC];
     double fs[NC];
+    DepositT<T> Ms[NC];  // the fixed-point mass of a scalar mass
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         s[c] = fixed_shift<T>(hdr[c], np) - g;
         fs[c] = ldexp(scale, -s[c]);
+        Ms[c] = fixed_mass<T>(mscalar, s[c]);
     }
     __syncthreads();
     const unsigned total = rpre[NR];
@@ -934,45 +986,36 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
         }
 #pragma unroll
         for (int k = 0; k < KB; k++) {
-            if (jb + k * TB + threadIdx.x >= total) continue;
+            // every lane stays active to the wave-wide test below; lanes past
+            // the end (reading a valid duplicate) deposit nothing
+            const bool valid = jb + k * TB + threadIdx.x < total;
             const int64_t p = pi[k];
             uint32_t id[3];
             T f[3];
-            cell_of<T, P2>(pd, mp, xs[k][0], xs[k][1], xs[k][2], p, id, f);
+            cell_of<T, DM>(pd, mp, xs[k][0], xs[k][1], xs[k][2], p, id, f);
             // local lower corner: -1 along the axes the particle was read from a
             // lower neighbour, else its offset in this tile
             const int a = (code[k] & 1) ? -1 : (int)(id[0] - (uint32_t)lo[0]);
             const int b = (code[k] & 2) ? -1 : (int)(id[1] - (uint32_t)lo[1]);
             const int cz = (code[k] & 4) ? -1 : (int)(id[2] - (uint32_t)lo[2]);
-            const bool in_x[2] = {a >= 0, a + 1 < ex};
+            const bool in_x[2] = {valid && a >= 0, valid && a + 1 < ex};
             const bool in_y[2] = {b >= 0, b + 1 < ey};
             const bool in_z[2] = {cz >= 0, cz + 1 < ez};
             const int base = (a * TY + b) * TZ + cz;
-            // all 8 corners in the tile (face class 0 of the own tile, most
-            // particles, stored together): unconditional deposits
+            // every lane of the wave has all 8 corners in the tile (face class
+            // 0 of the own tile, most particles, stored together): unconditional
+            // deposits; else each corner is tested.  One of the two per wave.
             const bool all_in = in_x[0] && in_x[1] && in_y[0] && in_y[1] && in_z[0] && in_z[1];
-            if (all_in) {
+            const bool wave_in = __all(all_in);
 #pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    DepositT<T> q[8];
-                    deposits<T>(f, ms[k][c], s[c], q);
+            for (int c = 0; c < NC; c++) {
+                DepositT<T> q[8];
+                deposits_of<T>(f, mass ? fixed_mass<T>(ms[k][c], s[c]) : Ms[c], q);
 #pragma unroll
-                    for (int qq = 0; qq < 8; qq++) {
-                        const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
+                for (int qq = 0; qq < 8; qq++) {
+                    const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
+                    if (wave_in || (in_x[dx] && in_y[dy] && in_z[dz]))
                         atomicAdd(&box[c * B3 + base + dx * TY * TZ + dy * TZ + dz], (Acc)(SAcc)q[qq]);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    DepositT<T> q[8];
-                    deposits<T>(f, ms[k][c], s[c], q);
-#pragma unroll
-                    for (int qq = 0; qq < 8; qq++) {
-                        const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
-                        if (in_x[dx] && in_y[dy] && in_z[dz] && q[qq] != 0)
-                            atomicAdd(&box[c * B3 + base + dx * TY * TZ + dy * TZ + dz], (Acc)(SAcc)q[qq]);
-                    }
                 }
             }
         }

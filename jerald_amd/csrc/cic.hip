@@ -143,9 +143,18 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     // (bins_in) skips the sort.
     const TileGeom tg = make_tiles(m, kTileSh);
     const int64_t nk = tg.nkeys();
-    // power-of-two meshes decode with 32-bit mask wraps (bin and deposit kernels
-    // must use the same decode)
+    if (m.n0 >= (1ll << 31) || m.n1 >= (1ll << 31) || m.n2 >= (1ll << 31) || nk >= (1ll << 32))
+        return fail(LDL_EUNSUPPORTED, "the tiled paint takes mesh axes below 2^31 cells and fewer than 2^29 tiles");
+    // power-of-two meshes decode with 32-bit mask wraps, unit-step lattice
+    // positions with power-of-two strides without run-time branches (the sort
+    // and deposit kernels must use the same decode)
     const bool p2 = is_pow2(m.n0) && is_pow2(m.n1) && is_pow2(m.n2);
+    const int dm = !p2 ? kDecAny : pd.lattice_unit_p2() ? kDecLU : kDecP2;
+    auto with_dm = [dm](auto&& f) {
+        if (dm == kDecLU) f(std::integral_constant<int, kDecLU>{});
+        else if (dm == kDecP2) f(std::integral_constant<int, kDecP2>{});
+        else f(std::integral_constant<int, kDecAny>{});
+    };
     const MeshP2 mp = mesh_p2(m);
     auto* counts = reinterpret_cast<unsigned*>(wb + w.counts);
     auto* segsum = reinterpret_cast<unsigned*>(wb + w.segsum);
@@ -165,8 +174,9 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
             constexpr int PPT = 8;
             KTimer _t("paint_bin", st);
             const dim3 gb(grid_for((np + 256 * PPT - 1) / (256 * PPT), 1, 1 << 15));
-            if (p2) hipLaunchKernelGGL((k_bin_keys<T, PPT, true>), gb, dim3(256), 0, st, pos, pd, np, m, mp, tg, counts, key);
-            else hipLaunchKernelGGL((k_bin_keys<T, PPT, false>), gb, dim3(256), 0, st, pos, pd, np, m, mp, tg, counts, key);
+            with_dm([&](auto D) {
+                hipLaunchKernelGGL((k_bin_keys<T, PPT, decltype(D)::value>), gb, dim3(256), 0, st, pos, pd, np, mp, tg, counts, key);
+            });
         }
         {
             KTimer _t("paint_scan", st);
@@ -184,12 +194,10 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     const dim3 gt((unsigned)tg.ntiles);
     if (nc == 1) {
         KTimer _t("paint_deposit1", st);
-        if (p2)
-            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileX, kTileY, kTileZ, kPaint1Block, true>), gt, dim3(kPaint1Block), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
-                               tg, offsets, sorted, hdr, out_scale, out);
-        else
-            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileX, kTileY, kTileZ, kPaint1Block, false>), gt, dim3(kPaint1Block), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
-                               tg, offsets, sorted, hdr, out_scale, out);
+        with_dm([&](auto D) {
+            hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileX, kTileY, kTileZ, kPaint1Block, decltype(D)::value>), gt, dim3(kPaint1Block), 0, st, pos, pd,
+                               np, mass, (T)mscalar, m, mp, tg, offsets, sorted, hdr, out_scale, out);
+        });
     } else if (sizeof(T) == 4 && wmesh) {
         // FP32 with the positions' unit-mass density known: 32-bit accumulators
         {
@@ -198,21 +206,17 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         }
         KTimer _t("paint_deposit3", st);
         if constexpr (sizeof(T) == 4) {
-            if (p2)
-                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileX, kTileY, kTileZ, kAcc32Block, true, true>), gt, dim3(kAcc32Block), 0, st, pos, pd, np, mass, (T)mscalar, m,
-                                   mp, tg, offsets, sorted, hdr, out_scale, out);
-            else
-                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileX, kTileY, kTileZ, kAcc32Block, false, true>), gt, dim3(kAcc32Block), 0, st, pos, pd, np, mass, (T)mscalar, m,
-                                   mp, tg, offsets, sorted, hdr, out_scale, out);
+            with_dm([&](auto D) {
+                hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileX, kTileY, kTileZ, kAcc32Block, decltype(D)::value, true>), gt, dim3(kAcc32Block), 0, st, pos,
+                                   pd, np, mass, (T)mscalar, m, mp, tg, offsets, sorted, hdr, out_scale, out);
+            });
         }
     } else {
         KTimer _t("paint_deposit3", st);
-        if (p2)
-            hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileX, kTileY, kTileZ, 1024, true>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
-                               tg, offsets, sorted, hdr, out_scale, out);
-        else
-            hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileX, kTileY, kTileZ, 1024, false>), gt, dim3(1024), 0, st, pos, pd, np, mass, (T)mscalar, m, mp,
-                               tg, offsets, sorted, hdr, out_scale, out);
+        with_dm([&](auto D) {
+            hipLaunchKernelGGL((k_paint_tiles<T, 3, kTileX, kTileY, kTileZ, 1024, decltype(D)::value>), gt, dim3(1024), 0, st, pos, pd, np, mass,
+                               (T)mscalar, m, mp, tg, offsets, sorted, hdr, out_scale, out);
+        });
     }
     LDL_LAUNCH_CHECK();
     return LDL_OK;

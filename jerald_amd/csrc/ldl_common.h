@@ -157,6 +157,22 @@ template <typename T> struct PosDec {
             idx[d] = (uint32_t)(base[d] + (int32_t)fl) & msk[d];
         }
     }
+
+    // decode_p2 for lattice positions with unit steps and power-of-two strides
+    // (lattice_unit_p2() on the host): no run-time branches on the form.
+    __device__ __forceinline__ void decode_lu(T x0, T x1, T x2, int64_t p, const uint32_t msk[3], uint32_t idx[3], T f[3]) const {
+        const uint32_t pp = (uint32_t)p;
+        const uint32_t rem = pp & (l12 - 1u);
+        const int32_t base[3] = {(int32_t)(pp >> sh12), (int32_t)(rem >> sh2), (int32_t)(rem & (l2 - 1u))};
+        const T x[3] = {x0, x1, x2};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            T fl = floor(x[d]);
+            f[d] = x[d] - fl;
+            idx[d] = (uint32_t)(base[d] + (int32_t)fl) & msk[d];
+        }
+    }
+    bool lattice_unit_p2() const { return kind == LDL_POS_LATTICE && unit && sh12 >= 0 && sh2 >= 0; }
 };
 
 template <typename T>
