@@ -360,10 +360,13 @@ template <typename T> struct StencilP2 {
     T f[3];
 };
 
-template <typename T>
+// LU: lattice positions with unit steps and power-of-two strides
+// (PosDec::lattice_unit_p2), decoded without run-time branches.
+template <typename T, bool LU = false>
 __device__ __forceinline__ void locate_p2(const PosDec<T>& pd, const MeshP2& mp, T x0, T x1, T x2, int64_t p, StencilP2<T>& st) {
     uint32_t idx[3];
-    pd.decode_p2(x0, x1, x2, p, mp.msk, idx, st.f);
+    if constexpr (LU) pd.decode_lu(x0, x1, x2, p, mp.msk, idx, st.f);
+    else pd.decode_p2(x0, x1, x2, p, mp.msk, idx, st.f);
     const uint32_t i1 = (idx[0] + 1u) & mp.msk[0], j1 = (idx[1] + 1u) & mp.msk[1];
     constexpr uint32_t lb = sizeof(T) == 4 ? 2 : 3;
     st.row[0] = ((idx[0] << mp.lg1) | idx[1]) << (mp.lg2 + lb);
@@ -457,11 +460,11 @@ template <typename T> struct GeomAny {
         }
     }
 };
-template <typename T> struct GeomP2 {
+template <typename T, bool LU = false> struct GeomP2 {
     using St = StencilP2<T>;
     MeshP2 mp;
     __device__ __forceinline__ void locate(const PosDec<T>& pd, T x0, T x1, T x2, int64_t p, St& st) const {
-        locate_p2(pd, mp, x0, x1, x2, p, st);
+        locate_p2<T, LU>(pd, mp, x0, x1, x2, p, st);
     }
     template <int NF>
     __device__ __forceinline__ void gather(const T* const (&F)[NF], const St& st, T (&v)[NF][8]) const { gather_p2<T, NF>(F, st, mp, v); }
@@ -491,10 +494,11 @@ inline bool p2_ok(const Mesh& m) {
     return is_pow2(m.n0) && is_pow2(m.n1) && is_pow2(m.n2) && m.n2 >= 2 &&
            m.cells() * (int64_t)sizeof(T) <= (int64_t(1) << 32);
 }
-// Calls f(geom) with the fastest policy that fits the mesh.
+// Calls f(geom) with the fastest policy that fits the mesh and the positions.
 template <typename T, typename Fn>
-inline void with_geom(const Mesh& m, Fn&& f) {
-    if (p2_ok<T>(m)) f(GeomP2<T>{mesh_p2(m)});
+inline void with_geom(const Mesh& m, const PosDec<T>& pd, Fn&& f) {
+    if (p2_ok<T>(m) && pd.lattice_unit_p2()) f(GeomP2<T, true>{mesh_p2(m)});
+    else if (p2_ok<T>(m)) f(GeomP2<T>{mesh_p2(m)});
     else f(GeomAny<T>{m});
 }
 // Same for kernels that also gather an interleaved 4-slot field.
@@ -503,8 +507,9 @@ inline bool p2_ok4(const Mesh& m) {
     return p2_ok<T>(m) && m.cells() * 4 * (int64_t)sizeof(T) <= (int64_t(1) << 32);
 }
 template <typename T, typename Fn>
-inline void with_geom4(const Mesh& m, Fn&& f) {
-    if (p2_ok4<T>(m)) f(GeomP2<T>{mesh_p2(m)});
+inline void with_geom4(const Mesh& m, const PosDec<T>& pd, Fn&& f) {
+    if (p2_ok4<T>(m) && pd.lattice_unit_p2()) f(GeomP2<T, true>{mesh_p2(m)});
+    else if (p2_ok4<T>(m)) f(GeomP2<T>{mesh_p2(m)});
     else f(GeomAny<T>{m});
 }
 

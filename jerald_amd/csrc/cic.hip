@@ -241,22 +241,22 @@ int readout_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* f
     KTimer _t("readout", st);
     switch (kind) {
         case 0:
-            with_geom<T>(m, [&](auto geo) {
+            with_geom<T>(m, pd, [&](auto geo) {
                 hipLaunchKernelGGL((k_readout<T, decltype(geo)>), dim3(g), dim3(kBlock), 0, st, pos, pd, np, sw, field, geo, vjpdim, out);
             });
             break;
         case 1:
-            with_geom<T>(m, [&](auto geo) {
+            with_geom<T>(m, pd, [&](auto geo) {
                 hipLaunchKernelGGL((k_readout3<T, decltype(geo)>), dim3(g), dim3(kBlock), 0, st, pos, pd, np, sw, fields3(field, m.cells()), geo, out);
             });
             break;
         case 2:
-            with_geom<T>(m, [&](auto geo) {
+            with_geom<T>(m, pd, [&](auto geo) {
                 hipLaunchKernelGGL((k_readout_grad<T, decltype(geo)>), dim3(g), dim3(kBlock), 0, st, pos, pd, np, sw, field, geo, aux, out);
             });
             break;
         case 3:
-            with_geom<T>(m, [&](auto geo) {
+            with_geom<T>(m, pd, [&](auto geo) {
                 hipLaunchKernelGGL((k_readout_grad3<T, decltype(geo)>), dim3(g), dim3(kBlock), 0, st, pos, pd, np, sw, fields3(field, m.cells()), aux, geo, out);
             });
             break;

@@ -1485,7 +1485,7 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     if (np > 0 && pos_out)
         {
             KTimer _t("readout3_update", st);
-            with_geom4<T>(m, [&](auto geo) {
+            with_geom4<T>(m, pd, [&](auto geo) {
                 hipLaunchKernelGGL((k_readout3_update<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, F, geo,
                                    s[0], s[1], s[2], pos_out);
             });
@@ -1607,7 +1607,7 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
         {
             if (gmax) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, gmax, (int64_t)3, 0ull);  // the paint has read it
             KTimer _t("sm_bwd_gather", st);
-            with_geom4<T>(m, [&](auto geo) {
+            with_geom4<T>(m, pd, [&](auto geo) {
                 hipLaunchKernelGGL((k_sm_bwd_gather<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, F, grho, geo,
                                    (T)fact, gout, gin, gmax);
             });
@@ -1868,7 +1868,7 @@ static int paint_vjp_pos(const T* pos, const ldl_pos_desc* pdesc, int64_t np, co
     if ((rc = make_posdec<T>(pdesc, nmesh, np, &pd))) return rc;
     const Sweep sw = make_sweep(pd, np);
     KTimer _t("paint_vjp_pos", st);
-    with_geom<T>(m, [&](auto geo) {
+    with_geom<T>(m, pd, [&](auto geo) {
         hipLaunchKernelGGL((k_paint_vjp_pos<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, grho, geo, (T)fact, g,
                            gmax);
     });
