@@ -11,9 +11,12 @@ cells), the rest on the Zel'dovich lattice; 2 sm layers, L2 loss with 1-cell
 Gaussian smoothing, target = the model at +-10% perturbed parameters plus 1%
 noise.  Runs the FP32 model with the 64-bit fixed-point paint and with
 floating-point atomics, ``reruns`` times each, and the FP64 fixed-point run
-of the same kernels as reference.  Reports bit-exactness across reruns and
-under a particle permutation, the spread of the float-atomic reruns, and the
-max/rms relative error of the density, the loss and every parameter gradient
+of the same kernels as reference, then the FP32 model in the benchmark's
+form (FP32 displacements from the 512^3 lattice, fixed-point paint).  Reports
+bit-exactness across reruns (loss, gradients and the density of every rerun)
+and under a particle permutation, the spread of the float-atomic reruns, and
+the max/rms relative error of the density (of the input positions; for the
+lattice form, after the layers), the loss and every parameter gradient
 against FP64.
 
 large: 1024^3 particles (FP32 displacements from the lattice) on a 1024^3
@@ -79,6 +82,7 @@ def precision(n=512, nmesh=1024, reruns=10, log=print):
     m64 = JERALDModel(X, target, pm64, **kw)
     l64, g64 = m64.loss_and_grad(p)
     rho64 = pm64.paint(X.to(torch.float64), 1.0)
+    rho64_final = m64.density(torch.tensor(p))  # after the layers: what the loss sees
     del m64
     L.workspaces.clear()
     torch.cuda.empty_cache()
@@ -95,16 +99,24 @@ def precision(n=512, nmesh=1024, reruns=10, log=print):
         grads = np.stack([r[1] for r in runs])
         L.workspaces.clear()
         torch.cuda.empty_cache()
+        # every rerun's density against the first (one extra mesh held at a time)
         J.set_paint_mode(mode)
         try:
-            rhos = [pm32.paint(X.to(torch.float32), 1.0) for _ in range(min(reruns, 3))]
+            X32 = X.to(torch.float32)
+            rho0 = pm32.paint(X32, 1.0)
+            dens_same = True
+            for _ in range(reruns - 1):
+                dens_same &= bool(torch.equal(rho0, pm32.paint(X32, 1.0)))
+            del X32
         finally:
             J.set_paint_mode("fixed")
+        rhos = [rho0]
         g64n = g64.cpu().numpy()
         r = {
             "loss_bitwise_identical": bool(np.all(losses == losses[0])),
             "grad_bitwise_identical": bool(np.all(grads == grads[0])),
-            "density_bitwise_identical": all(torch.equal(rhos[0], x) for x in rhos[1:]),
+            "density_bitwise_identical": dens_same,
+            "density_reruns": reruns,
             "loss_rerun_spread": float((losses.max() - losses.min()) / abs(l64.item())),
             "grad_rerun_spread_max": float(np.max((grads.max(0) - grads.min(0)) / np.abs(g64n))),
             "loss_rel_err": float(abs(losses[0] - l64.item()) / abs(l64.item())),
@@ -120,6 +132,28 @@ def precision(n=512, nmesh=1024, reruns=10, log=print):
         torch.cuda.empty_cache()
     out["fp32_fixed_point"] = res["fixed"]
     out["fp32_float_atomic"] = res["float"]
+    # the benchmark's form: FP32 displacements from the 512^3 lattice (step 2
+    # cells), fixed-point paint
+    log(f"[precision] FP32 lattice displacements, fixed-point paint, {reruns} runs")
+    m = JERALDModel(X, target, pm32, lattice=(n, n, n), **kw)
+    runs = []
+    for _ in range(reruns):
+        l, g = m.loss_and_grad(p)
+        runs.append((l.item(), g.cpu().numpy()))
+    losses = np.array([r[0] for r in runs])
+    grads = np.stack([r[1] for r in runs])
+    g64n = g64.cpu().numpy()
+    rho_final = m.density(torch.tensor(p))
+    out["fp32_lattice_fixed_point"] = {
+        "loss_bitwise_identical": bool(np.all(losses == losses[0])),
+        "grad_bitwise_identical": bool(np.all(grads == grads[0])),
+        "loss_rel_err": float(abs(losses[0] - l64.item()) / abs(l64.item())),
+        "grad_rel_err": [float(x) for x in np.abs(grads[0] - g64n) / np.abs(g64n)],
+        "final_density_rel_err_max_rms": _rel(rho_final, rho64_final),
+    }
+    del m, rho_final
+    L.workspaces.clear()
+    torch.cuda.empty_cache()
     out["loss_fp64"] = l64.item()
     return out
 
