@@ -60,6 +60,11 @@ typedef struct ldl_pos_desc {
                                     zero (every direct paint leaves it zero), skip clearing it */
 #define LDL_PAINT_DIRECT 4       /* with LDL_PAINT_FIXED: one device-scope 64-bit atomic per corner
                                     instead of the default tile-binned LDS accumulation (same bits) */
+#define LDL_PAINT_BINS_VALID 8   /* layer forward entry points, default tiled paint: ``bins`` already
+                                    holds the particle sort of these same positions (written by an
+                                    earlier forward of them); the density paint reads it instead of
+                                    sorting (same bits).  No counterpart in the reference: a backward
+                                    pass that recomputes a layer keeps the forward's sort. */
 
 const char* ldl_last_error(void);
 int ldl_version(void);
@@ -217,7 +222,9 @@ int64_t ldl_sm_spectrum_pitch(const int64_t nmesh[3], int32_t is_f64);
  * backward needs: rho (N0,N1,N2), U (N0,N1,Hp complex, = r2c of
  * (rho+1e-8)^gamma), forces F (N0,N1,N2,4) interleaved per cell as
  * (F_x, F_y, F_z, unused), and -- if ``bins`` is not NULL
- * (ldl_sm_bins_bytes) -- the particle sort of the fixed-point paint.
+ * (ldl_sm_bins_bytes) -- the particle sort of the fixed-point paint; with
+ * LDL_PAINT_BINS_VALID in paint_flags, ``bins`` is read (that sort, kept
+ * from an earlier forward of the same positions) instead of written.
  * pos_out may be NULL: the saved tensors only, no gather (a backward pass
  * that recomputes the layer from checkpointed positions). */
 int ldl_sm_layer_fwd_f32(const float* pos, const ldl_pos_desc* pd, int64_t np,

@@ -20,6 +20,7 @@ LDL_PAINT_FIXED = 0
 LDL_PAINT_FLOAT_ATOMIC = 1
 LDL_PAINT_WS_ZEROED = 2
 LDL_PAINT_DIRECT = 4
+LDL_PAINT_BINS_VALID = 8
 
 
 class LdlError(RuntimeError):

@@ -1417,9 +1417,13 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     const SmPtrs<T> P = sm_ptrs<T>(ws, w);
     const c2<T>* tw;
     if ((rc = twiddles<T>(&tw))) return rc;
-    // 1. density
-    if ((rc = paint_impl<T>(pos, pdesc, np, nullptr, 1.0, 1, nmesh, fact, pflags, rho, P.paint, w.paint, st,
-                            (pflags & 1) ? nullptr : reinterpret_cast<unsigned*>(bins))))
+    // 1. density (with LDL_PAINT_BINS_VALID from the sort an earlier forward of
+    // these positions wrote into bins)
+    const bool tiled = !(pflags & (LDL_PAINT_FLOAT_ATOMIC | LDL_PAINT_DIRECT));
+    const bool bins_in = tiled && bins && (pflags & LDL_PAINT_BINS_VALID);
+    if ((rc = paint_impl<T>(pos, pdesc, np, nullptr, 1.0, 1, nmesh, fact, pflags & ~LDL_PAINT_BINS_VALID, rho, P.paint, w.paint, st,
+                            tiled && !bins_in ? reinterpret_cast<unsigned*>(bins) : nullptr,
+                            bins_in ? reinterpret_cast<const unsigned*>(bins) : nullptr)))
         return rc;
     const KTabs<T> kt = build_tables<T>(P.tabs, m, pdesc->box, st);
     // half spectra: nh columns in rows of pitch hp, hc apart

@@ -23,7 +23,9 @@ Per layer the forward keeps the layer's input positions and, with
 backward recomputes each layer's density, spectrum and forces from them (the
 memory-lean mode for 1024^3) -- except the last layer's, which the backward
 consumes first, so keeping it costs no extra peak memory
-(``recompute_last=True`` recomputes it too).
+(``recompute_last=True`` recomputes it too).  With ``reuse_sort`` (default)
+a recomputed layer also keeps the forward's particle sort (4 bytes per
+particle), so its recomputation skips the sort (same bits).
 
 Extensions over the reference, all opt-in:
   * ``lattice=(l0, l1, l2)``: X is in lattice order and positions are held as
@@ -148,22 +150,26 @@ class JERALDModel:
         nb = L.load().ldl_sm_workspace_bytes(L.i64x3(self.nm), int(self.dtype == torch.float64), np_, self.paint_flags)
         return L.workspaces.get(self.device, f"sm{self.dtype}", nb)
 
-    def _layer_fwd(self, P, prm, fact, keep_bins=False, need_out=True):
+    def _layer_fwd(self, P, prm, fact, keep_bins=False, need_out=True, bins=None):
         """One layer: (positions out, rho, U, F, bins, S2); S2 (HI second
         source spectrum) is None for the other kinds; positions out is None
-        without ``need_out`` (the backward's recomputation)."""
+        without ``need_out`` (the backward's recomputation).  ``bins``: the
+        sort an earlier forward of the same positions wrote, read instead of
+        sorting again."""
         dt, dev, nm = self.dtype, self.device, self.nm
         rho = torch.empty(nm, dtype=dt, device=dev)
         hp = L.load().ldl_sm_spectrum_pitch(L.i64x3(nm), int(dt == torch.float64))  # 64-byte rows
         U = torch.empty(nm[0], nm[1], hp, 2, dtype=dt, device=dev)
         F = torch.empty(nm + (4,), dtype=dt, device=dev)  # forces, interleaved (x, y, z, unused) per cell
         out = torch.empty_like(P) if need_out else None
-        bins = None
-        if keep_bins:
+        flags = self.paint_flags
+        if bins is not None:
+            flags |= L.LDL_PAINT_BINS_VALID
+        elif keep_bins:
             nb = L.load().ldl_sm_bins_bytes(L.i64x3(nm), P.shape[0])
             bins = torch.empty(nb, dtype=torch.uint8, device=dev)
         ws = self._ws()
-        head = (L.ptr(P), self._pd(), P.shape[0], L.i64x3(nm), L.dvec(prm), float(fact), self.paint_flags)
+        head = (L.ptr(P), self._pd(), P.shape[0], L.i64x3(nm), L.dvec(prm), float(fact), flags)
         tail = (L.ptr(F), L.ptr(bins), L.ptr(ws), ws.numel(), L.stream_ptr(dev))
         S2 = None
         if self.kind == "HI":
@@ -288,7 +294,7 @@ class JERALDModel:
                                                        keep_bins=need_grad and self.reuse_sort)
             if need_grad:
                 keep = self.save == "all" or (i == self.Nstep - 1 and not self.recompute_last)
-                saved.append((P, rho, U, F, bins, S2) if keep else (P,))
+                saved.append((P, rho, U, F, bins, S2) if keep else (P, bins))
             P = out
         rho_f = self._paint_density(P, fact)
         q = p[self.npot * self.Nstep: self.npot * self.Nstep + self.nresp]
@@ -312,8 +318,8 @@ class JERALDModel:
             if len(sv) == 6:
                 Pi, rho, U, F, bins, S2 = sv
             else:
-                Pi = sv[0]
-                _, rho, U, F, bins, S2 = self._layer_fwd(Pi, prm, fact, keep_bins=self.reuse_sort, need_out=False)
+                Pi, bins = sv
+                _, rho, U, F, bins, S2 = self._layer_fwd(Pi, prm, fact, keep_bins=self.reuse_sort, need_out=False, bins=bins)
             gp = C_ptr_offset(grads, k * i)
             head = (L.ptr(Pi), self._pd(), Pi.shape[0], L.i64x3(nm), L.dvec(prm), float(fact), self.alpha_rule,
                     self.paint_flags)

@@ -351,6 +351,27 @@ def test_ldl_autograd_function_and_recompute_mode():
     np.testing.assert_allclose(m.evaluate(torch.tensor(p)).cpu().numpy(), orc.evaluate(p, X), rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("dt,lattice", [(torch.float32, True), (torch.float64, False)])
+def test_recomputation_keeps_the_forward_sort_bitwise(dt, lattice):
+    """A recomputed layer reads the sort its forward wrote (LDL_PAINT_BINS_VALID):
+    loss and gradients bitwise those of a recomputation that sorts again and of
+    storing every layer."""
+    N, L, nstep = 32, 25.6, 3
+    X, tgt, mk, p, _ = _problem(N, nstep, L, L1=False)
+    pm = PMesh(N, L, device=DEV, dtype=dt)
+    res = {}
+    for name, kw in (("kept", dict(save="positions", recompute_last=True)),
+                     ("resorted", dict(save="positions", recompute_last=True, reuse_sort=False)),
+                     ("stored", dict(save="all"))):
+        m = JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, kind="sm", L1=False, masktrain=mk,
+                        lattice=(N, N, N) if lattice else None, **kw)
+        l, g = m.loss_and_grad(torch.tensor(p))
+        res[name] = (l.item(), g.cpu().numpy())
+    for name in ("resorted", "stored"):
+        assert res[name][0] == res["kept"][0]
+        assert np.array_equal(res[name][1], res["kept"][1])
+
+
 def test_lossv_and_reference_smoothing():
     """Reference smoothing k^-n + 1 with knorm(0) = 1e-8 (W(0) = 1e8 + 1,
     model.py:102, :376-381) and train/validation masks (model.py:430-466).
