@@ -89,7 +89,7 @@ def kernel_bytes(name, n, b):
         "colpass3": 4 * b * Nh,                          # one force spectrum in and out
         "sm_x_fwd": 6 * b * Nh,                          # spectrum in, U out, P's x-inverse out
         "sm_x_bwd": 6 * b * Nh,                          # spectrum + U in, 1 out
-        "grad6": 5 * b * Nm,                             # potential in, interleaved forces (4 per cell) out
+        "grad6": 4 * b * Nm,                             # potential in, interleaved forces (3 per cell) out
         "div6": 4 * b * Nm,                              # 3 gradient meshes in, divergence out
         "x_realfilter": 4 * b * Nh,
         "gauss_line": 2 * b * Nm,                        # real mesh in and out (complex pairs)

@@ -220,8 +220,8 @@ int64_t ldl_sm_spectrum_pitch(const int64_t nmesh[3], int32_t is_f64);
 /* Forward of one layer.  Inputs pos (Np,3).  Writes pos_out (Np,3) = pos +
  * s / cell (same kind as pos; may alias pos), and the saved tensors the
  * backward needs: rho (N0,N1,N2), U (N0,N1,Hp complex, = r2c of
- * (rho+1e-8)^gamma), forces F (N0,N1,N2,4) interleaved per cell as
- * (F_x, F_y, F_z, unused), and -- if ``bins`` is not NULL
+ * (rho+1e-8)^gamma), forces F (N0,N1,N2,3) interleaved per cell as
+ * (F_x, F_y, F_z), and -- if ``bins`` is not NULL
  * (ldl_sm_bins_bytes) -- the particle sort of the fixed-point paint; with
  * LDL_PAINT_BINS_VALID in paint_flags, ``bins`` is read (that sort, kept
  * from an earlier forward of the same positions) instead of written.

@@ -419,18 +419,15 @@ inline Fields3<T> fields3(const T* F3, int64_t cells) {
     return Fields3<T>{{F3, F3 + cells, F3 + 2 * cells}};
 }
 
-// One cell of an interleaved 4-slot field (x, y, z, unused): one 16-byte load
-// in FP32, a 16-byte and an 8-byte load in FP64.
-template <typename T> struct Cell4 {
+// One cell of an interleaved 3-component field (x, y, z packed, 3 T per
+// cell): one 12-byte load in FP32 (elements are 4-byte aligned), 24 bytes in
+// FP64.
+template <typename T> struct Cell3 {
     T v[3];
 };
-__device__ __forceinline__ Cell4<float> load_cell4(const float* p) {
-    const float4 a = *reinterpret_cast<const float4*>(p);
-    return Cell4<float>{{a.x, a.y, a.z}};
-}
-__device__ __forceinline__ Cell4<double> load_cell4(const double* p) {
-    const double2 a = *reinterpret_cast<const double2*>(p);
-    return Cell4<double>{{a.x, a.y, p[2]}};
+template <typename T>
+__device__ __forceinline__ Cell3<T> load_cell3(const T* p) {
+    return Cell3<T>{{p[0], p[1], p[2]}};
 }
 
 // Mesh geometry policies of the gather kernels: GeomAny works on every mesh;
@@ -446,12 +443,12 @@ template <typename T> struct GeomAny {
 #pragma unroll
         for (int f = 0; f < NF; f++) gather8(F[f], st, m.n2, v[f]);
     }
-    // the three components of an interleaved field (cell c at F4[4c .. 4c+2])
-    __device__ __forceinline__ void gather4(const T* __restrict__ F4, const St& st, T (&v)[3][8]) const {
+    // the three components of an interleaved field (cell c at F3i[3c .. 3c+2])
+    __device__ __forceinline__ void gather3i(const T* __restrict__ F3i, const St& st, T (&v)[3][8]) const {
         const int64_t k1 = st.k + 1 == m.n2 ? 0 : st.k + 1;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const Cell4<T> a = load_cell4(F4 + 4 * (st.row[r] + st.k)), b = load_cell4(F4 + 4 * (st.row[r] + k1));
+            const Cell3<T> a = load_cell3(F3i + 3 * (st.row[r] + st.k)), b = load_cell3(F3i + 3 * (st.row[r] + k1));
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 v[c][2 * r] = a.v[c];
@@ -469,16 +466,16 @@ template <typename T, bool LU = false> struct GeomP2 {
     template <int NF>
     __device__ __forceinline__ void gather(const T* const (&F)[NF], const St& st, T (&v)[NF][8]) const { gather_p2<T, NF>(F, st, mp, v); }
     // the three components of an interleaved field; byte offsets are the
-    // 1-slot row offsets times 4, so the field's bytes must fit 32 bits (p2_ok4)
-    __device__ __forceinline__ void gather4(const T* __restrict__ F4, const St& st, T (&v)[3][8]) const {
-        constexpr uint32_t cb = 4 * sizeof(T);
+    // 1-slot row offsets times 3, so the field's bytes must fit 32 bits (p2_ok3i)
+    __device__ __forceinline__ void gather3i(const T* __restrict__ F3i, const St& st, T (&v)[3][8]) const {
+        constexpr uint32_t cb = 3 * sizeof(T);
         const uint32_t k0 = st.k * cb, k1 = ((st.k + 1u) & mp.msk[2]) * cb;
-        const char* b = reinterpret_cast<const char*>(F4);
-        Cell4<T> a[4], c[4];
+        const char* b = reinterpret_cast<const char*>(F3i);
+        Cell3<T> a[4], c[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            a[r] = load_cell4(reinterpret_cast<const T*>(b + ((st.row[r] << 2) + k0)));
-            c[r] = load_cell4(reinterpret_cast<const T*>(b + ((st.row[r] << 2) + k1)));
+            a[r] = load_cell3(reinterpret_cast<const T*>(b + (st.row[r] * 3u + k0)));
+            c[r] = load_cell3(reinterpret_cast<const T*>(b + (st.row[r] * 3u + k1)));
         }
 #pragma unroll
         for (int r = 0; r < 4; r++)
@@ -501,15 +498,15 @@ inline void with_geom(const Mesh& m, const PosDec<T>& pd, Fn&& f) {
     else if (p2_ok<T>(m)) f(GeomP2<T>{mesh_p2(m)});
     else f(GeomAny<T>{m});
 }
-// Same for kernels that also gather an interleaved 4-slot field.
+// Same for kernels that also gather an interleaved 3-component field.
 template <typename T>
-inline bool p2_ok4(const Mesh& m) {
-    return p2_ok<T>(m) && m.cells() * 4 * (int64_t)sizeof(T) <= (int64_t(1) << 32);
+inline bool p2_ok3i(const Mesh& m) {
+    return p2_ok<T>(m) && m.cells() * 3 * (int64_t)sizeof(T) <= (int64_t(1) << 32);
 }
 template <typename T, typename Fn>
-inline void with_geom4(const Mesh& m, const PosDec<T>& pd, Fn&& f) {
-    if (p2_ok4<T>(m) && pd.lattice_unit_p2()) f(GeomP2<T, true>{mesh_p2(m)});
-    else if (p2_ok4<T>(m)) f(GeomP2<T>{mesh_p2(m)});
+inline void with_geom3i(const Mesh& m, const PosDec<T>& pd, Fn&& f) {
+    if (p2_ok3i<T>(m) && pd.lattice_unit_p2()) f(GeomP2<T, true>{mesh_p2(m)});
+    else if (p2_ok3i<T>(m)) f(GeomP2<T>{mesh_p2(m)});
     else f(GeomAny<T>{m});
 }
 

@@ -256,10 +256,11 @@ template <typename T> struct Tri3RowLoad {
 };
 
 
-// Rows taken component-fastest (Tri3RowLoad) into the interleaved field F4
-// (x, y, z, 0): the power-of-two z pass hands over whole threes of rows
-// (store3, two cells per call); the any-length pass single rows (store2).
-template <typename T> struct Tri3Store4 {
+// Rows taken component-fastest (Tri3RowLoad) into the interleaved field F3i
+// (x, y, z packed per cell): the power-of-two z pass hands over whole threes
+// of rows (store3, two cells per call); the any-length pass single rows
+// (store2).
+template <typename T> struct Tri3Store3 {
     static constexpr int NR = 0;
     static constexpr int TRIPLE = 1;
     T* out;
@@ -267,21 +268,18 @@ template <typename T> struct Tri3Store4 {
     __device__ __forceinline__ void reduce_out(int64_t, const double*) const {}
     __device__ __forceinline__ void store2(int64_t R, int m, c2<T> v, double*) const {
         const int64_t r = R / 3;
-        T* o = out + 4 * (r * n2 + 2 * m) + (R - 3 * r);
+        T* o = out + 3 * (r * n2 + 2 * m) + (R - 3 * r);
         o[0] = v.x;
-        o[4] = v.y;
+        o[3] = v.y;
     }
     __device__ __forceinline__ void store3(int64_t r, int m, c2<T> a, c2<T> b, c2<T> c) const {
-        T* o = out + 4 * (r * n2 + 2 * m);
-        if constexpr (sizeof(T) == 4) {
-            reinterpret_cast<float4*>(o)[0] = float4{a.x, b.x, c.x, 0.0f};
-            reinterpret_cast<float4*>(o)[1] = float4{a.y, b.y, c.y, 0.0f};
-        } else {
-            reinterpret_cast<double2*>(o)[0] = double2{a.x, b.x};
-            reinterpret_cast<double2*>(o)[1] = double2{c.x, 0.0};
-            reinterpret_cast<double2*>(o)[2] = double2{a.y, b.y};
-            reinterpret_cast<double2*>(o)[3] = double2{c.y, 0.0};
-        }
+        T* o = out + 3 * (r * n2 + 2 * m);  // two adjacent cells: 6 consecutive values
+        o[0] = a.x;
+        o[1] = b.x;
+        o[2] = c.x;
+        o[3] = a.y;
+        o[4] = b.y;
+        o[5] = c.y;
     }
 };
 
@@ -545,7 +543,7 @@ struct StTile {
     }
 };
 
-// Forces F (N0, N1, N2, 4), component 3 zero, from the potential phi.
+// Forces F (N0, N1, N2, 3), interleaved per cell, from the potential phi.
 template <typename T>
 __global__ void __launch_bounds__(256) k_grad6(const T* __restrict__ phi, T* __restrict__ F, StGeom g, T cx, T cy, T cz) {
     __shared__ T pl[kStTY + 6][kStTZ + 6];
@@ -586,13 +584,10 @@ __global__ void __launch_bounds__(256) k_grad6(const T* __restrict__ phi, T* __r
                 const T dx = st_diff(q[c][0], q[c][1], q[c][2], q[c][4], q[c][5], q[c][6]);
                 const T dy = st_diff(pl[ly - 3][lz], pl[ly - 2][lz], pl[ly - 1][lz], pl[ly + 1][lz], pl[ly + 2][lz], pl[ly + 3][lz]);
                 const T dz = st_diff(pl[ly][lz - 3], pl[ly][lz - 2], pl[ly][lz - 1], pl[ly][lz + 1], pl[ly][lz + 2], pl[ly][lz + 3]);
-                T* o = F + 4 * ((int64_t)x * ps + (int64_t)(tl.y0 + 4 * tl.yg + c) * g.n2 + (tl.z0 + tl.tz));
-                if constexpr (sizeof(T) == 4) {
-                    *reinterpret_cast<float4*>(o) = float4{-cx * dx, -cy * dy, -cz * dz, 0.0f};
-                } else {
-                    reinterpret_cast<double2*>(o)[0] = double2{-cx * dx, -cy * dy};
-                    reinterpret_cast<double2*>(o)[1] = double2{-cz * dz, 0.0};
-                }
+                T* o = F + 3 * ((int64_t)x * ps + (int64_t)(tl.y0 + 4 * tl.yg + c) * g.n2 + (tl.z0 + tl.tz));
+                o[0] = -cx * dx;
+                o[1] = -cy * dy;
+                o[2] = -cz * dz;
             }
         }
         if (more) {
@@ -866,11 +861,11 @@ __global__ void __launch_bounds__(256) k_realfilter_mult(c2<T>* __restrict__ S, 
 // Pair sweep: each thread gathers for two particles, both particles' position
 // loads and then both particles' field loads issued before any is consumed.
 // Loads of a missing second particle read particle np - 1; only its stores
-// are skipped.  The forces F4 are interleaved (x, y, z, unused) per cell.
+// are skipped.  The forces F3i are interleaved (x, y, z) per cell.
 
 // pos_out = pos + readout3(F) * step (step = 1/cell in mesh units, 1 in box units)
 template <typename T, typename G>
-__global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ F4,
+__global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ F3i,
                                                          G geo, T s0, T s1, T s2, T* __restrict__ out) {
     LDL_SWEEP2(sw, np, p0, p1) {
         const int64_t pp[2] = {p0 < np ? p0 : np - 1, p1 < np ? p1 : np - 1};
@@ -884,7 +879,7 @@ __global__ void __launch_bounds__(256) k_readout3_update(const T* __restrict__ p
 #pragma unroll
         for (int k = 0; k < 2; k++) geo.locate(pd, x[k][0], x[k][1], x[k][2], pp[k], st[k]);
 #pragma unroll
-        for (int k = 0; k < 2; k++) geo.gather4(F4, st[k], v[k]);
+        for (int k = 0; k < 2; k++) geo.gather3i(F3i, st[k], v[k]);
 #pragma unroll
         for (int k = 0; k < 2; k++) {
             const T o0 = x[k][0] + interp(v[k][0], st[k].f, -1, pd.cellinv) * s0;
@@ -927,7 +922,7 @@ __device__ __forceinline__ void block_absmax3(const T (&mx)[3], unsigned long lo
 // g_in = g + sum_d g_d grad F_d + fact grad g_rho  (box units); with gmax,
 // also the per-component max |g_in|.
 template <typename T, typename G>
-__global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ F4,
+__global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos, PosDec<T> pd, int64_t np, Sweep sw, const T* __restrict__ F3i,
                                                        const T* __restrict__ grho, G geo, T fact, const T* __restrict__ gout,
                                                        T* __restrict__ gin, unsigned long long* __restrict__ gmax) {
     T mx[3] = {T(0), T(0), T(0)};
@@ -948,7 +943,7 @@ __global__ void __launch_bounds__(256) k_sm_bwd_gather(const T* __restrict__ pos
         for (int k = 0; k < 2; k++) geo.locate(pd, x[k][0], x[k][1], x[k][2], pp[k], st[k]);
 #pragma unroll
         for (int k = 0; k < 2; k++) {
-            geo.gather4(F4, st[k], v[k]);
+            geo.gather3i(F3i, st[k], v[k]);
             geo.template gather<1>(fr, st[k], vr[k]);
         }
 #pragma unroll
@@ -1479,7 +1474,7 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     } else {
         if ((rc = colpass<T, +1>(m.n1, 3 * m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st, "colpass3"))) return rc;
         if ((rc = zpass_c2r<T>(3 * nrows, m.n2, tw, Tri3RowLoad<T>{P.spec3, P.spec3 + hc, P.spec3 + 2 * hc, hp},
-                               Tri3Store4<T>{F, m.n2}, st, nullptr, "zpass_c2r3")))
+                               Tri3Store3<T>{F, m.n2}, st, nullptr, "zpass_c2r3")))
             return rc;
     }
     // 5. readout + position update (skipped without pos_out: the backward's
@@ -1489,7 +1484,7 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     if (np > 0 && pos_out)
         {
             KTimer _t("readout3_update", st);
-            with_geom4<T>(m, pd, [&](auto geo) {
+            with_geom3i<T>(m, pd, [&](auto geo) {
                 hipLaunchKernelGGL((k_readout3_update<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, F, geo,
                                    s[0], s[1], s[2], pos_out);
             });
@@ -1611,7 +1606,7 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
         {
             if (gmax) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, gmax, (int64_t)3, 0ull);  // the paint has read it
             KTimer _t("sm_bwd_gather", st);
-            with_geom4<T>(m, pd, [&](auto geo) {
+            with_geom3i<T>(m, pd, [&](auto geo) {
                 hipLaunchKernelGGL((k_sm_bwd_gather<T, decltype(geo)>), dim3(sweep2_grid(sw)), dim3(kBlk), 0, st, pos, pd, np, sw, F, grho, geo,
                                    (T)fact, gout, gin, gmax);
             });

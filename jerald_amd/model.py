@@ -160,7 +160,7 @@ class JERALDModel:
         rho = torch.empty(nm, dtype=dt, device=dev)
         hp = L.load().ldl_sm_spectrum_pitch(L.i64x3(nm), int(dt == torch.float64))  # 64-byte rows
         U = torch.empty(nm[0], nm[1], hp, 2, dtype=dt, device=dev)
-        F = torch.empty(nm + (4,), dtype=dt, device=dev)  # forces, interleaved (x, y, z, unused) per cell
+        F = torch.empty(nm + (3,), dtype=dt, device=dev)  # forces, interleaved (x, y, z) per cell
         out = torch.empty_like(P) if need_out else None
         flags = self.paint_flags
         if bins is not None:
