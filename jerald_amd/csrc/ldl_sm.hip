@@ -192,21 +192,34 @@ struct RedMap {
     int n;
     int map[12];
 };
+// One pass over the rows: thread t sums column j of rows t, t + 256, ... in
+// that order, then the waves and the four wave sums in a fixed order.
 __global__ void __launch_bounds__(256) k_reduce_cols(const double* __restrict__ part, int64_t nblocks, int stride, RedMap rm,
                                                      double* __restrict__ out, int assign) {
-    __shared__ double red[4];
-    for (int j = 0; j < rm.n; j++) {
-        double s = 0.0;
-        for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) s += part[b * stride + j];
-        s = wave_sum(s);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const double t = (red[0] + red[1]) + (red[2] + red[3]);
-            if (assign) out[rm.map[j]] = t;
-            else out[rm.map[j]] += t;
+    constexpr int MX = 12;
+    __shared__ double red[MX][4];
+    double s[MX];
+#pragma unroll
+    for (int j = 0; j < MX; j++) s[j] = 0.0;
+    for (int64_t b = threadIdx.x; b < nblocks; b += 256) {
+        const double* row = part + b * stride;
+#pragma unroll
+        for (int j = 0; j < MX; j++)
+            if (j < rm.n) s[j] += row[j];
+    }
+#pragma unroll
+    for (int j = 0; j < MX; j++) {
+        if (j < rm.n) {
+            const double v = wave_sum(s[j]);
+            if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = v;
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < rm.n) {
+        const int j = threadIdx.x;
+        const double t = (red[j][0] + red[j][1]) + (red[j][2] + red[j][3]);
+        if (assign) out[rm.map[j]] = t;
+        else out[rm.map[j]] += t;
     }
 }
 
