@@ -206,9 +206,10 @@ int ldl_spectrum_zweight_f64(const double* in, double* out, const int64_t nmesh[
  * Positions are LDL_POS_LATTICE displacements (mesh units) or physical. */
 size_t ldl_sm_workspace_bytes(const int64_t nmesh[3], int32_t is_f64, int64_t np, int32_t paint_flags);
 
-/* Size of a layer's particle sort (tile offsets + sorted indices, uint32): the
- * forward's density paint can record it and the backward's 3-component paint
- * of the same positions reuse it instead of sorting again. */
+/* Size of a layer's particle sort (tile offsets + sorted indices + the FP32
+ * density's largest cell, uint32): the forward's density paint can record it
+ * and the backward's 3-component paint of the same positions reuse it instead
+ * of sorting again (and of scanning the density for its largest cell). */
 size_t ldl_sm_bins_bytes(const int64_t nmesh[3], int64_t np);
 
 /* Row pitch Hp, in complex elements, of the half spectra the layer entry

@@ -877,11 +877,14 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned*
 // the largest CIC weight sum of any cell, and every shift drops by g with
 // 2^g > 2 W + 1, so a cell's |sum| stays below 2^29 plus 4 per contribution:
 // the int32 sums equal the int64 ones of the same deposits.
+//
+// vmax (FP32, one component, may be NULL): max |cell| of the painted mesh, as
+// FP32 bits, by one atomicMax per workgroup into a zeroed slot.
 template <typename T, int NC, int TX, int TY, int TZ, int TB, int DM, bool ACC32 = false>
 __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, PosDec<T> pd, int64_t np, const T* __restrict__ mass,
                                                      T mscalar, Mesh m, MeshP2 mp, TileGeom tg, const unsigned* __restrict__ offsets,
                                                      const unsigned* __restrict__ sorted, const unsigned long long* __restrict__ hdr,
-                                                     double scale, T* __restrict__ out) {
+                                                     double scale, T* __restrict__ out, unsigned* __restrict__ vmax = nullptr) {
     static_assert(!ACC32 || sizeof(T) == 4, "32-bit accumulation takes the 32-bit FP32 deposits");
     using Acc = typename std::conditional<ACC32, unsigned int, unsigned long long>::type;
     using SAcc = typename std::conditional<ACC32, int, long long>::type;
@@ -1024,13 +1027,40 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     }
     __syncthreads();
     const int64_t cells = m.cells();
+    T mx = T(0);
     for (int li = threadIdx.x; li < B3; li += TB) {
         const int cc = li % TZ, bb = (li / TZ) % TY, aa = li / (TZ * TY);
         if (aa >= ex || bb >= ey || cc >= ez) continue;
         const int64_t gi = ((lo[0] + aa) * m.n1 + (lo[1] + bb)) * m.n2 + (lo[2] + cc);
 #pragma unroll
-        for (int c = 0; c < NC; c++) out[c * cells + gi] = (T)((double)(SAcc)box[c * B3 + li] * fs[c]);
+        for (int c = 0; c < NC; c++) {
+            const T v = (T)((double)(SAcc)box[c * B3 + li] * fs[c]);
+            out[c * cells + gi] = v;
+            if constexpr (NC == 1 && sizeof(T) == 4) mx = fmaxf(mx, fabsf(v));
+        }
     }
+    if constexpr (NC == 1 && sizeof(T) == 4) {
+        if (vmax) {  // uniform: the workgroup's largest cell, one atomic
+            __shared__ float wmx[TB / 64];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+            if ((threadIdx.x & 63) == 0) wmx[threadIdx.x >> 6] = mx;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                float b = wmx[0];
+#pragma unroll
+                for (int i = 1; i < TB / 64; i++) b = fmaxf(b, wmx[i]);
+                atomicMax(vmax, __float_as_uint(b));  // non-negative floats order like their bits
+            }
+        }
+    }
+}
+
+// slot (FP32 bits of max |w| from the 1-component paint that wrote w) -> the
+// double bits k_mesh_absmax would give for w and scale: (double)max|w| * scale.
+static __global__ void k_vmax_scaled(const unsigned* __restrict__ slot, double scale, unsigned long long* __restrict__ out) {
+    const double v = (double)__uint_as_float(*slot) * scale;
+    if (v > 0.0) *out = (unsigned long long)__double_as_longlong(v);
 }
 
 size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np, int32_t flags);
@@ -1042,7 +1072,8 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
                double wscale = 0.0, const unsigned long long* absmax_in = nullptr);
 // Particle sort by tile and face class that a 1-component paint can
 // hand to a later 3-component paint of the same positions: offsets
-// (8 ntiles + 1) then sorted particle indices (np), uint32.
+// (8 ntiles + 1), sorted particle indices (np), then (FP32) the FP32 bits of
+// the painted mesh's largest |cell|, uint32.
 size_t paint_bins_bytes(const Mesh& m, int64_t np);
 template <typename T>
 int readout_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* field, const int64_t nmesh[3],

@@ -67,7 +67,7 @@ size_t paint_ws_bytes(const Mesh& m, int nc, int64_t np, int32_t flags) {
 
 size_t paint_bins_bytes(const Mesh& m, int64_t np) {
     const TileGeom tg = make_tiles(m, kTileSh);
-    return (size_t)(tg.nkeys() + 1 + (np < 0 ? 0 : np)) * 4;
+    return (size_t)(tg.nkeys() + 1 + (np < 0 ? 0 : np) + 1) * 4;
 }
 
 template <typename T>
@@ -161,9 +161,17 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
     auto* key = reinterpret_cast<unsigned*>(wb + w.key);
     unsigned* offsets = reinterpret_cast<unsigned*>(wb + w.offsets);
     unsigned* sorted = reinterpret_cast<unsigned*>(wb + w.sorted);
+    // the FP32 1-component paint that hands its sort over also records its
+    // largest cell there; a 3-component paint handed that sort reads it in place
+    // of scanning the density
+    unsigned* vmax_out = nullptr;
     if (bins_out) {
         offsets = bins_out;
         sorted = bins_out + nk + 1;
+        if (nc == 1 && sizeof(T) == 4) {
+            vmax_out = bins_out + nk + 1 + np;
+            hipLaunchKernelGGL(k_fill<unsigned>, dim3(1), dim3(64), 0, st, vmax_out, (int64_t)1, 0u);
+        }
     }
     if (bins_in) {
         offsets = const_cast<unsigned*>(bins_in);
@@ -196,11 +204,13 @@ int paint_impl(const T* pos, const ldl_pos_desc* pdesc, int64_t np, const T* mas
         KTimer _t("paint_deposit1", st);
         with_dm([&](auto D) {
             hipLaunchKernelGGL((k_paint_tiles<T, 1, kTileX, kTileY, kTileZ, kPaint1Block, decltype(D)::value>), gt, dim3(kPaint1Block), 0, st, pos, pd,
-                               np, mass, (T)mscalar, m, mp, tg, offsets, sorted, hdr, out_scale, out);
+                               np, mass, (T)mscalar, m, mp, tg, offsets, sorted, hdr, out_scale, out, vmax_out);
         });
     } else if (sizeof(T) == 4 && wmesh) {
         // FP32 with the positions' unit-mass density known: 32-bit accumulators
-        {
+        if (bins_in) {  // recorded by the 1-component paint that wrote the sort and wmesh
+            hipLaunchKernelGGL(k_vmax_scaled, dim3(1), dim3(1), 0, st, bins_in + nk + 1 + np, wscale, hdr + 3);
+        } else {
             KTimer _t("paint_absmax", st);
             hipLaunchKernelGGL(k_mesh_absmax<T>, dim3(grid_for(cells, kBlock, 2048)), dim3(kBlock), 0, st, wmesh, cells, wscale, hdr + 3);
         }
