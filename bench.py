@@ -85,6 +85,7 @@ def kernel_bytes(name, n, b):
         "zpass_c2r_gamma": 2 * b * Nh + 2 * b * Nm,     # spectrum in, density in, its cotangent out
         "zpass_c2r_loss": 2 * b * Nh + b * Nm,          # smoothed residual in, g_ds out (no masks)
         "zpass_c2r_resp": 2 * b * Nh + 2 * b * Nm,      # smoothed g_ds in, density in, g_rho out
+        "zpass_c2r_loss2": 2 * b * Nh + 3 * b * Nm,     # S^2-smoothed residual in, target and density in, g_rho out
         "colpass": 4 * b * Nh,
         "colpass3": 4 * b * Nh,                          # one force spectrum in and out
         "sm_x_fwd": 6 * b * Nh,                          # spectrum in, U out, P's x-inverse out
@@ -111,7 +112,7 @@ def op_bytes(n, b):
 
 KERNELS = ("paint_absmax", "paint_bin", "paint_scan", "paint_scatter", "paint_deposit1", "paint_deposit3",
            "paint_deposit", "paint_convert", "zpass_r2c", "zpass_r2c3", "zpass_c2r", "zpass_c2r3", "zpass_c2r_gamma", "zpass_c2r_loss",
-           "zpass_c2r_resp", "colpass",
+           "zpass_c2r_resp", "zpass_c2r_loss2", "colpass",
            "colpass3", "grad6", "div6", "sm_x_fwd", "sm_x_bwd", "x_realfilter", "gauss_line", "readout3_update", "sm_bwd_gather", "paint_vjp_pos",
            "loss")
 

@@ -1272,6 +1272,58 @@ template <typename T> struct RespBwdStore {
     }
 };
 
+// z c2r epilogue of q = S^2 r for the unmasked L2 loss with a real, even
+// smoothing weight W (S = F^-1 W F is symmetric): sum_x (S r)^2 = sum_x r q and
+// the residual's gradient is 2 q / npix, so one smoothing with W^2 gives the
+// loss partial, and the response backward follows in the same pass.  dc[0] *
+// dcw adds W(0)^2 times the residual's mean back (as LossStore's).  The
+// residual r = Fi - target is formed again here (the smoothing's line passes
+// overwrite their input), with k_loss_diff's arithmetic.
+template <typename T> struct LossSq2Store {
+    static constexpr int NR = 5;
+    const T* target;
+    const T* rho;
+    T* grho;
+    RespPar<T> rp;
+    const double* npix;
+    int64_t n2;
+    T scale;
+    double* part;
+    const double* dc;
+    double dcw;
+    __device__ __forceinline__ void store2(int64_t row, int m, c2<T> v, double* acc) const {
+        const int64_t i0 = row * n2 + 2 * m;
+        const T sh = (T)(dc[0] * dcw);
+        const T q[2] = {v.x * scale + sh, v.y * scale + sh};
+        const T ginv = (T)(2.0 / npix[0]);
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int64_t i = i0 + e;
+            const T r = rho[i];
+            const T f2 = rp.field2 ? rp.field2[i] : T(1);
+            T base, base1, lr, y, F, Fi;
+            response_adj(rp, i, r, f2, base, base1, lr, y, F, Fi);
+            acc[0] += (double)(Fi - target[i]) * (double)q[e];
+            const T gy = response_bwd(rp, q[e] * ginv, f2, y, F);
+            if (!rp.ident) {
+                acc[1] += (double)(gy * rp.w * base) * (double)lr;
+                acc[2] += (double)(gy * base);
+                acc[3] += (double)gy;
+            }
+            const T gr = response_grho(rp, i, gy, base1, r);
+            acc[4] += (double)gr;
+            grho[i] = gr;
+        }
+    }
+    __device__ __forceinline__ void reduce_out(int64_t b, const double* acc) const {
+        part[b * kLossCols + 0] = acc[0];
+        part[b * kLossCols + 1] = acc[1];
+        part[b * kLossCols + 2] = acc[2];
+        part[b * kLossCols + 3] = acc[3];
+        part[b * kLossCols + 5] = acc[4];
+    }
+};
+
 // out = {loss / npix, g_mu, g_w, g_b, loss_v / npix_v[, g_zs]}
 __global__ void k_loss_finish(const double* __restrict__ sums, const double* __restrict__ npix, double* __restrict__ out, int nout) {
     if (threadIdx.x == 0) {
@@ -1718,7 +1770,7 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
         double* gsum = P.part + 17;
         const double invMd = 1.0 / (double)cells;
         const double W0 = smooth_kind == 1 ? std::pow(1e-8, -smooth_par) + 1.0 : 1.0;  // W at knorm(0) = 1e-8
-        auto smooth = [&](T* src, const double* msum) -> int {  // S = W(k) r2c(src - mean) / M, inverse x and y passes
+        auto smooth = [&](T* src, const double* msum, double par) -> int {  // S = W(k) r2c(src - mean) / M, inverse x and y passes
             int rc2;
             if (smooth_kind == 2 && fft_mesh_pow2(m)) {
                 // separable Gaussian: y and x line filters in place on the real
@@ -1726,7 +1778,7 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
                 // the r2c pass stores the half spectrum
                 c2<T>* sc = reinterpret_cast<c2<T>*>(src);
                 const int64_t n2c = m.n2 / 2;
-                const T hs2 = (T)(0.5 * smooth_par * smooth_par);
+                const T hs2 = (T)(0.5 * par * par);
                 dispatch_n(m.n1, [&](auto N) {
                     constexpr int NN = decltype(N)::value;
                     const ColGeom g = col_geom<T, NN>(m.n0, n2c, m.n1 * n2c, n2c);
@@ -1751,13 +1803,13 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
                     ColGeom g = col_geom<T, NN>(m.n1, nh, hp, m.n1 * hp);
                     KTimer _t("x_realfilter", st);
                     hipLaunchKernelGGL((k_x_realfilter<T, NN>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, S, kt,
-                                       (int)smooth_kind, (T)smooth_par, invM);
+                                       (int)smooth_kind, (T)par, invM);
                 });
             } else {
                 const int64_t hc = spec_cells<T>(m);
                 if ((rc2 = colpass<T, -1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
                 hipLaunchKernelGGL(k_realfilter_mult<T>, dim3(grid_for(hc, 256, 1 << 16)), dim3(256), 0, st, S, hc, m.n1, nh, hp, kt, (int)smooth_kind,
-                                   (T)smooth_par, invM);
+                                   (T)par, invM);
                 if ((rc2 = colpass<T, +1>(m.n0, m.n1, nh, hp, m.n1 * hp, tw, PlainColOp<T>{S, S, T(1)}, st))) return rc2;
             }
             return colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{S, S, T(1)}, st);
@@ -1769,17 +1821,28 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
         hipLaunchKernelGGL(k_loss_diff<T>, dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff, dpart);
         const RedMap r1{1, {0}};
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, dpart, (int64_t)gb, 1, r1, dsum, 1);
-        if ((rc = smooth(diff, dsum))) return rc;
-        if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
-                               LossStore<T>{mask, maskv, grho ? gds : nullptr, npix, m.n2, T(1), p_norm == 1, bpart, dsum, W0 * invMd},
-                               st, nullptr, "zpass_c2r_loss")))
-            return rc;
-        if (grho) {
-            hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart + 6, nzb, kLossCols, r1, gsum, 1);
-            if ((rc = smooth(gds, gsum))) return rc;
-            if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart, gsum, W0 * invMd},
-                                   st, nullptr, "zpass_c2r_resp")))
+        if (smooth_kind == 2 && p_norm == 2 && !mask && !maskv && grho) {
+            // unmasked L2 with the Gaussian: one smoothing with W^2 = the
+            // Gaussian of width sigma sqrt(2) (W(0)^2 = 1) gives the loss and
+            // g_rho (LossSq2Store)
+            if ((rc = smooth(diff, dsum, smooth_par * std::sqrt(2.0)))) return rc;
+            if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
+                                   LossSq2Store<T>{target, rho, grho, rp, npix, m.n2, T(1), bpart, dsum, invMd}, st, nullptr,
+                                   "zpass_c2r_loss2")))
                 return rc;
+        } else {
+            if ((rc = smooth(diff, dsum, smooth_par))) return rc;
+            if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
+                                   LossStore<T>{mask, maskv, grho ? gds : nullptr, npix, m.n2, T(1), p_norm == 1, bpart, dsum, W0 * invMd},
+                                   st, nullptr, "zpass_c2r_loss")))
+                return rc;
+            if (grho) {
+                hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart + 6, nzb, kLossCols, r1, gsum, 1);
+                if ((rc = smooth(gds, gsum, smooth_par))) return rc;
+                if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp}, RespBwdStore<T>{rho, grho, rp, m.n2, T(1), bpart, gsum, W0 * invMd},
+                                       st, nullptr, "zpass_c2r_resp")))
+                    return rc;
+            }
         }
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, bpart, nzb, kLossCols, r6, sums, 1);
     }

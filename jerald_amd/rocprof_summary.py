@@ -33,6 +33,7 @@ _TIMERS = (
     (("k_zpass_c2r", "GammaBwdStore"), "zpass_c2r_gamma"),
     (("k_zpass_c2r", "LossStore"), "zpass_c2r_loss"),
     (("k_zpass_c2r", "RespBwdStore"), "zpass_c2r_resp"),
+    (("k_zpass_c2r", "LossSq2Store"), "zpass_c2r_loss2"),
     (("k_zpass_c2r", "Tri3Store3"), "zpass_c2r3"),
     (("k_zpass_c2r",), "zpass_c2r"),
     (("k_zpass_r2c",), "zpass_r2c"),

@@ -351,6 +351,29 @@ def test_ldl_autograd_function_and_recompute_mode():
     np.testing.assert_allclose(m.evaluate(torch.tensor(p)).cpu().numpy(), orc.evaluate(p, X), rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("kind,N", [("sm", 32), ("dm", 32), ("sm", 30)])
+def test_unmasked_l2_single_smoothing_matches_two_smoothings(kind, N):
+    """Unmasked L2 with the Gaussian: sum (S r)^2 = <r, S^2 r> and the residual's
+    gradient 2 S^2 r, from one smoothing with the Gaussian of width sigma sqrt(2).
+    An all-ones train mask takes the path that smooths the residual and then
+    its gradient; both agree to FP64 rounding (power-of-two line passes and a
+    mixed-radix mesh)."""
+    L, nstep = 0.8 * N, 2
+    X = O.lattice_positions(N, L) + O.zeldovich_displacement(N, L, 1.5, 11)
+    rng = np.random.default_rng(12)
+    tgt = rng.random((N, N, N)) + 0.5
+    p = np.array([0.8, 1.2, 0.3, 1.5, 0.2] * nstep + ([1.1, 0.9, 0.05] if kind == "sm" else []))
+    pm = PMesh(N, L, device=DEV, dtype=torch.float64)
+    res = []
+    for mk in (None, np.ones((N, N, N))):
+        m = JERALDModel(torch.tensor(X), tgt, pm, Nstep=nstep, kind=kind, L1=False, masktrain=mk, smoothing=("gaussian", 1.3 * L / N))
+        l, g = m.loss_and_grad(torch.tensor(p))
+        res.append((l.item(), g.cpu().numpy()))
+    (l0, g0), (l1, g1) = res
+    assert abs(l0 - l1) <= 1e-12 * abs(l1)
+    np.testing.assert_allclose(g0, g1, rtol=1e-10, atol=1e-14 * np.abs(g1).max())
+
+
 @pytest.mark.parametrize("dt,lattice", [(torch.float32, True), (torch.float64, False)])
 def test_recomputation_keeps_the_forward_sort_bitwise(dt, lattice):
     """A recomputed layer reads the sort its forward wrote (LDL_PAINT_BINS_VALID):
