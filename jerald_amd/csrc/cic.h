@@ -891,7 +891,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     constexpr int B3 = TX * TY * TZ;
     constexpr int NR = 20;  // 1 own range + 12 + 6 + 1 neighbour ranges
     __shared__ Acc box[NC * B3];
-    __shared__ unsigned rstart[NR], rpre[NR + 1];
+    __shared__ unsigned rstart[NR], rpre[NR + 1], rn0;
     __shared__ unsigned char rcode[NR];
     // XCD-contiguous tile order: the tiles an XCD runs together are z-neighbours,
     // whose particles share cache lines of the Lagrangian-ordered arrays
@@ -915,6 +915,7 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
             if (r == 0) {
                 start = offsets[8 * t];
                 len = offsets[8 * t + 8] - start;
+                rn0 = offsets[8 * t + 1] - start;  // own face class 0 comes first
             } else {
                 const int64_t nx = (dir & 1) ? (tx == 0 ? tg.nt[0] - 1 : tx - 1) : tx;
                 const int64_t ny = (dir & 2) ? (ty == 0 ? tg.nt[1] - 1 : ty - 1) : ty;
@@ -953,35 +954,51 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     }
     __syncthreads();
     const unsigned total = rpre[NR];
+    // The own tile's face class 0 (its first rn0 particles, most of them) has
+    // all 8 corners of every particle in the tile.
+    const unsigned r0 = __builtin_amdgcn_readfirstlane(rstart[0]), nfast = __builtin_amdgcn_readfirstlane(rn0);
     // KB particles per thread per round; all their loads are issued before any
     // deposit so the dependent sorted[j] -> pos[p] chains overlap (4 keeps the
     // 32-bit three-component paint at 74 VGPRs, 6 waves per SIMD, and the
-    // one-component paint at 62)
+    // one-component paint at 62).  A round takes TB KB particles, wave w the 64
+    // at w 64 + k TB for k < KB; a wave skips its sub-batches that start past
+    // the end (the waves of a tile stay within one sub-batch of each other),
+    // and a wave whose sub-batches all lie in face class 0 deposits without
+    // range lookups or corner tests.
     constexpr int KB = (ACC32 || NC == 1) ? 4 : 6;
-    for (unsigned jb = 0; jb < total; jb += TB * KB) {
+    const unsigned lane = threadIdx.x & 63u;
+    auto round = [&](const unsigned wb, const unsigned nk, auto fast_tag) {
+        constexpr bool FAST = decltype(fast_tag)::value;
         unsigned pi[KB];
         int code[KB];
 #pragma unroll
         for (int k = 0; k < KB; k++) {
-            unsigned j = jb + k * TB + threadIdx.x;
-            j = j < total ? j : total - 1;
-            int r = 0;
-            if (j >= rpre[1]) {  // binary search over the neighbour ranges 1..NR-1
-                int a = 1, b = NR - 1;
-                while (a < b) {
-                    const int mid = (a + b + 1) >> 1;
-                    if (rpre[mid] <= j) a = mid;
-                    else b = mid - 1;
+            if constexpr (FAST) {
+                pi[k] = sorted[r0 + wb + k * TB + lane];
+                code[k] = 0;
+            } else {
+                if (k >= (int)nk) break;
+                unsigned j = wb + k * TB + lane;
+                j = j < total ? j : total - 1;
+                int r = 0;
+                if (j >= rpre[1]) {  // binary search over the neighbour ranges 1..NR-1
+                    int a = 1, b = NR - 1;
+                    while (a < b) {
+                        const int mid = (a + b + 1) >> 1;
+                        if (rpre[mid] <= j) a = mid;
+                        else b = mid - 1;
+                    }
+                    r = a;
                 }
-                r = a;
+                pi[k] = sorted[rstart[r] + (j - rpre[r])];
+                code[k] = rcode[r];
             }
-            pi[k] = sorted[rstart[r] + (j - rpre[r])];
-            code[k] = rcode[r];
         }
         T xs[KB][3];
         T ms[KB][NC];
 #pragma unroll
         for (int k = 0; k < KB; k++) {
+            if (!FAST && k >= (int)nk) break;
             const T* q = pos + 3 * (int64_t)pi[k];
             xs[k][0] = q[0];
             xs[k][1] = q[1];
@@ -991,39 +1008,62 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
         }
 #pragma unroll
         for (int k = 0; k < KB; k++) {
-            // every lane stays active to the wave-wide test below; lanes past
-            // the end (reading a valid duplicate) deposit nothing
-            const bool valid = jb + k * TB + threadIdx.x < total;
+            if (!FAST && k >= (int)nk) break;
             const int64_t p = pi[k];
             uint32_t id[3];
             T f[3];
             cell_of<T, DM>(pd, mp, xs[k][0], xs[k][1], xs[k][2], p, id, f);
-            // local lower corner: -1 along the axes the particle was read from a
-            // lower neighbour, else its offset in this tile
-            const int a = (code[k] & 1) ? -1 : (int)(id[0] - (uint32_t)lo[0]);
-            const int b = (code[k] & 2) ? -1 : (int)(id[1] - (uint32_t)lo[1]);
-            const int cz = (code[k] & 4) ? -1 : (int)(id[2] - (uint32_t)lo[2]);
-            const bool in_x[2] = {valid && a >= 0, valid && a + 1 < ex};
-            const bool in_y[2] = {b >= 0, b + 1 < ey};
-            const bool in_z[2] = {cz >= 0, cz + 1 < ez};
-            const int base = (a * TY + b) * TZ + cz;
-            // every lane of the wave has all 8 corners in the tile (face class
-            // 0 of the own tile, most particles, stored together): unconditional
-            // deposits; else each corner is tested.  One of the two per wave.
-            const bool all_in = in_x[0] && in_x[1] && in_y[0] && in_y[1] && in_z[0] && in_z[1];
-            const bool wave_in = __all(all_in);
+            if constexpr (FAST) {
+                const int base = ((int)(id[0] - (uint32_t)lo[0]) * TY + (int)(id[1] - (uint32_t)lo[1])) * TZ +
+                                 (int)(id[2] - (uint32_t)lo[2]);
 #pragma unroll
-            for (int c = 0; c < NC; c++) {
-                DepositT<T> q[8];
-                deposits_of<T>(f, mass ? fixed_mass<T>(ms[k][c], s[c]) : Ms[c], q);
+                for (int c = 0; c < NC; c++) {
+                    DepositT<T> q[8];
+                    deposits_of<T>(f, mass ? fixed_mass<T>(ms[k][c], s[c]) : Ms[c], q);
 #pragma unroll
-                for (int qq = 0; qq < 8; qq++) {
-                    const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
-                    if (wave_in || (in_x[dx] && in_y[dy] && in_z[dz]))
+                    for (int qq = 0; qq < 8; qq++) {
+                        const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
                         atomicAdd(&box[c * B3 + base + dx * TY * TZ + dy * TZ + dz], (Acc)(SAcc)q[qq]);
+                    }
+                }
+            } else {
+                // every lane stays active to the wave-wide test below; lanes
+                // past the end (reading a valid duplicate) deposit nothing
+                const bool valid = wb + k * TB + lane < total;
+                // local lower corner: -1 along the axes the particle was read
+                // from a lower neighbour, else its offset in this tile
+                const int a = (code[k] & 1) ? -1 : (int)(id[0] - (uint32_t)lo[0]);
+                const int b = (code[k] & 2) ? -1 : (int)(id[1] - (uint32_t)lo[1]);
+                const int cz = (code[k] & 4) ? -1 : (int)(id[2] - (uint32_t)lo[2]);
+                const bool in_x[2] = {valid && a >= 0, valid && a + 1 < ex};
+                const bool in_y[2] = {b >= 0, b + 1 < ey};
+                const bool in_z[2] = {cz >= 0, cz + 1 < ez};
+                const int base = (a * TY + b) * TZ + cz;
+                // every lane of the wave has all 8 corners in the tile:
+                // unconditional deposits; else each corner is tested.  One of
+                // the two per wave.
+                const bool all_in = in_x[0] && in_x[1] && in_y[0] && in_y[1] && in_z[0] && in_z[1];
+                const bool wave_in = __all(all_in);
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    DepositT<T> q[8];
+                    deposits_of<T>(f, mass ? fixed_mass<T>(ms[k][c], s[c]) : Ms[c], q);
+#pragma unroll
+                    for (int qq = 0; qq < 8; qq++) {
+                        const int dx = qq >> 2, dy = (qq >> 1) & 1, dz = qq & 1;
+                        if (wave_in || (in_x[dx] && in_y[dy] && in_z[dz]))
+                            atomicAdd(&box[c * B3 + base + dx * TY * TZ + dy * TZ + dz], (Acc)(SAcc)q[qq]);
+                    }
                 }
             }
         }
+    };
+    for (unsigned jb = 0; jb < total; jb += TB * KB) {
+        const unsigned wb = __builtin_amdgcn_readfirstlane(jb + (threadIdx.x - lane));  // the wave's first particle
+        if (wb >= total) break;                                                       // and of every later round
+        const unsigned nk = (total - wb + TB - 1) / TB;  // sub-batches with a particle (KB or more: all)
+        if (wb + (KB - 1) * TB + 64 <= nfast) round(wb, (unsigned)KB, std::true_type{});
+        else round(wb, nk < (unsigned)KB ? nk : (unsigned)KB, std::false_type{});
     }
     __syncthreads();
     const int64_t cells = m.cells();
