@@ -837,8 +837,11 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned*
     constexpr unsigned EMPTY = 0xFFFFFFFFu;
     const int lane = threadIdx.x & 63;
     const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
-    const int64_t nchunks = (np + CH - 1) / CH;
-    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    // XCD-contiguous chunks: each XCD writes the ranks of its own band of
+    // particles, so the partial cache lines of a key's range fill in one L2
+    const int64_t nchunks = (np + CH - 1) / CH, per = (nchunks + gridDim.x - 1) / gridDim.x;
+    const int64_t c0 = xcd_tile(blockIdx.x, gridDim.x) * per, c1 = c0 + per < nchunks ? c0 + per : nchunks;
+    for (int64_t ch = c0; ch < c1; ch++) {
         unsigned k[PPT], base[PPT];
         int hl[PPT];
 #pragma unroll

@@ -84,9 +84,10 @@ template <int R, int DIR, typename C> __device__ __forceinline__ void dft(C* v) 
 // ------------------------------------------------------------ LDS FFT
 // In-place transform of the P pencils held in lds.  twN[m * TWS] =
 // exp(-2 pi i m / N): the workgroup's LDS copy of the twiddle table.
-template <typename T, int N, int P, int TPP, int DIR, int TWS = 1, int Ns = 1>
+// Stages Ns .. NE (exclusive) of the transform: all of them by default.
+template <typename T, int N, int P, int TPP, int DIR, int TWS = 1, int Ns = 1, int NE = N>
 __device__ __forceinline__ void lds_fft(c2<T>* lds, const c2<T>* twN) {
-    if constexpr (Ns < N) {
+    if constexpr (Ns < NE) {
         using C = c2<T>;
         constexpr int R = radix_of(N / Ns);
         constexpr int NB = N / R;
@@ -120,7 +121,87 @@ __device__ __forceinline__ void lds_fft(c2<T>* lds, const c2<T>* twN) {
             for (int r = 0; r < R; r++) lds[(base + r * Ns) * LS + p] = v[b][r];
         }
         __syncthreads();
-        lds_fft<T, N, P, TPP, DIR, TWS, Ns * R>(lds, twN);
+        lds_fft<T, N, P, TPP, DIR, TWS, Ns * R, NE>(lds, twN);
+    }
+}
+
+// Ns of the last stage of an N-point transform.
+constexpr int fft_last_ns(int n) {
+    int ns = 1;
+    while (ns * radix_of(n / ns) < n) ns *= radix_of(n / ns);
+    return ns;
+}
+
+// The same transform with its first and last stages in registers.  Thread
+// (q, p) = (threadIdx / P, threadIdx % P) holds x[s] = element q + TPP s of
+// pencil p: the elements reg_load gives it and reg_store takes from it.  The
+// first-stage butterflies j = q + TPP b read elements j + (N / R) r, which are
+// x[b + (N / R / TPP) r], and the last-stage ones write elements j + (N / R) r
+// in natural order, which are x[b + (N / R / TPP) r] again; so a load, the
+// transform and a store make two LDS round trips fewer than with lds_fft, and
+// an elementwise step between two transforms none.
+//
+// fft_from_regs: the first stage on x, written to lds, then the stages below
+// NE in lds (NE = N: the whole transform, in lds in natural order).  Other
+// threads may still read lds on entry (the first write waits at a barrier).
+template <typename T, int N, int P, int TPP, int DIR, int TWS = 1, int NE = N>
+__device__ __forceinline__ void fft_from_regs(c2<T> (&x)[N / TPP], c2<T>* lds, const c2<T>* twN) {
+    using C = c2<T>;
+    constexpr int R1 = radix_of(N), LS = P + 1, B1 = N / R1 / TPP;
+    static_assert(B1 * R1 * TPP == N, "threads per pencil must divide the butterfly count");
+    const int p = threadIdx.x % P, q = threadIdx.x / P;
+    __syncthreads();  // earlier readers of lds are done
+#pragma unroll
+    for (int b = 0; b < B1; b++) {
+        C v[R1];
+#pragma unroll
+        for (int r = 0; r < R1; r++) v[r] = x[b + B1 * r];
+        dft<R1, DIR>(v);
+        const int j = q + b * TPP;
+#pragma unroll
+        for (int r = 0; r < R1; r++) lds[(j * R1 + r) * LS + p] = v[r];
+    }
+    __syncthreads();
+    lds_fft<T, N, P, TPP, DIR, TWS, R1, NE>(lds, twN);
+}
+
+// fft_to_regs: the last stage from lds into x, every earlier stage done (and
+// followed by a barrier).  Other threads may still read lds on return.
+template <typename T, int N, int P, int TPP, int DIR, int TWS = 1>
+__device__ __forceinline__ void fft_to_regs(c2<T> (&x)[N / TPP], const c2<T>* lds, const c2<T>* twN) {
+    using C = c2<T>;
+    constexpr int NL = fft_last_ns(N), RL = N / NL, BL = NL / TPP, LS = P + 1;
+    static_assert(BL * RL * TPP == N, "threads per pencil must divide the butterfly count");
+    const int p = threadIdx.x % P, q = threadIdx.x / P;
+#pragma unroll
+    for (int b = 0; b < BL; b++) {
+        const int j = q + b * TPP;  // < NL: its twiddle index is j itself
+        C v[RL];
+#pragma unroll
+        for (int r = 0; r < RL; r++) v[r] = lds[(j + r * NL) * LS + p];
+        if constexpr (NL > 1) {
+#pragma unroll
+            for (int r = 1; r < RL; r++) {
+                C w = twN[r * j * TWS];
+                if constexpr (DIR > 0) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft<RL, DIR>(v);
+#pragma unroll
+        for (int r = 0; r < RL; r++) x[b + BL * r] = v[r];
+    }
+}
+
+// The whole transform of x, in place.
+template <typename T, int N, int P, int TPP, int DIR, int TWS = 1>
+__device__ __forceinline__ void reg_fft(c2<T> (&x)[N / TPP], c2<T>* lds, const c2<T>* twN) {
+    if constexpr (radix_of(N) == N) {  // one stage: the thread holds the whole pencil
+        static_assert(TPP == 1, "a one-stage transform is one thread per pencil");
+        dft<N, DIR>(x);
+    } else {
+        fft_from_regs<T, N, P, TPP, DIR, TWS, fft_last_ns(N)>(x, lds, twN);
+        fft_to_regs<T, N, P, TPP, DIR, TWS>(x, lds, twN);
     }
 }
 
@@ -150,6 +231,34 @@ __device__ __forceinline__ void tile_load(c2<T>* lds, int pv, F&& f) {
         const int i = threadIdx.x + s * TH;
         const int n = i / P, p = i - n * P;
         lds[n * (P + 1) + p] = p < pv ? v[s] : c2<T>{T(0), T(0)};
+    }
+}
+
+// tile_load into registers: x[s] = element q + TPP s of pencil p (reg_fft).
+template <typename T, int N, int P, int TH, class F>
+__device__ __forceinline__ void reg_load(c2<T> (&x)[N * P / TH], int pv, F&& f) {
+    constexpr int NL = N * P / TH;
+#pragma unroll
+    for (int s = 0; s < NL; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        x[s] = f(n, p < pv ? p : pv - 1);
+    }
+    if ((int)threadIdx.x % P >= pv) {
+#pragma unroll
+        for (int s = 0; s < NL; s++) x[s] = c2<T>{T(0), T(0)};
+    }
+}
+
+// tile_store from registers.
+template <typename T, int N, int P, int TH, class F>
+__device__ __forceinline__ void reg_store(const c2<T> (&x)[N * P / TH], int pv, F&& f) {
+    constexpr int NL = N * P / TH;
+#pragma unroll
+    for (int s = 0; s < NL; s++) {
+        const int i = threadIdx.x + s * TH;
+        const int n = i / P, p = i - n * P;
+        if (p < pv) f(n, p, x[s]);
     }
 }
 
@@ -224,10 +333,10 @@ __global__ void __launch_bounds__((FftCfg<T, N>::THREADS)) k_colpass(ColGeom g, 
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
     stage_twiddles<T, N, TH>(twl, tw);
-    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return op.load(base + n * g.es + p, n, plane, col0 + p); });
-    __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, DIR>(lds, twl);
-    tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, c2<T> v) { op.store(base + n * g.es + p, n, plane, col0 + p, v); });
+    c2<T> x[N * P / TH];
+    reg_load<T, N, P, TH>(x, pv, [&](int n, int p) { return op.load(base + n * g.es + p, n, plane, col0 + p); });
+    reg_fft<T, N, P, Cfg::TPP, DIR>(x, lds, twl);
+    reg_store<T, N, P, TH>(x, pv, [&](int n, int p, c2<T> v) { op.store(base + n * g.es + p, n, plane, col0 + p, v); });
 }
 
 // ------------------------------------------------------------ z passes

@@ -399,9 +399,9 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
     stage_twiddles<T, N, TH>(twl, tw);
-    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return U[base + n * g.es + p]; });
-    __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
+    C x[EPT];
+    reg_load<T, N, P, TH>(x, pv, [&](int n, int p) { return U[base + n * g.es + p]; });
+    fft_from_regs<T, N, P, Cfg::TPP, -1>(x, lds, twl);
     // each thread rewrites only its own elements: no barrier between
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
@@ -424,8 +424,9 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
         lds[n * LS + p] = pk;
     }
     __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
-    tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
+    lds_fft<T, N, P, Cfg::TPP, +1, 1, 1, fft_last_ns(N)>(lds, twl);
+    fft_to_regs<T, N, P, Cfg::TPP, +1>(x, lds, twl);
+    reg_store<T, N, P, TH>(x, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
 }
 
 // Backward: X FFT of Q's y,z-transformed spectrum (Q = sum_d i D_d times
@@ -450,9 +451,18 @@ k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
     stage_twiddles<T, N, TH>(twl, tw);
-    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return Sq[base + n * g.es + p]; });
-    __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
+    // register-ended transforms, except where their registers would cost
+    // occupancy (the HI kind's extra partials) or spill (1024-thread groups)
+    constexpr bool RE = !HI && TH <= 512;
+    C x[EPT];
+    if constexpr (RE) {
+        reg_load<T, N, P, TH>(x, pv, [&](int n, int p) { return Sq[base + n * g.es + p]; });
+        fft_from_regs<T, N, P, Cfg::TPP, -1>(x, lds, twl);
+    } else {
+        tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return Sq[base + n * g.es + p]; });
+        __syncthreads();
+        lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
+    }
     constexpr int NP = HI ? 9 : 4;
     double acc[NP] = {};
     const double apot = cpot * alpha;
@@ -473,8 +483,14 @@ k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>
         lds[n * LS + p] = y;
     }
     __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
-    tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
+    if constexpr (RE) {
+        lds_fft<T, N, P, Cfg::TPP, +1, 1, 1, fft_last_ns(N)>(lds, twl);
+        fft_to_regs<T, N, P, Cfg::TPP, +1>(x, lds, twl);
+        reg_store<T, N, P, TH>(x, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
+    } else {
+        lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
+        tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
+    }
     block_sum<NP, TH>(acc);
     if (threadIdx.x == 0) {
 #pragma unroll
@@ -708,7 +724,7 @@ k_gauss_line(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, co
              const double* __restrict__ msum, double inv) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
-    constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS;
+    constexpr int P = Cfg::P, TH = Cfg::THREADS;
     __shared__ C lds[Cfg::LDS_ELEMS];
     __shared__ C twl[N];
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -718,22 +734,21 @@ k_gauss_line(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, co
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
     stage_twiddles<T, N, TH>(twl, tw);
     const T mu = msum ? (T)(msum[0] * inv) : T(0);
-    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) {
+    C x[N * P / TH];
+    reg_load<T, N, P, TH>(x, pv, [&](int n, int p) {
         const C v = io[base + n * g.es + p];
         return C{v.x - mu, v.y - mu};
     });
-    __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
-    for (int i = threadIdx.x; i < N * P; i += TH) {
-        const int n = i / P, p = i - n * P;
+    reg_fft<T, N, P, Cfg::TPP, -1>(x, lds, twl);
+#pragma unroll
+    for (int s = 0; s < N * P / TH; s++) {
+        const int n = (threadIdx.x + s * TH) / P;
         const T k = kd[n];
         const T w = pexp(-hs2 * k * k);  // unnormalised both ways: the r2c store applies 1/M
-        const C v = lds[n * LS + p];
-        lds[n * LS + p] = C{v.x * w, v.y * w};
+        x[s] = C{x[s].x * w, x[s].y * w};
     }
-    __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
-    tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { io[base + n * g.es + p] = v; });
+    reg_fft<T, N, P, Cfg::TPP, +1>(x, lds, twl);
+    reg_store<T, N, P, TH>(x, pv, [&](int n, int p, C v) { io[base + n * g.es + p] = v; });
 }
 
 // z axis of the separable Gaussian: half-spectrum bins times
@@ -769,9 +784,9 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
     stage_twiddles<T, N, TH>(twl, tw);
-    tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return io[base + n * g.es + p]; });
-    __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
+    C x[N * P / TH];
+    reg_load<T, N, P, TH>(x, pv, [&](int n, int p) { return io[base + n * g.es + p]; });
+    fft_from_regs<T, N, P, Cfg::TPP, -1>(x, lds, twl);
     for (int i = threadIdx.x; i < N * P; i += TH) {
         const int n = i / P, p = i - n * P;
         if (p >= pv) continue;
@@ -787,8 +802,9 @@ k_x_realfilter(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ io, 
         lds[n * LS + p] = C{v.x * w, v.y * w};
     }
     __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
-    tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { io[base + n * g.es + p] = v; });
+    lds_fft<T, N, P, Cfg::TPP, +1, 1, 1, fft_last_ns(N)>(lds, twl);
+    fft_to_regs<T, N, P, Cfg::TPP, +1>(x, lds, twl);
+    reg_store<T, N, P, TH>(x, pv, [&](int n, int p, C v) { io[base + n * g.es + p] = v; });
 }
 
 // ------------------------------------ spectral kernels of the any-length path
