@@ -1054,43 +1054,62 @@ template <typename T> struct RespPar {
 __device__ __forceinline__ float relu12(float x) { return x > 1e-12f ? x : 1e-12f; }
 __device__ __forceinline__ double relu12(double x) { return x > 1e-12 ? x : 1e-12; }
 
-// Forward response value at cell i: y, F, Fi (model.py:344, :358-362)
-template <typename T>
+// Forward response value at cell i: y, F, Fi (model.py:344, :358-362).
+// PLAIN as for response_adj below.
+template <typename T, bool PLAIN = false>
 __device__ __forceinline__ void response(const RespPar<T>& rp, int64_t i, T rho, T f2, T& base, T& y, T& F, T& Fi) {
-    if (rp.ident) {
+    if (!PLAIN && rp.ident) {
         base = y = F = rho;
     } else {
-        base = ppow(rp.hi ? relu12(rho + rp.zs) : rho + T(1e-8), rp.mu);
+        const bool hi = !PLAIN && rp.hi;
+        base = ppow(hi ? relu12(rho + rp.zs) : rho + T(1e-8), rp.mu);
         y = rp.w * base + rp.b;
-        if (rp.hi) y -= rp.dep[i];
+        if (hi) y -= rp.dep[i];
         F = relu12(y);
     }
-    Fi = rp.index == T(1) ? F : ppow(F, rp.index);
-    Fi = Fi * f2;
+    if constexpr (PLAIN) {
+        Fi = F;
+    } else {
+        Fi = rp.index == T(1) ? F : ppow(F, rp.index);
+        Fi = Fi * f2;
+    }
 }
 
 // Same, also returning x^(mu-1) and ln x (x = rho+1e-8, HI ReLU(rho+zs)) for
-// the adjoint.
-template <typename T>
+// the adjoint.  PLAIN: the caller guarantees the sm response with index 1
+// and no second field (plain_response), so those branches compile away; the
+// arithmetic, and so every bit, is the general path's.
+template <typename T, bool PLAIN = false>
 __device__ __forceinline__ void response_adj(const RespPar<T>& rp, int64_t i, T rho, T f2, T& base, T& base1, T& lr, T& y, T& F,
                                              T& Fi) {
-    if (rp.ident) {
+    if (!PLAIN && rp.ident) {
         base = y = F = rho;
         base1 = T(1);
         lr = T(0);
     } else {
-        pow_pair_log(rp.hi ? relu12(rho + rp.zs) : rho + T(1e-8), rp.mu, base, base1, lr);
+        const bool hi = !PLAIN && rp.hi;
+        pow_pair_log(hi ? relu12(rho + rp.zs) : rho + T(1e-8), rp.mu, base, base1, lr);
         y = rp.w * base + rp.b;
-        if (rp.hi) y -= rp.dep[i];
+        if (hi) y -= rp.dep[i];
         F = relu12(y);
     }
-    Fi = rp.index == T(1) ? F : ppow(F, rp.index);
-    Fi = Fi * f2;
+    if constexpr (PLAIN) {
+        Fi = F;  // F^1 times a unit field
+    } else {
+        Fi = rp.index == T(1) ? F : ppow(F, rp.index);
+        Fi = Fi * f2;
+    }
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ bool plain_response(const RespPar<T>& rp) {
+    return !rp.ident && !rp.hi && rp.index == T(1) && rp.field2 == nullptr;
 }
 
 // d loss / d y, given d loss / d Fi (times field2 and the index power)
-template <typename T>
+template <typename T, bool PLAIN = false>
 __device__ __forceinline__ T response_bwd(const RespPar<T>& rp, T gFi, T f2, T y, T F) {
+    if constexpr (PLAIN) return y > T(1e-12) ? gFi : T(0);
     T g = gFi * f2;
     if (rp.index != T(1)) g = g * rp.index * ppow(F, rp.index - T(1));
     return (rp.ident || y > T(1e-12)) ? g : T(0);
@@ -1098,8 +1117,9 @@ __device__ __forceinline__ T response_bwd(const RespPar<T>& rp, T gFi, T f2, T y
 
 // d loss / d rho from d loss / d y (the mu, w, b partials are zero for
 // ident); HI also writes dL/d dep = -gy.
-template <typename T>
+template <typename T, bool PLAIN = false>
 __device__ __forceinline__ T response_grho(const RespPar<T>& rp, int64_t i, T gy, T base1, T rho) {
+    if constexpr (PLAIN) return gy * rp.w * rp.mu * base1;
     if (rp.ident) return gy;
     if (rp.hi) {
         if (rp.gdep) rp.gdep[i] = -gy;
@@ -1165,14 +1185,14 @@ __global__ void __launch_bounds__(256) k_loss_direct(const T* __restrict__ rho, 
 }
 
 // diff = Fi - target (smoothed-loss path) and per-block FP64 sums of diff
-template <typename T>
+template <typename T, bool PLAIN = false>
 __global__ void __launch_bounds__(256) k_loss_diff(const T* __restrict__ rho, const T* __restrict__ target, RespPar<T> rp, int64_t n,
                                                    T* __restrict__ diff, double* __restrict__ dpart) {
     double a[1] = {0.0};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const T f2 = rp.field2 ? rp.field2[i] : T(1);
+        const T f2 = PLAIN ? T(1) : rp.field2 ? rp.field2[i] : T(1);
         T base, y, F, Fi;
-        response(rp, i, rho[i], f2, base, y, F, Fi);
+        response<T, PLAIN>(rp, i, rho[i], f2, base, y, F, Fi);
         const T d = Fi - target[i];
         diff[i] = d;
         a[0] += (double)d;
@@ -1295,7 +1315,7 @@ template <typename T> struct RespBwdStore {
 // dcw adds W(0)^2 times the residual's mean back (as LossStore's).  The
 // residual r = Fi - target is formed again here (the smoothing's line passes
 // overwrite their input), with k_loss_diff's arithmetic.
-template <typename T> struct LossSq2Store {
+template <typename T, bool PLAIN = false> struct LossSq2Store {
     static constexpr int NR = 5;
     const T* target;
     const T* rho;
@@ -1316,17 +1336,17 @@ template <typename T> struct LossSq2Store {
         for (int e = 0; e < 2; e++) {
             const int64_t i = i0 + e;
             const T r = rho[i];
-            const T f2 = rp.field2 ? rp.field2[i] : T(1);
+            const T f2 = PLAIN ? T(1) : rp.field2 ? rp.field2[i] : T(1);
             T base, base1, lr, y, F, Fi;
-            response_adj(rp, i, r, f2, base, base1, lr, y, F, Fi);
+            response_adj<T, PLAIN>(rp, i, r, f2, base, base1, lr, y, F, Fi);
             acc[0] += (double)(Fi - target[i]) * (double)q[e];
-            const T gy = response_bwd(rp, q[e] * ginv, f2, y, F);
-            if (!rp.ident) {
+            const T gy = response_bwd<T, PLAIN>(rp, q[e] * ginv, f2, y, F);
+            if (PLAIN || !rp.ident) {
                 acc[1] += (double)(gy * rp.w * base) * (double)lr;
                 acc[2] += (double)(gy * base);
                 acc[3] += (double)gy;
             }
-            const T gr = response_grho(rp, i, gy, base1, r);
+            const T gr = response_grho<T, PLAIN>(rp, i, gy, base1, r);
             acc[4] += (double)gr;
             grho[i] = gr;
         }
@@ -1834,7 +1854,10 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
         // every block writes the loss columns; the gradient columns stay zero without g_rho
         hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nzb * kLossCols, 256)), dim3(256), 0, st, bpart, nzb * kLossCols, 0.0);
         double* dpart = bpart + nzb * kLossCols;
-        hipLaunchKernelGGL(k_loss_diff<T>, dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff, dpart);
+        if (plain_response(rp))
+            hipLaunchKernelGGL((k_loss_diff<T, true>), dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff, dpart);
+        else
+            hipLaunchKernelGGL((k_loss_diff<T, false>), dim3(gb), dim3(256), 0, st, rho, target, rp, cells, diff, dpart);
         const RedMap r1{1, {0}};
         hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, dpart, (int64_t)gb, 1, r1, dsum, 1);
         if (smooth_kind == 2 && p_norm == 2 && !mask && !maskv && grho) {
@@ -1842,10 +1865,15 @@ static int sm_loss(int kind, const T* rho, const T* target, const T* mask, const
             // Gaussian of width sigma sqrt(2) (W(0)^2 = 1) gives the loss and
             // g_rho (LossSq2Store)
             if ((rc = smooth(diff, dsum, smooth_par * std::sqrt(2.0)))) return rc;
-            if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
-                                   LossSq2Store<T>{target, rho, grho, rp, npix, m.n2, T(1), bpart, dsum, invMd}, st, nullptr,
-                                   "zpass_c2r_loss2")))
-                return rc;
+            if (plain_response(rp))
+                rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
+                                  LossSq2Store<T, true>{target, rho, grho, rp, npix, m.n2, T(1), bpart, dsum, invMd}, st, nullptr,
+                                  "zpass_c2r_loss2");
+            else
+                rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
+                                  LossSq2Store<T>{target, rho, grho, rp, npix, m.n2, T(1), bpart, dsum, invMd}, st, nullptr,
+                                  "zpass_c2r_loss2");
+            if (rc) return rc;
         } else {
             if ((rc = smooth(diff, dsum, smooth_par))) return rc;
             if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{S, hp},
