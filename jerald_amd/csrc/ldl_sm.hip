@@ -383,14 +383,20 @@ __device__ __forceinline__ T xbwd_cell(const SmFilter<T>& fl, const Src2<T>& s2,
 
 // Forward: X FFT, save U = . / N^3, P = 1e-2 alpha G U, inverse X FFT of P
 // into out.  The derivative symbols are applied in real space afterwards
-// (k_grad6).
+// (k_grad6).  FP32 512-thread workgroups without the HI source keep the
+// whole tile in registers from the load to the store (RE: 128 VGPRs, the
+// two workgroups per CU the LDS allows); the others run the filter through
+// LDS, which would otherwise spill.  Element indices of a tile fit 32 bits
+// (every axis is at most 1024), so its accesses are 32-bit offsets from a
+// uniform base.
 template <typename T, int N, bool HI>
-__global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
+__global__ void __launch_bounds__((FftCfg<T, N>::THREADS), (FftCfg<T, N>::THREADS == 512 && sizeof(T) == 4 && !HI ? 4 : 1))
 k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>* __restrict__ out, KTabs<T> kt,
            SmFilter<T> fl, T a1e2, T invM, Src2<T> s2) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
     constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
+    constexpr bool RE = TH == 512 && sizeof(T) == 4 && !HI;
     __shared__ C lds[Cfg::LDS_ELEMS];
     __shared__ C twl[N];
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -398,20 +404,24 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
     const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
+    C* Ub = U + base;
+    C* Ob = out + base;
+    const uint32_t es = (uint32_t)g.es;
     stage_twiddles<T, N, TH>(twl, tw);
     C x[EPT];
-    reg_load<T, N, P, TH>(x, pv, [&](int n, int p) { return U[base + n * g.es + p]; });
-    fft_from_regs<T, N, P, Cfg::TPP, -1>(x, lds, twl);
+    reg_load<T, N, P, TH>(x, pv, [&](int n, int p) { return Ub[(uint32_t)n * es + (uint32_t)p]; });
+    if constexpr (RE) reg_fft<T, N, P, Cfg::TPP, -1>(x, lds, twl);
+    else fft_from_regs<T, N, P, Cfg::TPP, -1>(x, lds, twl);
     // each thread rewrites only its own elements: no barrier between
 #pragma unroll
     for (int s = 0; s < EPT; s++) {
         const int i = threadIdx.x + s * TH;
         const int n = i / P, p = i - n * P;
-        C u = lds[n * LS + p];
+        C u = RE ? x[s] : lds[n * LS + p];
         u = C{u.x * invM, u.y * invM};
         C pk = C{T(0), T(0)};
         if (p < pv) {
-            U[base + n * g.es + p] = u;
+            Ub[(uint32_t)n * es + (uint32_t)p] = u;
             const T k2 = kt.knorm2(n, plane, col0 + p);
             const T G = fl.G(k2);
             pk = C{u.x * a1e2 * G, u.y * a1e2 * G};
@@ -421,12 +431,17 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
                 pk = C{pk.x + G2 * sv.x, pk.y + G2 * sv.y};
             }
         }
-        lds[n * LS + p] = pk;
+        if constexpr (RE) x[s] = pk;
+        else lds[n * LS + p] = pk;
     }
-    __syncthreads();
-    lds_fft<T, N, P, Cfg::TPP, +1, 1, 1, fft_last_ns(N)>(lds, twl);
-    fft_to_regs<T, N, P, Cfg::TPP, +1>(x, lds, twl);
-    reg_store<T, N, P, TH>(x, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
+    if constexpr (RE) {
+        reg_fft<T, N, P, Cfg::TPP, +1>(x, lds, twl);
+    } else {
+        __syncthreads();
+        lds_fft<T, N, P, Cfg::TPP, +1, 1, 1, fft_last_ns(N)>(lds, twl);
+        fft_to_regs<T, N, P, Cfg::TPP, +1>(x, lds, twl);
+    }
+    reg_store<T, N, P, TH>(x, pv, [&](int n, int p, C v) { Ob[(uint32_t)n * es + (uint32_t)p] = v; });
 }
 
 // Backward: X FFT of Q's y,z-transformed spectrum (Q = sum_d i D_d times
@@ -450,19 +465,16 @@ k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>
     const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
     const int64_t base = plane * g.plane_stride + col0;
     const int pv = (int)((g.ncols - col0) < P ? (g.ncols - col0) : P);
+    const C* Sb = Sq + base;
+    const C* Ub = U + base;
+    C* Ob = out + base;
+    const uint32_t es = (uint32_t)g.es;  // tile element indices fit 32 bits, as in k_sm_x_fwd
     stage_twiddles<T, N, TH>(twl, tw);
-    // register-ended transforms, except where their registers would cost
-    // occupancy (the HI kind's extra partials) or spill (1024-thread groups)
-    constexpr bool RE = !HI && TH <= 512;
+    // the first forward and last inverse stages in registers; the epilogue
+    // through LDS (in registers its FP64 partials would spill)
     C x[EPT];
-    if constexpr (RE) {
-        reg_load<T, N, P, TH>(x, pv, [&](int n, int p) { return Sq[base + n * g.es + p]; });
-        fft_from_regs<T, N, P, Cfg::TPP, -1>(x, lds, twl);
-    } else {
-        tile_load<T, N, P, TH>(lds, pv, [&](int n, int p) { return Sq[base + n * g.es + p]; });
-        __syncthreads();
-        lds_fft<T, N, P, Cfg::TPP, -1>(lds, twl);
-    }
+    reg_load<T, N, P, TH>(x, pv, [&](int n, int p) { return Sb[(uint32_t)n * es + (uint32_t)p]; });
+    fft_from_regs<T, N, P, Cfg::TPP, -1>(x, lds, twl);
     constexpr int NP = HI ? 9 : 4;
     double acc[NP] = {};
     const double apot = cpot * alpha;
@@ -475,22 +487,17 @@ k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>
         C y = C{T(0), T(0)};
         if (p < pv) {
             const int64_t kz = col0 + p;
-            const int64_t idx = base + n * g.es + p;
-            const T G = xbwd_cell<T, HI>(fl, s2, kt.knorm2(n, plane, kz), q, U[idx], idx, xbwd_weights(kz, n2, zdbl, alpha_rule), cpot,
-                                         apot, gkh, gkl, acc);
+            const uint32_t e = (uint32_t)n * es + (uint32_t)p;
+            const T G = xbwd_cell<T, HI>(fl, s2, kt.knorm2(n, plane, kz), q, Ub[e], base + e, xbwd_weights(kz, n2, zdbl, alpha_rule),
+                                         cpot, apot, gkh, gkl, acc);
             y = C{a1e2 * G * q.x, a1e2 * G * q.y};
         }
         lds[n * LS + p] = y;
     }
     __syncthreads();
-    if constexpr (RE) {
-        lds_fft<T, N, P, Cfg::TPP, +1, 1, 1, fft_last_ns(N)>(lds, twl);
-        fft_to_regs<T, N, P, Cfg::TPP, +1>(x, lds, twl);
-        reg_store<T, N, P, TH>(x, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
-    } else {
-        lds_fft<T, N, P, Cfg::TPP, +1>(lds, twl);
-        tile_store<T, N, P, TH>(lds, pv, [&](int n, int p, C v) { out[base + n * g.es + p] = v; });
-    }
+    lds_fft<T, N, P, Cfg::TPP, +1, 1, 1, fft_last_ns(N)>(lds, twl);
+    fft_to_regs<T, N, P, Cfg::TPP, +1>(x, lds, twl);
+    reg_store<T, N, P, TH>(x, pv, [&](int n, int p, C v) { Ob[(uint32_t)n * es + (uint32_t)p] = v; });
     block_sum<NP, TH>(acc);
     if (threadIdx.x == 0) {
 #pragma unroll
