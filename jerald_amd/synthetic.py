@@ -1,5 +1,6 @@
 """Seeded synthetic inputs for the benchmarks and studies, built on the GPU
-with torch (input setup only, not part of any timed path).
+with torch and libldl's own FP64 transforms (input setup only, not part of
+any timed path).
 
 * ``zeldovich_gpu``: lattice + Zel'dovich displacement, the recipe of the
   headline configuration: white noise n_k, delta_k = sqrt(P(k)) n_k with
@@ -15,12 +16,14 @@ import math
 
 import torch
 
+from . import jaxpops as jpo
+
 
 def zeldovich_gpu(n, box, rms_cells, seed, device):
     """(n^3, 3) float64 positions in box units, lattice order (z fastest)."""
     g = torch.Generator(device=device).manual_seed(seed)
     white = torch.randn((n, n, n), generator=g, device=device, dtype=torch.float64)
-    wk = torch.fft.rfftn(white)
+    wk = jpo.pfft(white)  # unnormalised r2c
     del white
     cell = box / n
     k1 = torch.fft.fftfreq(n, d=cell, device=device, dtype=torch.float64) * 2 * math.pi
@@ -32,7 +35,7 @@ def zeldovich_gpu(n, box, rms_cells, seed, device):
     del wk, pk
     psi = torch.empty((n ** 3, 3), device=device, dtype=torch.float64)
     for d, kk in enumerate((k1[:, None, None], k1[None, :, None], kz[None, None, :])):
-        psi[:, d] = torch.fft.irfftn(1j * kk * dk, s=(n, n, n)).reshape(-1)
+        psi[:, d] = jpo.pifft(1j * kk * dk, (n, n, n)).reshape(-1) / float(n) ** 3  # normalised c2r
     del dk, k2, k2s
     psi *= rms_cells * cell / torch.sqrt(torch.mean(psi ** 2))
     i = torch.arange(n, device=device, dtype=torch.float64) * cell
