@@ -275,24 +275,32 @@ __device__ __forceinline__ void tile_store(const c2<T>* lds, int pv, F&& f) {
 }
 
 // Sum NR per-thread FP64 values over the workgroup; thread 0 gets the totals.
+// block_sum into caller-provided shared memory: NR * ceil(TH / 64) doubles
+// that no thread reads any more (say the caller's tile after a barrier).
 template <int NR, int TH>
-__device__ __forceinline__ void block_sum(double (&v)[NR]) {
-    __shared__ double red[NR][(TH + 63) / 64];
+__device__ __forceinline__ void block_sum_in(double (&v)[NR], double* red) {
+    constexpr int NW = (TH + 63) / 64;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         double x = wave_sum(v[r]);
-        if (lane == 0) red[r][w] = x;
+        if (lane == 0) red[r * NW + w] = x;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             double x = 0.0;
-            for (int i = 0; i < (TH + 63) / 64; i++) x += red[r][i];
+            for (int i = 0; i < NW; i++) x += red[r * NW + i];
             v[r] = x;
         }
     }
+}
+
+template <int NR, int TH>
+__device__ __forceinline__ void block_sum(double (&v)[NR]) {
+    __shared__ double red[NR * ((TH + 63) / 64)];
+    block_sum_in<NR, TH>(v, red);
 }
 
 // ------------------------------------------------------- column passes
@@ -302,9 +310,8 @@ struct ColGeom {
     int64_t planes, ncols, plane_stride, es, tiles_per_plane, ntiles;
 };
 
-template <typename T, int N>
+template <typename T, int N, int P = FftCfg<T, N>::P>
 inline ColGeom col_geom(int64_t planes, int64_t ncols, int64_t plane_stride, int64_t es) {
-    constexpr int P = FftCfg<T, N>::P;
     ColGeom g{planes, ncols, plane_stride, es, (ncols + P - 1) / P, 0};
     g.ntiles = g.tiles_per_plane * planes;
     return g;

@@ -389,15 +389,15 @@ __device__ __forceinline__ T xbwd_cell(const SmFilter<T>& fl, const Src2<T>& s2,
 // LDS, which would otherwise spill.  Element indices of a tile fit 32 bits
 // (every axis is at most 1024), so its accesses are 32-bit offsets from a
 // uniform base.
-template <typename T, int N, bool HI>
-__global__ void __launch_bounds__((FftCfg<T, N>::THREADS), (FftCfg<T, N>::THREADS == 512 && sizeof(T) == 4 && !HI ? 4 : 1))
+template <typename T, int N, bool HI, int P = FftCfg<T, N>::P>
+__global__ void __launch_bounds__((P * FftCfg<T, N>::TPP), (P * FftCfg<T, N>::TPP == 512 && sizeof(T) == 4 && !HI ? 4 : 1))
 k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>* __restrict__ out, KTabs<T> kt,
            SmFilter<T> fl, T a1e2, T invM, Src2<T> s2) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
-    constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
+    constexpr int LS = P + 1, TH = P * Cfg::TPP, EPT = N * P / TH;
     constexpr bool RE = TH == 512 && sizeof(T) == 4 && !HI;
-    __shared__ C lds[Cfg::LDS_ELEMS];
+    __shared__ C lds[N * LS];
     __shared__ C twl[N];
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t plane = tile / g.tiles_per_plane;
@@ -450,15 +450,20 @@ k_sm_x_fwd(ColGeom g, const c2<T>* __restrict__ tw, c2<T>* __restrict__ U, c2<T>
 // kh, kl, nu; HI also the second source's) against the saved U, then one
 // inverse X FFT of cpot alpha G Q into out (may alias Sq: each workgroup
 // reads its tile whole before it writes).
-template <typename T, int N, bool HI>
-__global__ void __launch_bounds__((FftCfg<T, N>::THREADS))
+// Pencils per workgroup of the backward x pass: FP32 N = 512 takes 8 (its
+// tile, with the reduction in the tile's LDS, is 40 KB: four workgroups per
+// CU; 0.513 -> 0.496 ms at 512^3, where the forward pass is faster at 16).
+template <typename T, int N> constexpr int xbwd_pencils() { return sizeof(T) == 4 && N == 512 ? 8 : FftCfg<T, N>::P; }
+
+template <typename T, int N, bool HI, int P = xbwd_pencils<T, N>()>
+__global__ void __launch_bounds__((P * FftCfg<T, N>::TPP))
 k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ U, c2<T>* out,
            int64_t n2, int64_t zdbl, KTabs<T> kt, SmFilter<T> fl, T a1e2, double gkh, double gkl, double cpot, double alpha,
            int alpha_rule, Src2<T> s2, double* __restrict__ part) {
     using Cfg = FftCfg<T, N>;
     using C = c2<T>;
-    constexpr int P = Cfg::P, LS = Cfg::LS, TH = Cfg::THREADS, EPT = N * P / TH;
-    __shared__ C lds[Cfg::LDS_ELEMS];
+    constexpr int LS = P + 1, TH = P * Cfg::TPP, EPT = N * P / TH;
+    __shared__ C lds[N * LS];
     __shared__ C twl[N];
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int64_t plane = tile / g.tiles_per_plane;
@@ -498,7 +503,9 @@ k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>
     lds_fft<T, N, P, Cfg::TPP, +1, 1, 1, fft_last_ns(N)>(lds, twl);
     fft_to_regs<T, N, P, Cfg::TPP, +1>(x, lds, twl);
     reg_store<T, N, P, TH>(x, pv, [&](int n, int p, C v) { Ob[(uint32_t)n * es + (uint32_t)p] = v; });
-    block_sum<NP, TH>(acc);
+    __syncthreads();  // the tile's last reads are done: its LDS holds the reduction
+    static_assert(NP * (TH / 64) * sizeof(double) <= sizeof(lds), "reduction fits the tile");
+    block_sum_in<NP, TH>(acc, reinterpret_cast<double*>(lds));
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int j = 0; j < NP; j++) part[blockIdx.x * NP + j] = acc[j];
@@ -1550,12 +1557,13 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
         dispatch_n(m.n0, [&](auto N) {
             constexpr int NN = decltype(N)::value;
             ColGeom g = col_geom<T, NN>(m.n1, nh, hp, m.n1 * hp);
+            constexpr int XTH = FftCfg<T, NN>::THREADS;
             KTimer _t("sm_x_fwd", st);
             if (hi)
-                hipLaunchKernelGGL((k_sm_x_fwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3, kt,
+                hipLaunchKernelGGL((k_sm_x_fwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(XTH), 0, st, g, tw, U, P.spec3, kt,
                                    fl, a1e2, invM, s2);
             else
-                hipLaunchKernelGGL((k_sm_x_fwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, tw, U, P.spec3,
+                hipLaunchKernelGGL((k_sm_x_fwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(XTH), 0, st, g, tw, U, P.spec3,
                                    kt, fl, a1e2, invM, s2);
         });
     } else {
@@ -1672,14 +1680,15 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     if (fft_mesh_pow2(m)) {
         dispatch_n(m.n0, [&](auto N) {
             constexpr int NN = decltype(N)::value;
-            ColGeom g = col_geom<T, NN>(m.n1, nh, hp, m.n1 * hp);
+            ColGeom g = col_geom<T, NN, xbwd_pencils<T, NN>()>(m.n1, nh, hp, m.n1 * hp);
+            constexpr int XTH = xbwd_pencils<T, NN>() * FftCfg<T, NN>::TPP;
             ntx = g.ntiles;
             KTimer _t("sm_x_bwd", st);
             if (hi)
-                hipLaunchKernelGGL((k_sm_x_bwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, P.spec3, tw, U, S,
+                hipLaunchKernelGGL((k_sm_x_bwd<T, NN, true>), dim3((unsigned)g.ntiles), dim3(XTH), 0, st, g, P.spec3, tw, U, S,
                                    m.n2, zdbl, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
             else
-                hipLaunchKernelGGL((k_sm_x_bwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(FftCfg<T, NN>::THREADS), 0, st, g, P.spec3, tw, U,
+                hipLaunchKernelGGL((k_sm_x_bwd<T, NN, false>), dim3((unsigned)g.ntiles), dim3(XTH), 0, st, g, P.spec3, tw, U,
                                    S, m.n2, zdbl, kt, fl, a1e2, gkh, gkl, cpot, prm[0], (int)alpha_rule, s2, P.part);
         });
     } else {
