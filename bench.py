@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--cpu1-n", type=int, default=128, help="per-side size of the 1-thread CPU sample (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the plain device copy timed during setup (its HBM counters calibrate traffic)")
+    ap.add_argument("--fft-reps", type=int, default=5, help="timed standalone r2c and c2r transforms of the mesh (0: skip)")
     return ap.parse_args()
 
 
@@ -156,6 +157,37 @@ def copy_probe(mib, dev):
     return {"bytes": by, "ms": round(ms, 4), "GBps": round(by / ms / 1e6, 1)}
 
 
+def fft_op(n, dev, reps):
+    """The FFT as one operation: standalone FP32 r2c (ldl_pfft) and c2r
+    (ldl_pifft) of an n^3 mesh, timed with HIP events on the launch stream,
+    against the operation's algorithmic bytes (the real mesh and the half
+    spectrum, each moved once).  The transform runs as three passes (z rows,
+    y columns, x columns), each reading and writing the whole array, so the
+    operation's rate is about a third of a pass's."""
+    import torch
+    from jerald_amd import jaxpops as jpo
+    if reps <= 0:
+        return None
+    g = torch.Generator(device=dev).manual_seed(87)
+    x = torch.randn((n, n, n), device=dev, dtype=torch.float32, generator=g)
+    X = jpo.pfft(x)
+    jpo.pifft(X, (n, n, n))
+    torch.cuda.synchronize()
+    by = 4 * n ** 3 + 8 * n * n * (n // 2 + 1)
+    out = {"passes": 3, "algorithmic_bytes": by}
+    for name, fn in (("r2c", lambda: jpo.pfft(x)), ("c2r", lambda: jpo.pifft(X, (n, n, n)))):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"avg_ms": round(ms, 4), "GBps_alg": round(by / ms / 1e6, 1), "frac": round(by / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    del x, X
+    return out
+
+
 def timed(model, p, steps, warmup):
     """(seconds for ``steps`` loss+gradient steps after ``warmup``, last result)."""
     import torch
@@ -189,6 +221,7 @@ def main():
     cell = box / n
     p = params0(nstep)
     probe = copy_probe(args.copy_probe_mib, dev)
+    fftop = fft_op(n, dev, args.fft_reps)
     log(f"[bench] building inputs n={n} box={box}")
     X = zeldovich_gpu(n, box, 2.0, 83, dev)
     pm32 = PMesh(n, box, device=dev, dtype=torch.float32)
@@ -364,6 +397,7 @@ def main():
         "fp64_plain_gpu": fp64,
         "memory_modes": variants,
         "copy_probe": probe,
+        "fft_op": fftop,
         "loss_fp32": loss32,
         "ops": ops,
         "kernels": kernel_table,
