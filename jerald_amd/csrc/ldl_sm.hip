@@ -526,13 +526,18 @@ k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>
 // painted gradient meshes g_d into sum_d diff_d(g_d) / (60 h_d), whose
 // transform is the sum_d i D_d of theirs.
 //
-// A workgroup of 256 threads covers kStTY (y) x kStTZ (z) cells (each thread
+// A workgroup of 4 st_tz<T>() threads covers kStTY (y) x st_tz<T>() (z) cells (each thread
 // 4 cells along y at one z) and marches through kStXC x planes: the x
 // difference comes from a 7-plane register queue per cell, the y and z
 // differences from the current plane staged in LDS with 3-cell halos.  All
 // indices are wrapped, so a tile larger than the mesh reads valid periodic
 // values and only its stores are bounded.
-constexpr int kStTZ = 64, kStTY = 16, kStXC = 32;
+constexpr int kStTY = 16, kStXC = 32;
+// z width of a tile: its 3-cell z halo costs two extra cache lines per row, so
+// wider rows fetch less (FP32, 512^3: grad6 0.500 -> 0.466 ms, div6 0.471 ->
+// 0.432 ms from 64 to 256); FP64 keeps 128 (at 1024 threads its divergence
+// would spill)
+template <typename T> constexpr int st_tz() { return sizeof(T) == 4 ? 256 : 128; }
 
 struct StGeom {
     int n0, n1, n2, tiles_y, tiles_z, chunks;
@@ -545,7 +550,7 @@ __device__ __forceinline__ T st_diff(T m3, T m2, T m1, T p1, T p2, T p3) {
     return T(45) * (p1 - m1) - T(9) * (p2 - m2) + (p3 - m3);
 }
 
-struct StTile {
+template <int TZ> struct StTile {
     int x0, x1, y0, z0, tz, yg;
     int zw, yw[4];
     bool zok, yok[4];
@@ -558,9 +563,9 @@ struct StTile {
         x0 = xc * kStXC;
         x1 = x0 + kStXC < g.n0 ? x0 + kStXC : g.n0;
         y0 = tyt * kStTY;
-        z0 = tzt * kStTZ;
-        tz = threadIdx.x & 63;
-        yg = threadIdx.x >> 6;
+        z0 = tzt * TZ;
+        tz = threadIdx.x % TZ;
+        yg = threadIdx.x / TZ;
         zw = st_wrap(z0 + tz, g.n2);
         zok = z0 + tz < g.n2;
 #pragma unroll
@@ -576,11 +581,11 @@ struct StTile {
             hy[h] = r < 6 ? (r < 3 ? r : kStTY + r) : -1;
             hyw[h] = st_wrap(y0 + (hy[h] < 0 ? 0 : hy[h]) - 3, g.n1);
         }
-        // z halo: kStTY rows x 6 columns (0..2, kStTZ+3..kStTZ+5), threads 0..95
+        // z halo: kStTY rows x 6 columns (0..2, TZ+3..TZ+5), threads 0..95
         const int tt = threadIdx.x;
         zrow = tt < 6 * kStTY ? tt / 6 : -1;
         const int k = tt - (tt / 6) * 6;
-        zcol = k < 3 ? k : kStTZ + k;
+        zcol = k < 3 ? k : TZ + k;
         zyw = st_wrap(y0 + (zrow < 0 ? 0 : zrow), g.n1);
         zzw = st_wrap(z0 + zcol - 3, g.n2);
     }
@@ -588,9 +593,10 @@ struct StTile {
 
 // Forces F (N0, N1, N2, 3), interleaved per cell, from the potential phi.
 template <typename T>
-__global__ void __launch_bounds__(256) k_grad6(const T* __restrict__ phi, T* __restrict__ F, StGeom g, T cx, T cy, T cz) {
-    __shared__ T pl[kStTY + 6][kStTZ + 6];
-    const StTile tl(g);
+__global__ void __launch_bounds__(4 * st_tz<T>()) k_grad6(const T* __restrict__ phi, T* __restrict__ F, StGeom g, T cx, T cy, T cz) {
+    constexpr int TZ = st_tz<T>();
+    __shared__ T pl[kStTY + 6][TZ + 6];
+    const StTile<TZ> tl(g);
     if (tl.x0 >= g.n0) return;  // uniform over the workgroup
     const int64_t ps = (int64_t)g.n1 * g.n2;
     auto at = [&](int x, int y, int z) { return phi[(int64_t)st_wrap(x, g.n0) * ps + (int64_t)y * g.n2 + z]; };
@@ -647,10 +653,11 @@ __global__ void __launch_bounds__(256) k_grad6(const T* __restrict__ phi, T* __r
 // Divergence out = sum_d c_d diff_d(g_d) of the meshes g (3 consecutive
 // (N0, N1, N2) meshes) into a real mesh.
 template <typename T>
-__global__ void __launch_bounds__(256) k_div6(const T* __restrict__ gm, T* __restrict__ out, StGeom g, T cx, T cy, T cz) {
-    __shared__ T ly_[kStTY + 6][kStTZ];
-    __shared__ T lz_[kStTY][kStTZ + 6];
-    const StTile tl(g);
+__global__ void __launch_bounds__(4 * st_tz<T>()) k_div6(const T* __restrict__ gm, T* __restrict__ out, StGeom g, T cx, T cy, T cz) {
+    constexpr int TZ = st_tz<T>();
+    __shared__ T ly_[kStTY + 6][TZ];
+    __shared__ T lz_[kStTY][TZ + 6];
+    const StTile<TZ> tl(g);
     if (tl.x0 >= g.n0) return;  // uniform over the workgroup
     const int64_t ps = (int64_t)g.n1 * g.n2, cells = ps * g.n0;
     const T* gx = gm;
@@ -713,13 +720,13 @@ __global__ void __launch_bounds__(256) k_div6(const T* __restrict__ gm, T* __res
     }
 }
 
-static StGeom st_geom(const Mesh& m) {
+template <typename T> static StGeom st_geom(const Mesh& m) {
     StGeom g;
     g.n0 = (int)m.n0;
     g.n1 = (int)m.n1;
     g.n2 = (int)m.n2;
     g.tiles_y = (int)((m.n1 + kStTY - 1) / kStTY);
-    g.tiles_z = (int)((m.n2 + kStTZ - 1) / kStTZ);
+    g.tiles_z = (int)((m.n2 + st_tz<T>() - 1) / st_tz<T>());
     g.chunks = (int)((m.n0 + kStXC - 1) / kStXC);
     return g;
 }
@@ -1582,9 +1589,9 @@ static int sm_layer_fwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
         if ((rc = colpass<T, +1>(m.n1, m.n0, nh, m.n1 * hp, hp, tw, PlainColOp<T>{P.spec3, P.spec3, T(1)}, st))) return rc;
         if ((rc = zpass_c2r<T>(nrows, m.n2, tw, HalfRowLoad<T>{P.spec3, hp}, RealRowStore<T>{phi, m.n2, T(1)}, st, nullptr)))
             return rc;
-        const StGeom sg = st_geom(m);
+        const StGeom sg = st_geom<T>(m);
         KTimer _t("grad6", st);
-        hipLaunchKernelGGL(k_grad6<T>, dim3((unsigned)(sg.chunks * sg.tiles_y * sg.tiles_z)), dim3(256), 0, st, phi, F, sg,
+        hipLaunchKernelGGL(k_grad6<T>, dim3((unsigned)(sg.chunks * sg.tiles_y * sg.tiles_z)), dim3(4 * st_tz<T>()), 0, st, phi, F, sg,
                            (T)((double)m.n0 / (60.0 * pdesc->box[0])), (T)((double)m.n1 / (60.0 * pdesc->box[1])),
                            (T)((double)m.n2 / (60.0 * pdesc->box[2])));
     } else {
@@ -1644,10 +1651,10 @@ static int sm_layer_bwd(int kind, const T* pos, const ldl_pos_desc* pdesc, int64
     const bool p2 = fft_mesh_pow2(m);
     if (p2) {
         T* dv = reinterpret_cast<T*>(P.spec3 + 2 * hc);  // a real mesh in the free third spectrum
-        const StGeom sg = st_geom(m);
+        const StGeom sg = st_geom<T>(m);
         {
             KTimer _t("div6", st);
-            hipLaunchKernelGGL(k_div6<T>, dim3((unsigned)(sg.chunks * sg.tiles_y * sg.tiles_z)), dim3(256), 0, st, P.mesh3, dv, sg,
+            hipLaunchKernelGGL(k_div6<T>, dim3((unsigned)(sg.chunks * sg.tiles_y * sg.tiles_z)), dim3(4 * st_tz<T>()), 0, st, P.mesh3, dv, sg,
                                (T)((double)m.n0 / (60.0 * pdesc->box[0])), (T)((double)m.n1 / (60.0 * pdesc->box[1])),
                                (T)((double)m.n2 / (60.0 * pdesc->box[2])));
         }
