@@ -532,7 +532,7 @@ k_sm_x_bwd(ColGeom g, const c2<T>* Sq, const c2<T>* __restrict__ tw, const c2<T>
 // differences from the current plane staged in LDS with 3-cell halos.  All
 // indices are wrapped, so a tile larger than the mesh reads valid periodic
 // values and only its stores are bounded.
-constexpr int kStTY = 16, kStXC = 32;
+constexpr int kStTY = 16, kStXC = 64;
 // z width of a tile: its 3-cell z halo costs two extra cache lines per row, so
 // wider rows fetch less (FP32, 512^3: grad6 0.500 -> 0.466 ms, div6 0.471 ->
 // 0.432 ms from 64 to 256); FP64 keeps 128 (at 1024 threads its divergence
