@@ -171,6 +171,7 @@ def large(n=1024, nstep=3, steps=2, sub=256, log=print):
     m = JERALDModel(X, target, pm, Nstep=nstep, kind="sm", L1=False, smoothing=("gaussian", cell), lattice=(n, n, n),
                     save="positions")
     del X
+    L.workspaces.clear()  # the input generator's FP64 transform workspace
     torch.cuda.empty_cache()
     m.loss_and_grad(p)
     torch.cuda.synchronize()
