@@ -1006,8 +1006,19 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
             xs[k][0] = q[0];
             xs[k][1] = q[1];
             xs[k][2] = q[2];
+        }
+        if (mass) {  // uniform: a particle's NC masses in one load
 #pragma unroll
-            for (int c = 0; c < NC; c++) ms[k][c] = mass ? mass[(int64_t)pi[k] * NC + c] : mscalar;
+            for (int k = 0; k < KB; k++) {
+                if (!FAST && k >= (int)nk) break;
+#pragma unroll
+                for (int c = 0; c < NC; c++) ms[k][c] = mass[(int64_t)pi[k] * NC + c];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < KB; k++)
+#pragma unroll
+                for (int c = 0; c < NC; c++) ms[k][c] = mscalar;
         }
 #pragma unroll
         for (int k = 0; k < KB; k++) {
