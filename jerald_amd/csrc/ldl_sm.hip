@@ -43,10 +43,22 @@ static_assert(kBlk == kChunk, "gathers take one particle per thread");
 // axis keeps N/2+1 entries, so its last one is -pi/cell) and the 6th-order
 // finite-difference gradient symbol (45 sin kD - 9 sin 2kD + sin 3kD)/(30 D)
 // (model.py:200-201), evaluated in FP64.
+template <typename T> struct AxisTabs {
+    T* k[3];
+    T* D[3];
+    int64_t n[3], count[3];
+    double cell[3];
+};
 template <typename T>
-__global__ void k_axis_tables(T* __restrict__ k, T* __restrict__ D, int64_t n, int64_t count, double cell) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= count) return;
+__global__ void k_axis_tables(AxisTabs<T> a) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int d = 0;  // the three axes' tables, one after the other
+    while (d < 2 && i >= a.count[d]) i -= a.count[d++];
+    if (i >= a.count[d]) return;
+    const int64_t n = a.n[d];
+    const double cell = a.cell[d];
+    T* __restrict__ k = a.k[d];
+    T* __restrict__ D = a.D[d];
     const int64_t m = 2 * i < n ? i : i - n;  // fftfreq order, odd n included: 0..(n-1)/2, -(n-1)/2..-1
     const double th = 2.0 * (double)m / (double)n;  // k cell / pi
     k[i] = (T)(2.0 * M_PI * (double)m / ((double)n * cell));
@@ -174,15 +186,19 @@ static KTabs<T> build_tables(T* tabs, const Mesh& m, const double box[3], hipStr
     const int64_t cnt[3] = {m.n0, m.n1, m.nh()};
     const int64_t nn[3] = {m.n0, m.n1, m.n2};
     KTabs<T> kt;
+    AxisTabs<T> a;
     T* p = tabs;
     for (int d = 0; d < 3; d++) {
-        T* kk = p;
-        T* DD = p + cnt[d];
-        hipLaunchKernelGGL(k_axis_tables<T>, dim3(grid_for(cnt[d], kBlk)), dim3(kBlk), 0, st, kk, DD, nn[d], cnt[d], box[d] / (double)nn[d]);
-        kt.k[d] = kk;
-        kt.D[d] = DD;
+        a.k[d] = p;
+        a.D[d] = p + cnt[d];
+        a.n[d] = nn[d];
+        a.count[d] = cnt[d];
+        a.cell[d] = box[d] / (double)nn[d];
+        kt.k[d] = a.k[d];
+        kt.D[d] = a.D[d];
         p += 2 * cnt[d];
     }
+    hipLaunchKernelGGL(k_axis_tables<T>, dim3(grid_for(cnt[0] + cnt[1] + cnt[2], kBlk)), dim3(kBlk), 0, st, a);
     return kt;
 }
 
@@ -193,15 +209,29 @@ struct RedMap {
     int map[12];
 };
 // One pass over the rows: thread t sums column j of rows t, t + 256, ... in
-// that order, then the waves and the four wave sums in a fixed order.
+// that order, then the waves and the four wave sums in a fixed order.  Four
+// rows' loads are issued before they are added (in the same order).
 __global__ void __launch_bounds__(256) k_reduce_cols(const double* __restrict__ part, int64_t nblocks, int stride, RedMap rm,
                                                      double* __restrict__ out, int assign) {
-    constexpr int MX = 12;
+    constexpr int MX = 12, U = 4;
     __shared__ double red[MX][4];
     double s[MX];
 #pragma unroll
     for (int j = 0; j < MX; j++) s[j] = 0.0;
-    for (int64_t b = threadIdx.x; b < nblocks; b += 256) {
+    int64_t b = threadIdx.x;
+    for (; b + (U - 1) * 256 < nblocks; b += U * 256) {
+        double v[U][MX];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int j = 0; j < MX; j++) v[u][j] = j < rm.n ? part[(b + u * 256) * stride + j] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int j = 0; j < MX; j++)
+                if (j < rm.n) s[j] += v[u][j];
+    }
+    for (; b < nblocks; b += 256) {
         const double* row = part + b * stride;
 #pragma unroll
         for (int j = 0; j < MX; j++)
