@@ -1,6 +1,6 @@
 """Text summaries of rocprofv3 output for the LDL step benchmark.
 
-Two commands, each reading the CSV files rocprofv3 wrote
+Three commands, each reading the CSV files rocprofv3 wrote
 (``--output-format csv``) under a directory, and printing a summary:
 
 ``stats DIR [--bench-json FILE]``
@@ -15,6 +15,13 @@ Two commands, each reading the CSV files rocprofv3 wrote
     device-to-device copy of B bytes (``__amd_rocclr_copyBuffer``).  The data
     lines are what ``bench.py`` reads:
     ``<timer name> <n> <dispatches> <fetch bytes> <write bytes>``.
+
+``texture DIR [DIR ...]``
+    the texture (vector memory) path of every timed kernel from ``--pmc``
+    runs of the TA, TD and TCP counters below (``_sum`` over the CUs), each run
+    with ``GRBM_GUI_ACTIVE``: busy and stall cycles as a share of CU cycles
+    (256 CUs x GRBM_GUI_ACTIVE / 8, the counter summing the 8 XCDs), and the
+    mean L2 read latency of the TCP's requests in cycles.
 
 Kernels are named by the timer the library brackets them with (the names of
 ``ldl_timer_read``), so the summary and the benchmark's own per-kernel table
@@ -189,6 +196,52 @@ def traffic(fetch_dir, write_dir, n, probe_bytes, alg=None):
     return "\n".join(lines)
 
 
+# (column, counter): TA busy / address and data stalled by the texture cache
+# (TC), TD busy / stalled by TC, TCP stalled on pending misses / tag conflicts
+_TEX = [("TA busy", "TA_TA_BUSY_sum"), ("TA addrTC", "TA_ADDR_STALLED_BY_TC_CYCLES_sum"),
+        ("TA dataTC", "TA_DATA_STALLED_BY_TC_CYCLES_sum"), ("TD busy", "TD_TD_BUSY_sum"),
+        ("TD TCst", "TD_TC_STALL_sum"), ("TCP pend", "TCP_PENDING_STALL_CYCLES_sum"),
+        ("TCP tagc", "TCP_READ_TAGCONFLICT_STALL_CYCLES_sum")]
+
+
+def texture(dirs, ncu=256, nxcd=8):
+    """Per timer: mean dispatch time, each _TEX counter as % of CU cycles (each
+    run normalised by its own GRBM_GUI_ACTIVE) and the mean L2 read latency."""
+    val = collections.defaultdict(lambda: collections.defaultdict(float))
+    dur = collections.defaultdict(list)
+    for i, d in enumerate(dirs):
+        grbm = collections.defaultdict(float)
+        sums = collections.defaultdict(lambda: collections.defaultdict(float))
+        for f in _csvs(d, "counter_collection.csv"):
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    t = timer_of(r["Kernel_Name"])
+                    if not t or "<double" in r["Kernel_Name"]:
+                        continue
+                    c, v = r["Counter_Name"], float(r["Counter_Value"])
+                    if c == "GRBM_GUI_ACTIVE":
+                        grbm[t] += v
+                        if i == 0:
+                            dur[t].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+                    else:
+                        sums[t][c] += v
+        for t, cs in sums.items():
+            cyc = ncu * grbm[t] / nxcd
+            for c, v in cs.items():
+                val[t][c] = 100.0 * v / cyc if cyc else 0.0
+                val[t]["_raw_" + c] = v
+    cols = [(h, c) for h, c in _TEX if any(c in v for v in val.values())]
+    lines = ["# % of CU cycles; L2 lat = TCP_TCC_READ_REQ_LATENCY_sum / TCP_TCC_READ_REQ_sum (cycles per request)",
+             f"{'timer':18s} {'us':>7s} " + " ".join(f"{h:>9s}" for h, _ in cols) + f" {'L2 lat':>7s}"]
+    for t in sorted(val, key=lambda t: -(sum(dur[t]) / max(len(dur[t]), 1))):
+        v = val[t]
+        rq = v.get("_raw_TCP_TCC_READ_REQ_sum", 0.0)
+        lat = v.get("_raw_TCP_TCC_READ_REQ_LATENCY_sum", 0.0) / rq if rq else float("nan")
+        us = sum(dur[t]) / len(dur[t]) if dur[t] else float("nan")
+        lines.append(f"{t:18s} {us:7.0f} " + " ".join(f"{v.get(c, float('nan')):9.1f}" for _, c in cols) + f" {lat:7.0f}")
+    return "\n".join(lines)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -201,9 +254,13 @@ def main(argv=None):
     t.add_argument("write_dir")
     t.add_argument("--n", type=int, required=True)
     t.add_argument("--probe-bytes", type=int, required=True, help="bytes the copy probe moves (read + write)")
+    x = sub.add_parser("texture")
+    x.add_argument("dirs", nargs="+")
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         print(stats(a.dir, a.bench_json, a.top))
+    elif a.cmd == "texture":
+        print(texture(a.dirs))
     else:
         root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
         sys.path.insert(0, root)

@@ -65,3 +65,22 @@ def test_committed_traffic_summary_is_read_by_bench():
     for k in ("sm_bwd_gather", "readout3_update", "paint_deposit3", "grad6"):
         t = bench.measured_traffic(k, 512)
         assert t is not None and t > 0
+
+
+def test_texture_shares_of_cu_cycles(tmp_path):
+    gather = "void ldl::k_sm_bwd_gather<float, ldl::GeomP2<float> >(float const*)"
+    # one dispatch: GRBM_GUI_ACTIVE sums 8 XCDs of 1000 cycles; 256 CUs -> 256000 CU cycles
+    d = tmp_path / "g"
+    d.mkdir()
+    with open(d / "pmc_counter_collection.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, _COLS)
+        w.writeheader()
+        row = {c: 0 for c in _COLS} | {"Dispatch_Id": 1, "Kernel_Name": gather, "Start_Timestamp": 1000, "End_Timestamp": 3000}
+        for c, v in (("GRBM_GUI_ACTIVE", 8000.0), ("TD_TD_BUSY_sum", 128000.0), ("TCP_PENDING_STALL_CYCLES_sum", 64000.0),
+                     ("TCP_TCC_READ_REQ_sum", 10.0), ("TCP_TCC_READ_REQ_LATENCY_sum", 6000.0)):
+            w.writerow(row | {"Counter_Name": c, "Counter_Value": v})
+    lines = R.texture([str(d)]).splitlines()
+    head = lines[1].split()
+    vals = dict(zip(["timer", "us", "TD busy", "TCP pend", "L2 lat"], lines[2].split()))
+    assert "TD" in head and vals["timer"] == "sm_bwd_gather" and vals["us"] == "2"
+    assert vals["TD busy"] == "50.0" and vals["TCP pend"] == "25.0" and vals["L2 lat"] == "600"
