@@ -961,14 +961,15 @@ __global__ void __launch_bounds__(TB) k_paint_tiles(const T* __restrict__ pos, P
     // all 8 corners of every particle in the tile.
     const unsigned r0 = __builtin_amdgcn_readfirstlane(rstart[0]), nfast = __builtin_amdgcn_readfirstlane(rn0);
     // KB particles per thread per round; all their loads are issued before any
-    // deposit so the dependent sorted[j] -> pos[p] chains overlap (4 keeps the
-    // 32-bit three-component paint at 74 VGPRs, 6 waves per SIMD, and the
-    // one-component paint at 62).  A round takes TB KB particles, wave w the 64
+    // deposit so the dependent sorted[j] -> pos[p] chains overlap (measured at
+    // 512^3: the one-component paint fastest at 6, the 32-bit three-component
+    // paint at 4, 59 VGPRs; FP64 three-component 4, whose 1024-thread groups
+    // would spill at 6).  A round takes TB KB particles, wave w the 64
     // at w 64 + k TB for k < KB; a wave skips its sub-batches that start past
     // the end (the waves of a tile stay within one sub-batch of each other),
     // and a wave whose sub-batches all lie in face class 0 deposits without
     // range lookups or corner tests.
-    constexpr int KB = (ACC32 || NC == 1) ? 4 : 6;
+    constexpr int KB = NC == 1 ? 6 : (ACC32 || sizeof(T) == 8) ? 4 : 6;
     const unsigned lane = threadIdx.x & 63u;
     auto round = [&](const unsigned wb, const unsigned nk, auto fast_tag) {
         constexpr bool FAST = decltype(fast_tag)::value;
