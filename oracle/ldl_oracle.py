@@ -655,6 +655,6 @@ def zeldovich_displacement(nside, box, rms_cells, seed, ncell=None):
 
 
 def default_params(nstep):
-    """Reference-order parameters for BASELINE's config: alpha=0.01, gamma=1,
-    kh(low-k cutoff)=0.1, kl(high-k cutoff)=1.0, nu=0 per layer; mu=1, w=1, b=0."""
+    """Reference-order parameters of the 64^3 one-layer FP64 workload: alpha=0.01,
+    gamma=1, kh(low-k cutoff)=0.1, kl(high-k cutoff)=1.0, nu=0 per layer; mu=1, w=1, b=0."""
     return np.array([0.01, 1.0, 0.1, 1.0, 0.0] * nstep + [1.0, 1.0, 0.0], dtype=np.float64)
