@@ -334,7 +334,11 @@ __global__ void __launch_bounds__((FftCfg<T, N>::THREADS)) k_colpass(ColGeom g, 
     constexpr int P = Cfg::P, TH = Cfg::THREADS;
     __shared__ c2<T> lds[Cfg::LDS_ELEMS];
     __shared__ c2<T> twl[N];
-    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    // walks each XCD's range backward: between z passes (which walk forward,
+    // like the rows they write) each column pass and the z pass after it start
+    // on the lines written last, which the 256 MB memory-side cache still
+    // holds (y passes 0.2205 -> 0.212 ms, the z c2r after them 0.241 -> 0.229)
+    const int64_t tile = xcd_tile_rev(blockIdx.x, gridDim.x);
     const int64_t plane = tile / g.tiles_per_plane;
     const int64_t col0 = (tile - plane * g.tiles_per_plane) * P;
     const int64_t base = plane * g.plane_stride + col0;

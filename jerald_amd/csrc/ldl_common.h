@@ -265,6 +265,15 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t bid, int64_t nb) {
     return x * q + (x < r ? x : r) + local;
 }
 
+// The same ranges, each walked from its end: a pass that reads what the
+// previous pass wrote in xcd_tile order starts on the lines written last.
+__device__ __forceinline__ int64_t xcd_tile_rev(int64_t bid, int64_t nb) {
+    const int64_t x = bid & 7, local = bid >> 3;
+    const int64_t q = nb >> 3, r = nb & 7;
+    const int64_t len = q + (x < r ? 1 : 0);
+    return x * q + (x < r ? x : r) + (len - 1 - local);
+}
+
 // Order in which the workgroups of a particle gather sweep the particles.
 // A workgroup takes chunks of kChunk consecutive particles (one per thread);
 // its chunk id comes from xcd_tile, so each XCD owns a contiguous run of chunk
