@@ -825,47 +825,78 @@ static __global__ void __launch_bounds__(1024) k_scan_apply(const unsigned* coun
 }
 
 // Counting-sort pass 2: particle indices grouped by key.  cursor[k] starts
-// at the key's offset; each run of equal keys in a wavefront reserves its
-// ranks with one atomic (PPT particles per thread, all atomics issued before
-// any result is used) and places its particles in lane order.  Rank order
-// inside a key depends on atomic order, which no paint result depends on
-// (integer sums).  Whole wavefronts step together (shuffles over all lanes).
+// at the key's offset.  A workgroup reserves the ranks of a chunk in two
+// levels: each run of equal keys in a wavefront takes its local ranks with one
+// LDS atomic on a window of kScatterWin counters from the chunk's smallest key
+// (the chunk's rows share their tiles, so its runs fall on a few dozen keys),
+// then each key the window holds takes a block of global ranks with one
+// device atomic.  A run whose key lies outside the window reserves with a
+// device atomic of its own.  Particles are placed in lane order within a run;
+// rank order inside a key depends on atomic order, which no paint result
+// depends on (integer sums).  Whole wavefronts step together (shuffles over
+// all lanes); every workgroup runs the same number of barriers per chunk.
+constexpr int kScatterWin = 1024;
+
 template <int PPT>
 __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned* __restrict__ key, unsigned* __restrict__ cursor,
                                                      unsigned* __restrict__ sorted) {
-    constexpr int CH = 256 * PPT;
+    constexpr int CH = 256 * PPT, W = kScatterWin;
     constexpr unsigned EMPTY = 0xFFFFFFFFu;
-    const int lane = threadIdx.x & 63;
+    __shared__ unsigned win[W];  // local counts, then the global base of each key
+    __shared__ unsigned wmin[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
     // XCD-contiguous chunks: each XCD writes the ranks of its own band of
     // particles, so the partial cache lines of a key's range fill in one L2
     const int64_t nchunks = (np + CH - 1) / CH, per = (nchunks + gridDim.x - 1) / gridDim.x;
     const int64_t c0 = xcd_tile(blockIdx.x, gridDim.x) * per, c1 = c0 + per < nchunks ? c0 + per : nchunks;
     for (int64_t ch = c0; ch < c1; ch++) {
-        unsigned k[PPT], base[PPT];
-        int hl[PPT];
+        unsigned k[PPT], loc[PPT];
+        unsigned m = EMPTY;
 #pragma unroll
         for (int q = 0; q < PPT; q++) {
             const int64_t p = ch * CH + q * 256 + threadIdx.x;
             k[q] = p < np ? key[p] : EMPTY;
+            m = k[q] < m ? k[q] : m;
         }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned y = __shfl_xor(m, o);
+            m = y < m ? y : m;
+        }
+        for (int i = threadIdx.x; i < W; i += 256) win[i] = 0u;
+        if (lane == 0) wmin[wv] = m;
+        __syncthreads();
+        // the chunk's first particle exists (ch < nchunks), so kb is a key
+        const unsigned kb0 = wmin[0] < wmin[1] ? wmin[0] : wmin[1], kb1 = wmin[2] < wmin[3] ? wmin[2] : wmin[3];
+        const unsigned kb = kb0 < kb1 ? kb0 : kb1;
 #pragma unroll
         for (int q = 0; q < PPT; q++) {
             const unsigned prev = __shfl_up(k[q], 1);
             const bool head = lane == 0 || prev != k[q];
             const unsigned long long heads = __ballot(head);
-            hl[q] = 63 - __clzll(heads & upto);
+            const int hl = 63 - __clzll(heads & upto);
             const unsigned long long after = heads & ~upto;
             const int next = after ? __ffsll((long long)after) - 1 : 64;
-            base[q] = 0u;
-            if (head && k[q] != EMPTY) base[q] = atomicAdd(&cursor[k[q]], (unsigned)(next - lane));
+            unsigned b = 0u;
+            if (head && k[q] != EMPTY) {
+                const unsigned len = (unsigned)(next - lane);
+                b = k[q] - kb < (unsigned)W ? atomicAdd(&win[k[q] - kb], len) : atomicAdd(&cursor[k[q]], len);
+            }
+            loc[q] = __shfl(b, hl) + (unsigned)(lane - hl);
         }
+        __syncthreads();
+        for (int i = threadIdx.x; i < W; i += 256) {
+            const unsigned c = win[i];
+            if (c) win[i] = atomicAdd(&cursor[kb + (unsigned)i], c);
+        }
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < PPT; q++) {
-            const unsigned b = __shfl(base[q], hl[q]);
             const int64_t p = ch * CH + q * 256 + threadIdx.x;
-            if (p < np) sorted[b + (unsigned)(lane - hl[q])] = (unsigned)p;
+            if (p < np) sorted[(k[q] - kb < (unsigned)W ? win[k[q] - kb] : 0u) + loc[q]] = (unsigned)p;
         }
+        __syncthreads();  // the next chunk clears the window
     }
 }
 
