@@ -689,14 +689,21 @@ __device__ __forceinline__ unsigned tile_key(const uint32_t id[3], const TileGeo
 // Counting-sort pass 1: key of each particle, and the particle counts per
 // key.  Consecutive lanes hold consecutive particles, and particles stored in
 // lattice order fall into a few runs of equal keys per wavefront: each run's
-// first lane adds the run's length with one global atomic.  All PPT positions
-// of a thread are loaded before any is decoded (np >= 1).
+// first lane adds the run's length to an LDS window of kSortWin counters
+// from the chunk's smallest key (or, outside it, to the global count), and
+// the window's nonzero counters are added to the global counts once per
+// chunk.  All PPT positions of a thread are loaded before any is decoded
+// (np >= 1).
+constexpr int kSortWin = 1024;  // key window of the sort's per-workgroup counters
+
 template <typename T, int PPT, int DM>
 __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, PosDec<T> pd, int64_t np, MeshP2 mp, TileGeom tg,
                                                   unsigned* __restrict__ counts, unsigned* __restrict__ key) {
-    constexpr int CH = 256 * PPT;
+    constexpr int CH = 256 * PPT, W = kSortWin;
     constexpr unsigned EMPTY = 0xFFFFFFFFu;
-    const int lane = threadIdx.x & 63;
+    __shared__ unsigned win[W];
+    __shared__ unsigned wmin[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
     const int64_t nchunks = (np + CH - 1) / CH;
     for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
@@ -721,6 +728,20 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
                 st_stream(key + p, k[q]);
             }
         }
+        unsigned m = EMPTY;
+#pragma unroll
+        for (int q = 0; q < PPT; q++) m = k[q] < m ? k[q] : m;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned y = __shfl_xor(m, o);
+            m = y < m ? y : m;
+        }
+        for (int i = threadIdx.x; i < W; i += 256) win[i] = 0u;
+        if (lane == 0) wmin[wv] = m;
+        __syncthreads();
+        // the chunk's first particle exists (ch < nchunks), so kb is a key
+        const unsigned kb0 = wmin[0] < wmin[1] ? wmin[0] : wmin[1], kb1 = wmin[2] < wmin[3] ? wmin[2] : wmin[3];
+        const unsigned kb = kb0 < kb1 ? kb0 : kb1;
 #pragma unroll
         for (int q = 0; q < PPT; q++) {
             const unsigned prev = __shfl_up(k[q], 1);
@@ -728,8 +749,18 @@ __global__ void __launch_bounds__(256) k_bin_keys(const T* __restrict__ pos, Pos
             const unsigned long long heads = __ballot(head);
             const unsigned long long after = heads & ~upto;
             const int next = after ? __ffsll((long long)after) - 1 : 64;
-            if (head && k[q] != EMPTY) atomicAdd(&counts[k[q]], (unsigned)(next - lane));
+            if (head && k[q] != EMPTY) {
+                const unsigned len = (unsigned)(next - lane);
+                if (k[q] - kb < (unsigned)W) atomicAdd(&win[k[q] - kb], len);
+                else atomicAdd(&counts[k[q]], len);
+            }
         }
+        __syncthreads();
+        for (int i = threadIdx.x; i < W; i += 256) {
+            const unsigned c = win[i];
+            if (c) atomicAdd(&counts[kb + (unsigned)i], c);
+        }
+        __syncthreads();  // the next chunk clears the window
     }
 }
 
@@ -827,7 +858,7 @@ static __global__ void __launch_bounds__(1024) k_scan_apply(const unsigned* coun
 // Counting-sort pass 2: particle indices grouped by key.  cursor[k] starts
 // at the key's offset.  A workgroup reserves the ranks of a chunk in two
 // levels: each run of equal keys in a wavefront takes its local ranks with one
-// LDS atomic on a window of kScatterWin counters from the chunk's smallest key
+// LDS atomic on a window of kSortWin counters from the chunk's smallest key
 // (the chunk's rows share their tiles, so its runs fall on a few dozen keys),
 // then each key the window holds takes a block of global ranks with one
 // device atomic.  A run whose key lies outside the window reserves with a
@@ -835,12 +866,10 @@ static __global__ void __launch_bounds__(1024) k_scan_apply(const unsigned* coun
 // rank order inside a key depends on atomic order, which no paint result
 // depends on (integer sums).  Whole wavefronts step together (shuffles over
 // all lanes); every workgroup runs the same number of barriers per chunk.
-constexpr int kScatterWin = 1024;
-
 template <int PPT>
 __global__ void __launch_bounds__(256) k_bin_scatter(int64_t np, const unsigned* __restrict__ key, unsigned* __restrict__ cursor,
                                                      unsigned* __restrict__ sorted) {
-    constexpr int CH = 256 * PPT, W = kScatterWin;
+    constexpr int CH = 256 * PPT, W = kSortWin;
     constexpr unsigned EMPTY = 0xFFFFFFFFu;
     __shared__ unsigned win[W];  // local counts, then the global base of each key
     __shared__ unsigned wmin[4];
